@@ -1,0 +1,62 @@
+"""sl3dpp_amd: the MI355X (gfx950) implementation of the data-parallel core of
+sl3dpp's 3D lateral-viscosity step, loaded over its C ABI
+(include/sl3dpp_amd.h) with ctypes.
+
+The HIP library is required: importing raises if libsl3dpp_amd.so is not
+built, and there is no CPU fallback on any product path.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libsl3dpp_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "sl3dpp_amd.h")
+
+if not os.path.exists(LIBRARY_PATH):
+    raise ImportError(
+        f"{LIBRARY_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)"
+    )
+
+lib = ctypes.CDLL(LIBRARY_PATH)
+lib.sl3_last_error.restype = ctypes.c_char_p
+lib.sl3_version.restype = ctypes.c_char_p
+
+
+class Sl3Error(RuntimeError):
+    """A failed C-ABI call: status code and the library's message."""
+
+    def __init__(self, status, message):
+        super().__init__(f"sl3 status {status}: {message}")
+        self.status = status
+
+
+class InvalidArgument(Sl3Error, ValueError):
+    """The reference's std::invalid_argument."""
+
+
+class FactorisationError(Sl3Error):
+    """The reference's std::runtime_error from a failed factorisation."""
+
+
+def check(status):
+    """Raises the exception the reference would throw for a status code."""
+    if status == 0:
+        return
+    message = lib.sl3_last_error().decode(errors="replace")
+    if status == -1:
+        raise InvalidArgument(status, message)
+    if status > 0:
+        raise FactorisationError(status, message)
+    raise Sl3Error(status, message)
+
+
+def synchronize():
+    check(lib.sl3_synchronize())
+
+
+def version():
+    return lib.sl3_version().decode()
+
+
+from . import band  # noqa: E402,F401  (the band solvers' Python mirror)

@@ -1,0 +1,119 @@
+// Shared plumbing of the HIP library: status codes, the last-error string,
+// HIP error checks and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "sl3dpp_amd.h"
+
+namespace sl3amd {
+
+// The message of the last failed call on this thread (sl3_last_error).
+void SetLastError(const std::string& message);
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgumentError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+// A factorisation that failed outright (the reference's std::runtime_error
+// from DegreeOperator::Factorise, src/DegreeOperator.cpp:848-851).
+struct NumericalError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define SL3_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t err_ = (call);                                                \
+    if (err_ != hipSuccess) {                                                \
+      throw ::sl3amd::HipError(std::string(#call) + ": " +                   \
+                               hipGetErrorString(err_) + " (" __FILE__ ":" + \
+                               std::to_string(__LINE__) + ")");              \
+    }                                                                        \
+  } while (0)
+
+// Runs a C-ABI body, mapping exceptions to status codes.
+template <class F>
+int Guard(F&& body) {
+  try {
+    body();
+    return SL3_OK;
+  } catch (const ArgumentError& e) {
+    SetLastError(e.what());
+    return SL3_EINVAL;
+  } catch (const NumericalError& e) {
+    SetLastError(e.what());
+    return SL3_ENUMERIC;
+  } catch (const HipError& e) {
+    SetLastError(e.what());
+    return SL3_EHIP;
+  } catch (const std::bad_alloc&) {
+    SetLastError("out of host memory");
+    return SL3_ENOMEM;
+  } catch (const std::exception& e) {
+    SetLastError(e.what());
+    return SL3_EINTERNAL;
+  }
+}
+
+inline int CeilDiv(long long a, long long b) {
+  return static_cast<int>((a + b - 1) / b);
+}
+
+// A device buffer owned by the library (freed on destruction).
+template <class T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(size_t count) { Resize(count); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : ptr_(o.ptr_), count_(o.count_) {
+    o.ptr_ = nullptr;
+    o.count_ = 0;
+  }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) {
+      Free();
+      ptr_ = o.ptr_;
+      count_ = o.count_;
+      o.ptr_ = nullptr;
+      o.count_ = 0;
+    }
+    return *this;
+  }
+  ~DeviceArray() { Free(); }
+  void Resize(size_t count) {
+    if (count == count_) return;
+    Free();
+    if (count > 0) {
+      SL3_HIP(hipMalloc(reinterpret_cast<void**>(&ptr_), count * sizeof(T)));
+    }
+    count_ = count;
+  }
+  void Upload(const T* host, size_t count, hipStream_t stream) {
+    Resize(count);
+    if (count > 0) {
+      SL3_HIP(hipMemcpyAsync(ptr_, host, count * sizeof(T),
+                             hipMemcpyHostToDevice, stream));
+    }
+  }
+  T* data() const { return ptr_; }
+  size_t size() const { return count_; }
+
+ private:
+  void Free() {
+    if (ptr_ != nullptr) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    count_ = 0;
+  }
+  T* ptr_ = nullptr;
+  size_t count_ = 0;
+};
+
+}  // namespace sl3amd
