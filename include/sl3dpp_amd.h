@@ -89,6 +89,153 @@ int sl3_band_lu_solve_batched(int count, const int* n, const int* kl,
                               const int64_t* b_off, const int64_t* row_stride,
                               const int64_t* col_stride, double* b_dev);
 
+/* ===================================================================== *
+ * The Earth and its radial set-up (host): a layered body with properties *
+ * linear in radius per layer, meshed into GLL elements, one Maxwell       *
+ * layer (include/sl3dpp/RadialMesh.hpp, Material.hpp, Rheology.hpp:       *
+ * Maxwell; src/Viscoelastic.cpp:114-176 for the memory points' moduli).   *
+ * SI inputs; everything inside is non-dimensional as the reference's.    *
+ * ===================================================================== */
+#define SL3_MAX_LAYERS 8
+typedef struct {
+  int layers;
+  double r0[SL3_MAX_LAYERS], r1[SL3_MAX_LAYERS];         /* m */
+  double rho0[SL3_MAX_LAYERS], rho1[SL3_MAX_LAYERS];     /* kg m^-3 */
+  double kappa0[SL3_MAX_LAYERS], kappa1[SL3_MAX_LAYERS]; /* Pa */
+  double mu0[SL3_MAX_LAYERS], mu1[SL3_MAX_LAYERS];       /* Pa; 0 = fluid */
+  int elements[SL3_MAX_LAYERS];
+  int ngll;
+  int viscous_layer;
+  double viscosity_boundary; /* m */
+  double viscosity_upper;    /* Pa s above the boundary */
+  double viscosity_lower;    /* Pa s at or below it */
+} sl3_earth_spec;
+
+typedef struct sl3_earth sl3_earth;
+int sl3_earth_create(const sl3_earth_spec* spec, sl3_earth** out);
+int sl3_earth_destroy(sl3_earth* earth);
+/* The non-dimensional scales: length (m), time (s), mass (kg). */
+int sl3_earth_scales(const sl3_earth* earth, double* length, double* time,
+                     double* mass);
+/* Mesh element count and nodes per element. */
+int sl3_earth_mesh(const sl3_earth* earth, int* elements, int* ngll);
+
+/* Host arrays of the set-up (no device work): "radius" (element x gll
+ * node radii), "jac" (per element), "gll_weight", "gll_deriv" (ngll x ngll),
+ * "point_value", "point_deriv" (points x ngll), "point_weight",
+ * "point_radius" (element x point), "point_mu", "point_itau" (the Maxwell
+ * branch at the points, element x point), "fluid" (per element),
+ * "surface_gravity", "memory_elements" (first, count). dst NULL returns the
+ * size through *count. */
+int sl3_earth_export(const sl3_earth* earth, const char* name, double* dst,
+                     int64_t* count);
+/* One assembled degree system (host; DegreeOperator / ToroidalOperator,
+ * src/DegreeOperator.cpp:24-474, src/ToroidalOperator.cpp:14-147): kind 0
+ * spheroidal, 1 toroidal, with the memory points' shear of (scheme, dt)
+ * (dt = 0 the elastic operators'; dt < 0 none). Returns ndof, kd, the first
+ * element, the pinned dof (or -1); fills, where non-NULL, the unscaled
+ * upper band ((kd+1) x ndof), the dof map (per global mesh node: U, V, phi
+ * for spheroidal, W for toroidal; -1 absent) and the unit load vector
+ * (spheroidal, ndof). */
+int sl3_earth_degree_system(const sl3_earth* earth, int l, int kind, int scheme,
+                            double dt, double eps, int* ndof, int* kd,
+                            int* first_element, int* pinned, double* band,
+                            int* dofs, double* load);
+
+/* ===================================================================== *
+ * Viscoelastic3D (include/sl3dpp/Viscoelastic3D.hpp:89-675): the 3D      *
+ * lateral-viscosity step on the GPU. One Maxwell branch, Gauss memory    *
+ * points, the fixed-point step solver with adaptive Anderson mixing.     *
+ * ===================================================================== */
+typedef struct {
+  double eps;           /* truncation level, 1e-8 */
+  double oversampling;  /* 1.5 */
+  int max_iterations;   /* 40 */
+  double rtol;          /* 1e-11 */
+  double damping;       /* 1 */
+  int anderson_memory;  /* 5 */
+  double anderson_rate; /* 0.5; <= 0 for Anderson from the first pass */
+} sl3_step_options;
+
+typedef struct {
+  int iterations;
+  double residual;
+  int converged;
+  int mixed_passes;
+  float start_ms, passes_ms, finish_ms; /* device time of the phases */
+} sl3_step_info;
+
+enum { SL3_EXPONENTIAL_TRAPEZOID = 0, SL3_BACKWARD_EULER = 1 };
+
+typedef struct sl3_operator sl3_operator;
+typedef struct sl3_state sl3_state;
+
+void sl3_step_options_default(sl3_step_options* options);
+/* delta_dev: delta(r, theta, phi) sampled on every memory node's grid,
+ * [node][theta][phi] on the device (viscosity eta_ref exp(delta)), or NULL
+ * for none (src/Viscoelastic3D.cpp:226-287). The earth must outlive it. */
+int sl3_operator_create(const sl3_earth* earth, int lmax, const double* delta_dev,
+                        const sl3_step_options* options, sl3_operator** out);
+int sl3_operator_destroy(sl3_operator* op);
+int sl3_operator_set_options(sl3_operator* op, const sl3_step_options* options);
+/* Replaces the lateral field: itau(x) = itau_ref(node) exp(-delta(x)) from
+ * delta_dev ([node][theta][phi], device; NULL for delta = 0). */
+int sl3_operator_set_delta(sl3_operator* op, const double* delta_dev);
+/* Memory nodes, grid shape, doubles of a spectral tensor (device layout). */
+int sl3_operator_shape(const sl3_operator* op, int* nodes, int* ntheta,
+                       int* nphi, int64_t* spectral_doubles);
+/* Host arrays: node radii (nodes), colatitudes and their weights (ntheta). */
+int sl3_operator_geometry(const sl3_operator* op, double* radii, double* theta,
+                          double* wtheta);
+
+/* LoadCallable3D: write degree l's m >= 0 columns at time t as (re, im)
+ * pairs into columns (2(l+1) doubles) and return 1, or return 0 for none. */
+typedef int (*sl3_load_fn)(int l, double t, double* columns, void* user);
+
+int sl3_state_zero(sl3_operator* op, sl3_state** out);
+int sl3_state_destroy(sl3_state* state);
+int sl3_state_time(const sl3_state* state, double* t);
+/* Step (src/Viscoelastic3D.cpp:1982-2030), in place on the state.
+ * start_dev: a warm start for the fixed point (device layout spectral
+ * tensor), or NULL for the step's own start strain. */
+int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme,
+             sl3_load_fn load, void* user, const double* start_dev,
+             sl3_step_info* info);
+
+/* Host copies for checks and outputs. Names:
+ *   state:    "coefficients" (the strain coefficients, reference l-major
+ *             layout per node, blocks spin -2, -1, 0 end to end, complex),
+ *             "coefficients_device" (device layout), "m0" "m1" "m2"
+ *             (memory grid components), "strain0" "strain1" "strain2";
+ *   operator: "weight" "beta" "itau" (grid fields), "fixed" "analysed"
+ *             (device layout spectral), "solution:<task>" (the last pass's
+ *             solution of a task, ndof x (l+1) complex column-major),
+ *             "tasks" (per task l, kind, ndof), "residuals" (last step).
+ * dst NULL returns the size in doubles through *count. */
+int sl3_state_export(const sl3_operator* op, const sl3_state* state,
+                     const char* name, double* dst, int64_t* count);
+int sl3_operator_export(sl3_operator* op, const char* name, double* dst,
+                        int64_t* count);
+/* Copies a spectral tensor between the device layout and the reference's
+ * (both device pointers). */
+int sl3_spectral_to_reference(const sl3_operator* op, const double* device_dev,
+                              double* reference_dev);
+int sl3_spectral_from_reference(const sl3_operator* op, const double* reference_dev,
+                                double* device_dev);
+
+/* ===================================================================== *
+ * Batched generalised spherical-harmonic transforms over the operator's *
+ * memory nodes on its grid: SurfaceGrid::SynthesiseSpinBatch /          *
+ * ExpandSpinBatch (spin -2, -1) and SynthesiseBatch / ExpandBatch       *
+ * (spin 0, real), include/sl3dpp/SurfaceGrid.hpp:164-253. Coefficients   *
+ * in the reference layout per node, end to end; fields [node][theta]     *
+ * [phi] (complex for spins -2, -1, real for 0). Device pointers.         *
+ * ===================================================================== */
+int sl3_gsh_synthesise(const sl3_operator* op, int spin, const double* coef_dev,
+                       double* field_dev);
+int sl3_gsh_analyse(const sl3_operator* op, int spin, const double* field_dev,
+                    double* coef_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,7 @@ static cplx cadd(cplx a, cplx b) { return C(a.re + b.re, a.im + b.im); }
 static cplx csub(cplx a, cplx b) { return C(a.re - b.re, a.im - b.im); }
 static cplx cscale(double s, cplx a) { return C(s * a.re, s * a.im); }
 static cplx ctimes_i(cplx a) { return C(-a.im, a.re); }
+static cplx cdiv(cplx a, double d) { return C(a.re / d, a.im / d); }
 
 /* include/sl3dpp/SpinComponents.hpp:53-72 */
 void sl3o_deviatoric_components(int l, const double* v, double r, double* A) {
@@ -50,13 +51,14 @@ void sl3o_deviatoric_components(int l, const double* v, double r, double* A) {
   const double zeta2 = l * (l + 1.0);
   const double zeta = sqrt(zeta2);
   cplx out[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  out[2] = cscale(1.0 / (3.0 * r), cadd(csub(rdU, U), cscale(0.5 * zeta2, V)));
+  /* complex / real divides each part, as std::complex does */
+  out[2] = cdiv(cadd(csub(rdU, U), cscale(0.5 * zeta2, V)), 3.0 * r);
   if (l >= 1) {
     const cplx tangential = cadd(csub(rdV, V), U);
     const cplx twist = csub(W, rdW);
-    const double f = zeta / (2.0 * sqrt(2.0) * r);
-    out[1] = cscale(f, cadd(tangential, ctimes_i(twist)));
-    out[3] = cscale(f, csub(tangential, ctimes_i(twist)));
+    const double den = 2.0 * sqrt(2.0) * r;
+    out[1] = cdiv(cscale(zeta, cadd(tangential, ctimes_i(twist))), den);
+    out[3] = cdiv(cscale(zeta, csub(tangential, ctimes_i(twist))), den);
   }
   if (l >= 2) {
     const double f = 0.5 * zeta * sqrt(zeta2 - 2.0) / r;

@@ -1,0 +1,555 @@
+/*
+ * step.c -- the oracle's 3D step (see step.h). Test infrastructure only.
+ */
+#include "step.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct sl3o_mixer sl3o_mixer;
+sl3o_mixer* sl3o_mixer_create(int memory);
+void sl3o_mixer_destroy(sl3o_mixer* mx);
+int sl3o_mixer_update(sl3o_mixer* mx, size_t n, double* x, const double* gx);
+
+static double g_residuals[4096];
+static int g_residual_count = 0;
+
+int sl3o_last_residuals(double* out, int capacity) {
+  const int n = g_residual_count < capacity ? g_residual_count : capacity;
+  memcpy(out, g_residuals, sizeof(double) * n);
+  return g_residual_count;
+}
+
+/* ---- layouts (include/sl3dpp/SurfaceGrid.hpp:116-118, 191-200) ---------- */
+
+static int64_t block_size(int b, int lmax) {
+  const int64_t L1 = lmax + 1;
+  return b == 2 ? L1 * (lmax + 2) / 2 : L1 * L1 - (b == 0 ? 4 : 1);
+}
+static int spin_of(int b) { return b == 0 ? -2 : (b == 1 ? -1 : 0); }
+static int64_t block_offset(int b, int lmax, int nodes) {
+  int64_t off = 0;
+  for (int k = 0; k < b; ++k) off += 2 * block_size(k, lmax) * nodes;
+  return off;
+}
+int64_t sl3o_spectral_doubles(int lmax, int nodes) {
+  return block_offset(3, lmax, nodes);
+}
+static int64_t index_of(int b, int l, int m) {
+  if (b == 2) return (int64_t)l * (l + 1) / 2 + m;
+  const int N = spin_of(b);
+  return (int64_t)l * l - N * N + m + l;
+}
+/* pointer to the (re, im) pair of (block, node, l, m) */
+static double* coef_at(double* s, int lmax, int nodes, int b, int node, int l, int m) {
+  return s + block_offset(b, lmax, nodes) + 2 * (node * block_size(b, lmax) + index_of(b, l, m));
+}
+
+/* ---- transforms of a whole tensor, node by node ------------------------- */
+
+static void synthesise_tensor(const sl3o_problem* p, const double* s, double* const grid[3]) {
+  const size_t fs = (size_t)p->ntheta * p->nphi;
+#pragma omp parallel for schedule(dynamic)
+  for (int node = 0; node < p->nodes; ++node) {
+    for (int b = 0; b < 3; ++b) {
+      const double* c = s + block_offset(b, p->lmax, p->nodes) + 2 * node * block_size(b, p->lmax);
+      double* f = grid[b] + (b == 2 ? 1 : 2) * node * fs;
+      sl3o_synthesise(p->lmax, spin_of(b), b == 2, p->ntheta, p->nphi, p->theta, c, f);
+    }
+  }
+}
+
+static void analyse_tensor(const sl3o_problem* p, double* const grid[3], double* s) {
+  const size_t fs = (size_t)p->ntheta * p->nphi;
+#pragma omp parallel for schedule(dynamic)
+  for (int node = 0; node < p->nodes; ++node) {
+    for (int b = 0; b < 3; ++b) {
+      double* c = s + block_offset(b, p->lmax, p->nodes) + 2 * node * block_size(b, p->lmax);
+      const double* f = grid[b] + (b == 2 ? 1 : 2) * node * fs;
+      sl3o_analyse(p->lmax, spin_of(b), b == 2, p->ntheta, p->nphi, p->theta, p->wtheta, f, c);
+    }
+  }
+}
+
+/* ---- GridCoupling::Gather / Scatter (src/GridCoupling.cpp:595-633) ------- */
+
+static void gather(const sl3o_problem* p, const double* s, int node, int l, int m, double A[10]) {
+  const double sign = (m % 2 == 0) ? 1.0 : -1.0;
+  double* w = (double*)s;
+  memset(A, 0, sizeof(double) * 10);
+  const double* r = coef_at(w, p->lmax, p->nodes, 2, node, l, m);
+  A[4] = r[0];
+  A[5] = r[1];
+  if (l >= 1) {
+    const double* a = coef_at(w, p->lmax, p->nodes, 1, node, l, m);
+    const double* b = coef_at(w, p->lmax, p->nodes, 1, node, l, -m);
+    A[2] = a[0];
+    A[3] = a[1];
+    A[6] = sign * b[0];
+    A[7] = -sign * b[1];
+  }
+  if (l >= 2) {
+    const double* a = coef_at(w, p->lmax, p->nodes, 0, node, l, m);
+    const double* b = coef_at(w, p->lmax, p->nodes, 0, node, l, -m);
+    A[0] = a[0];
+    A[1] = a[1];
+    A[8] = sign * b[0];
+    A[9] = -sign * b[1];
+  }
+}
+
+static void scatter(const sl3o_problem* p, double* s, int node, int l, int m, const double A[10]) {
+  const double sign = (m % 2 == 0) ? 1.0 : -1.0;
+  for (int c = 0; c < 2; ++c) {
+    const int N = spin_of(c);
+    if (l < abs(N)) continue;
+    double* a = coef_at(s, p->lmax, p->nodes, c, node, l, m);
+    a[0] += A[2 * c];
+    a[1] += A[2 * c + 1];
+    if (m > 0) {
+      double* b = coef_at(s, p->lmax, p->nodes, c, node, l, -m);
+      b[0] += sign * A[2 * (4 - c)];
+      b[1] += -sign * A[2 * (4 - c) + 1];
+    }
+  }
+  double* r = coef_at(s, p->lmax, p->nodes, 2, node, l, m);
+  r[0] += A[4];
+  r[1] += A[5];
+}
+
+static int first_node(const sl3o_problem* p, int first_element) {
+  for (int node = 0; node < p->nodes; ++node) {
+    if (p->elem0 + node / p->np >= first_element) return node;
+  }
+  return p->nodes;
+}
+
+/* ---- the degree systems --------------------------------------------------- */
+
+typedef struct {
+  int l, kind, ndof, kd, cholesky, pinned, first_element;
+  double* scale;
+  double* factor;
+  int* piv;
+  double* unit;
+  const int* dofs;
+} task_t;
+
+/* src/DegreeOperator.cpp:775-852, src/ToroidalOperator.cpp:190-239 */
+static int factorise(task_t* t, const double* band) {
+  const int n = t->ndof, kd = t->kd, ld = kd + 1;
+  t->scale = (double*)malloc(sizeof(double) * n);
+  int positive = 1;
+  for (int j = 0; j < n; ++j) {
+    const double d = band[(size_t)j * ld + kd];
+    positive = positive && d > 0.0;
+    t->scale[j] = 1.0 / sqrt(fabs(d) > 0.0 ? fabs(d) : 1.0);
+    if (t->kind == 1 && !(d > 0.0)) return 1;
+  }
+  t->piv = NULL;
+  t->cholesky = 0;
+  if (positive) {
+    t->factor = (double*)calloc((size_t)ld * n, sizeof(double));
+    for (int j = 0; j < n; ++j)
+      for (int i = (j - kd > 0 ? j - kd : 0); i <= j; ++i)
+        t->factor[(size_t)j * ld + kd + i - j] = band[(size_t)j * ld + kd + i - j] * t->scale[i] * t->scale[j];
+    if (t->pinned >= 0) {
+      const int pp = t->pinned;
+      for (int j = (pp - kd > 0 ? pp - kd : 0); j <= (pp + kd < n - 1 ? pp + kd : n - 1); ++j) {
+        const int i = pp < j ? pp : j, k = pp < j ? j : pp;
+        t->factor[(size_t)k * ld + kd + i - k] = i == k ? 1.0 : 0.0;
+      }
+    }
+    if (sl3o_band_cholesky_factor(t->factor, n, kd) == 0) {
+      t->cholesky = 1;
+      return 0;
+    }
+    free(t->factor);
+    if (t->kind == 1) return 1;
+  }
+  const int ldlu = 3 * kd + 1;
+  t->factor = (double*)calloc((size_t)ldlu * n, sizeof(double));
+  for (int j = 0; j < n; ++j) {
+    for (int i = (j - kd > 0 ? j - kd : 0); i <= (j + kd < n - 1 ? j + kd : n - 1); ++i) {
+      const int r = i <= j ? i : j, c = i <= j ? j : i;
+      t->factor[(size_t)j * ldlu + 2 * kd + i - j] =
+          band[(size_t)c * ld + kd + r - c] * t->scale[r] * t->scale[c];
+    }
+  }
+  t->piv = (int*)malloc(sizeof(int) * n);
+  return sl3o_band_lu_factor(t->factor, n, kd, kd, t->piv) != 0;
+}
+
+/* BeginSolve + solve + FinishSolve on nrhs real columns (column-major). */
+static void solve(const task_t* t, double* x, int nrhs) {
+  const int n = t->ndof;
+  for (int k = 0; k < nrhs; ++k) {
+    for (int j = 0; j < n; ++j) x[(size_t)k * n + j] *= t->scale[j];
+    if (t->pinned >= 0) x[(size_t)k * n + t->pinned] = 0.0;
+  }
+  if (t->cholesky) {
+    sl3o_band_cholesky_solve(t->factor, n, t->kd, x, nrhs);
+  } else {
+    sl3o_band_lu_solve(t->factor, n, t->kd, t->kd, t->piv, x, nrhs);
+  }
+  for (int k = 0; k < nrhs; ++k)
+    for (int j = 0; j < n; ++j) x[(size_t)k * n + j] *= t->scale[j];
+}
+
+/* ---- force (src/GridCoupling.cpp:691-809) and solves --------------------- */
+
+static void force_and_solve(const sl3o_problem* p, task_t* tasks, int ntasks,
+                            const double* weighted, const double* amp, double** solutions) {
+#pragma omp parallel for schedule(dynamic)
+  for (int ti = 0; ti < ntasks; ++ti) {
+    task_t* t = &tasks[ti];
+    const int l = t->l, cols = l + 1, nd = t->ndof, ngll = p->ngll;
+    double* b = (double*)calloc((size_t)2 * nd * cols, sizeof(double)); /* complex, column-major */
+    const int fn = first_node(p, t->first_element);
+    for (int node = fn; node < p->nodes; ++node) {
+      const int e = p->elem0 + node / p->np, q = node % p->np;
+      const int le = e - p->elem0;
+      const double r = p->radius[node], jac = p->jac[le];
+      for (int m = 0; m < cols; ++m) {
+        double A[10], rows[12];
+        gather(p, weighted, node, l, m, A);
+        sl3o_force_rows(l, A, r, rows);
+        const double point = jac * p->pweight[q] * r;
+        for (int i = 0; i < ngll; ++i) {
+          const double v = point * p->pvalue[q * ngll + i];
+          const int* d = t->dofs + (le * ngll + i) * 2;
+          if (t->kind == 1) {
+            if (d[0] >= 0) {
+              b[2 * ((size_t)m * nd + d[0])] += v * rows[4];
+              b[2 * ((size_t)m * nd + d[0]) + 1] += v * rows[5];
+            }
+          } else {
+            if (d[0] >= 0) {
+              b[2 * ((size_t)m * nd + d[0])] += v * rows[0];
+              b[2 * ((size_t)m * nd + d[0]) + 1] += v * rows[1];
+            }
+            if (d[1] >= 0) {
+              b[2 * ((size_t)m * nd + d[1])] += v * rows[2];
+              b[2 * ((size_t)m * nd + d[1]) + 1] += v * rows[3];
+            }
+          }
+        }
+        for (int j = 0; j < ngll; ++j) {
+          const double deriv = p->pweight[q] * p->pderiv[q * ngll + j] * r;
+          const int* d = t->dofs + (le * ngll + j) * 2;
+          if (t->kind == 1) {
+            if (d[0] >= 0) {
+              b[2 * ((size_t)m * nd + d[0])] += deriv * rows[10];
+              b[2 * ((size_t)m * nd + d[0]) + 1] += deriv * rows[11];
+            }
+          } else {
+            if (d[0] >= 0) {
+              b[2 * ((size_t)m * nd + d[0])] += deriv * rows[6];
+              b[2 * ((size_t)m * nd + d[0]) + 1] += deriv * rows[7];
+            }
+            if (d[1] >= 0) {
+              b[2 * ((size_t)m * nd + d[1])] += deriv * rows[8];
+              b[2 * ((size_t)m * nd + d[1]) + 1] += deriv * rows[9];
+            }
+          }
+        }
+      }
+    }
+    /* SolveForces (src/Viscoelastic3D.cpp:912-939): [Re | Im] real columns */
+    double* x = (double*)malloc(sizeof(double) * 2 * nd * cols);
+    for (int c = 0; c < cols; ++c) {
+      for (int d = 0; d < nd; ++d) {
+        x[(size_t)c * nd + d] = b[2 * ((size_t)c * nd + d)];
+        x[(size_t)(cols + c) * nd + d] = b[2 * ((size_t)c * nd + d) + 1];
+      }
+    }
+    solve(t, x, 2 * cols);
+    for (int c = 0; c < cols; ++c) {
+      for (int d = 0; d < nd; ++d) {
+        double re = x[(size_t)c * nd + d], im = x[(size_t)(cols + c) * nd + d];
+        /* AddUnitChannels (:1029-1073) */
+        if (t->kind == 0 && amp != NULL) {
+          const size_t at = (size_t)l * (l + 1) / 2 + c;
+          re += amp[2 * at] * t->unit[d];
+          im += amp[2 * at + 1] * t->unit[d];
+        }
+        solutions[ti][2 * ((size_t)c * nd + d)] = re;
+        solutions[ti][2 * ((size_t)c * nd + d) + 1] = im;
+      }
+    }
+    free(x);
+    free(b);
+  }
+}
+
+/* ---- strain (src/GridCoupling.cpp:635-689; Viscoelastic3D.cpp:832-865) --- */
+
+static void strain_of(const sl3o_problem* p, const task_t* tasks, double** solutions, double* s) {
+  memset(s, 0, sizeof(double) * sl3o_spectral_doubles(p->lmax, p->nodes));
+  const int L = p->lmax, ngll = p->ngll;
+#pragma omp parallel for schedule(dynamic)
+  for (int i0 = 0; i0 <= L; ++i0) {
+    const int l = L - i0;
+    const task_t* ts = &tasks[l];
+    const task_t* tt = l >= 1 ? &tasks[L + l] : NULL;
+    const double* x = solutions[l];
+    const double* w = l >= 1 ? solutions[L + l] : NULL;
+    const int fn = first_node(p, ts->first_element);
+    for (int node = fn; node < p->nodes; ++node) {
+      const int e = p->elem0 + node / p->np, q = node % p->np, le = e - p->elem0;
+      const double r = p->radius[node], jac = p->jac[le];
+      for (int c = 0; c <= l; ++c) {
+        double v[12] = {0};
+        for (int i = 0; i < ngll; ++i) {
+          const int* d = ts->dofs + (le * ngll + i) * 2;
+          double U[2] = {0, 0}, V[2] = {0, 0}, W[2] = {0, 0};
+          if (d[0] >= 0) {
+            U[0] = x[2 * ((size_t)c * ts->ndof + d[0])];
+            U[1] = x[2 * ((size_t)c * ts->ndof + d[0]) + 1];
+          }
+          if (d[1] >= 0) {
+            V[0] = x[2 * ((size_t)c * ts->ndof + d[1])];
+            V[1] = x[2 * ((size_t)c * ts->ndof + d[1]) + 1];
+          }
+          if (tt) {
+            const int dw = tt->dofs[(le * ngll + i) * 2];
+            if (dw >= 0) {
+              W[0] = w[2 * ((size_t)c * tt->ndof + dw)];
+              W[1] = w[2 * ((size_t)c * tt->ndof + dw) + 1];
+            }
+          }
+          const double val = p->pvalue[q * ngll + i];
+          for (int k = 0; k < 2; ++k) {
+            v[0 + k] += val * U[k];
+            v[2 + k] += val * V[k];
+            v[4 + k] += val * W[k];
+          }
+          const double bb = r * p->pderiv[q * ngll + i] / jac;
+          for (int k = 0; k < 2; ++k) {
+            v[6 + k] += bb * U[k];
+            v[8 + k] += bb * V[k];
+            v[10 + k] += bb * W[k];
+          }
+        }
+        double A[10];
+        sl3o_deviatoric_components(l, v, r, A);
+        scatter(p, s, node, l, c, A);
+      }
+    }
+  }
+}
+
+/* ---- RelativeChange (src/Viscoelastic3D.cpp:1636-1669) ------------------- */
+
+static double relative_change(const double* next, const double* now, int64_t n) {
+  double scale = 0.0, diff = 0.0;
+  int finite = 1;
+  for (int64_t i = 0; i < n / 2; ++i) {
+    const double nr = next[2 * i], ni = next[2 * i + 1];
+    finite = finite && isfinite(nr) && isfinite(ni);
+    const double a = nr * nr + ni * ni;
+    const double dr = nr - now[2 * i], di = ni - now[2 * i + 1];
+    const double b = dr * dr + di * di;
+    if (a > scale) scale = a;
+    if (b > diff) diff = b;
+  }
+  if (!finite) return NAN;
+  return scale > 0.0 ? sqrt(diff) / sqrt(scale) : 0.0;
+}
+
+/* ---- the step -------------------------------------------------------------- */
+
+int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
+              const double* amp, const sl3o_options* o, sl3o_info* info) {
+  const int L = p->lmax;
+  const int ntasks = 2 * L + 1;
+  const size_t fs = (size_t)p->ntheta * p->nphi;
+  const size_t G = fs * p->nodes;
+  const int64_t S = sl3o_spectral_doubles(L, p->nodes);
+  int status = 0;
+  if (dt == 0.0) scheme = 1;
+  task_t* tasks = (task_t*)calloc(ntasks, sizeof(task_t));
+  for (int ti = 0; ti < ntasks; ++ti) {
+    task_t* t = &tasks[ti];
+    t->kind = ti <= L ? 0 : 1;
+    t->l = ti <= L ? ti : ti - L;
+    t->ndof = p->ndof[ti];
+    t->kd = p->kd[ti];
+    t->first_element = p->first_element[ti];
+    t->pinned = p->pinned[ti];
+    t->dofs = p->dofs + p->dof_off[ti];
+    if (factorise(t, p->bands + p->band_off[ti]) != 0) status = 1;
+  }
+  if (status != 0) goto done;
+  for (int l = 0; l <= L; ++l) {
+    task_t* t = &tasks[l];
+    t->unit = (double*)malloc(sizeof(double) * t->ndof);
+    memcpy(t->unit, p->loads + p->load_off[l], sizeof(double) * t->ndof);
+    solve(t, t->unit, 1);
+  }
+  {
+    double* s = (double*)calloc(S, sizeof(double));
+    double* sNext = (double*)calloc(S, sizeof(double));
+    double* fixedS = (double*)calloc(S, sizeof(double));
+    double* analysed = (double*)calloc(S, sizeof(double));
+    double* grid[3];
+    double* d0[3];
+    for (int b = 0; b < 3; ++b) {
+      grid[b] = (double*)calloc((b == 2 ? 1 : 2) * G, sizeof(double));
+      d0[b] = st->has_strain ? st->strain[b] : (double*)calloc((b == 2 ? 1 : 2) * G, sizeof(double));
+    }
+    double* beta = (double*)malloc(sizeof(double) * G);
+    double* weight = (double*)malloc(sizeof(double) * G);
+    double** sol = (double**)malloc(sizeof(double*) * ntasks);
+    for (int ti = 0; ti < ntasks; ++ti) {
+      sol[ti] = (double*)calloc((size_t)2 * tasks[ti].ndof * (tasks[ti].l + 1), sizeof(double));
+    }
+    /* s0 and d0 (:1252-1263) */
+    if (st->has_coef) memcpy(s, st->coef, sizeof(double) * S);
+    if (!st->has_strain) synthesise_tensor(p, s, d0);
+    /* StartStep's sweep (:1283-1332), one branch */
+    for (int node = 0; node < p->nodes; ++node) {
+      const double mu = p->mu_node[node];
+      const double href = dt * p->itau_ref_node[node];
+      double refBeta;
+      if (scheme == 1) {
+        refBeta = 1.0 - sl3o_backward_euler_weight(href);
+      } else {
+        double e, a, bb;
+        sl3o_trapezoid_weights(href, &e, &a, &bb);
+        refBeta = bb;
+      }
+      for (size_t x = node * fs; x < (node + 1) * fs; ++x) {
+        const double h = dt * p->itau[x];
+        double ew, aw, bw;
+        if (scheme == 1) {
+          const double w = sl3o_backward_euler_weight(h);
+          ew = w;
+          aw = 0.0;
+          bw = 1.0 - w;
+        } else {
+          sl3o_trapezoid_weights(h, &ew, &aw, &bw);
+        }
+        beta[x] = bw;
+        weight[x] = 0.0 + mu * (bw - refBeta);
+        for (int c = 0; c < 2; ++c) {
+          for (int k = 0; k < 2; ++k) {
+            double* mm = &st->m[c][2 * x + k];
+            *mm = ew * *mm + aw * d0[c][2 * x + k];
+            grid[c][2 * x + k] = 0.0 + mu * *mm;
+          }
+        }
+        st->m[2][x] = ew * st->m[2][x] + aw * d0[2][x];
+        grid[2][x] = 0.0 + mu * st->m[2][x];
+      }
+    }
+    analyse_tensor(p, grid, fixedS);
+    for (size_t x = 0; x < G; ++x) {
+      for (int c = 0; c < 2; ++c) {
+        grid[c][2 * x] = weight[x] * d0[c][2 * x];
+        grid[c][2 * x + 1] = weight[x] * d0[c][2 * x + 1];
+      }
+      grid[2][x] = weight[x] * d0[2][x];
+    }
+    analyse_tensor(p, grid, analysed);
+    for (int64_t i = 0; i < S; ++i) analysed[i] += fixedS[i];
+
+    /* Converge (:1916-1974) */
+    sl3o_mixer* mixer = NULL;
+    const int adaptive = o->anderson_rate > 0.0;
+    if (o->anderson_memory > 0 && !adaptive) mixer = sl3o_mixer_create(o->anderson_memory);
+    double previous = 0.0;
+    info->iterations = 0;
+    info->residual = 0.0;
+    info->converged = 0;
+    info->mixed_passes = 0;
+    g_residual_count = 0;
+    for (int it = 0; it < o->max_iterations; ++it) {
+      if (it > 0) {
+        synthesise_tensor(p, s, grid);
+        for (size_t x = 0; x < G; ++x) {
+          for (int c = 0; c < 2; ++c) {
+            grid[c][2 * x] *= weight[x];
+            grid[c][2 * x + 1] *= weight[x];
+          }
+          grid[2][x] *= weight[x];
+        }
+        analyse_tensor(p, grid, analysed);
+        for (int64_t i = 0; i < S; ++i) analysed[i] += fixedS[i];
+      }
+      force_and_solve(p, tasks, ntasks, analysed, amp, sol);
+      strain_of(p, tasks, sol, sNext);
+      const double residual = relative_change(sNext, s, S);
+      info->iterations = it + 1;
+      info->residual = residual;
+      if (g_residual_count < 4096) g_residuals[g_residual_count++] = residual;
+      if (!isfinite(residual)) break;
+      if (residual <= o->rtol) {
+        info->converged = 1;
+        memcpy(s, sNext, sizeof(double) * S);
+        break;
+      }
+      if (o->damping != 1.0) {
+        for (int64_t i = 0; i < S; ++i) sNext[i] = s[i] + o->damping * (sNext[i] - s[i]);
+      }
+      if (adaptive && !mixer && o->anderson_memory > 0 && it > 0 &&
+          residual > o->anderson_rate * previous) {
+        mixer = sl3o_mixer_create(o->anderson_memory);
+      }
+      previous = residual;
+      if (mixer) {
+        sl3o_mixer_update(mixer, (size_t)S, s, sNext);
+        ++info->mixed_passes;
+      } else {
+        double* t = s;
+        s = sNext;
+        sNext = t;
+      }
+    }
+    sl3o_mixer_destroy(mixer);
+    /* Finish (:1900-1914) */
+    synthesise_tensor(p, s, grid);
+    for (size_t x = 0; x < G; ++x) {
+      for (int c = 0; c < 2; ++c) {
+        st->m[c][2 * x] += beta[x] * grid[c][2 * x];
+        st->m[c][2 * x + 1] += beta[x] * grid[c][2 * x + 1];
+      }
+      st->m[2][x] += beta[x] * grid[2][x];
+    }
+    for (int b = 0; b < 3; ++b) {
+      memcpy(st->strain[b], grid[b], sizeof(double) * (b == 2 ? 1 : 2) * G);
+    }
+    memcpy(st->coef, s, sizeof(double) * S);
+    st->has_strain = 1;
+    st->has_coef = 1;
+    st->t += dt;
+    for (int b = 0; b < 3; ++b) {
+      free(grid[b]);
+      if (d0[b] != st->strain[b]) free(d0[b]);
+    }
+    for (int ti = 0; ti < ntasks; ++ti) free(sol[ti]);
+    free(sol);
+    free(beta);
+    free(weight);
+    free(s);
+    free(sNext);
+    free(fixedS);
+    free(analysed);
+  }
+done:
+  for (int ti = 0; ti < ntasks; ++ti) {
+    free(tasks[ti].scale);
+    free(tasks[ti].factor);
+    free(tasks[ti].piv);
+    free(tasks[ti].unit);
+  }
+  free(tasks);
+  return status;
+}

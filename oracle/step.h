@@ -1,0 +1,92 @@
+/*
+ * step.h -- the oracle's restatement of one 3D lateral-viscosity step
+ * (Viscoelastic3D::Step with the fixed point, src/Viscoelastic3D.cpp:
+ * 1223-1424, 1636-1669, 1758-1974, 1982-2030; GridCoupling,
+ * src/GridCoupling.cpp:263-809; the degree solves, src/DegreeOperator.cpp:
+ * 775-901, src/ToroidalOperator.cpp:190-272), FP64 on the CPU, in the
+ * reference's own layouts. Test infrastructure only (see oracle.h).
+ *
+ * The problem's radial set-up (the assembled, unscaled degree bands, dof
+ * maps, memory points) is an input: it is host set-up, not the path under
+ * test. Everything from the factorisation on is restated here.
+ */
+#ifndef SL3_ORACLE_STEP_H
+#define SL3_ORACLE_STEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int lmax, ntheta, nphi;
+  const double* theta;   /* ntheta colatitudes */
+  const double* wtheta;  /* their Gauss-Legendre weights */
+  int nodes, np, ngll, elem0, elements;
+  const double* radius;  /* per memory node */
+  const double* jac;     /* per memory element */
+  const double* pweight; /* np */
+  const double* pvalue;  /* np x ngll */
+  const double* pderiv;  /* np x ngll */
+  const double* mu_node;
+  const double* itau_ref_node;
+  const double* itau;    /* nodes x ntheta x nphi */
+  /* tasks: spheroidal l = 0..lmax, then toroidal l = 1..lmax */
+  const int* ndof;
+  const int* kd;
+  const int* first_element;
+  const int* pinned;        /* -1 or the pinned dof */
+  const int64_t* band_off;  /* unscaled upper bands (kd+1) x ndof */
+  const double* bands;
+  const int64_t* dof_off;   /* memory elements x ngll x 2 */
+  const int* dofs;
+  const int64_t* load_off;  /* spheroidal tasks' unit loads (ndof) */
+  const double* loads;
+} sl3o_problem;
+
+typedef struct {
+  int max_iterations;
+  double rtol, damping;
+  int anderson_memory;
+  double anderson_rate;
+} sl3o_options;
+
+typedef struct {
+  int iterations;
+  double residual;
+  int converged;
+  int mixed_passes;
+} sl3o_info;
+
+/* The state (caller-owned arrays): m and strain grid components (c0, c1
+ * complex as (re, im) pairs, r real; nodes x ntheta x nphi values), the
+ * strain coefficients in the reference layout (blocks spin -2, -1, 0 end to
+ * end, node-major, complex). Without coefficients the state must be the
+ * zero state under a zero load (the elastic start strain is then zero). */
+typedef struct {
+  double t;
+  double* m[3];
+  double* strain[3];
+  int has_strain;
+  double* coef;
+  int has_coef;
+} sl3o_state;
+
+/* Size in doubles of the reference-layout spectral tensor. */
+int64_t sl3o_spectral_doubles(int lmax, int nodes);
+
+/* One step in place; amp: the load channel's columns at the step's end
+ * time, complex per (l, m) at l(l+1)/2 + m (NULL for none). scheme 0 the
+ * exponential trapezoid, 1 backward Euler. Returns 0, or non-zero when a
+ * factorisation fails. */
+int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
+              const double* amp, const sl3o_options* o, sl3o_info* info);
+
+/* The residual history of the last sl3o_step (count passes). */
+int sl3o_last_residuals(double* out, int capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,490 @@
+// C ABI of the Earth set-up, the 3D step and the batched transforms
+// (include/sl3dpp_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "common.hpp"
+#include "coupling.hpp"
+#include "kernels.hpp"
+#include "step.hpp"
+
+struct sl3_earth {
+  std::unique_ptr<sl3amd::Earth> earth;
+};
+
+struct sl3_operator {
+  const sl3_earth* earth = nullptr;
+  std::unique_ptr<sl3amd::Viscoelastic3D> op;
+};
+
+struct sl3_state {
+  std::unique_ptr<sl3amd::DeviceState> state;
+};
+
+using namespace sl3amd;
+
+namespace {
+
+void Check(bool ok, const std::string& what) {
+  if (!ok) throw ArgumentError(what);
+}
+
+StepOptions ToOptions(const sl3_step_options* o) {
+  StepOptions s;
+  if (o == nullptr) return s;
+  s.eps = o->eps;
+  s.oversampling = o->oversampling;
+  s.maxIterations = o->max_iterations;
+  s.rtol = o->rtol;
+  s.damping = o->damping;
+  s.andersonMemory = o->anderson_memory;
+  s.andersonRate = o->anderson_rate;
+  Check(s.andersonMemory >= 0 && s.andersonMemory <= 8, "anderson memory in [0, 8]");
+  Check(s.maxIterations >= 1, "max_iterations >= 1");
+  Check(s.eps > 0.0 && s.eps < 1.0, "eps in (0, 1)");
+  Check(s.oversampling >= 1.0, "oversampling >= 1");
+  return s;
+}
+
+// Copies n doubles from the device to the host (dst), or reports n.
+void Export(const double* src_dev, size_t n, double* dst, int64_t* count) {
+  if (count) *count = static_cast<int64_t>(n);
+  if (dst == nullptr) return;
+  hipStream_t s = Stream();
+  SL3_HIP(hipMemcpyAsync(dst, src_dev, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SL3_HIP(hipStreamSynchronize(s));
+}
+
+void ExportHost(const std::vector<double>& v, double* dst, int64_t* count) {
+  if (count) *count = static_cast<int64_t>(v.size());
+  if (dst) std::memcpy(dst, v.data(), sizeof(double) * v.size());
+}
+
+size_t ReferenceBlockSize(int block, int lmax) {
+  const size_t L1 = static_cast<size_t>(lmax + 1);
+  return block == 2 ? L1 * (lmax + 2) / 2 : L1 * L1 - (block == 0 ? 4 : 1);
+}
+
+void ToReference(const Coupling& c, const double* dev, double* ref, hipStream_t s) {
+  size_t refOff = 0;
+  for (int b = 0; b < 3; ++b) {
+    LaunchToReferenceLayout(dev + c.SpectralOffset(b), ref + refOff, c.RowOffsetDevice(b),
+                            SurfaceGrid::SpinOf(b), c.LMax(), c.NodeCount(), s);
+    refOff += 2 * ReferenceBlockSize(b, c.LMax()) * c.NodeCount();
+  }
+}
+
+void FromReference(const Coupling& c, const double* ref, double* dev, hipStream_t s) {
+  size_t refOff = 0;
+  for (int b = 0; b < 3; ++b) {
+    LaunchToDeviceLayout(ref + refOff, dev + c.SpectralOffset(b), c.RowOffsetDevice(b),
+                         SurfaceGrid::SpinOf(b), c.LMax(), c.NodeCount(), s);
+    refOff += 2 * ReferenceBlockSize(b, c.LMax()) * c.NodeCount();
+  }
+}
+
+int BlockOfSpin(int spin) {
+  if (spin == -2) return 0;
+  if (spin == -1) return 1;
+  if (spin == 0) return 2;
+  throw ArgumentError("the stored spins are -2, -1 and 0");
+}
+
+}  // namespace
+
+extern "C" {
+
+int sl3_earth_create(const sl3_earth_spec* spec, sl3_earth** out) {
+  return Guard([&] {
+    Check(spec != nullptr && out != nullptr, "null argument");
+    Check(spec->layers >= 1 && spec->layers <= SL3_MAX_LAYERS, "1 to 8 layers");
+    EarthSpec es;
+    for (int i = 0; i < spec->layers; ++i) {
+      es.layers.push_back({spec->r0[i], spec->r1[i], spec->rho0[i], spec->rho1[i],
+                           spec->kappa0[i], spec->kappa1[i], spec->mu0[i], spec->mu1[i],
+                           spec->elements[i]});
+    }
+    es.ngll = spec->ngll;
+    es.viscousLayer = spec->viscous_layer;
+    es.viscosityBoundary = spec->viscosity_boundary;
+    es.viscosityUpper = spec->viscosity_upper;
+    es.viscosityLower = spec->viscosity_lower;
+    auto e = std::make_unique<sl3_earth>();
+    e->earth = BuildEarth(es);
+    *out = e.release();
+  });
+}
+
+int sl3_earth_destroy(sl3_earth* earth) {
+  delete earth;
+  return SL3_OK;
+}
+
+int sl3_earth_scales(const sl3_earth* earth, double* length, double* time,
+                     double* mass) {
+  return Guard([&] {
+    Check(earth != nullptr, "null earth");
+    const auto& s = earth->earth->scales;
+    if (length) *length = s.length;
+    if (time) *time = s.Time();
+    if (mass) *mass = s.Mass();
+  });
+}
+
+int sl3_earth_mesh(const sl3_earth* earth, int* elements, int* ngll) {
+  return Guard([&] {
+    Check(earth != nullptr, "null earth");
+    if (elements) *elements = earth->earth->mesh->ElementCount();
+    if (ngll) *ngll = earth->earth->mesh->NGll();
+  });
+}
+
+int sl3_earth_export(const sl3_earth* earth, const char* name, double* dst,
+                     int64_t* count) {
+  return Guard([&] {
+    Check(earth != nullptr && name != nullptr, "null argument");
+    const auto& e = *earth->earth;
+    const auto& mesh = *e.mesh;
+    const auto& pts = e.reference.points;
+    const std::string key(name);
+    std::vector<double> v;
+    if (key == "radius") {
+      v = mesh.Radii();
+    } else if (key == "jac") {
+      for (int k = 0; k < mesh.ElementCount(); ++k) v.push_back(mesh.Jacobian(k));
+    } else if (key == "gll_weight") {
+      v = mesh.Weights();
+    } else if (key == "gll_deriv") {
+      for (int k = 0; k < mesh.NGll(); ++k)
+        for (int i = 0; i < mesh.NGll(); ++i) v.push_back(mesh.Derivative(k, i));
+    } else if (key == "point_value") {
+      v = pts.value;
+    } else if (key == "point_deriv") {
+      v = pts.derivative;
+    } else if (key == "point_weight") {
+      v = pts.weight;
+    } else if (key == "point_radius") {
+      for (int k = 0; k < mesh.ElementCount(); ++k)
+        for (int p = 0; p < pts.count; ++p) v.push_back(pts.RadiusAt(mesh, k, p));
+    } else if (key == "point_mu") {
+      v = e.reference.pointMu;
+    } else if (key == "point_itau") {
+      v = e.reference.pointItau;
+    } else if (key == "fluid") {
+      for (int k = 0; k < mesh.ElementCount(); ++k) v.push_back(e.material.IsFluid(k));
+    } else if (key == "surface_gravity") {
+      v.push_back(e.material.surfaceGravity);
+    } else if (key == "memory_elements") {
+      int first = -1, n = 0;
+      for (int k = 0; k < mesh.ElementCount(); ++k) {
+        if (!e.material.IsFluid(k) && e.rheology.ElementActive(mesh, k)) {
+          if (first < 0) first = k;
+          ++n;
+        }
+      }
+      v = {static_cast<double>(first), static_cast<double>(n)};
+    } else {
+      throw ArgumentError("unknown earth export: " + key);
+    }
+    if (count) *count = static_cast<int64_t>(v.size());
+    if (dst) std::memcpy(dst, v.data(), sizeof(double) * v.size());
+  });
+}
+
+int sl3_earth_degree_system(const sl3_earth* earth, int l, int kind, int scheme,
+                            double dt, double eps, int* ndof, int* kd,
+                            int* first_element, int* pinned, double* band,
+                            int* dofs, double* load) {
+  return Guard([&] {
+    Check(earth != nullptr, "null earth");
+    Check(kind == 0 || kind == 1, "kind 0 (spheroidal) or 1 (toroidal)");
+    const auto& e = *earth->earth;
+    std::vector<double> shear;
+    PointShear ps{&e.reference.points, &shear};
+    const PointShear* psp = nullptr;
+    if (dt >= 0.0) {
+      shear = e.reference.PointShearModulus(scheme, dt);
+      psp = &ps;
+    }
+    const auto& mesh = *e.mesh;
+    if (kind == 0) {
+      DegreeOperator op(e.reference.nodal, l, eps, psp);
+      if (ndof) *ndof = op.DofCount();
+      if (kd) *kd = op.Kd();
+      if (first_element) *first_element = op.FirstElement();
+      if (pinned) *pinned = -1;
+      if (band) std::memcpy(band, op.Band().data(), sizeof(double) * op.Band().size());
+      if (dofs) {
+        for (int gn = 0; gn < mesh.NodeCount(); ++gn)
+          for (int c = 0; c < 3; ++c) dofs[gn * 3 + c] = op.Dof(gn, Component(c));
+      }
+      if (load) {
+        const auto b = op.Load();
+        std::memcpy(load, b.data(), sizeof(double) * b.size());
+      }
+    } else {
+      ToroidalOperator op(e.reference.nodal, l, eps, psp);
+      if (ndof) *ndof = op.DofCount();
+      if (kd) *kd = op.Kd();
+      if (first_element) *first_element = op.FirstElement();
+      if (pinned) *pinned = op.Pinned().empty() ? -1 : op.Pinned()[0];
+      if (band) std::memcpy(band, op.Band().data(), sizeof(double) * op.Band().size());
+      if (dofs) {
+        for (int gn = 0; gn < mesh.NodeCount(); ++gn) dofs[gn] = op.Dof(gn);
+      }
+    }
+  });
+}
+
+void sl3_step_options_default(sl3_step_options* o) {
+  if (o == nullptr) return;
+  const StepOptions s;
+  o->eps = s.eps;
+  o->oversampling = s.oversampling;
+  o->max_iterations = s.maxIterations;
+  o->rtol = s.rtol;
+  o->damping = s.damping;
+  o->anderson_memory = s.andersonMemory;
+  o->anderson_rate = s.andersonRate;
+}
+
+int sl3_operator_create(const sl3_earth* earth, int lmax, const double* delta_dev,
+                        const sl3_step_options* options, sl3_operator** out) {
+  return Guard([&] {
+    Check(earth != nullptr && out != nullptr, "null argument");
+    Check(lmax >= 2, "lmax >= 2");
+    auto op = std::make_unique<sl3_operator>();
+    op->earth = earth;
+    op->op = std::make_unique<Viscoelastic3D>(*earth->earth, lmax, delta_dev,
+                                              ToOptions(options));
+    *out = op.release();
+  });
+}
+
+int sl3_operator_destroy(sl3_operator* op) {
+  return Guard([&] {
+    if (op) SL3_HIP(hipStreamSynchronize(Stream()));
+    delete op;
+  });
+}
+
+int sl3_operator_set_options(sl3_operator* op, const sl3_step_options* options) {
+  return Guard([&] {
+    Check(op != nullptr && options != nullptr, "null argument");
+    const auto o = ToOptions(options);
+    const auto& cur = op->op->Options();
+    Check(o.eps == cur.eps && o.oversampling == cur.oversampling,
+          "eps and oversampling are fixed at construction");
+    op->op->MutableOptions() = o;
+  });
+}
+
+int sl3_operator_set_delta(sl3_operator* op, const double* delta_dev) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    op->op->SetDelta(delta_dev);
+  });
+}
+
+int sl3_operator_shape(const sl3_operator* op, int* nodes, int* ntheta, int* nphi,
+                       int64_t* spectral_doubles) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    const auto& c = op->op->GetCoupling();
+    if (nodes) *nodes = c.NodeCount();
+    if (ntheta) *ntheta = c.Grid().NTheta();
+    if (nphi) *nphi = c.Grid().NPhi();
+    if (spectral_doubles) *spectral_doubles = static_cast<int64_t>(c.SpectralDoubles());
+  });
+}
+
+int sl3_operator_geometry(const sl3_operator* op, double* radii, double* theta,
+                          double* wtheta) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    const auto& c = op->op->GetCoupling();
+    if (radii) std::memcpy(radii, c.NodeRadii().data(), sizeof(double) * c.NodeCount());
+    if (theta) {
+      std::memcpy(theta, c.Grid().CoLatitudes().data(), sizeof(double) * c.Grid().NTheta());
+    }
+    if (wtheta) {
+      std::memcpy(wtheta, c.Grid().CoLatitudeWeights().data(),
+                  sizeof(double) * c.Grid().NTheta());
+    }
+  });
+}
+
+int sl3_state_zero(sl3_operator* op, sl3_state** out) {
+  return Guard([&] {
+    Check(op != nullptr && out != nullptr, "null argument");
+    auto s = std::make_unique<sl3_state>();
+    s->state = op->op->ZeroState();
+    SL3_HIP(hipStreamSynchronize(Stream()));
+    *out = s.release();
+  });
+}
+
+int sl3_state_destroy(sl3_state* state) {
+  return Guard([&] {
+    if (state) SL3_HIP(hipStreamSynchronize(Stream()));
+    delete state;
+  });
+}
+
+int sl3_state_time(const sl3_state* state, double* t) {
+  return Guard([&] {
+    Check(state != nullptr && t != nullptr, "null argument");
+    *t = state->state->t;
+  });
+}
+
+int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme, sl3_load_fn load,
+             void* user, const double* start_dev, sl3_step_info* info) {
+  return Guard([&] {
+    Check(op != nullptr && state != nullptr, "null argument");
+    LoadFn fn;
+    if (load) {
+      fn = [load, user](int l, double t, double* columns) {
+        return load(l, t, columns, user) != 0;
+      };
+    }
+    const auto r = op->op->Step(*state->state, dt, scheme, fn, start_dev);
+    if (info) {
+      info->iterations = r.iterations;
+      info->residual = r.residual;
+      info->converged = r.converged ? 1 : 0;
+      info->mixed_passes = r.mixedPasses;
+      info->start_ms = op->op->lastTiming.startStep;
+      info->passes_ms = op->op->lastTiming.passes;
+      info->finish_ms = op->op->lastTiming.finish;
+    }
+  });
+}
+
+int sl3_state_export(const sl3_operator* op, const sl3_state* state, const char* name,
+                     double* dst, int64_t* count) {
+  return Guard([&] {
+    Check(op != nullptr && state != nullptr && name != nullptr, "null argument");
+    const auto& c = op->op->GetCoupling();
+    const auto& st = *state->state;
+    const std::string key(name);
+    hipStream_t s = Stream();
+    if (key == "coefficients" || key == "coefficients_device") {
+      Check(st.hasCoefficients, "the state carries no strain coefficients");
+      const size_t n = c.SpectralDoubles();
+      if (key == "coefficients_device") return Export(st.coefficients.data(), n, dst, count);
+      DeviceArray<double> ref(n);
+      ToReference(c, st.coefficients.data(), ref.data(), s);
+      return Export(ref.data(), n, dst, count);
+    }
+    for (int b = 0; b < 3; ++b) {
+      if (key == "m" + std::to_string(b)) return Export(st.m[b].data(), st.m[b].size(), dst, count);
+      if (key == "strain" + std::to_string(b)) {
+        Check(st.hasStrain, "the state carries no strain grid");
+        return Export(st.strain[b].data(), st.strain[b].size(), dst, count);
+      }
+    }
+    throw ArgumentError("unknown state export: " + key);
+  });
+}
+
+int sl3_operator_export(sl3_operator* op, const char* name, double* dst, int64_t* count) {
+  return Guard([&] {
+    Check(op != nullptr && name != nullptr, "null argument");
+    auto& v = *op->op;
+    const auto& c = v.GetCoupling();
+    const std::string key(name);
+    const size_t G = c.GridValueCount();
+    if (key == "weight") return Export(v.Weight(), G, dst, count);
+    if (key == "beta") return Export(v.Beta(), G, dst, count);
+    if (key == "itau") return Export(v.Itau(), G, dst, count);
+    if (key == "fixed") return Export(v.FixedForce(), c.SpectralDoubles(), dst, count);
+    if (key == "analysed") return Export(v.Analysed(), c.SpectralDoubles(), dst, count);
+    if (key == "residuals") return ExportHost(v.residualHistory, dst, count);
+    if (key == "mu_node") return ExportHost(v.MuNode(), dst, count);
+    if (key == "itau_ref_node") return ExportHost(v.ItauRefNode(), dst, count);
+    if (key == "tasks") {
+      std::vector<double> t;
+      for (const auto& task : v.Tasks()) {
+        t.push_back(task.l);
+        t.push_back(task.kind);
+        t.push_back(task.ndof);
+        t.push_back(task.firstNode);
+      }
+      return ExportHost(t, dst, count);
+    }
+    if (key.rfind("solution:", 0) == 0) {
+      const int task = std::stoi(key.substr(9));
+      Check(task >= 0 && task < static_cast<int>(v.Tasks().size()), "no such task");
+      return ExportHost(v.SolutionOfTask(task), dst, count);
+    }
+    throw ArgumentError("unknown operator export: " + key);
+  });
+}
+
+int sl3_spectral_to_reference(const sl3_operator* op, const double* device_dev,
+                              double* reference_dev) {
+  return Guard([&] {
+    Check(op && device_dev && reference_dev, "null argument");
+    ToReference(op->op->GetCoupling(), device_dev, reference_dev, Stream());
+    SL3_HIP(hipStreamSynchronize(Stream()));
+  });
+}
+
+int sl3_spectral_from_reference(const sl3_operator* op, const double* reference_dev,
+                                double* device_dev) {
+  return Guard([&] {
+    Check(op && device_dev && reference_dev, "null argument");
+    FromReference(op->op->GetCoupling(), reference_dev, device_dev, Stream());
+    SL3_HIP(hipStreamSynchronize(Stream()));
+  });
+}
+
+int sl3_gsh_synthesise(const sl3_operator* op, int spin, const double* coef_dev,
+                       double* field_dev) {
+  return Guard([&] {
+    Check(op && coef_dev && field_dev, "null argument");
+    const int b = BlockOfSpin(spin);
+    const auto& c = op->op->GetCoupling();
+    hipStream_t s = Stream();
+    const size_t blockDoubles = c.SpectralOffset(b + 1) - c.SpectralOffset(b);
+    DeviceArray<double> dev(blockDoubles), fourier(c.FourierDoubles(b));
+    LaunchToDeviceLayout(coef_dev, dev.data(), c.RowOffsetDevice(b), spin, c.LMax(),
+                         c.NodeCount(), s);
+    c.SynthesiseBlockToFourier(b, dev.data(), fourier.data(), s);
+    RingLaunch r;
+    r.block = b;
+    r.op = kRingSynthesise;
+    r.fourierIn = fourier.data();
+    r.gridOut = field_dev;
+    c.Ring(r, s);
+    SL3_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int sl3_gsh_analyse(const sl3_operator* op, int spin, const double* field_dev,
+                    double* coef_dev) {
+  return Guard([&] {
+    Check(op && coef_dev && field_dev, "null argument");
+    const int b = BlockOfSpin(spin);
+    const auto& c = op->op->GetCoupling();
+    hipStream_t s = Stream();
+    const size_t blockDoubles = c.SpectralOffset(b + 1) - c.SpectralOffset(b);
+    DeviceArray<double> dev(blockDoubles), fourier(c.FourierDoubles(b));
+    RingLaunch r;
+    r.block = b;
+    r.op = kRingAnalyse;
+    r.gridIn = field_dev;
+    r.fourierOut = fourier.data();
+    c.Ring(r, s);
+    c.AnalyseBlockFromFourier(b, fourier.data(), dev.data(), s);
+    LaunchToReferenceLayout(dev.data(), coef_dev, c.RowOffsetDevice(b), spin, c.LMax(),
+                            c.NodeCount(), s);
+    SL3_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

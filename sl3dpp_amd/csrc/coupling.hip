@@ -1,0 +1,569 @@
+// The grid coupling's per-degree maps and the fixed point's vector sweeps
+// for gfx950, FP64 (all HBM-bound):
+//  * the memory force (GridCoupling::SpheroidalForce / ToroidalForce with
+//    Gather and ForceRowsFromComponents, src/GridCoupling.cpp:613-809,
+//    include/sl3dpp/SpinComponents.hpp:124-145), written straight into the
+//    solves' scaled right-hand sides (DegreeOperator::BeginSolve,
+//    src/DegreeOperator.cpp:861-886);
+//  * strain recovery (GridCoupling::AddStrainOfSolution and Scatter,
+//    src/GridCoupling.cpp:595-689; DeviatoricComponents,
+//    SpinComponents.hpp:53-72) from the unscaled solutions plus the load
+//    channel (FinishSolve, AddUnitChannels: src/DegreeOperator.cpp:888-901,
+//    src/Viscoelastic3D.cpp:1029-1073);
+//  * RelativeChange (src/Viscoelastic3D.cpp:1636-1669) and the Anderson
+//    sweeps (include/sl3dpp/detail/AndersonMixer.hpp:123-218), reduced in a
+//    fixed block order so that results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "coupling.hpp"
+
+namespace sl3amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr double kSqrt2 = 1.41421356237309504880;
+
+struct Cplx {
+  double re, im;
+};
+__device__ inline Cplx C(double re, double im) { return {re, im}; }
+__device__ inline Cplx operator+(Cplx a, Cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ inline Cplx operator-(Cplx a, Cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ inline Cplx operator*(double s, Cplx a) { return {s * a.re, s * a.im}; }
+__device__ inline Cplx TimesI(Cplx a) { return {-a.im, a.re}; }
+__device__ inline Cplx Conj(Cplx a) { return {a.re, -a.im}; }
+__device__ inline Cplx Div(Cplx a, double d) { return {a.re / d, a.im / d}; }
+
+__device__ inline int LMin(int m, int spin) {
+  const int am = m < 0 ? -m : m, an = spin < 0 ? -spin : spin;
+  return am > an ? am : an;
+}
+
+__device__ inline size_t RowOf(const SpectralView& v, int b, int l, int m) {
+  const int mlo = b == 2 ? 0 : -v.lmax;
+  const int spin = b == 0 ? -2 : (b == 1 ? -1 : 0);
+  return static_cast<size_t>(v.rowOffset[b][m - mlo] + l - LMin(m, spin));
+}
+
+__device__ inline Cplx Load(const SpectralView& v, int b, int l, int m, int node,
+                            int nodes) {
+  const double* p = v.block[b] + 2 * (RowOf(v, b, l, m) * nodes + node);
+  return C(p[0], p[1]);
+}
+
+__device__ inline void Store(const SpectralView& v, int b, int l, int m, int node,
+                             int nodes, Cplx z) {
+  double* p = v.block[b] + 2 * (RowOf(v, b, l, m) * nodes + node);
+  p[0] = z.re;
+  p[1] = z.im;
+}
+
+// GridCoupling::Gather (src/GridCoupling.cpp:613-633): the five components
+// of column m >= 0, the derived ones by A^{(N)}_{lm} = (-1)^m
+// conj(A^{(-N)}_{l,-m}).
+__device__ inline void Gather(const SpectralView& v, int l, int m, int node,
+                              int nodes, Cplx A[5]) {
+  const double sign = (m & 1) ? -1.0 : 1.0;
+  A[0] = A[1] = A[3] = A[4] = C(0.0, 0.0);
+  A[2] = Load(v, 2, l, m, node, nodes);
+  if (l >= 1) {
+    A[1] = Load(v, 1, l, m, node, nodes);
+    A[3] = sign * Conj(Load(v, 1, l, -m, node, nodes));
+  }
+  if (l >= 2) {
+    A[0] = Load(v, 0, l, m, node, nodes);
+    A[4] = sign * Conj(Load(v, 0, l, -m, node, nodes));
+  }
+}
+
+// ForceRowsFromComponents: u, v, w, du, dv, dw.
+__device__ inline void ForceRows(int l, const Cplx A[5], double r, Cplx rows[6]) {
+  const double zeta2 = l * (l + 1.0);
+  const double zeta = sqrt(zeta2);
+  const double root = sqrt(zeta2 - 2.0 > 0.0 ? zeta2 - 2.0 : 0.0);
+  const Cplx p13 = A[1] + A[3], m13 = A[1] - A[3];
+  const Cplx p04 = A[0] + A[4], m04 = A[0] - A[4];
+  rows[0] = (kSqrt2 * zeta) * p13 - 4.0 * A[2];
+  rows[1] = (zeta * root) * p04 + (2.0 * zeta2) * A[2] - (kSqrt2 * zeta) * p13;
+  rows[2] = TimesI((zeta * root) * m04 - (kSqrt2 * zeta) * m13);
+  rows[3] = (4.0 * r) * A[2];
+  rows[4] = (kSqrt2 * zeta * r) * p13;
+  rows[5] = TimesI((kSqrt2 * zeta * r) * m13);
+}
+
+constexpr int kChunkElements = 4;
+constexpr int kMaxPoints = 6;  // memory points per element the LDS holds
+constexpr int kChunkOrders = 16;
+
+// One workgroup: one task, a chunk of memory elements, every order m in
+// chunks; the rows of every point of the chunk (and of the element below,
+// whose top node the chunk's bottom node shares) staged in LDS.
+__global__ void __launch_bounds__(kThreads)
+    ForceKernel(const DegreeTask* tasks, RadialGeometry g, SpectralView w,
+                const double* scale, const int* dofs, double* rhs) {
+  constexpr int maxPoints = (kChunkElements + 1) * kMaxPoints;
+  __shared__ Cplx rowsL[maxPoints * kChunkOrders][4];
+  const DegreeTask t = tasks[blockIdx.y];
+  const int l = t.l;
+  const bool tor = t.kind == 1;
+  const int elemEnd = g.elem0 + g.elements;
+  const int eLo = max(g.elem0, t.firstElement);
+  const int e0 = g.elem0 + blockIdx.x * kChunkElements;
+  const int e1 = min(e0 + kChunkElements, elemEnd);
+  if (e1 <= eLo) return;
+  const int eStart = max(eLo, e0 - 1);
+  const int points = (e1 - eStart) * g.np;
+  const int nm = g.ngll - 1;
+  // owned GLL nodes: [gidx(max(e0, eLo), 0), gidx(e1, 0)) plus the top node
+  // of the last memory element
+  const int gnFirst = max(e0, eLo) * nm;
+  const int gnEnd = e1 * nm + (e1 == elemEnd ? 1 : 0);
+  const int owned = gnEnd - gnFirst;
+  const int* dofTable = dofs + t.dofOff;
+  const double* sc = scale + t.scaleOff;
+  double* b = rhs + t.rhsOff;
+  for (int mc = 0; mc <= l; mc += kChunkOrders) {
+    const int mCount = min(kChunkOrders, l + 1 - mc);
+    // stage 1: rows of (point, m)
+    for (int e = threadIdx.x; e < points * mCount; e += blockDim.x) {
+      const int p = e % points, mi = e / points;
+      const int elem = eStart + p / g.np, q = p % g.np;
+      const int node = (elem - g.elem0) * g.np + q;
+      Cplx A[5], rows[6];
+      Gather(w, l, mc + mi, node, g.nodes, A);
+      ForceRows(l, A, g.radius[node], rows);
+      Cplx* dst = rowsL[mi * maxPoints + p];
+      if (tor) {
+        dst[0] = rows[2];
+        dst[1] = rows[5];
+      } else {
+        dst[0] = rows[0];
+        dst[1] = rows[1];
+        dst[2] = rows[3];
+        dst[3] = rows[4];
+      }
+    }
+    __syncthreads();
+    // stage 2: each owned node's sums, m fastest
+    for (int e = threadIdx.x; e < owned * mCount; e += blockDim.x) {
+      const int mi = e % mCount, gl = e / mCount;
+      const int gn = gnFirst + gl;
+      const int m = mc + mi;
+      int elems[2], locals[2], count = 0;
+      const int ea = gn / nm, ia = gn % nm;
+      if (ia == 0 && ea - 1 >= eStart && ea - 1 < e1) {
+        elems[count] = ea - 1;
+        locals[count++] = nm;
+      }
+      if (ea >= eStart && ea < e1) {
+        elems[count] = ea;
+        locals[count++] = ia;
+      }
+      Cplx U = C(0.0, 0.0), V = C(0.0, 0.0);
+      for (int k = 0; k < count; ++k) {
+        const int elem = elems[k], i = locals[k];
+        const double jac = g.jac[elem - g.elem0];
+        for (int q = 0; q < g.np; ++q) {
+          const int p = (elem - eStart) * g.np + q;
+          const int node = (elem - g.elem0) * g.np + q;
+          const double r = g.radius[node];
+          const double point = jac * g.weight[q] * r;
+          const double v = point * g.value[q * g.ngll + i];
+          const double deriv = g.weight[q] * g.deriv[q * g.ngll + i] * r;
+          const Cplx* rw = rowsL[mi * maxPoints + p];
+          if (tor) {
+            U = U + v * rw[0];
+            U = U + deriv * rw[1];
+          } else {
+            U = U + v * rw[0];
+            V = V + v * rw[1];
+            U = U + deriv * rw[2];
+            V = V + deriv * rw[3];
+          }
+        }
+      }
+      const int elem = count > 0 ? elems[count - 1] : ea;
+      const int i = count > 0 ? locals[count - 1] : ia;
+      const int* d = dofTable + ((elem - g.elem0) * g.ngll + i) * 2;
+      const int dU = d[0], dV = tor ? -1 : d[1];
+      if (dU >= 0) {
+        const double s = dU == t.pinned ? 0.0 : sc[dU];
+        b[static_cast<size_t>(dU) * t.cols + m] = U.re * s;
+        b[static_cast<size_t>(dU) * t.cols + l + 1 + m] = U.im * s;
+      }
+      if (dV >= 0) {
+        const double s = sc[dV];
+        b[static_cast<size_t>(dV) * t.cols + m] = V.re * s;
+        b[static_cast<size_t>(dV) * t.cols + l + 1 + m] = V.im * s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One thread per (degree, order, memory node), node fastest.
+__global__ void __launch_bounds__(kThreads)
+    StrainKernel(const DegreeTask* sph, const DegreeTask* tor, int lmax,
+                 RadialGeometry g, const double* scale, const int* dofs,
+                 const double* rhs, const double* units, const double* amp,
+                 SpectralView out) {
+  const int pair = blockIdx.y;
+  int l = static_cast<int>((sqrt(8.0 * pair + 1.0) - 1.0) * 0.5);
+  while (l * (l + 1) / 2 > pair) --l;
+  while ((l + 1) * (l + 2) / 2 <= pair) ++l;
+  const int m = pair - l * (l + 1) / 2;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= g.nodes) return;
+  const DegreeTask ts = sph[l];
+  Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
+  if (node >= ts.firstNode) {
+    const int elem = g.elem0 + node / g.np, q = node % g.np;
+    const double r = g.radius[node];
+    const double jac = g.jac[elem - g.elem0];
+    Cplx U = C(0, 0), V = C(0, 0), W = C(0, 0), rdU = C(0, 0), rdV = C(0, 0),
+         rdW = C(0, 0);
+    const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
+    const bool tor1 = l >= 1;
+    DegreeTask tt;
+    if (tor1) tt = tor[l - 1];
+    for (int i = 0; i < g.ngll; ++i) {
+      const int* d = dofs + ts.dofOff + ((elem - g.elem0) * g.ngll + i) * 2;
+      Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
+      const double* bs = rhs + ts.rhsOff;
+      const double* ss = scale + ts.scaleOff;
+      if (d[0] >= 0) {
+        const size_t at = static_cast<size_t>(d[0]) * ts.cols;
+        Ui = ss[d[0]] * C(bs[at + m], bs[at + l + 1 + m]);
+        if (ts.unitOff >= 0) Ui = Ui + units[ts.unitOff + d[0]] * a;
+      }
+      if (d[1] >= 0) {
+        const size_t at = static_cast<size_t>(d[1]) * ts.cols;
+        Vi = ss[d[1]] * C(bs[at + m], bs[at + l + 1 + m]);
+        if (ts.unitOff >= 0) Vi = Vi + units[ts.unitOff + d[1]] * a;
+      }
+      if (tor1) {
+        const int dw = dofs[tt.dofOff + ((elem - g.elem0) * g.ngll + i) * 2];
+        if (dw >= 0) {
+          const size_t at = static_cast<size_t>(dw) * tt.cols;
+          const double* bt = rhs + tt.rhsOff;
+          Wi = scale[tt.scaleOff + dw] * C(bt[at + m], bt[at + l + 1 + m]);
+        }
+      }
+      const double v = g.value[q * g.ngll + i];
+      U = U + v * Ui;
+      V = V + v * Vi;
+      W = W + v * Wi;
+      const double bb = r * g.deriv[q * g.ngll + i] / jac;
+      rdU = rdU + bb * Ui;
+      rdV = rdV + bb * Vi;
+      rdW = rdW + bb * Wi;
+    }
+    // DeviatoricComponents
+    const double zeta2 = l * (l + 1.0);
+    const double zeta = sqrt(zeta2);
+    A[2] = Div(rdU - U + (0.5 * zeta2) * V, 3.0 * r);
+    if (l >= 1) {
+      const Cplx tangential = rdV - V + U;
+      const Cplx twist = W - rdW;
+      const double den = 2.0 * kSqrt2 * r;
+      A[1] = Div(zeta * (tangential + TimesI(twist)), den);
+      A[3] = Div(zeta * (tangential - TimesI(twist)), den);
+    }
+    if (l >= 2) {
+      const double f = 0.5 * zeta * sqrt(zeta2 - 2.0) / r;
+      A[0] = f * (V - TimesI(W));
+      A[4] = f * (V + TimesI(W));
+    }
+  }
+  // Scatter (src/GridCoupling.cpp:595-611), every slot written once.
+  const double sign = (m & 1) ? -1.0 : 1.0;
+  if (l >= 2) {
+    Store(out, 0, l, m, node, g.nodes, A[0]);
+    if (m > 0) Store(out, 0, l, -m, node, g.nodes, sign * Conj(A[4]));
+  }
+  if (l >= 1) {
+    Store(out, 1, l, m, node, g.nodes, A[1]);
+    if (m > 0) Store(out, 1, l, -m, node, g.nodes, sign * Conj(A[3]));
+  }
+  Store(out, 2, l, m, node, g.nodes, A[2]);
+}
+
+__global__ void FillKernel(double* dst, size_t count, double value) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < count; i += stride) {
+    dst[i] = value;
+  }
+}
+
+__global__ void CopyKernel(double* dst, const double* src, size_t count) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < count; i += stride) {
+    dst[i] = src[i];
+  }
+}
+
+// Block b covers [n b / B, n (b + 1) / B): a fixed partition of the index
+// range, so the reductions below group the same terms on every run.
+__device__ inline void BlockRange(size_t n, size_t& lo, size_t& hi) {
+  lo = n * blockIdx.x / gridDim.x;
+  hi = n * (blockIdx.x + 1) / gridDim.x;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    RelativeChangeKernel(const double* next, const double* now, size_t pairs,
+                         double* partials) {
+  __shared__ double s1[kThreads], s2[kThreads], s3[kThreads];
+  size_t lo, hi;
+  BlockRange(pairs, lo, hi);
+  double scale = 0.0, diff = 0.0, bad = 0.0;
+  for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const double nr = next[2 * i], ni = next[2 * i + 1];
+    const double dr = nr - now[2 * i], di = ni - now[2 * i + 1];
+    if (!(isfinite(nr) && isfinite(ni))) bad = 1.0;
+    scale = fmax(scale, nr * nr + ni * ni);
+    diff = fmax(diff, dr * dr + di * di);
+  }
+  s1[threadIdx.x] = scale;
+  s2[threadIdx.x] = diff;
+  s3[threadIdx.x] = bad;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      s3[threadIdx.x] = fmax(s3[threadIdx.x], s3[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x] = s1[0];
+    partials[3 * blockIdx.x + 1] = s2[0];
+    partials[3 * blockIdx.x + 2] = s3[0];
+  }
+}
+
+// Sums (or maxima) of `width` columns of partials, in block order.
+__global__ void ReducePartialsKernel(const double* partials, int blocks, int width,
+                                     int useMax, double* out) {
+  const int c = threadIdx.x;
+  if (c >= width) return;
+  double s = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    const double v = partials[b * width + c];
+    s = useMax ? fmax(s, v) : s + v;
+  }
+  out[c] = s;
+}
+
+__global__ void AndersonFirstKernel(double* x, const double* gx, double* xPrev,
+                                    double* fPrev, size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    xPrev[i] = x[i];
+    fPrev[i] = gx[i] - x[i];
+    x[i] = gx[i];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    AndersonSweepKernel(AndersonSweep s, size_t n, double* partials) {
+  __shared__ double red[kThreads][17];
+  size_t lo, hi;
+  BlockRange(n, lo, hi);
+  double col[8] = {}, gam[8] = {};
+  double* dFs = s.dF[s.slot];
+  double* dGs = s.dG[s.slot];
+  for (size_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+    const double xs = s.x[j];
+    const double f = s.gx[j] - xs;
+    const double dx = xs - s.xPrev[j];
+    const double df = f - s.fPrev[j];
+    dFs[j] = df;
+    dGs[j] = dx + df;
+    s.xPrev[j] = xs;
+    s.fPrev[j] = f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i < s.m) {
+        const double di = i == s.slot ? df : s.dF[i][j];
+        col[i] += di * df;
+        gam[i] += di * f;
+      }
+    }
+  }
+  for (int i = 0; i < 8; ++i) {
+    red[threadIdx.x][i] = col[i];
+    red[threadIdx.x][8 + i] = gam[i];
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      for (int i = 0; i < 16; ++i) red[threadIdx.x][i] += red[threadIdx.x + w][i];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 16) partials[blockIdx.x * 16 + threadIdx.x] = red[0][threadIdx.x];
+}
+
+struct Gamma {
+  double v[8];
+};
+
+__global__ void AndersonMixKernel(AndersonSweep s, Gamma gamma, size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t j = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
+       j += stride) {
+    double value = s.gx[j];
+    for (int i = 0; i < s.m; ++i) value -= gamma.v[i] * s.dG[i][j];
+    s.x[j] = value;
+  }
+}
+
+__device__ inline void RefIndexToLm(int spin, long long idx, int& l, int& m) {
+  if (spin == 0) {
+    l = static_cast<int>((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while (static_cast<long long>(l) * (l + 1) / 2 > idx) --l;
+    while (static_cast<long long>(l + 1) * (l + 2) / 2 <= idx) ++l;
+    m = static_cast<int>(idx - static_cast<long long>(l) * (l + 1) / 2);
+  } else {
+    const long long t = idx + spin * spin;
+    l = static_cast<int>(sqrt(static_cast<double>(t)));
+    while (static_cast<long long>(l) * l > t) --l;
+    while (static_cast<long long>(l + 1) * (l + 1) <= t) ++l;
+    m = static_cast<int>(t - static_cast<long long>(l) * l - l);
+  }
+}
+
+__global__ void LayoutKernel(const double* src, double* dst, const int* rowOffset,
+                             int spin, int lmax, int nodes, long long size,
+                             int toDevice) {
+  const long long total = size * nodes;
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  const int mlo = spin == 0 ? 0 : -lmax;
+  for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+       e < total; e += stride) {
+    const long long idx = e % size;
+    const int node = static_cast<int>(e / size);
+    int l, m;
+    RefIndexToLm(spin, idx, l, m);
+    const long long row = rowOffset[m - mlo] + l - LMin(m, spin);
+    const long long devAt = 2 * (row * nodes + node);
+    const long long refAt = 2 * (static_cast<long long>(node) * size + idx);
+    if (toDevice) {
+      dst[devAt] = src[refAt];
+      dst[devAt + 1] = src[refAt + 1];
+    } else {
+      dst[refAt] = src[devAt];
+      dst[refAt + 1] = src[devAt + 1];
+    }
+  }
+}
+
+int GridFor(size_t n) {
+  const size_t blocks = (n + kThreads - 1) / kThreads;
+  return static_cast<int>(blocks < 65536 ? (blocks == 0 ? 1 : blocks) : 65536);
+}
+
+}  // namespace
+
+void LaunchFill(double* dst, size_t count, double value, hipStream_t stream) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(FillKernel, dim3(GridFor(count)), dim3(kThreads), 0, stream,
+                     dst, count, value);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchCopy(double* dst, const double* src, size_t count, hipStream_t stream) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(CopyKernel, dim3(GridFor(count)), dim3(kThreads), 0, stream,
+                     dst, src, count);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchForce(const DegreeTask* tasks_dev, int taskCount, int lmax,
+                 const RadialGeometry& g, const SpectralView& weighted,
+                 const double* scale, const int* dofs, double* rhs,
+                 hipStream_t stream) {
+  (void)lmax;
+  if (taskCount == 0 || g.elements == 0) return;
+  if (g.np > kMaxPoints) throw ArgumentError("at most 6 memory points per element");
+  const dim3 grid(CeilDiv(g.elements, kChunkElements), taskCount);
+  hipLaunchKernelGGL(ForceKernel, grid, dim3(kThreads), 0, stream, tasks_dev, g,
+                     weighted, scale, dofs, rhs);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev,
+                  int lmax, const RadialGeometry& g, const double* scale,
+                  const int* dofs, const double* rhs, const double* units,
+                  const double* amp, SpectralView out, hipStream_t stream) {
+  const int pairs = (lmax + 1) * (lmax + 2) / 2;
+  const dim3 grid(CeilDiv(g.nodes, kThreads), pairs);
+  hipLaunchKernelGGL(StrainKernel, grid, dim3(kThreads), 0, stream, sphTasks_dev,
+                     torTasks_dev, lmax, g, scale, dofs, rhs, units, amp, out);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchRelativeChange(const double* next, const double* now, size_t n,
+                          double* partials, int partialBlocks, double* result,
+                          hipStream_t stream) {
+  hipLaunchKernelGGL(RelativeChangeKernel, dim3(partialBlocks), dim3(kThreads), 0,
+                     stream, next, now, n / 2, partials);
+  SL3_HIP(hipGetLastError());
+  hipLaunchKernelGGL(ReducePartialsKernel, dim3(1), dim3(64), 0, stream, partials,
+                     partialBlocks, 3, 1, result);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPrev,
+                         size_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(AndersonFirstKernel, dim3(GridFor(n)), dim3(kThreads), 0,
+                     stream, x, gx, xPrev, fPrev, n);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchAndersonSweep(const AndersonSweep& s, size_t n, double* partials,
+                         int partialBlocks, double* sums, hipStream_t stream) {
+  hipLaunchKernelGGL(AndersonSweepKernel, dim3(partialBlocks), dim3(kThreads), 0,
+                     stream, s, n, partials);
+  SL3_HIP(hipGetLastError());
+  hipLaunchKernelGGL(ReducePartialsKernel, dim3(1), dim3(64), 0, stream, partials,
+                     partialBlocks, 16, 0, sums);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, size_t n,
+                       hipStream_t stream) {
+  Gamma g{};
+  for (int i = 0; i < s.m && i < 8; ++i) g.v[i] = gamma[i];
+  hipLaunchKernelGGL(AndersonMixKernel, dim3(GridFor(n)), dim3(kThreads), 0, stream,
+                     s, g, n);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchToDeviceLayout(const double* ref, double* dev, const int* rowOffset,
+                          int spin, int lmax, int nodes, hipStream_t stream) {
+  const long long size = spin == 0 ? static_cast<long long>(lmax + 1) * (lmax + 2) / 2
+                                   : static_cast<long long>(lmax + 1) * (lmax + 1) -
+                                         spin * spin;
+  hipLaunchKernelGGL(LayoutKernel, dim3(GridFor(size * nodes)), dim3(kThreads), 0,
+                     stream, ref, dev, rowOffset, spin, lmax, nodes, size, 1);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchToReferenceLayout(const double* dev, double* ref, const int* rowOffset,
+                             int spin, int lmax, int nodes, hipStream_t stream) {
+  const long long size = spin == 0 ? static_cast<long long>(lmax + 1) * (lmax + 2) / 2
+                                   : static_cast<long long>(lmax + 1) * (lmax + 1) -
+                                         spin * spin;
+  hipLaunchKernelGGL(LayoutKernel, dim3(GridFor(size * nodes)), dim3(kThreads), 0,
+                     stream, dev, ref, rowOffset, spin, lmax, nodes, size, 0);
+  SL3_HIP(hipGetLastError());
+}
+
+}  // namespace sl3amd

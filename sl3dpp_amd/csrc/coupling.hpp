@@ -1,0 +1,98 @@
+// Launch interface of the coupling kernels (coupling.hip): the memory
+// force on the degree systems, strain recovery from their solutions, the
+// residual and the Anderson sweeps, and layout permutations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sl3amd {
+
+// One degree system of the step's batch (spheroidal or toroidal).
+struct DegreeTask {
+  int l = 0;
+  int kind = 0;  // 0 spheroidal, 1 toroidal
+  int ndof = 0;
+  int cols = 0;         // 2 (l + 1) real columns: Re of m = 0..l, then Im
+  int firstNode = 0;    // the degree's first memory node
+  int firstElement = 0;
+  int pinned = -1;      // a pinned dof (toroidal l = 1), or -1
+  int pad = 0;
+  long long rhsOff = 0;    // [dof][cols] row-major
+  long long scaleOff = 0;  // ndof doubles
+  long long dofOff = 0;    // memory elements x ngll x (U, V) or (W)
+  long long unitOff = -1;  // the unit load response (ndof), spheroidal
+};
+
+// The radial geometry the force and strain maps read (device pointers).
+struct RadialGeometry {
+  int nodes = 0;       // memory nodes = memory elements x np
+  int np = 0;          // points per element
+  int ngll = 0;
+  int elem0 = 0;       // first memory element
+  int elements = 0;    // memory elements (contiguous)
+  const double* radius = nullptr;  // per node
+  const double* jac = nullptr;     // per memory element
+  const double* weight = nullptr;  // per point (reference weights)
+  const double* value = nullptr;   // (np, ngll)
+  const double* deriv = nullptr;   // (np, ngll)
+};
+
+struct SpectralView {
+  double* block[3] = {};           // device layout blocks (see grid.hpp)
+  const int* rowOffset[3] = {};    // per m - mlo
+  int lmax = 0;
+};
+
+void LaunchFill(double* dst, size_t count, double value, hipStream_t stream);
+void LaunchCopy(double* dst, const double* src, size_t count, hipStream_t stream);
+
+// Forces of the analysed tensor onto every task's right-hand sides, scaled
+// for the solve (BeginSolve), pinned dofs zeroed; rows outside the memory
+// elements are left as they are (the caller fills zeros first).
+void LaunchForce(const DegreeTask* tasks_dev, int taskCount, int lmax,
+                 const RadialGeometry& g, const SpectralView& weighted,
+                 const double* scale, const int* dofs, double* rhs,
+                 hipStream_t stream);
+
+// Strain of the solutions into the spectral iterate (every value written):
+// solutions unscaled (FinishSolve) and the load channel's unit responses
+// added with amplitudes amp (complex per (l, m), offset l(l+1)/2 + m).
+void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev,
+                  int lmax, const RadialGeometry& g, const double* scale,
+                  const int* dofs, const double* rhs, const double* units,
+                  const double* amp, SpectralView out, hipStream_t stream);
+
+// max |n|^2, max |n - o|^2 and a non-finite flag over n values, into
+// result[0..2] (device).
+void LaunchRelativeChange(const double* next, const double* now, size_t n,
+                          double* partials, int partialBlocks, double* result,
+                          hipStream_t stream);
+
+// Anderson (AndersonMixer.hpp:123-218) on a flat vector of n doubles.
+struct AndersonSweep {
+  double* x;            // in: iterate; (second sweep) out: mixed
+  const double* gx;     // the pass's image
+  double* xPrev;
+  double* fPrev;
+  double* dF[8];
+  double* dG[8];
+  int m;                // active history columns
+  int slot;             // the slot being written
+};
+void LaunchAndersonFirst(double* x, const double* gx, double* xPrev,
+                         double* fPrev, size_t n, hipStream_t stream);
+void LaunchAndersonSweep(const AndersonSweep& s, size_t n, double* partials,
+                         int partialBlocks, double* sums, hipStream_t stream);
+void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, size_t n,
+                       hipStream_t stream);
+
+// Reference layout (per node, l-major, SurfaceGrid::SpinIndex / Index)
+// <-> device layout, for one block over `nodes` nodes.
+void LaunchToDeviceLayout(const double* ref, double* dev, const int* rowOffset,
+                          int spin, int lmax, int nodes, hipStream_t stream);
+void LaunchToReferenceLayout(const double* dev, double* ref, const int* rowOffset,
+                             int spin, int lmax, int nodes, hipStream_t stream);
+
+}  // namespace sl3amd

@@ -1,0 +1,964 @@
+// The 3D step on the GPU (see step.hpp).
+#include "step.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <limits>
+#include <cstring>
+#include <future>
+#include <string>
+#include <thread>
+
+#include "kernels.hpp"
+
+namespace sl3amd {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// itau(x) = itau_ref(node) exp(-delta(x)) (src/Viscoelastic3D.cpp:250-287)
+__global__ void ItauKernel(const double* delta, const double* itauRef, double* itau,
+                           size_t perNode, size_t total) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += stride) {
+    const double factor = delta ? exp(-delta[i]) : 1.0;
+    itau[i] = itauRef[i / perNode] * factor;
+  }
+}
+
+// Converge's damping (src/Viscoelastic3D.cpp:1949-1957): n = o + d (n - o).
+__global__ void DampKernel(double* next, const double* now, double d, size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    next[i] = now[i] + d * (next[i] - now[i]);
+  }
+}
+
+int Blocks(size_t n) {
+  const size_t b = (n + 255) / 256;
+  return static_cast<int>(std::min<size_t>(b == 0 ? 1 : b, 65536));
+}
+
+// dgelss with rcond = -1 on the symmetric Gram system, as the minimum-norm
+// least-squares solution through a cyclic Jacobi eigen-decomposition
+// (singular values of a symmetric matrix are its |eigenvalues|).
+void GramLeastSquares(int m, const double* gram, const double* rhs, double* gamma) {
+  double a[64], v[64];
+  for (int i = 0; i < m * m; ++i) a[i] = gram[i];
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) v[i * m + j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) off += a[p * m + q] * a[p * m + q];
+    if (off == 0.0) break;
+    for (int p = 0; p < m; ++p) {
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = a[p * m + q];
+        if (apq == 0.0) continue;
+        const double tau = (a[q * m + q] - a[p * m + p]) / (2.0 * apq);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::abs(tau) + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+        for (int k = 0; k < m; ++k) {
+          const double akp = a[k * m + p], akq = a[k * m + q];
+          a[k * m + p] = c * akp - s * akq;
+          a[k * m + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double apk = a[p * m + k], aqk = a[q * m + k];
+          a[p * m + k] = c * apk - s * aqk;
+          a[q * m + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double vkp = v[k * m + p], vkq = v[k * m + q];
+          v[k * m + p] = c * vkp - s * vkq;
+          v[k * m + q] = s * vkp + c * vkq;
+        }
+      }
+    }
+  }
+  double smax = 0.0;
+  for (int i = 0; i < m; ++i) smax = std::max(smax, std::abs(a[i * m + i]));
+  const double cut = 2.220446049250313e-16 * smax;
+  for (int i = 0; i < m; ++i) gamma[i] = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const double lam = a[k * m + k];
+    if (!(std::abs(lam) > cut)) continue;
+    double proj = 0.0;
+    for (int i = 0; i < m; ++i) proj += v[i * m + k] * rhs[i];
+    proj /= lam;
+    for (int i = 0; i < m; ++i) gamma[i] += v[i * m + k] * proj;
+  }
+}
+
+template <class F>
+void ParallelFor(int count, F&& body) {
+  const int threads = std::max(1, std::min<int>(8, static_cast<int>(std::thread::hardware_concurrency())));
+  std::vector<std::future<void>> jobs;
+  std::atomic<int> next{0};
+  for (int t = 0; t < threads; ++t) {
+    jobs.push_back(std::async(std::launch::async, [&] {
+      for (int i = next++; i < count; i = next++) body(i);
+    }));
+  }
+  for (auto& j : jobs) j.get();
+}
+
+}  // namespace
+
+// ---- Coupling ---------------------------------------------------------------
+
+Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps)
+    : lmax_(lmax), grid_(lmax, oversampling) {
+  hipStream_t s = Stream();
+  const auto& mesh = *earth.mesh;
+  const auto& points = earth.reference.points;
+  const int np = points.count, n = mesh.NGll();
+  // memory elements: solid, with an active branch (GridCoupling.cpp:35-52)
+  std::vector<int> elements;
+  for (int e = 0; e < mesh.ElementCount(); ++e) {
+    if (!earth.material.IsFluid(e) && earth.rheology.ElementActive(mesh, e)) {
+      elements.push_back(e);
+    }
+  }
+  if (elements.empty()) throw ArgumentError("no active branch: nothing to couple");
+  for (size_t i = 1; i < elements.size(); ++i) {
+    if (elements[i] != elements[i - 1] + 1) {
+      throw ArgumentError("the relaxing elements must be contiguous");
+    }
+  }
+  geometry_.np = np;
+  geometry_.ngll = n;
+  geometry_.elem0 = elements.front();
+  geometry_.elements = static_cast<int>(elements.size());
+  geometry_.nodes = geometry_.elements * np;
+  std::vector<double> jac;
+  for (int e : elements) {
+    jac.push_back(mesh.Jacobian(e));
+    for (int p = 0; p < np; ++p) {
+      nodeElement_.push_back(e);
+      radiusHost_.push_back(points.RadiusAt(mesh, e, p));
+    }
+  }
+  firstElement_.resize(lmax + 1);
+  firstNodeHost_.resize(lmax + 1);
+  for (int l = 0; l <= lmax; ++l) {
+    firstElement_[l] = l > 0 ? mesh.StartElement(l, eps) : 0;
+    int first = geometry_.nodes;
+    for (int node = 0; node < geometry_.nodes; ++node) {
+      if (nodeElement_[node] >= firstElement_[l]) {
+        first = node;
+        break;
+      }
+    }
+    firstNodeHost_[l] = first;
+  }
+  reachHost_.assign(geometry_.nodes, 0);
+  for (int node = 0; node < geometry_.nodes; ++node) {
+    int reach = -1;
+    for (int l = 0; l <= lmax; ++l) {
+      if (firstNodeHost_[l] <= node) reach = l;
+    }
+    reachHost_[node] = std::max(reach, 0);
+  }
+  radius_.Upload(radiusHost_.data(), radiusHost_.size(), s);
+  jac_.Upload(jac.data(), jac.size(), s);
+  weight_.Upload(points.weight.data(), points.weight.size(), s);
+  value_.Upload(points.value.data(), points.value.size(), s);
+  deriv_.Upload(points.derivative.data(), points.derivative.size(), s);
+  geometry_.radius = radius_.data();
+  geometry_.jac = jac_.data();
+  geometry_.weight = weight_.data();
+  geometry_.value = value_.data();
+  geometry_.deriv = deriv_.data();
+  firstNode_.Upload(firstNodeHost_.data(), firstNodeHost_.size(), s);
+  reach_.Upload(reachHost_.data(), reachHost_.size(), s);
+
+  std::vector<int> orders;
+  spectralOffset_[0] = 0;
+  for (int b = 0; b < 3; ++b) {
+    const auto& lay = grid_.Layout(b);
+    const auto table = grid_.WignerTable(b);
+    table_[b].Upload(table.data(), table.size(), s);
+    rowOffset_[b].Upload(lay.rowOffset.data(), lay.rowOffset.size(), s);
+    for (int m = lay.mlo; m <= lmax; ++m) {
+      orders.push_back(b);
+      orders.push_back(m);
+    }
+    spectralOffset_[b + 1] =
+        spectralOffset_[b] + static_cast<size_t>(lay.Rows()) * geometry_.nodes * 2;
+    plan_.table[b] = table_[b].data();
+    plan_.rowOffset[b] = rowOffset_[b].data();
+    plan_.mlo[b] = lay.mlo;
+    plan_.spin[b] = lay.spin;
+    SL3_HIP(hipStreamSynchronize(s));  // the host table is freed next
+  }
+  orders_.Upload(orders.data(), orders.size(), s);
+  orderCount_ = static_cast<int>(orders.size() / 2);
+  plan_.orders = orders_.data();
+  plan_.orderCount = orderCount_;
+  plan_.lmax = lmax;
+  plan_.nodes = geometry_.nodes;
+  plan_.ntheta = grid_.NTheta();
+
+  const int nphi = grid_.NPhi();
+  dft_.R = nphi % 4 == 0 ? 4 : 2;
+  dft_.P = nphi / dft_.R;
+  std::vector<double> cp(dft_.P), sp(dft_.P), cn(nphi), sn(nphi);
+  for (int j = 0; j < dft_.P; ++j) {
+    cp[j] = std::cos(2.0 * kPi * j / dft_.P);
+    sp[j] = std::sin(2.0 * kPi * j / dft_.P);
+  }
+  for (int j = 0; j < nphi; ++j) {
+    cn[j] = std::cos(2.0 * kPi * j / nphi);
+    sn[j] = std::sin(2.0 * kPi * j / nphi);
+  }
+  cosP_.Upload(cp.data(), cp.size(), s);
+  sinP_.Upload(sp.data(), sp.size(), s);
+  cosN_.Upload(cn.data(), cn.size(), s);
+  sinN_.Upload(sn.data(), sn.size(), s);
+  wtheta_.Upload(grid_.CoLatitudeWeights().data(), grid_.NTheta(), s);
+  dft_.cosP = cosP_.data();
+  dft_.sinP = sinP_.data();
+  dft_.cosN = cosN_.data();
+  dft_.sinN = sinN_.data();
+  if (RingLds(nphi, dft_.P) > 160 * 1024) {
+    throw ArgumentError("grid too large for the ring kernel's LDS staging");
+  }
+  SL3_HIP(hipStreamSynchronize(s));
+}
+
+size_t Coupling::FourierDoubles(int block) const {
+  return static_cast<size_t>(grid_.Layout(block).Orders()) * geometry_.nodes *
+         grid_.NTheta() * 2;
+}
+
+SpectralView Coupling::View(double* spectral) const {
+  SpectralView v;
+  for (int b = 0; b < 3; ++b) {
+    v.block[b] = spectral + spectralOffset_[b];
+    v.rowOffset[b] = rowOffset_[b].data();
+  }
+  v.lmax = lmax_;
+  return v;
+}
+
+void Coupling::SynthesiseToFourier(const double* spectral, double* const fourier[3],
+                                   bool truncated, hipStream_t s) const {
+  const double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
+                             spectral + spectralOffset_[2]};
+  LaunchLegendreSynthesis(plan_, blocks, fourier, truncated ? reach_.data() : nullptr, s);
+}
+
+void Coupling::AnalyseFromFourier(const double* const fourier[3], double* spectral,
+                                  const double* addend, bool truncated,
+                                  hipStream_t s) const {
+  double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
+                       spectral + spectralOffset_[2]};
+  const double* adds[3] = {};
+  if (addend) {
+    for (int b = 0; b < 3; ++b) adds[b] = addend + spectralOffset_[b];
+  }
+  LaunchLegendreAnalysis(plan_, fourier, blocks, addend ? adds : nullptr,
+                         truncated ? firstNode_.data() : nullptr, s);
+}
+
+namespace {
+int OrderStart(const SurfaceGrid& g, int block) {
+  int z = 0;
+  for (int b = 0; b < block; ++b) z += g.Layout(b).Orders();
+  return z;
+}
+}  // namespace
+
+void Coupling::SynthesiseBlockToFourier(int block, const double* spectral,
+                                        double* fourier, hipStream_t s) const {
+  LegendrePlan p = plan_;
+  p.orders = plan_.orders + 2 * OrderStart(grid_, block);
+  p.orderCount = grid_.Layout(block).Orders();
+  const double* blocks[3] = {};
+  double* fouriers[3] = {};
+  blocks[block] = spectral;
+  fouriers[block] = fourier;
+  LaunchLegendreSynthesis(p, blocks, fouriers, nullptr, s);
+}
+
+void Coupling::AnalyseBlockFromFourier(int block, const double* fourier,
+                                       double* spectral, hipStream_t s) const {
+  LegendrePlan p = plan_;
+  p.orders = plan_.orders + 2 * OrderStart(grid_, block);
+  p.orderCount = grid_.Layout(block).Orders();
+  const double* fouriers[3] = {};
+  double* blocks[3] = {};
+  blocks[block] = spectral;
+  fouriers[block] = fourier;
+  LaunchLegendreAnalysis(p, fouriers, blocks, nullptr, nullptr, s);
+}
+
+void Coupling::Ring(RingLaunch r, hipStream_t s) const {
+  r.tables = dft_;
+  r.lmax = lmax_;
+  r.nodes = geometry_.nodes;
+  r.ntheta = grid_.NTheta();
+  r.nphi = grid_.NPhi();
+  r.wtheta = wtheta_.data();
+  LaunchRing(r, s);
+}
+
+void Coupling::Synthesise(const double* spectral, double* const grid[3],
+                          double* const fourier[3], hipStream_t s) const {
+  SynthesiseToFourier(spectral, fourier, false, s);
+  for (int b = 0; b < 3; ++b) {
+    RingLaunch r;
+    r.block = b;
+    r.op = kRingSynthesise;
+    r.fourierIn = fourier[b];
+    r.gridOut = grid[b];
+    Ring(r, s);
+  }
+}
+
+void Coupling::Analyse(const double* const grid[3], double* spectral,
+                       double* const fourier[3], hipStream_t s) const {
+  for (int b = 0; b < 3; ++b) {
+    RingLaunch r;
+    r.block = b;
+    r.op = kRingAnalyse;
+    r.gridIn = grid[b];
+    r.fourierOut = fourier[b];
+    Ring(r, s);
+  }
+  AnalyseFromFourier(fourier, spectral, nullptr, false, s);
+}
+
+// ---- Viscoelastic3D ---------------------------------------------------------
+
+Viscoelastic3D::Viscoelastic3D(const Earth& earth, int lmax, const double* delta_dev,
+                               StepOptions options)
+    : earth_(&earth),
+      options_(options),
+      coupling_(earth, lmax, options.oversampling, options.eps) {
+  hipStream_t s = Stream();
+  const auto& geo = coupling_.Geometry();
+  const int np = geo.np;
+  for (int node = 0; node < geo.nodes; ++node) {
+    const int e = coupling_.NodeElement()[node];
+    const int p = node % np;
+    muNodeHost_.push_back(earth.reference.pointMu[static_cast<size_t>(e) * np + p]);
+    itauRefHost_.push_back(earth.reference.pointItau[static_cast<size_t>(e) * np + p]);
+  }
+  muNode_.Upload(muNodeHost_.data(), muNodeHost_.size(), s);
+  itauRefNode_.Upload(itauRefHost_.data(), itauRefHost_.size(), s);
+  const size_t G = coupling_.GridValueCount();
+  SetDelta(delta_dev);
+  BuildTasks();
+  weight_.Resize(G);
+  beta_.Resize(G);
+  const size_t S = coupling_.SpectralDoubles();
+  fixed_.Resize(S);
+  analysed_.Resize(S);
+  s_.Resize(S);
+  sNext_.Resize(S);
+  for (int b = 0; b < 3; ++b) {
+    fourierA_[b].Resize(coupling_.FourierDoubles(b));
+    fourierB_[b].Resize(coupling_.FourierDoubles(b));
+  }
+  rhs_.Resize(rhsDoubles_);
+  amp_.Resize(static_cast<size_t>(lmax + 1) * (lmax + 2));
+  partials_.Resize(1024 * 16);
+  result_.Resize(16);
+  for (auto& e : events_) SL3_HIP(hipEventCreate(&e));
+  SL3_HIP(hipStreamSynchronize(s));
+}
+
+void Viscoelastic3D::SetDelta(const double* delta_dev) {
+  hipStream_t s = Stream();
+  const size_t G = coupling_.GridValueCount();
+  itau_.Resize(G);
+  hipLaunchKernelGGL(ItauKernel, dim3(Blocks(G)), dim3(256), 0, s, delta_dev,
+                     itauRefNode_.data(), itau_.data(), coupling_.Grid().FieldSize(), G);
+  SL3_HIP(hipGetLastError());
+  SL3_HIP(hipStreamSynchronize(s));
+}
+
+void Viscoelastic3D::BuildTasks() {
+  const auto& mesh = *earth_->mesh;
+  const auto& geo = coupling_.Geometry();
+  const int L = coupling_.LMax();
+  const int n = mesh.NGll();
+  // the dof maps (the elastic operators': any point shear gives the same)
+  const PointShear elastic{&earth_->reference.points, &earth_->reference.elasticShear};
+  std::vector<std::unique_ptr<DegreeOperator>> sph(L + 1);
+  std::vector<std::unique_ptr<ToroidalOperator>> tor(L + 1);
+  ParallelFor(2 * L + 1, [&](int i) {
+    if (i <= L) {
+      sph[i] = std::make_unique<DegreeOperator>(earth_->reference.nodal, i, options_.eps, &elastic);
+    } else {
+      const int l = i - L;
+      tor[l] = std::make_unique<ToroidalOperator>(earth_->reference.nodal, l, options_.eps, &elastic);
+    }
+  });
+  tasks_.clear();
+  dofsHost_.clear();
+  rhsDoubles_ = scaleDoubles_ = unitDoubles_ = 0;
+  auto add = [&](int l, int kind) {
+    DegreeTask t;
+    t.l = l;
+    t.kind = kind;
+    t.cols = 2 * (l + 1);
+    t.firstElement = coupling_.FirstElementOfDegree(l);
+    t.firstNode = coupling_.FirstNodes()[l];
+    t.ndof = kind == 0 ? sph[l]->DofCount() : tor[l]->DofCount();
+    t.pinned = (kind == 1 && !tor[l]->Pinned().empty()) ? tor[l]->Pinned()[0] : -1;
+    if (kind == 1 && tor[l]->Pinned().size() > 1) {
+      throw ArgumentError("one solid region above the fluid core is supported");
+    }
+    t.rhsOff = static_cast<long long>(rhsDoubles_);
+    rhsDoubles_ += static_cast<size_t>(t.ndof) * t.cols;
+    t.scaleOff = static_cast<long long>(scaleDoubles_);
+    scaleDoubles_ += t.ndof;
+    if (kind == 0) {
+      t.unitOff = static_cast<long long>(unitDoubles_);
+      unitDoubles_ += t.ndof;
+    }
+    t.dofOff = static_cast<long long>(dofsHost_.size());
+    for (int e = geo.elem0; e < geo.elem0 + geo.elements; ++e) {
+      for (int i = 0; i < n; ++i) {
+        const int gn = mesh.GlobalIndex(e, i);
+        if (e < t.firstElement) {
+          dofsHost_.push_back(-1);
+          dofsHost_.push_back(-1);
+        } else if (kind == 0) {
+          dofsHost_.push_back(sph[l]->Dof(gn, Component::U));
+          dofsHost_.push_back(sph[l]->Dof(gn, Component::V));
+        } else {
+          dofsHost_.push_back(tor[l]->Dof(gn));
+          dofsHost_.push_back(-1);
+        }
+      }
+    }
+    tasks_.push_back(t);
+  };
+  for (int l = 0; l <= L; ++l) add(l, 0);
+  for (int l = 1; l <= L; ++l) add(l, 1);
+  hipStream_t s = Stream();
+  tasksDev_.Upload(tasks_.data(), tasks_.size(), s);
+  dofs_.Upload(dofsHost_.data(), dofsHost_.size(), s);
+  SL3_HIP(hipStreamSynchronize(s));
+}
+
+Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
+  if (dt == 0.0) scheme = kBackwardEuler;  // every scheme's zero-length step
+  for (auto it = flavours_.begin(); it != flavours_.end(); ++it) {
+    if ((*it)->scheme == scheme && (*it)->dt == dt) {
+      flavours_.splice(flavours_.begin(), flavours_, it);
+      return *flavours_.front();
+    }
+  }
+  while (flavours_.size() >= 4) flavours_.pop_back();
+  auto f = std::make_unique<Flavour>();
+  f->scheme = scheme;
+  f->dt = dt;
+  f->pointShear = earth_->reference.PointShearModulus(scheme, dt);
+  const int L = coupling_.LMax();
+  const PointShear ps{&earth_->reference.points, &f->pointShear};
+  std::vector<std::unique_ptr<DegreeOperator>> sph(L + 1);
+  std::vector<std::unique_ptr<ToroidalOperator>> tor(L + 1);
+  ParallelFor(2 * L + 1, [&](int i) {
+    if (i <= L) {
+      sph[i] = std::make_unique<DegreeOperator>(earth_->reference.nodal, i, options_.eps, &ps);
+    } else {
+      const int l = i - L;
+      tor[l] = std::make_unique<ToroidalOperator>(earth_->reference.nodal, l, options_.eps, &ps);
+    }
+  });
+  hipStream_t s = Stream();
+  // The scaling, and the bands for the Cholesky attempt (spheroidal with a
+  // positive diagonal, every toroidal system).
+  std::vector<double> scale(scaleDoubles_), bands;
+  std::vector<BandSystem> chol;
+  std::vector<int> cholTask;
+  for (size_t ti = 0; ti < tasks_.size(); ++ti) {
+    const auto& t = tasks_[ti];
+    const auto& sc = t.kind == 0 ? sph[t.l]->Scale() : tor[t.l]->Scale();
+    std::copy(sc.begin(), sc.end(), scale.begin() + t.scaleOff);
+    if (t.kind == 0 && !sph[t.l]->PositiveDiagonal()) continue;
+    const auto band = t.kind == 0 ? sph[t.l]->ScaledUpperBand() : tor[t.l]->ScaledUpperBand();
+    BandSystem b;
+    b.n = t.ndof;
+    b.kl = t.kind == 0 ? sph[t.l]->Kd() : tor[t.l]->Kd();
+    b.nrhs = t.cols;
+    b.ab_off = static_cast<long long>(bands.size());
+    b.b_off = t.rhsOff;
+    b.row_stride = t.cols;
+    b.col_stride = 1;
+    bands.insert(bands.end(), band.begin(), band.end());
+    chol.push_back(b);
+    cholTask.push_back(static_cast<int>(ti));
+    f->kdMax = std::max(f->kdMax, b.kl);
+  }
+  f->usedCholesky.assign(tasks_.size(), 0);
+  {
+    DeviceArray<double> cholBands;
+    cholBands.Upload(bands.data(), bands.size(), s);
+    DeviceArray<BandSystem> sysDev;
+    sysDev.Upload(chol.data(), chol.size(), s);
+    DeviceArray<int> info(chol.size());
+    LaunchCholeskyFactor(sysDev.data(), static_cast<int>(chol.size()), f->kdMax,
+                         cholBands.data(), info.data(), s);
+    std::vector<int> infoHost(chol.size());
+    SL3_HIP(hipMemcpyAsync(infoHost.data(), info.data(), sizeof(int) * chol.size(),
+                           hipMemcpyDeviceToHost, s));
+    SL3_HIP(hipMemcpyAsync(bands.data(), cholBands.data(), sizeof(double) * bands.size(),
+                           hipMemcpyDeviceToHost, s));
+    SL3_HIP(hipStreamSynchronize(s));
+    // keep the successes; the spheroidal failures fall back to LU
+    // (src/DegreeOperator.cpp:823-851), a toroidal one is an error
+    std::vector<double> factors;
+    std::vector<BandSystem> keep, lu;
+    std::vector<double> luBands;
+    std::vector<int> luTask;
+    for (size_t k = 0; k < chol.size(); ++k) {
+      const auto& t = tasks_[cholTask[k]];
+      if (infoHost[k] != 0) {
+        if (t.kind == 1) {
+          throw NumericalError("toroidal degree " + std::to_string(t.l) +
+                               ": Cholesky failed (info = " + std::to_string(infoHost[k]) + ")");
+        }
+        continue;
+      }
+      BandSystem b = chol[k];
+      b.ab_off = static_cast<long long>(factors.size());
+      factors.insert(factors.end(), bands.begin() + chol[k].ab_off,
+                     bands.begin() + chol[k].ab_off + static_cast<long long>(b.kl + 1) * b.n);
+      keep.push_back(b);
+      f->usedCholesky[cholTask[k]] = 1;
+    }
+    for (size_t ti = 0; ti < tasks_.size(); ++ti) {
+      const auto& t = tasks_[ti];
+      if (t.kind != 0 || f->usedCholesky[ti]) continue;
+      const auto band = sph[t.l]->ScaledLuBand();
+      BandSystem b;
+      b.n = t.ndof;
+      b.kl = b.ku = sph[t.l]->Kd();
+      b.nrhs = t.cols;
+      b.ab_off = static_cast<long long>(luBands.size());
+      b.piv_off = static_cast<long long>(luTask.size() ? 0 : 0);
+      b.b_off = t.rhsOff;
+      b.row_stride = t.cols;
+      b.col_stride = 1;
+      luBands.insert(luBands.end(), band.begin(), band.end());
+      lu.push_back(b);
+      luTask.push_back(static_cast<int>(ti));
+    }
+    long long pivTotal = 0;
+    for (auto& b : lu) {
+      b.piv_off = pivTotal;
+      pivTotal += b.n;
+    }
+    const size_t luBase = factors.size();
+    for (auto& b : lu) b.ab_off += static_cast<long long>(luBase);
+    factors.insert(factors.end(), luBands.begin(), luBands.end());
+    f->factors.Upload(factors.data(), factors.size(), s);
+    f->pivots.Resize(std::max<long long>(pivTotal, 1));
+    if (!lu.empty()) {
+      DeviceArray<BandSystem> luDev;
+      luDev.Upload(lu.data(), lu.size(), s);
+      DeviceArray<int> luInfo(lu.size());
+      int klMax = 0;
+      for (const auto& b : lu) klMax = std::max(klMax, b.kl);
+      LaunchLuFactor(luDev.data(), static_cast<int>(lu.size()), klMax, klMax,
+                     f->factors.data(), f->pivots.data(), luInfo.data(), s);
+      std::vector<int> li(lu.size());
+      SL3_HIP(hipMemcpyAsync(li.data(), luInfo.data(), sizeof(int) * lu.size(),
+                             hipMemcpyDeviceToHost, s));
+      SL3_HIP(hipStreamSynchronize(s));
+      for (size_t k = 0; k < lu.size(); ++k) {
+        if (li[k] != 0) {
+          throw NumericalError("degree " + std::to_string(tasks_[luTask[k]].l) +
+                               ": the banded LU factorisation failed (info = " +
+                               std::to_string(li[k]) + ")");
+        }
+      }
+    }
+    f->cholesky = keep;
+    f->lu = lu;
+    SL3_HIP(hipStreamSynchronize(s));  // the host staging vectors end here
+  }
+  f->scale.Upload(scale.data(), scale.size(), s);
+  f->choleskyDev.Upload(f->cholesky.data(), f->cholesky.size(), s);
+  f->luDev.Upload(f->lu.data(), f->lu.size(), s);
+  const auto ct = SolveTasks(f->cholesky);
+  const auto lt = SolveTasks(f->lu);
+  f->choleskyTasks.Upload(ct.data(), ct.size(), s);
+  f->luTasks.Upload(lt.data(), lt.size(), s);
+  f->choleskyTaskCount = static_cast<int>(ct.size());
+  f->luTaskCount = static_cast<int>(lt.size());
+
+  // The unit load responses (Viscoelastic3D::AddUnitChannels' cache,
+  // src/Viscoelastic3D.cpp:1042-1055): Load() solved with this flavour.
+  std::vector<double> unitRhs(unitDoubles_, 0.0);
+  for (const auto& t : tasks_) {
+    if (t.kind != 0) continue;
+    const auto load = sph[t.l]->Load();
+    for (int d = 0; d < t.ndof; ++d) {
+      unitRhs[t.unitOff + d] = load[d] * scale[t.scaleOff + d];  // BeginSolve
+    }
+  }
+  DeviceArray<double> unitDev;
+  unitDev.Upload(unitRhs.data(), unitRhs.size(), s);
+  auto single = [&](std::vector<BandSystem> sys) {
+    std::vector<BandSystem> out;
+    for (auto b : sys) {
+      int ti = -1;
+      for (size_t k = 0; k < tasks_.size(); ++k) {
+        if (tasks_[k].rhsOff == b.b_off) ti = static_cast<int>(k);
+      }
+      if (ti < 0 || tasks_[ti].kind != 0) continue;
+      b.nrhs = 1;
+      b.b_off = tasks_[ti].unitOff;
+      b.row_stride = 1;
+      b.col_stride = b.n;
+      out.push_back(b);
+    }
+    return out;
+  };
+  const auto uc = single(f->cholesky), ul = single(f->lu);
+  DeviceArray<BandSystem> ucDev, ulDev;
+  ucDev.Upload(uc.data(), uc.size(), s);
+  ulDev.Upload(ul.data(), ul.size(), s);
+  const auto uct = SolveTasks(uc), ult = SolveTasks(ul);
+  DeviceArray<int2> uctDev, ultDev;
+  uctDev.Upload(uct.data(), uct.size(), s);
+  ultDev.Upload(ult.data(), ult.size(), s);
+  LaunchCholeskySolve(ucDev.data(), uctDev.data(), static_cast<int>(uct.size()),
+                      f->kdMax, f->factors.data(), unitDev.data(), s);
+  int kvMax = 0;
+  for (const auto& b : f->lu) kvMax = std::max(kvMax, b.kl + b.ku);
+  LaunchLuSolve(ulDev.data(), ultDev.data(), static_cast<int>(ult.size()), kvMax,
+                f->factors.data(), f->pivots.data(), unitDev.data(), s);
+  std::vector<double> units(unitDoubles_);
+  SL3_HIP(hipMemcpyAsync(units.data(), unitDev.data(), sizeof(double) * units.size(),
+                         hipMemcpyDeviceToHost, s));
+  SL3_HIP(hipStreamSynchronize(s));
+  for (const auto& t : tasks_) {
+    if (t.kind != 0) continue;
+    for (int d = 0; d < t.ndof; ++d) units[t.unitOff + d] *= scale[t.scaleOff + d];  // FinishSolve
+  }
+  f->units.Upload(units.data(), units.size(), s);
+  SL3_HIP(hipStreamSynchronize(s));
+  flavours_.push_front(std::move(f));
+  return *flavours_.front();
+}
+
+int Viscoelastic3D::CholeskyFallbacks(int scheme, double dt) {
+  auto& f = FlavourOf(scheme, dt);
+  int count = 0;
+  for (size_t k = 0; k < tasks_.size(); ++k) count += f.usedCholesky[k] ? 0 : 1;
+  return count;
+}
+
+std::unique_ptr<DeviceState> Viscoelastic3D::ZeroState() const {
+  auto state = std::make_unique<DeviceState>();
+  hipStream_t s = Stream();
+  for (int b = 0; b < 3; ++b) {
+    state->m[b].Resize(coupling_.GridDoubles(b));
+    LaunchFill(state->m[b].data(), state->m[b].size(), 0.0, s);
+  }
+  return state;
+}
+
+void Viscoelastic3D::SolveAll(const double* weighted, Flavour& f) {
+  hipStream_t s = Stream();
+  LaunchFill(rhs_.data(), rhs_.size(), 0.0, s);
+  const SpectralView w = coupling_.View(const_cast<double*>(weighted));
+  LaunchForce(tasksDev_.data(), static_cast<int>(tasks_.size()), coupling_.LMax(),
+              coupling_.Geometry(), w, f.scale.data(), dofs_.data(), rhs_.data(), s);
+  LaunchCholeskySolve(f.choleskyDev.data(), f.choleskyTasks.data(), f.choleskyTaskCount,
+                      f.kdMax, f.factors.data(), rhs_.data(), s);
+  int kvMax = 0;
+  for (const auto& b : f.lu) kvMax = std::max(kvMax, b.kl + b.ku);
+  LaunchLuSolve(f.luDev.data(), f.luTasks.data(), f.luTaskCount, kvMax, f.factors.data(),
+                f.pivots.data(), rhs_.data(), s);
+}
+
+void Viscoelastic3D::Amplitudes(const LoadFn& load, double t) {
+  const int L = coupling_.LMax();
+  std::vector<double> amp(static_cast<size_t>(L + 1) * (L + 2), 0.0);
+  if (load) {
+    std::vector<double> cols(2 * (L + 1));
+    for (int l = 0; l <= L; ++l) {
+      std::fill(cols.begin(), cols.end(), 0.0);
+      if (!load(l, t, cols.data())) continue;
+      for (int m = 0; m <= l; ++m) {
+        const size_t at = static_cast<size_t>(l) * (l + 1) / 2 + m;
+        amp[2 * at] = cols[2 * m];
+        amp[2 * at + 1] = cols[2 * m + 1];
+      }
+    }
+  }
+  hipStream_t s = Stream();
+  SL3_HIP(hipMemcpyAsync(amp_.data(), amp.data(), sizeof(double) * amp.size(),
+                         hipMemcpyHostToDevice, s));
+  SL3_HIP(hipStreamSynchronize(s));
+}
+
+double Viscoelastic3D::RelativeChange(const double* next, const double* now) {
+  hipStream_t s = Stream();
+  LaunchRelativeChange(next, now, coupling_.SpectralDoubles(), partials_.data(), 1024,
+                       result_.data(), s);
+  double r[3];
+  SL3_HIP(hipMemcpyAsync(r, result_.data(), sizeof(r), hipMemcpyDeviceToHost, s));
+  SL3_HIP(hipStreamSynchronize(s));
+  if (r[2] != 0.0) return std::numeric_limits<double>::quiet_NaN();
+  return r[0] > 0.0 ? std::sqrt(r[1]) / std::sqrt(r[0]) : 0.0;
+}
+
+void Viscoelastic3D::Pass(const double* s, double* sNext, bool useStartForce, Flavour& f) {
+  hipStream_t st = Stream();
+  if (!useStartForce) {
+    double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
+    double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
+    coupling_.SynthesiseToFourier(s, A, true, st);
+    for (int b = 0; b < 3; ++b) {
+      RingLaunch r;
+      r.block = b;
+      r.op = kRingPass;
+      r.fourierIn = A[b];
+      r.fourierOut = B[b];
+      r.weight = weight_.data();
+      coupling_.Ring(r, st);
+    }
+    coupling_.AnalyseFromFourier(B, analysed_.data(), fixed_.data(), true, st);
+  }
+  SolveAll(analysed_.data(), f);
+  const auto* sph = tasksDev_.data();
+  const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
+  LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), f.scale.data(),
+               dofs_.data(), rhs_.data(), f.units.data(), amp_.data(),
+               coupling_.View(sNext), st);
+}
+
+void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const LoadFn& load,
+                                               double* out) {
+  hipStream_t s = Stream();
+  Flavour& f = FlavourOf(kBackwardEuler, 0.0);
+  double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
+  for (int b = 0; b < 3; ++b) {
+    RingLaunch r;
+    r.block = b;
+    r.op = kRingNodeScaled;
+    r.gridIn = state.m[b].data();
+    r.fourierOut = B[b];
+    r.muNode = muNode_.data();
+    coupling_.Ring(r, s);
+  }
+  coupling_.AnalyseFromFourier(B, analysed_.data(), nullptr, true, s);
+  SolveAll(analysed_.data(), f);
+  Amplitudes(load, state.t);
+  const auto* sph = tasksDev_.data();
+  const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
+  LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), f.scale.data(),
+               dofs_.data(), rhs_.data(), f.units.data(), amp_.data(),
+               coupling_.View(out), s);
+}
+
+StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
+                              const LoadFn& load, const double* start) {
+  if (dt < 0.0) throw ArgumentError("dt must be non-negative");
+  if (scheme != kExponentialTrapezoid && scheme != kBackwardEuler) {
+    throw ArgumentError("the 3D step is implicit: ExponentialTrapezoid or BackwardEuler");
+  }
+  if (dt == 0.0) scheme = kBackwardEuler;
+  hipStream_t s = Stream();
+  const size_t S = coupling_.SpectralDoubles();
+  double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
+  double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
+  Flavour& f = FlavourOf(scheme, dt);
+  const double tEnd = state.t + dt;
+
+  SL3_HIP(hipEventRecord(events_[0], s));
+  // s0 (src/Viscoelastic3D.cpp:1252-1263)
+  if (state.hasCoefficients) {
+    LaunchCopy(s_.data(), state.coefficients.data(), S, s);
+  } else if (state.hasStrain) {
+    const double* grid[3] = {state.strain[0].data(), state.strain[1].data(),
+                             state.strain[2].data()};
+    coupling_.Analyse(grid, s_.data(), B, s);
+  } else {
+    ElasticStrainCoefficients(state, load, s_.data());
+  }
+  // StartStep's sweep (:1283-1376)
+  if (!state.hasStrain) coupling_.SynthesiseToFourier(s_.data(), A, false, s);
+  for (int b = 0; b < 3; ++b) {
+    RingLaunch r;
+    r.block = b;
+    r.op = state.hasStrain ? kRingStart : kRingStartFromSpectral;
+    r.gridIn = state.hasStrain ? state.strain[b].data() : nullptr;
+    r.fourierIn = A[b];
+    r.fourierOut = B[b];   // the fixed force mu m~
+    r.fourierOut2 = A[b];  // W d0 (each workgroup rewrites only its own rings)
+    r.memIn = state.m[b].data();
+    r.memOut = state.m[b].data();
+    r.itau = itau_.data();
+    r.betaOut = beta_.data();
+    r.weightOut = weight_.data();
+    r.muNode = muNode_.data();
+    r.itauRefNode = itauRefNode_.data();
+    r.dt = dt;
+    r.scheme = scheme;
+    coupling_.Ring(r, s);
+  }
+  coupling_.AnalyseFromFourier(B, fixed_.data(), nullptr, true, s);
+  coupling_.AnalyseFromFourier(A, analysed_.data(), fixed_.data(), true, s);
+  Amplitudes(load, tEnd);
+  SL3_HIP(hipEventRecord(events_[1], s));
+
+  // Converge (:1916-1974)
+  StepInfo info;
+  residualHistory.clear();
+  bool warm = false;
+  if (start != nullptr) {
+    LaunchCopy(s_.data(), start, S, s);
+    warm = true;
+  }
+  const bool adaptive = options_.andersonRate > 0.0;
+  const int memory = options_.andersonMemory;
+  bool mixing = memory > 0 && !adaptive;
+  bool mixerStarted = false;
+  int count = 0, slot = 0;
+  std::vector<double> gram(static_cast<size_t>(std::max(memory, 1)) * std::max(memory, 1), 0.0);
+  if (memory > 0 && history_.size() != 2 * static_cast<size_t>(memory) * S) {
+    history_.Resize(2 * static_cast<size_t>(memory) * S);
+    xPrev_.Resize(S);
+    fPrev_.Resize(S);
+  }
+  double previous = 0.0;
+  for (int it = 0; it < options_.maxIterations; ++it) {
+    Pass(s_.data(), sNext_.data(), it == 0 && !warm, f);
+    const double residual = RelativeChange(sNext_.data(), s_.data());
+    info.iterations = it + 1;
+    info.residual = residual;
+    residualHistory.push_back(residual);
+    if (!std::isfinite(residual)) break;
+    if (residual <= options_.rtol) {
+      info.converged = true;
+      std::swap(s_, sNext_);
+      break;
+    }
+    if (options_.damping != 1.0) {
+      hipLaunchKernelGGL(DampKernel, dim3(Blocks(S)), dim3(256), 0, s, sNext_.data(),
+                         s_.data(), options_.damping, S);
+      SL3_HIP(hipGetLastError());
+    }
+    if (adaptive && !mixing && memory > 0 && it > 0 &&
+        residual > options_.andersonRate * previous) {
+      mixing = true;
+    }
+    previous = residual;
+    if (mixing) {
+      if (!mixerStarted) {
+        LaunchAndersonFirst(s_.data(), sNext_.data(), xPrev_.data(), fPrev_.data(), S, s);
+        mixerStarted = true;
+      } else {
+        const int m = std::min(count + 1, memory);
+        AndersonSweep sw{};
+        sw.x = s_.data();
+        sw.gx = sNext_.data();
+        sw.xPrev = xPrev_.data();
+        sw.fPrev = fPrev_.data();
+        for (int i = 0; i < memory && i < 8; ++i) {
+          sw.dF[i] = history_.data() + static_cast<size_t>(i) * S;
+          sw.dG[i] = history_.data() + static_cast<size_t>(memory + i) * S;
+        }
+        sw.m = m;
+        sw.slot = slot;
+        LaunchAndersonSweep(sw, S, partials_.data(), 1024, result_.data(), s);
+        double sums[16];
+        SL3_HIP(hipMemcpyAsync(sums, result_.data(), sizeof(sums), hipMemcpyDeviceToHost, s));
+        SL3_HIP(hipStreamSynchronize(s));
+        count = m;
+        for (int i = 0; i < m; ++i) {
+          gram[slot * memory + i] = sums[i];
+          gram[i * memory + slot] = sums[i];
+        }
+        slot = (slot + 1) % memory;
+        double g[64], gamma[8];
+        for (int i = 0; i < m; ++i)
+          for (int j = 0; j < m; ++j) g[i * m + j] = gram[i * memory + j];
+        GramLeastSquares(m, g, sums + 8, gamma);
+        LaunchAndersonMix(sw, gamma, S, s);
+      }
+      ++info.mixedPasses;
+    } else {
+      std::swap(s_, sNext_);
+    }
+  }
+  SL3_HIP(hipEventRecord(events_[2], s));
+
+  // Finish (:1900-1914): d = Synthesise(strain), m+ = m~ + beta d
+  coupling_.SynthesiseToFourier(s_.data(), A, true, s);
+  for (int b = 0; b < 3; ++b) {
+    if (!state.hasStrain) state.strain[b].Resize(coupling_.GridDoubles(b));
+    RingLaunch r;
+    r.block = b;
+    r.op = kRingFinish;
+    r.fourierIn = A[b];
+    r.gridOut = state.strain[b].data();
+    r.memOut = state.m[b].data();
+    r.beta = beta_.data();
+    coupling_.Ring(r, s);
+  }
+  state.hasStrain = true;
+  if (state.coefficients.size() != S) state.coefficients.Resize(S);
+  LaunchCopy(state.coefficients.data(), s_.data(), S, s);
+  state.hasCoefficients = true;
+  state.t = tEnd;
+  SL3_HIP(hipEventRecord(events_[3], s));
+  SL3_HIP(hipStreamSynchronize(s));
+  SL3_HIP(hipEventElapsedTime(&lastTiming.startStep, events_[0], events_[1]));
+  SL3_HIP(hipEventElapsedTime(&lastTiming.passes, events_[1], events_[2]));
+  SL3_HIP(hipEventElapsedTime(&lastTiming.finish, events_[2], events_[3]));
+  return info;
+}
+
+std::vector<double> Viscoelastic3D::SolutionOfTask(int task) {
+  const auto& t = tasks_.at(task);
+  Flavour& f = *flavours_.front();
+  hipStream_t s = Stream();
+  std::vector<double> raw(static_cast<size_t>(t.ndof) * t.cols), scale(t.ndof),
+      units(t.kind == 0 ? t.ndof : 0), amp(2 * (t.l + 1));
+  SL3_HIP(hipMemcpyAsync(raw.data(), rhs_.data() + t.rhsOff, sizeof(double) * raw.size(),
+                         hipMemcpyDeviceToHost, s));
+  SL3_HIP(hipMemcpyAsync(scale.data(), f.scale.data() + t.scaleOff,
+                         sizeof(double) * t.ndof, hipMemcpyDeviceToHost, s));
+  if (t.kind == 0) {
+    SL3_HIP(hipMemcpyAsync(units.data(), f.units.data() + t.unitOff,
+                           sizeof(double) * t.ndof, hipMemcpyDeviceToHost, s));
+    SL3_HIP(hipMemcpyAsync(amp.data(),
+                           amp_.data() + static_cast<size_t>(t.l) * (t.l + 1),
+                           sizeof(double) * amp.size(), hipMemcpyDeviceToHost, s));
+  }
+  SL3_HIP(hipStreamSynchronize(s));
+  // the reference's Solutions layout: ndof x (l + 1) complex, column-major
+  std::vector<double> out(static_cast<size_t>(t.ndof) * (t.l + 1) * 2);
+  for (int m = 0; m <= t.l; ++m) {
+    for (int d = 0; d < t.ndof; ++d) {
+      double re = raw[static_cast<size_t>(d) * t.cols + m] * scale[d];
+      double im = raw[static_cast<size_t>(d) * t.cols + t.l + 1 + m] * scale[d];
+      if (t.kind == 0) {
+        re += amp[2 * m] * units[d];
+        im += amp[2 * m + 1] * units[d];
+      }
+      out[2 * (static_cast<size_t>(m) * t.ndof + d)] = re;
+      out[2 * (static_cast<size_t>(m) * t.ndof + d) + 1] = im;
+    }
+  }
+  return out;
+}
+
+}  // namespace sl3amd

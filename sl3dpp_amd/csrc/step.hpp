@@ -1,0 +1,206 @@
+// The 3D lateral-viscosity step on the GPU: the device-resident grid
+// coupling, the per-(scheme, dt) flavours of factorised degree systems,
+// and Viscoelastic3D's Step (StartStep, the Converge fixed point with
+// adaptive Anderson mixing, Finish), mirroring the reference's
+// include/sl3dpp/Viscoelastic3D.hpp:89-675 for one Maxwell branch, Gauss
+// memory points, one node block and one process.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <list>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "coupling.hpp"
+#include "grid.hpp"
+#include "kernels.hpp"
+#include "model.hpp"
+#include "operators.hpp"
+#include "transform.hpp"
+
+namespace sl3amd {
+
+// Viscoelastic3D::Options (include/sl3dpp/Viscoelastic3D.hpp:89-187), the
+// fixed-point subset.
+struct StepOptions {
+  double eps = 1e-8;
+  double oversampling = 1.5;
+  int maxIterations = 40;
+  double rtol = 1e-11;
+  double damping = 1.0;
+  int andersonMemory = 5;
+  double andersonRate = 0.5;
+};
+
+// Viscoelastic3D::StepInfo (include/sl3dpp/Viscoelastic3D.hpp:357-375).
+struct StepInfo {
+  int iterations = 0;
+  double residual = 0.0;
+  bool converged = false;
+  int mixedPasses = 0;
+};
+
+// LoadCallable3D: the m >= 0 columns of degree l at time t, (re, im) pairs,
+// or false for none.
+using LoadFn = std::function<bool(int l, double t, double* columns)>;
+
+// The device grid coupling (GridCoupling + SurfaceGrid): memory nodes,
+// Wigner tables, DFT tables, layouts.
+class Coupling {
+ public:
+  Coupling(const Earth& earth, int lmax, double oversampling, double eps);
+  int LMax() const { return lmax_; }
+  int NodeCount() const { return geometry_.nodes; }
+  const SurfaceGrid& Grid() const { return grid_; }
+  size_t GridValueCount() const { return static_cast<size_t>(geometry_.nodes) * grid_.FieldSize(); }
+  // doubles of each grid component (complex: 2 per value) and of a
+  // spectral tensor (three blocks end to end, device layout)
+  size_t GridDoubles(int block) const { return (block == 2 ? 1 : 2) * GridValueCount(); }
+  size_t SpectralDoubles() const { return spectralOffset_[3]; }
+  size_t SpectralOffset(int block) const { return spectralOffset_[block]; }
+  size_t FourierDoubles(int block) const;
+  const RadialGeometry& Geometry() const { return geometry_; }
+  const std::vector<int>& FirstNodes() const { return firstNodeHost_; }
+  const std::vector<double>& NodeRadii() const { return radiusHost_; }
+  const std::vector<int>& NodeElement() const { return nodeElement_; }
+  int FirstElementOfDegree(int l) const { return firstElement_[l]; }
+  SpectralView View(double* spectral) const;
+  const int* RowOffsetDevice(int block) const { return rowOffset_[block].data(); }
+
+  // Batched transforms of all three blocks over every node.
+  void SynthesiseToFourier(const double* spectral, double* const fourier[3],
+                           bool truncated, hipStream_t s) const;
+  void AnalyseFromFourier(const double* const fourier[3], double* spectral,
+                          const double* addend, bool truncated, hipStream_t s) const;
+  void Ring(RingLaunch r, hipStream_t s) const;  // fills tables and sizes
+  // One block's Legendre stage alone (the C-ABI transforms).
+  void SynthesiseBlockToFourier(int block, const double* spectral, double* fourier,
+                                hipStream_t s) const;
+  void AnalyseBlockFromFourier(int block, const double* fourier, double* spectral,
+                               hipStream_t s) const;
+  void Synthesise(const double* spectral, double* const grid[3], double* const fourier[3],
+                  hipStream_t s) const;
+  void Analyse(const double* const grid[3], double* spectral, double* const fourier[3],
+               hipStream_t s) const;
+
+ private:
+  int lmax_;
+  SurfaceGrid grid_;
+  RadialGeometry geometry_;
+  std::vector<int> nodeElement_;
+  std::vector<double> radiusHost_;
+  std::vector<int> firstElement_;  // per degree
+  std::vector<int> firstNodeHost_, reachHost_;
+  DeviceArray<double> radius_, jac_, weight_, value_, deriv_;
+  DeviceArray<double> table_[3];
+  DeviceArray<int> rowOffset_[3];
+  DeviceArray<int> orders_;
+  int orderCount_ = 0;
+  DeviceArray<int> firstNode_, reach_;
+  DeviceArray<double> wtheta_, cosP_, sinP_, cosN_, sinN_;
+  DftTables dft_;
+  size_t spectralOffset_[4] = {};
+  LegendrePlan plan_;
+};
+
+// The factorised systems of one (scheme, dt): Viscoelastic3D::Flavour
+// (src/Viscoelastic3D.cpp:158-167, 535-558).
+struct Flavour {
+  int scheme = 0;
+  double dt = 0.0;
+  std::vector<double> pointShear;
+  DeviceArray<double> scale, units, factors;
+  DeviceArray<int> pivots;
+  std::vector<BandSystem> cholesky, lu;  // the solve batches
+  DeviceArray<BandSystem> choleskyDev, luDev;
+  DeviceArray<int2> choleskyTasks, luTasks;
+  int choleskyTaskCount = 0, luTaskCount = 0;
+  int kdMax = 0;
+  std::vector<char> usedCholesky;  // per task
+};
+
+// Viscoelastic3D::State on the device: one branch's memory tensor, the
+// strain on the grid and its coefficients (device layout).
+struct DeviceState {
+  double t = 0.0;
+  DeviceArray<double> m[3];
+  bool hasStrain = false;
+  DeviceArray<double> strain[3];
+  bool hasCoefficients = false;
+  DeviceArray<double> coefficients;
+};
+
+class Viscoelastic3D {
+ public:
+  // delta_dev: delta(r, theta, phi) sampled on every memory node's grid
+  // ([node][theta][phi], device), or null for none.
+  Viscoelastic3D(const Earth& earth, int lmax, const double* delta_dev,
+                 StepOptions options);
+  const Coupling& GetCoupling() const { return coupling_; }
+  const StepOptions& Options() const { return options_; }
+  StepOptions& MutableOptions() { return options_; }
+  std::unique_ptr<DeviceState> ZeroState() const;
+  void SetDelta(const double* delta_dev);
+  // Step (src/Viscoelastic3D.cpp:1982-2030) for ExponentialTrapezoid or
+  // BackwardEuler, in place on the state. `start` (device spectral, or
+  // null) is the warm start of Converge (:1916-1921).
+  StepInfo Step(DeviceState& state, double dt, int scheme, const LoadFn& load,
+                const double* start = nullptr);
+  // The last pass's solutions, unscaled with channels, per task: host
+  // export for the surface outputs and tests.
+  std::vector<double> SolutionOfTask(int task);
+  const std::vector<DegreeTask>& Tasks() const { return tasks_; }
+  Flavour& FlavourOf(int scheme, double dt);
+  // Timings of the last Step's phases (device events), milliseconds.
+  struct Timing {
+    float startStep = 0, passes = 0, finish = 0;
+  } lastTiming;
+  // Per-pass residuals of the last step.
+  std::vector<double> residualHistory;
+  // The device buffers, for export.
+  const double* Weight() const { return weight_.data(); }
+  const double* Beta() const { return beta_.data(); }
+  const double* Itau() const { return itau_.data(); }
+  const double* FixedForce() const { return fixed_.data(); }
+  const double* Analysed() const { return analysed_.data(); }
+  const double* Rhs() const { return rhs_.data(); }
+  const double* AmplitudesDevice() const { return amp_.data(); }
+  const int* Dofs() const { return dofs_.data(); }
+  const std::vector<int>& DofsHost() const { return dofsHost_; }
+  const std::vector<double>& MuNode() const { return muNodeHost_; }
+  const std::vector<double>& ItauRefNode() const { return itauRefHost_; }
+  int CholeskyFallbacks(int scheme, double dt);
+
+  // One pass: Respond(s) + AddChannels + StrainOf into sNext.
+  void Pass(const double* s, double* sNext, bool useStartForce, Flavour& f);
+
+ private:
+  void BuildTasks();
+  void SolveAll(const double* weighted, Flavour& f);
+  void Amplitudes(const LoadFn& load, double t);
+  double RelativeChange(const double* next, const double* now);
+  void ElasticStrainCoefficients(DeviceState& state, const LoadFn& load, double* out);
+
+  const Earth* earth_;
+  StepOptions options_;
+  Coupling coupling_;
+  std::vector<double> muNodeHost_, itauRefHost_;
+  DeviceArray<double> muNode_, itauRefNode_, itau_;
+  std::vector<DegreeTask> tasks_;  // spheroidal l = 0..L, then toroidal 1..L
+  DeviceArray<DegreeTask> tasksDev_;
+  std::vector<int> dofsHost_;
+  DeviceArray<int> dofs_;
+  size_t rhsDoubles_ = 0, scaleDoubles_ = 0, unitDoubles_ = 0;
+  std::list<std::unique_ptr<Flavour>> flavours_;
+  // the open step's buffers
+  DeviceArray<double> weight_, beta_, fixed_, analysed_, rhs_, amp_, s_, sNext_;
+  DeviceArray<double> fourierA_[3], fourierB_[3];
+  DeviceArray<double> partials_, result_;
+  DeviceArray<double> history_, xPrev_, fPrev_;
+  hipEvent_t events_[4] = {};
+};
+
+}  // namespace sl3amd

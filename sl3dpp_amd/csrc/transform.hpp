@@ -1,0 +1,76 @@
+// Launch interface of the transform kernels (transform.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace sl3amd {
+
+// The Legendre stage's tables for the three stored blocks (spin -2, -1
+// complex over all orders; spin 0 real over m >= 0), device pointers.
+struct LegendrePlan {
+  const double* table[3] = {};
+  const int* rowOffset[3] = {};
+  int mlo[3] = {};
+  int spin[3] = {};
+  const int* orders = nullptr;  // (block, m) per launch z, device
+  int orderCount = 0;
+  int lmax = 0, nodes = 0, ntheta = 0;
+};
+
+// The longitude DFT's tables: nphi = R P; cos/sin(2 pi j / P) and
+// cos/sin(2 pi j / nphi), device pointers.
+struct DftTables {
+  int R = 1, P = 1;
+  const double* cosP = nullptr;
+  const double* sinP = nullptr;
+  const double* cosN = nullptr;
+  const double* sinN = nullptr;
+};
+
+enum RingOp {
+  kRingSynthesise = 0,        // F -> grid
+  kRingAnalyse = 1,           // grid -> F
+  kRingWeighted = 2,          // grid x weight -> F
+  kRingPass = 3,              // F -> ring x weight -> F (no grid round trip)
+  kRingStart = 4,             // step start from a stored strain grid
+  kRingStartFromSpectral = 5, // step start, d0 synthesised from F
+  kRingFinish = 6,            // F -> d grid, m += beta d
+  kRingNodeScaled = 7,        // grid x mu(node) -> F
+};
+
+struct RingLaunch {
+  DftTables tables;
+  int lmax = 0, nodes = 0, ntheta = 0, nphi = 0;
+  int block = 0;
+  int op = kRingAnalyse;
+  const double* wtheta = nullptr;
+  const double* fourierIn = nullptr;
+  double* fourierOut = nullptr;
+  double* fourierOut2 = nullptr;
+  const double* gridIn = nullptr;
+  double* gridOut = nullptr;
+  const double* weight = nullptr;
+  const double* memIn = nullptr;
+  double* memOut = nullptr;
+  const double* itau = nullptr;
+  double* betaOut = nullptr;
+  double* weightOut = nullptr;
+  const double* beta = nullptr;
+  const double* muNode = nullptr;
+  const double* itauRefNode = nullptr;
+  double dt = 0.0;
+  int scheme = 0;
+};
+
+size_t RingLds(int nphi, int P);
+void LaunchLegendreSynthesis(const LegendrePlan& p, const double* const spectral[3],
+                             double* const fourier[3], const int* reach,
+                             hipStream_t stream);
+void LaunchLegendreAnalysis(const LegendrePlan& p, const double* const fourier[3],
+                            double* const spectral[3], const double* const addend[3],
+                            const int* firstNode, hipStream_t stream);
+void LaunchRing(const RingLaunch& r, hipStream_t stream);
+
+}  // namespace sl3amd

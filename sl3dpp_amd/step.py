@@ -1,0 +1,347 @@
+"""Python mirror of the 3D step's interface over the C ABI: the Earth set-up,
+Viscoelastic3D (Step, ZeroState, the exports) and the synthetic inputs of
+the benchmark configurations (the three-region Earth, the zonal load rate of
+the reference's step benchmark, the lateral viscosity field).
+
+Names and argument meaning follow include/sl3dpp/Viscoelastic3D.hpp; device
+arrays are torch tensors on the HIP device (memory and stream plumbing)."""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import check, lib
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+YEAR = 365.25 * 24.0 * 3600.0
+EXPONENTIAL_TRAPEZOID = 0
+BACKWARD_EULER = 1
+
+MAX_LAYERS = 8
+
+
+class EarthSpec(ctypes.Structure):
+    _fields_ = [
+        ("layers", ctypes.c_int),
+        ("r0", ctypes.c_double * MAX_LAYERS),
+        ("r1", ctypes.c_double * MAX_LAYERS),
+        ("rho0", ctypes.c_double * MAX_LAYERS),
+        ("rho1", ctypes.c_double * MAX_LAYERS),
+        ("kappa0", ctypes.c_double * MAX_LAYERS),
+        ("kappa1", ctypes.c_double * MAX_LAYERS),
+        ("mu0", ctypes.c_double * MAX_LAYERS),
+        ("mu1", ctypes.c_double * MAX_LAYERS),
+        ("elements", ctypes.c_int * MAX_LAYERS),
+        ("ngll", ctypes.c_int),
+        ("viscous_layer", ctypes.c_int),
+        ("viscosity_boundary", ctypes.c_double),
+        ("viscosity_upper", ctypes.c_double),
+        ("viscosity_lower", ctypes.c_double),
+    ]
+
+
+class StepOptions(ctypes.Structure):
+    _fields_ = [
+        ("eps", ctypes.c_double),
+        ("oversampling", ctypes.c_double),
+        ("max_iterations", ctypes.c_int),
+        ("rtol", ctypes.c_double),
+        ("damping", ctypes.c_double),
+        ("anderson_memory", ctypes.c_int),
+        ("anderson_rate", ctypes.c_double),
+    ]
+
+    @classmethod
+    def default(cls, **overrides):
+        o = cls()
+        lib.sl3_step_options_default(ctypes.byref(o))
+        for k, v in overrides.items():
+            setattr(o, k, v)
+        return o
+
+
+class StepInfo(ctypes.Structure):
+    """Viscoelastic3D::StepInfo plus the device time of the step's phases."""
+
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("residual", ctypes.c_double),
+        ("converged", ctypes.c_int),
+        ("mixed_passes", ctypes.c_int),
+        ("start_ms", ctypes.c_float),
+        ("passes_ms", ctypes.c_float),
+        ("finish_ms", ctypes.c_float),
+    ]
+
+
+LOAD_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                           c_double_p, ctypes.c_void_p)
+
+lib.sl3_step_options_default.restype = None
+
+
+def three_region_earth(core=70, mantle=56, lithosphere=2, ngll=6):
+    """The benchmark configurations' Earth: radius 6371 km; fluid core to
+    3480 km (rho 12000 -> 9900, kappa 1300 -> 650 GPa, mu 0); mantle to
+    6271 km (rho 5500 -> 3400, kappa 650 -> 130 GPa, mu 290 -> 65 GPa);
+    lithosphere (rho 3300, kappa 130 GPa, mu 65 GPa), all linear in r; a
+    Maxwell mantle of 5e20 Pa s above 5701 km and 1e22 below, the
+    lithosphere elastic. Elements per region and GLL nodes per element as
+    given (128 elements x 6 nodes for the 3D step)."""
+    s = EarthSpec()
+    s.layers = 3
+    G = 1e9
+    rows = [
+        (0.0, 3480e3, 12000.0, 9900.0, 1300 * G, 650 * G, 0.0, 0.0, core),
+        (3480e3, 6271e3, 5500.0, 3400.0, 650 * G, 130 * G, 290 * G, 65 * G, mantle),
+        (6271e3, 6371e3, 3300.0, 3300.0, 130 * G, 130 * G, 65 * G, 65 * G, lithosphere),
+    ]
+    for i, (r0, r1, d0, d1, k0, k1, m0, m1, n) in enumerate(rows):
+        s.r0[i], s.r1[i], s.rho0[i], s.rho1[i] = r0, r1, d0, d1
+        s.kappa0[i], s.kappa1[i], s.mu0[i], s.mu1[i] = k0, k1, m0, m1
+        s.elements[i] = n
+    s.ngll = ngll
+    s.viscous_layer = 1
+    s.viscosity_boundary = 5701e3
+    s.viscosity_upper = 5e20
+    s.viscosity_lower = 1e22
+    return s
+
+
+class Earth:
+    """The layered Earth, mesh, material, Maxwell rheology (host)."""
+
+    def __init__(self, spec=None):
+        self.spec = spec if spec is not None else three_region_earth()
+        self.handle = ctypes.c_void_p()
+        check(lib.sl3_earth_create(ctypes.byref(self.spec), ctypes.byref(self.handle)))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.sl3_earth_destroy(self.handle)
+            self.handle = None
+
+    def scales(self):
+        length, time, mass = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib.sl3_earth_scales(self.handle, ctypes.byref(length),
+                                   ctypes.byref(time), ctypes.byref(mass)))
+        return length.value, time.value, mass.value
+
+    def mesh(self):
+        e, n = ctypes.c_int(), ctypes.c_int()
+        check(lib.sl3_earth_mesh(self.handle, ctypes.byref(e), ctypes.byref(n)))
+        return e.value, n.value
+
+    def export(self, name):
+        count = ctypes.c_int64()
+        check(lib.sl3_earth_export(self.handle, name.encode(), None, ctypes.byref(count)))
+        out = np.zeros(count.value)
+        check(lib.sl3_earth_export(self.handle, name.encode(),
+                                   out.ctypes.data_as(c_double_p), ctypes.byref(count)))
+        return out
+
+    def degree_system(self, l, kind, scheme=EXPONENTIAL_TRAPEZOID, dt=-1.0, eps=1e-8):
+        """The assembled degree-l system: ndof, kd, first element, pinned
+        dof, unscaled upper band ((ndof, kd+1): row j is column j of the
+        band), dof map and unit load."""
+        ndof, kd, first, pinned = (ctypes.c_int() for _ in range(4))
+        check(lib.sl3_earth_degree_system(self.handle, l, kind, scheme, ctypes.c_double(dt),
+                                          ctypes.c_double(eps), ctypes.byref(ndof),
+                                          ctypes.byref(kd), ctypes.byref(first),
+                                          ctypes.byref(pinned), None, None, None))
+        elements, ngll = self.mesh()
+        nodes = elements * (ngll - 1) + 1
+        band = np.zeros((ndof.value, kd.value + 1))
+        dofs = np.zeros(nodes * (3 if kind == 0 else 1), dtype=np.int32)
+        load = np.zeros(ndof.value)
+        check(lib.sl3_earth_degree_system(
+            self.handle, l, kind, scheme, ctypes.c_double(dt), ctypes.c_double(eps),
+            ctypes.byref(ndof), ctypes.byref(kd), ctypes.byref(first), ctypes.byref(pinned),
+            band.ctypes.data_as(c_double_p), dofs.ctypes.data_as(c_int_p),
+            load.ctypes.data_as(c_double_p) if kind == 0 else None))
+        return dict(ndof=ndof.value, kd=kd.value, first_element=first.value,
+                    pinned=pinned.value, band=band,
+                    dofs=dofs.reshape(nodes, 3) if kind == 0 else dofs,
+                    load=load if kind == 0 else None)
+
+
+def zonal_load_rate(lmax, earth):
+    """The reference step benchmark's load (benchmarks/step_timing.cpp:
+    185-199): zonal columns sigma_l(t) = rate_l t with rate_l =
+    exp(-2 pi (l+1)/10.5) sqrt((2l+1)/4 pi) 5515 kg m^-2 / 30 yr for l >= 1,
+    in the non-dimensional units. Returns a LoadCallable3D for the C ABI."""
+    length, time, mass = earth.scales()
+    sigma = mass / length ** 2
+    rate = np.zeros(lmax + 1)
+    for l in range(1, lmax + 1):
+        rate[l] = (math.exp(-2.0 * math.pi * (l + 1) / 10.5)
+                   * math.sqrt((2.0 * l + 1) / (4.0 * math.pi)) * 5515.0 / (30.0 * YEAR))
+
+    def load(l, t, columns, user):
+        if l < 1 or l > lmax:
+            return 0
+        for k in range(2 * (l + 1)):
+            columns[k] = 0.0
+        columns[0] = rate[l] * (t * time) / sigma
+        return 1
+
+    return LOAD_FN(load), rate
+
+
+def delta_coefficients(lmax_field, nodes, seed=2026, rms=0.7, ar=0.9, lmax=None):
+    """The lateral log-viscosity field of the 3D step configurations: at each
+    memory node an isotropic Gaussian random field on the sphere with angular
+    power (1 + l/8)^-3 for l <= lmax_field (zero above) and rms `rms`,
+    successive nodes (in radius order) AR(1) with coefficient `ar`:
+    c_k = ar c_{k-1} + sqrt(1 - ar^2) xi_k. Spin-0 coefficients in the real
+    m >= 0 layout l(l+1)/2 + m up to `lmax`, complex, per node end to end."""
+    lmax = lmax_field if lmax is None else lmax
+    rng = np.random.default_rng(seed)
+    size = (lmax + 1) * (lmax + 2) // 2
+    power = np.array([(1.0 + l / 8.0) ** -3 if l <= lmax_field else 0.0
+                      for l in range(lmax + 1)])
+    variance = sum((2 * l + 1) * power[l] for l in range(lmax + 1)) / (4 * math.pi)
+    power *= rms ** 2 / variance
+    out = np.zeros((nodes, size), dtype=np.complex128)
+    prev = None
+    for k in range(nodes):
+        xi = np.zeros(size, dtype=np.complex128)
+        for l in range(lmax_field + 1):
+            base = l * (l + 1) // 2
+            sd = math.sqrt(power[l])
+            xi[base] = rng.normal(0.0, sd)
+            m = l
+            if m > 0:
+                re = rng.normal(0.0, sd / math.sqrt(2.0), size=l)
+                im = rng.normal(0.0, sd / math.sqrt(2.0), size=l)
+                xi[base + 1: base + l + 1] = re + 1j * im
+        cur = xi if prev is None else ar * prev + math.sqrt(1.0 - ar * ar) * xi
+        out[k] = cur
+        prev = cur
+    return out
+
+
+class State:
+    """Viscoelastic3D::State on the device (one Maxwell branch)."""
+
+    def __init__(self, op):
+        self.op = op
+        self.handle = ctypes.c_void_p()
+        check(lib.sl3_state_zero(op.handle, ctypes.byref(self.handle)))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.sl3_state_destroy(self.handle)
+            self.handle = None
+
+    @property
+    def t(self):
+        t = ctypes.c_double()
+        check(lib.sl3_state_time(self.handle, ctypes.byref(t)))
+        return t.value
+
+    def export(self, name):
+        count = ctypes.c_int64()
+        check(lib.sl3_state_export(self.op.handle, self.handle, name.encode(), None,
+                                   ctypes.byref(count)))
+        out = np.zeros(count.value)
+        check(lib.sl3_state_export(self.op.handle, self.handle, name.encode(),
+                                   out.ctypes.data_as(c_double_p), ctypes.byref(count)))
+        return out
+
+
+class Viscoelastic3D:
+    """The 3D lateral-viscosity step on the GPU (Viscoelastic3D of
+    include/sl3dpp/Viscoelastic3D.hpp). delta: a torch tensor on the device
+    holding delta on every memory node's grid, or None."""
+
+    def __init__(self, earth, lmax, delta=None, options=None):
+        import torch
+
+        self.earth = earth
+        self.lmax = lmax
+        self.options = options if options is not None else StepOptions.default()
+        self.handle = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        ptr = ctypes.c_void_p(delta.data_ptr()) if delta is not None else None
+        check(lib.sl3_operator_create(earth.handle, lmax, ptr, ctypes.byref(self.options),
+                                      ctypes.byref(self.handle)))
+        nodes, nt, nph = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        sd = ctypes.c_int64()
+        check(lib.sl3_operator_shape(self.handle, ctypes.byref(nodes), ctypes.byref(nt),
+                                     ctypes.byref(nph), ctypes.byref(sd)))
+        self.nodes, self.ntheta, self.nphi = nodes.value, nt.value, nph.value
+        self.spectral_doubles = sd.value
+        self.radii = np.zeros(self.nodes)
+        self.theta = np.zeros(self.ntheta)
+        self.wtheta = np.zeros(self.ntheta)
+        check(lib.sl3_operator_geometry(self.handle, self.radii.ctypes.data_as(c_double_p),
+                                        self.theta.ctypes.data_as(c_double_p),
+                                        self.wtheta.ctypes.data_as(c_double_p)))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.sl3_operator_destroy(self.handle)
+            self.handle = None
+
+    def set_options(self, **overrides):
+        for k, v in overrides.items():
+            setattr(self.options, k, v)
+        check(lib.sl3_operator_set_options(self.handle, ctypes.byref(self.options)))
+
+    def set_delta(self, delta):
+        import torch
+
+        torch.cuda.synchronize()
+        ptr = ctypes.c_void_p(delta.data_ptr()) if delta is not None else None
+        check(lib.sl3_operator_set_delta(self.handle, ptr))
+
+    def zero_state(self):
+        return State(self)
+
+    def step(self, state, dt, scheme=EXPONENTIAL_TRAPEZOID, load=None, start=None):
+        """Step (src/Viscoelastic3D.cpp:1982-2030) in place on the state."""
+        info = StepInfo()
+        ptr = ctypes.c_void_p(start.data_ptr()) if start is not None else None
+        check(lib.sl3_step(self.handle, state.handle, ctypes.c_double(dt), scheme,
+                           load, None, ptr, ctypes.byref(info)))
+        return info
+
+    def export(self, name):
+        count = ctypes.c_int64()
+        check(lib.sl3_operator_export(self.handle, name.encode(), None, ctypes.byref(count)))
+        out = np.zeros(count.value)
+        check(lib.sl3_operator_export(self.handle, name.encode(),
+                                      out.ctypes.data_as(c_double_p), ctypes.byref(count)))
+        return out
+
+    def synthesise(self, spin, coefficients, field):
+        """SurfaceGrid::Synthesise(Spin)Batch over the memory nodes, device
+        tensors in the reference layouts."""
+        import torch
+
+        torch.cuda.synchronize()
+        check(lib.sl3_gsh_synthesise(self.handle, spin, ctypes.c_void_p(coefficients.data_ptr()),
+                                     ctypes.c_void_p(field.data_ptr())))
+
+    def analyse(self, spin, field, coefficients):
+        import torch
+
+        torch.cuda.synchronize()
+        check(lib.sl3_gsh_analyse(self.handle, spin, ctypes.c_void_p(field.data_ptr()),
+                                  ctypes.c_void_p(coefficients.data_ptr())))
+
+    def delta_field(self, coefficients):
+        """delta on every node's grid from per-node spin-0 coefficients
+        (delta_coefficients), synthesised on the device."""
+        import torch
+
+        coef = torch.from_numpy(np.ascontiguousarray(coefficients).view(np.float64).ravel()).cuda()
+        field = torch.zeros(self.nodes * self.ntheta * self.nphi, dtype=torch.float64,
+                            device="cuda")
+        self.synthesise(0, coef, field)
+        return field

@@ -1,0 +1,141 @@
+"""The HIP 3D step against the oracle on identical inputs (the host set-up,
+the lateral field, the load):
+ * the batched GSH transforms (SurfaceGrid's Synthesise/ExpandSpinBatch) for
+   spins -2, -1, 0 against the oracle's direct sums, node by node (1e-12);
+ * one pass of the step (Converge capped at one pass: StartStep's sweep and
+   analyses, the force, the band solves, the channel, strain recovery,
+   Finish) at lmax 8 and at lmax 32, where the degree truncation cuts the
+   deeper memory nodes (1e-11 relative);
+ * whole converged steps with lateral viscosity, adaptive Anderson mixing
+   included, to the fixed-point tolerance (1e-8 relative at rtol 1e-11);
+ * the zero-lateral limit closing in two passes."""
+
+import ctypes
+
+import numpy as np
+import pytest
+from step_problem import OracleStep, amplitudes, small_earth_spec
+
+import sl3dpp_amd.step as st
+
+pytestmark = pytest.mark.gpu
+
+DP = ctypes.POINTER(ctypes.c_double)
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def earth():
+    import torch
+
+    assert torch.cuda.is_available(), "no HIP device"
+    return st.Earth(small_earth_spec())
+
+
+def make_operator(earth, lmax, delta_scale, options=None, seed=11):
+    op = st.Viscoelastic3D(earth, lmax, None, options)
+    if delta_scale != 0.0:
+        coef = st.delta_coefficients(min(lmax, 6), op.nodes, seed=seed, rms=delta_scale,
+                                     lmax=lmax)
+        field = op.delta_field(coef)
+        op.set_delta(field)
+    return op
+
+
+@pytest.mark.parametrize("lmax", [8, 32])
+@pytest.mark.parametrize("spin", [-2, -1, 0])
+def test_gsh_transforms_match_oracle(earth, oracle, lmax, spin):
+    import torch
+
+    print(f"[step] gsh lmax {lmax} spin {spin}", flush=True)
+    op = make_operator(earth, lmax, 0.0)
+    real = spin == 0
+    size = (lmax + 1) * (lmax + 2) // 2 if real else (lmax + 1) ** 2 - spin * spin
+    rng = np.random.default_rng(lmax + 3 * abs(spin))
+    coef = rng.normal(size=(op.nodes, size, 2))
+    if real:
+        for l in range(lmax + 1):
+            coef[:, l * (l + 1) // 2, 1] = 0.0
+    fs = op.ntheta * op.nphi
+    field = torch.zeros(op.nodes * fs * (1 if real else 2), dtype=torch.float64, device="cuda")
+    cdev = torch.from_numpy(coef.ravel().copy()).cuda()
+    op.synthesise(spin, cdev, field)
+    back = torch.zeros_like(cdev)
+    op.analyse(spin, field, back)
+    f = field.cpu().numpy().reshape(op.nodes, -1)
+    b = back.cpu().numpy().reshape(op.nodes, size, 2)
+    theta = np.ascontiguousarray(op.theta)
+    wtheta = np.ascontiguousarray(op.wtheta)
+    for node in [0, op.nodes // 2, op.nodes - 1]:
+        ref = np.zeros(f.shape[1])
+        c = np.ascontiguousarray(coef[node])
+        oracle.sl3o_synthesise(lmax, spin, int(real), op.ntheta, op.nphi,
+                               theta.ctypes.data_as(DP), c.ctypes.data_as(DP),
+                               ref.ctypes.data_as(DP))
+        assert rel(f[node], ref) < 1e-12
+        refc = np.zeros_like(c)
+        fn = np.ascontiguousarray(f[node])
+        oracle.sl3o_analyse(lmax, spin, int(real), op.ntheta, op.nphi,
+                            theta.ctypes.data_as(DP), wtheta.ctypes.data_as(DP),
+                            fn.ctypes.data_as(DP), refc.ctypes.data_as(DP))
+        assert rel(b[node], refc) < 1e-12
+        assert rel(b[node], c) < 1e-12  # round trip
+
+
+def run_pair(earth, oracle, lmax, delta_scale, steps, max_iterations, rtol):
+    L, T, M = earth.scales()
+    dt = 100 * st.YEAR / T
+    opts = st.StepOptions.default(max_iterations=max_iterations, rtol=rtol,
+                                  anderson_memory=6, anderson_rate=0.5)
+    op = make_operator(earth, lmax, delta_scale, opts)
+    itau = op.export("itau")
+    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, itau,
+                     theta=op.theta, wtheta=op.wtheta)
+    load, _ = st.zonal_load_rate(lmax, earth)
+    state = op.zero_state()
+    out = []
+    for k in range(steps):
+        print(f"[step] lmax {lmax} delta {delta_scale} step {k}", flush=True)
+        info = op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+        amp = amplitudes(load, lmax, (k + 1) * dt)
+        rinfo = ref.step(dt, st.EXPONENTIAL_TRAPEZOID, amp, max_iterations=max_iterations,
+                         rtol=rtol, anderson_memory=6, anderson_rate=0.5)
+        out.append((info, rinfo, state.export("coefficients"), ref.coef.copy(),
+                    [state.export(f"m{b}") for b in range(3)], [m.copy() for m in ref.m],
+                    [state.export(f"strain{b}") for b in range(3)],
+                    [s.copy() for s in ref.strain]))
+    return out
+
+
+@pytest.mark.parametrize("lmax", [8, 32])
+def test_one_pass_matches_oracle(earth, oracle, lmax):
+    results = run_pair(earth, oracle, lmax, 0.7, steps=2, max_iterations=1, rtol=1e-14)
+    for info, rinfo, c, rc, m, rm, d, rd in results:
+        assert info.iterations == rinfo.iterations == 1
+        assert abs(info.residual - rinfo.residual) <= 1e-10 * abs(rinfo.residual)
+        assert rel(c, rc) < 1e-11
+        for b in range(3):
+            assert rel(m[b], rm[b]) < 1e-11
+            assert rel(d[b], rd[b]) < 1e-11
+
+
+def test_converged_lateral_steps_match_oracle(earth, oracle):
+    results = run_pair(earth, oracle, 16, 1.5, steps=2, max_iterations=80, rtol=1e-11)
+    for info, rinfo, c, rc, m, rm, d, rd in results:
+        assert info.converged == 1 and rinfo.converged == 1
+        assert abs(info.iterations - rinfo.iterations) <= 1
+        assert info.mixed_passes > 0  # Anderson engaged
+        assert rel(c, rc) < 1e-8
+        for b in range(3):
+            assert rel(m[b], rm[b]) < 1e-8
+
+
+def test_zero_lateral_step_closes_in_two_passes(earth, oracle):
+    results = run_pair(earth, oracle, 8, 0.0, steps=2, max_iterations=40, rtol=1e-12)
+    for info, rinfo, c, rc, *_ in results:
+        assert info.converged == 1
+        assert info.iterations <= 3
+        assert rel(c, rc) < 1e-11
