@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""One 3D lateral-viscosity step at lmax 128 to residual 1e-8 on one MI355X.
+
+Workload (the 3D step configuration): the three-region Earth (fluid core to
+3480 km, Maxwell mantle 5e20 / 1e22 Pa s split at 5701 km, elastic
+lithosphere), 128 elements (70 / 56 / 2) x 6 GLL nodes, Gauss memory
+points (280 memory nodes), lmax 128 on the 194 x 388 Gauss-Legendre grid,
+eta = eta_ref exp(delta) with delta a seeded isotropic Gaussian random field
+of angular power (1 + l/8)^-3 to l = 64, rms 0.7, AR(1) 0.9 across the
+memory nodes; the exponential trapezoid, dt = 100 yr, the reference step
+benchmark's zonal load rate; the fixed point to rtol 1e-8 with adaptive
+Anderson mixing of depth 6. A "step" is one Viscoelastic3D::Step (StartStep,
+the fixed point's passes, Finish) from the state the previous step left.
+
+Prints one JSON line (the driver's contract). --gpus N > 1 exits: this build
+runs on one GPU.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "3D lateral-viscosity steps/s at lmax128, tol 1e-8: learned+batched vs plain"
+PEAK_HBM = 8.0e12       # B/s
+PEAK_FP64 = 78.6e12     # flop/s, MI355X dense FP64 (vector and matrix)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--lmax", type=int, default=128)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def work_models(op, lmax):
+    """Algorithmic work per launch of the step's kernels:
+    flops for the transforms (only the coefficients the degree truncation
+    keeps), HBM bytes for the vector sweeps."""
+    import numpy as np
+
+    tasks = op.export("tasks").reshape(-1, 4)
+    first_node = {int(t[0]): int(t[3]) for t in tasks if int(t[1]) == 0}
+    nodes, nt, nph = op.nodes, op.ntheta, op.nphi
+    active = 0  # (block, m, l, node) coefficients kept
+    for b, spin in enumerate([-2, -1, 0]):
+        mlo = 0 if spin == 0 else -lmax
+        for m in range(mlo, lmax + 1):
+            for l in range(max(abs(m), abs(spin)), lmax + 1):
+                active += nodes - first_node[l]
+    legendre_flops = 4.0 * nt * active  # complex coefficient x real table, 2 flops each
+    R = 4 if nph % 4 == 0 else 2
+    P = nph // R
+    H = (P - 1) // 2
+    orders = [2 * lmax + 1, 2 * lmax + 1, lmax + 1]
+    ring_flops = 0.0
+    for b in range(3):
+        per_dir = R * (12.0 * H * H) + R * orders[b] * 8.0
+        ring_flops += nodes * nt * (2 * per_dir + 2.0 * nph)
+    ring_flops /= 3.0  # per launch (one block per launch)
+    S = op.spectral_doubles * 8.0
+    return {
+        "legendre_synthesis": ("mfma", legendre_flops),
+        "legendre_analysis": ("mfma", legendre_flops),
+        "ring": ("mfma", ring_flops),
+        # the mix sweep: g and history (m = 6) read, x written
+        "anderson": ("hbm", None),
+        "residual": ("hbm", 2.0 * S),
+    }
+
+
+def cpu_baseline(op, earth, dt, load, passes_per_step, rtol):
+    """The oracle (the CPU restatement of the step, FP64, OpenMP) on the same
+    workload: one step capped at one pass and one capped at two, giving the
+    per-pass time and the once-per-step StartStep + Finish time; the CPU
+    steps/s is 1 / (start/finish + passes x pass) with the GPU run's mean
+    passes per step (the same algorithm; the iteration counts agree to one
+    pass in the parity tests)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from step_problem import OracleStep, amplitudes, oracle_library
+
+    import sl3dpp_amd.step as st
+
+    oracle = oracle_library()
+    itau = op.export("itau")
+    lmax = op.lmax
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    times = []
+    for cap in (1, 2):
+        p = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, itau,
+                       theta=op.theta, wtheta=op.wtheta)
+        amp = amplitudes(load, lmax, dt)
+        log(f"cpu baseline: oracle step capped at {cap} pass(es) ...")
+        t0 = time.perf_counter()
+        p.step(dt, st.EXPONENTIAL_TRAPEZOID, amp, max_iterations=cap, rtol=rtol,
+               anderson_memory=6, anderson_rate=0.5)
+        times.append(time.perf_counter() - t0)
+        log(f"cpu baseline: {times[-1]:.2f} s")
+        del p
+    per_pass = max(times[1] - times[0], 1e-9)
+    once = max(times[0] - per_pass, 0.0)
+    step_time = once + passes_per_step * per_pass
+    return {
+        "value": 1.0 / step_time,
+        "unit": "steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"the oracle (FP64 C/OpenMP restatement) at the full workload: one step "
+                   f"capped at 1 and at 2 passes ({times[0]:.1f} s, {times[1]:.1f} s) -> "
+                   f"{per_pass:.2f} s/pass, {once:.2f} s StartStep+Finish; scaled to "
+                   f"{passes_per_step:.1f} passes/step"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the step shards by radius across GPUs only by design, see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    import sl3dpp_amd
+    import sl3dpp_amd.step as st
+
+    assert torch.cuda.is_available(), "bench.py needs the HIP device"
+    lmax = args.lmax
+    earth = st.Earth(st.three_region_earth(70, 56, 2, 6))
+    length, tscale, mass = earth.scales()
+    dt = 100.0 * st.YEAR / tscale
+    opts = st.StepOptions.default(max_iterations=200, rtol=args.rtol, anderson_memory=6,
+                                  anderson_rate=0.5)
+    log("building the operator ...")
+    t0 = time.perf_counter()
+    op = st.Viscoelastic3D(earth, lmax, None, opts)
+    coef = st.delta_coefficients(64, op.nodes, seed=2026, rms=0.7, ar=0.9, lmax=lmax)
+    op.set_delta(op.delta_field(coef))
+    load, _ = st.zonal_load_rate(lmax, earth)
+    log(f"operator: {op.nodes} memory nodes, grid {op.ntheta} x {op.nphi}, "
+        f"{time.perf_counter() - t0:.1f} s")
+    state = op.zero_state()
+    for k in range(args.warmup):
+        info = op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+        log(f"warmup step {k}: {info.iterations} passes ({info.mixed_passes} mixed), "
+            f"residual {info.residual:.2e}, {info.start_ms + info.passes_ms + info.finish_ms:.1f} ms")
+    torch.cuda.synchronize()
+    sl3dpp_amd.synchronize()
+    sl3dpp_amd.profile_reset()
+    sl3dpp_amd.profile_enable(True)
+    passes, mixed, converged = [], [], []
+    torch.cuda.synchronize()
+    sl3dpp_amd.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        info = op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+        passes.append(info.iterations)
+        mixed.append(info.mixed_passes)
+        converged.append(bool(info.converged))
+        log(f"step {k}: {info.iterations} passes ({info.mixed_passes} mixed), residual "
+            f"{info.residual:.2e}, start {info.start_ms:.2f} ms, passes {info.passes_ms:.2f} ms, "
+            f"finish {info.finish_ms:.2f} ms")
+    sl3dpp_amd.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sl3dpp_amd.profile_enable(False)
+    kernels = sl3dpp_amd.profile_read()
+    steps_per_s = args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    # dominant kernel and its roofline
+    models = work_models(op, lmax)
+    top = max(kernels.items(), key=lambda kv: kv[1][0])
+    name, (total_ms, launches) = top
+    roofline = None
+    bound, work = models.get(name, ("hbm", None))
+    avg_s = total_ms / max(launches, 1) / 1e3
+    if work is not None and avg_s > 0:
+        achieved = work / avg_s
+        if bound == "mfma":
+            roofline = {"bound": "mfma", "kernel": name, "achieved": achieved / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64, "traffic": None,
+                        "work_per_launch": work, "avg_launch_ms": avg_s * 1e3}
+        else:
+            roofline = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9,
+                        "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM, "traffic": None,
+                        "work_per_launch": work, "avg_launch_ms": avg_s * 1e3}
+    breakdown = {k: {"total_ms": round(v[0], 3), "launches": v[1]} for k, v in kernels.items()}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(op, earth, dt, load, float(np.mean(passes)), args.rtol)
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("one 3D lateral-viscosity step, lmax 128, grid 194x388, 128 elements "
+                         "(70/56/2) x 6 GLL nodes, 280 Gauss memory nodes, delta GRF "
+                         "(1+l/8)^-3 to l=64 rms 0.7 AR(1) 0.9, exponential trapezoid "
+                         "dt 100 yr, zonal load rate, Anderson depth 6, rtol 1e-8"),
+            "variant": "plain GPU fixed point (no learned warm start, one realisation)",
+            "lmax": lmax, "grid": [op.ntheta, op.nphi], "memory_nodes": op.nodes,
+            "passes_per_step": passes, "mixed_passes": mixed, "converged": converged,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": breakdown,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

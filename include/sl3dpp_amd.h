@@ -37,6 +37,16 @@ const char* sl3_version(void);
 /* Waits for every call issued so far on the library's stream. */
 int sl3_synchronize(void);
 
+/* Kernel timing with device events on the library's stream (off by
+ * default): enable, run, then read a kernel's total milliseconds and launch
+ * count ("legendre_synthesis", "legendre_analysis", "ring", "force",
+ * "band_solve", "strain", "anderson", "residual", ...); names lists them,
+ * newline-separated. */
+int sl3_profile_enable(int on);
+int sl3_profile_reset(void);
+int sl3_profile_read(const char* kernel, double* total_ms, int64_t* launches);
+int sl3_profile_names(char* buffer, int64_t capacity);
+
 /* ===================================================================== *
  * Banded Cholesky and LU, FP64, LAPACK band storage (column-major bands, *
  * 0-based pivots). Replace sl3dpp::detail::BandCholeskyFactor,          *

@@ -53,27 +53,174 @@ static double* coef_at(double* s, int lmax, int nodes, int b, int node, int l, i
 }
 
 /* ---- transforms of a whole tensor, node by node ------------------------- */
+/* A table-driven form of sht.c's direct sums (same formulas, the Wigner d
+ * values and the longitude exponentials computed once per step): per node
+ * and block, F_m(theta) = sum_l c_l d^l_{mN}(theta) f_lm and
+ * f(phi_j) = sum_m F_m e^{i m phi_j}; analysis the adjoint with the
+ * Gauss-Legendre and uniform weights. */
+typedef struct {
+  int lmax, ntheta, nphi;
+  double* d[3];       /* [m - mlo][theta][l - lmin] */
+  int64_t* moff[3];   /* per m - mlo */
+  double* cosT;       /* [j][m + lmax] */
+  double* sinT;
+} tables_t;
+
+static int mlo_of(int b, int lmax) { return b == 2 ? 0 : -lmax; }
+static int lmin_of(int b, int m) {
+  const int am = abs(m), an = abs(spin_of(b));
+  return am > an ? am : an;
+}
+
+static void tables_build(tables_t* t, const sl3o_problem* p) {
+  const int L = p->lmax, nt = p->ntheta, nph = p->nphi;
+  t->lmax = L;
+  t->ntheta = nt;
+  t->nphi = nph;
+  for (int b = 0; b < 3; ++b) {
+    const int mlo = mlo_of(b, L), orders = L - mlo + 1;
+    t->moff[b] = (int64_t*)malloc(sizeof(int64_t) * (orders + 1));
+    int64_t off = 0;
+    for (int m = mlo; m <= L; ++m) {
+      t->moff[b][m - mlo] = off;
+      const int lmin = lmin_of(b, m);
+      if (lmin <= L) off += (int64_t)nt * (L - lmin + 1);
+    }
+    t->moff[b][orders] = off;
+    t->d[b] = (double*)malloc(sizeof(double) * (off > 0 ? off : 1));
+#pragma omp parallel for schedule(dynamic)
+    for (int m = mlo; m <= L; ++m) {
+      const int lmin = lmin_of(b, m);
+      if (lmin > L) continue;
+      for (int i = 0; i < nt; ++i) {
+        double* col = t->d[b] + t->moff[b][m - mlo] + (int64_t)i * (L - lmin + 1);
+        sl3o_wigner_d_column(L, spin_of(b), m, p->theta[i], col);
+        for (int l = lmin; l <= L; ++l) col[l - lmin] *= sqrt((2.0 * l + 1.0) / (4.0 * M_PI));
+      }
+    }
+  }
+  t->cosT = (double*)malloc(sizeof(double) * nph * (2 * L + 1));
+  t->sinT = (double*)malloc(sizeof(double) * nph * (2 * L + 1));
+  for (int j = 0; j < nph; ++j) {
+    for (int m = -L; m <= L; ++m) {
+      const double a = 2.0 * M_PI * (double)((int64_t)j * m % nph) / nph;
+      t->cosT[(int64_t)j * (2 * L + 1) + m + L] = cos(a);
+      t->sinT[(int64_t)j * (2 * L + 1) + m + L] = sin(a);
+    }
+  }
+}
+
+static void tables_free(tables_t* t) {
+  for (int b = 0; b < 3; ++b) {
+    free(t->d[b]);
+    free(t->moff[b]);
+  }
+  free(t->cosT);
+  free(t->sinT);
+}
+
+static tables_t* g_tables = NULL;
+
+static void synthesise_block(const tables_t* t, int b, const double* c, double* f) {
+  const int L = t->lmax, nt = t->ntheta, nph = t->nphi, mlo = mlo_of(b, L);
+  const int real = b == 2;
+  double* F = (double*)malloc(sizeof(double) * 2 * (2 * L + 1));
+  for (int i = 0; i < nt; ++i) {
+    for (int m = mlo; m <= L; ++m) {
+      double re = 0.0, im = 0.0;
+      const int lmin = lmin_of(b, m);
+      if (lmin <= L) {
+        const double* col = t->d[b] + t->moff[b][m - mlo] + (int64_t)i * (L - lmin + 1);
+        for (int l = lmin; l <= L; ++l) {
+          const int64_t k = index_of(b, l, m);
+          re += col[l - lmin] * c[2 * k];
+          im += col[l - lmin] * c[2 * k + 1];
+        }
+      }
+      F[2 * (m + L)] = re;
+      F[2 * (m + L) + 1] = im;
+    }
+    for (int j = 0; j < nph; ++j) {
+      const double* cs = t->cosT + (int64_t)j * (2 * L + 1) + L;
+      const double* sn = t->sinT + (int64_t)j * (2 * L + 1) + L;
+      double re = 0.0, im = 0.0;
+      for (int m = mlo; m <= L; ++m) {
+        const double fr = F[2 * (m + L)], fi = F[2 * (m + L) + 1];
+        double tr = fr * cs[m] - fi * sn[m], ti = fr * sn[m] + fi * cs[m];
+        if (real && m > 0) {
+          tr *= 2.0;
+          ti = 0.0;
+        }
+        re += tr;
+        im += ti;
+      }
+      if (real) {
+        f[(int64_t)i * nph + j] = re;
+      } else {
+        f[2 * ((int64_t)i * nph + j)] = re;
+        f[2 * ((int64_t)i * nph + j) + 1] = im;
+      }
+    }
+  }
+  free(F);
+}
+
+static void analyse_block(const tables_t* t, int b, const double* wtheta, const double* f,
+                          double* c) {
+  const int L = t->lmax, nt = t->ntheta, nph = t->nphi, mlo = mlo_of(b, L);
+  const int real = b == 2;
+  memset(c, 0, sizeof(double) * 2 * block_size(b, L));
+  for (int i = 0; i < nt; ++i) {
+    for (int m = mlo; m <= L; ++m) {
+      const int lmin = lmin_of(b, m);
+      if (lmin > L) continue;
+      double re = 0.0, im = 0.0;
+      for (int j = 0; j < nph; ++j) {
+        const double cs = t->cosT[(int64_t)j * (2 * L + 1) + m + L];
+        const double sn = -t->sinT[(int64_t)j * (2 * L + 1) + m + L];
+        if (real) {
+          const double v = f[(int64_t)i * nph + j];
+          re += v * cs;
+          im += v * sn;
+        } else {
+          const double fr = f[2 * ((int64_t)i * nph + j)], fi = f[2 * ((int64_t)i * nph + j) + 1];
+          re += fr * cs - fi * sn;
+          im += fr * sn + fi * cs;
+        }
+      }
+      const double w = 2.0 * M_PI / nph * wtheta[i];
+      re *= w;
+      im *= w;
+      const double* col = t->d[b] + t->moff[b][m - mlo] + (int64_t)i * (L - lmin + 1);
+      for (int l = lmin; l <= L; ++l) {
+        const int64_t k = index_of(b, l, m);
+        c[2 * k] += col[l - lmin] * re;
+        c[2 * k + 1] += col[l - lmin] * im;
+      }
+    }
+  }
+}
 
 static void synthesise_tensor(const sl3o_problem* p, const double* s, double* const grid[3]) {
   const size_t fs = (size_t)p->ntheta * p->nphi;
-#pragma omp parallel for schedule(dynamic)
+#pragma omp parallel for schedule(dynamic) collapse(2)
   for (int node = 0; node < p->nodes; ++node) {
     for (int b = 0; b < 3; ++b) {
       const double* c = s + block_offset(b, p->lmax, p->nodes) + 2 * node * block_size(b, p->lmax);
       double* f = grid[b] + (b == 2 ? 1 : 2) * node * fs;
-      sl3o_synthesise(p->lmax, spin_of(b), b == 2, p->ntheta, p->nphi, p->theta, c, f);
+      synthesise_block(g_tables, b, c, f);
     }
   }
 }
 
 static void analyse_tensor(const sl3o_problem* p, double* const grid[3], double* s) {
   const size_t fs = (size_t)p->ntheta * p->nphi;
-#pragma omp parallel for schedule(dynamic)
+#pragma omp parallel for schedule(dynamic) collapse(2)
   for (int node = 0; node < p->nodes; ++node) {
     for (int b = 0; b < 3; ++b) {
       double* c = s + block_offset(b, p->lmax, p->nodes) + 2 * node * block_size(b, p->lmax);
       const double* f = grid[b] + (b == 2 ? 1 : 2) * node * fs;
-      sl3o_analyse(p->lmax, spin_of(b), b == 2, p->ntheta, p->nphi, p->theta, p->wtheta, f, c);
+      analyse_block(g_tables, b, p->wtheta, f, c);
     }
   }
 }
@@ -388,6 +535,9 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
     if (factorise(t, p->bands + p->band_off[ti]) != 0) status = 1;
   }
   if (status != 0) goto done;
+  tables_t tables;
+  tables_build(&tables, p);
+  g_tables = &tables;
   for (int l = 0; l <= L; ++l) {
     task_t* t = &tasks[l];
     t->unit = (double*)malloc(sizeof(double) * t->ndof);
@@ -543,6 +693,8 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
     free(fixedS);
     free(analysed);
   }
+  tables_free(&tables);
+  g_tables = NULL;
 done:
   for (int ti = 0; ti < ntasks; ++ti) {
     free(tasks[ti].scale);

@@ -60,3 +60,25 @@ def version():
 
 
 from . import band  # noqa: E402,F401  (the band solvers' Python mirror)
+
+
+def profile_enable(on=True):
+    """Device-event timing of the library's kernels (off by default)."""
+    check(lib.sl3_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(lib.sl3_profile_reset())
+
+
+def profile_read():
+    """{kernel: (total_ms, launches)} since the last reset."""
+    buf = ctypes.create_string_buffer(1 << 14)
+    check(lib.sl3_profile_names(buf, len(buf)))
+    out = {}
+    for name in buf.value.decode().split():
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(lib.sl3_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        out[name] = (ms.value, n.value)
+    return out

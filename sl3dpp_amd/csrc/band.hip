@@ -273,6 +273,7 @@ size_t LuFactorLds(int klMax, int kuMax) {
 void LaunchCholeskyFactor(const BandSystem* systems_dev, int count, int kdMax,
                           double* ab_dev, int* info_dev, hipStream_t stream) {
   if (count == 0) return;
+  ProfileScope prof275("band_factor", stream);
   hipLaunchKernelGGL(CholeskyFactorKernel, dim3(count), dim3(kWave),
                      CholeskyFactorLds(kdMax), stream, systems_dev, ab_dev,
                      info_dev);
@@ -283,6 +284,7 @@ void LaunchLuFactor(const BandSystem* systems_dev, int count, int klMax,
                     int kuMax, double* ab_dev, int* piv_dev, int* info_dev,
                     hipStream_t stream) {
   if (count == 0) return;
+  ProfileScope prof286("band_factor", stream);
   hipLaunchKernelGGL(LuFactorKernel, dim3(count), dim3(kWave),
                      LuFactorLds(klMax, kuMax), stream, systems_dev, ab_dev,
                      piv_dev, info_dev);
@@ -293,6 +295,7 @@ void LaunchCholeskySolve(const BandSystem* systems_dev, const int2* tasks_dev,
                          int tasks, int kdMax, const double* ab_dev,
                          double* b_dev, hipStream_t stream) {
   if (tasks == 0) return;
+  ProfileScope prof297("band_solve", stream);
   hipLaunchKernelGGL(CholeskySolveKernel, dim3(tasks), dim3(kWave),
                      sizeof(double) * (kdMax + 1) * kWave, stream, systems_dev,
                      tasks_dev, ab_dev, b_dev);
@@ -303,6 +306,7 @@ void LaunchLuSolve(const BandSystem* systems_dev, const int2* tasks_dev,
                    int tasks, int kvMax, const double* ab_dev,
                    const int* piv_dev, double* b_dev, hipStream_t stream) {
   if (tasks == 0) return;
+  ProfileScope prof308("band_solve", stream);
   hipLaunchKernelGGL(LuSolveKernel, dim3(tasks), dim3(kWave),
                      sizeof(double) * (kvMax + 1) * kWave, stream, systems_dev,
                      tasks_dev, ab_dev, piv_dev, b_dev);

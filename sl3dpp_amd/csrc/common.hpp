@@ -61,6 +61,22 @@ int Guard(F&& body) {
   }
 }
 
+// Kernel timing on the library's stream, enabled by sl3_profile_enable:
+// device events around each launch of a named kernel, read back by
+// sl3_profile_read. No cost when disabled.
+class ProfileScope {
+ public:
+  ProfileScope(const char* name, hipStream_t stream);
+  ~ProfileScope();
+  ProfileScope(const ProfileScope&) = delete;
+  ProfileScope& operator=(const ProfileScope&) = delete;
+
+ private:
+  const char* name_;
+  hipStream_t stream_;
+  int slot_ = -1;
+};
+
 inline int CeilDiv(long long a, long long b) {
   return static_cast<int>((a + b - 1) / b);
 }

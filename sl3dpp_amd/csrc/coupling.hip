@@ -473,6 +473,7 @@ int GridFor(size_t n) {
 
 void LaunchFill(double* dst, size_t count, double value, hipStream_t stream) {
   if (count == 0) return;
+  ProfileScope prof475("fill", stream);
   hipLaunchKernelGGL(FillKernel, dim3(GridFor(count)), dim3(kThreads), 0, stream,
                      dst, count, value);
   SL3_HIP(hipGetLastError());
@@ -480,6 +481,7 @@ void LaunchFill(double* dst, size_t count, double value, hipStream_t stream) {
 
 void LaunchCopy(double* dst, const double* src, size_t count, hipStream_t stream) {
   if (count == 0) return;
+  ProfileScope prof483("copy", stream);
   hipLaunchKernelGGL(CopyKernel, dim3(GridFor(count)), dim3(kThreads), 0, stream,
                      dst, src, count);
   SL3_HIP(hipGetLastError());
@@ -493,6 +495,7 @@ void LaunchForce(const DegreeTask* tasks_dev, int taskCount, int lmax,
   if (taskCount == 0 || g.elements == 0) return;
   if (g.np > kMaxPoints) throw ArgumentError("at most 6 memory points per element");
   const dim3 grid(CeilDiv(g.elements, kChunkElements), taskCount);
+  ProfileScope prof497("force", stream);
   hipLaunchKernelGGL(ForceKernel, grid, dim3(kThreads), 0, stream, tasks_dev, g,
                      weighted, scale, dofs, rhs);
   SL3_HIP(hipGetLastError());
@@ -504,6 +507,7 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
                   const double* amp, SpectralView out, hipStream_t stream) {
   const int pairs = (lmax + 1) * (lmax + 2) / 2;
   const dim3 grid(CeilDiv(g.nodes, kThreads), pairs);
+  ProfileScope prof509("strain", stream);
   hipLaunchKernelGGL(StrainKernel, grid, dim3(kThreads), 0, stream, sphTasks_dev,
                      torTasks_dev, lmax, g, scale, dofs, rhs, units, amp, out);
   SL3_HIP(hipGetLastError());
@@ -512,6 +516,7 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
 void LaunchRelativeChange(const double* next, const double* now, size_t n,
                           double* partials, int partialBlocks, double* result,
                           hipStream_t stream) {
+  ProfileScope prof518("residual", stream);
   hipLaunchKernelGGL(RelativeChangeKernel, dim3(partialBlocks), dim3(kThreads), 0,
                      stream, next, now, n / 2, partials);
   SL3_HIP(hipGetLastError());
@@ -522,6 +527,7 @@ void LaunchRelativeChange(const double* next, const double* now, size_t n,
 
 void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPrev,
                          size_t n, hipStream_t stream) {
+  ProfileScope prof529("anderson", stream);
   hipLaunchKernelGGL(AndersonFirstKernel, dim3(GridFor(n)), dim3(kThreads), 0,
                      stream, x, gx, xPrev, fPrev, n);
   SL3_HIP(hipGetLastError());
@@ -529,6 +535,7 @@ void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPr
 
 void LaunchAndersonSweep(const AndersonSweep& s, size_t n, double* partials,
                          int partialBlocks, double* sums, hipStream_t stream) {
+  ProfileScope prof537("anderson", stream);
   hipLaunchKernelGGL(AndersonSweepKernel, dim3(partialBlocks), dim3(kThreads), 0,
                      stream, s, n, partials);
   SL3_HIP(hipGetLastError());
@@ -541,6 +548,7 @@ void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, size_t n,
                        hipStream_t stream) {
   Gamma g{};
   for (int i = 0; i < s.m && i < 8; ++i) g.v[i] = gamma[i];
+  ProfileScope prof550("anderson", stream);
   hipLaunchKernelGGL(AndersonMixKernel, dim3(GridFor(n)), dim3(kThreads), 0, stream,
                      s, g, n);
   SL3_HIP(hipGetLastError());
@@ -551,6 +559,7 @@ void LaunchToDeviceLayout(const double* ref, double* dev, const int* rowOffset,
   const long long size = spin == 0 ? static_cast<long long>(lmax + 1) * (lmax + 2) / 2
                                    : static_cast<long long>(lmax + 1) * (lmax + 1) -
                                          spin * spin;
+  ProfileScope prof561("layout", stream);
   hipLaunchKernelGGL(LayoutKernel, dim3(GridFor(size * nodes)), dim3(kThreads), 0,
                      stream, ref, dev, rowOffset, spin, lmax, nodes, size, 1);
   SL3_HIP(hipGetLastError());
@@ -561,6 +570,7 @@ void LaunchToReferenceLayout(const double* dev, double* ref, const int* rowOffse
   const long long size = spin == 0 ? static_cast<long long>(lmax + 1) * (lmax + 2) / 2
                                    : static_cast<long long>(lmax + 1) * (lmax + 1) -
                                          spin * spin;
+  ProfileScope prof572("layout", stream);
   hipLaunchKernelGGL(LayoutKernel, dim3(GridFor(size * nodes)), dim3(kThreads), 0,
                      stream, dev, ref, rowOffset, spin, lmax, nodes, size, 0);
   SL3_HIP(hipGetLastError());

@@ -379,6 +379,7 @@ void Viscoelastic3D::SetDelta(const double* delta_dev) {
   hipStream_t s = Stream();
   const size_t G = coupling_.GridValueCount();
   itau_.Resize(G);
+  ProfileScope prof381("itau", s);
   hipLaunchKernelGGL(ItauKernel, dim3(Blocks(G)), dim3(256), 0, s, delta_dev,
                      itauRefNode_.data(), itau_.data(), coupling_.Grid().FieldSize(), G);
   SL3_HIP(hipGetLastError());
@@ -851,6 +852,7 @@ StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
       break;
     }
     if (options_.damping != 1.0) {
+      ProfileScope prof854("damp", s);
       hipLaunchKernelGGL(DampKernel, dim3(Blocks(S)), dim3(256), 0, s, sNext_.data(),
                          s_.data(), options_.damping, S);
       SL3_HIP(hipGetLastError());

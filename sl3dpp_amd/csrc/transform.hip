@@ -632,6 +632,7 @@ void LaunchLegendreSynthesis(const LegendrePlan& p, const double* const spectral
   a.nodes = p.nodes;
   a.ntheta = p.ntheta;
   const dim3 grid(CeilDiv(2 * p.nodes, kTile), CeilDiv(p.ntheta, kTile), p.orderCount);
+  ProfileScope prof634("legendre_synthesis", stream);
   hipLaunchKernelGGL(LegendreSynthesisKernel, grid, dim3(kThreads), 0, stream, a);
   SL3_HIP(hipGetLastError());
 }
@@ -655,6 +656,7 @@ void LaunchLegendreAnalysis(const LegendrePlan& p, const double* const fourier[3
   a.nodes = p.nodes;
   a.ntheta = p.ntheta;
   const dim3 grid(CeilDiv(2 * p.nodes, kTile), CeilDiv(p.lmax + 1, kTile), p.orderCount);
+  ProfileScope prof658("legendre_analysis", stream);
   hipLaunchKernelGGL(LegendreAnalysisKernel, grid, dim3(kThreads), 0, stream, a);
   SL3_HIP(hipGetLastError());
 }
@@ -688,6 +690,7 @@ void LaunchRing(const RingLaunch& r, hipStream_t stream) {
   a.dt = r.dt;
   a.scheme = r.scheme;
   const dim3 grid(CeilDiv(r.ntheta, kRings), r.nodes);
+  ProfileScope prof692("ring", stream);
   hipLaunchKernelGGL(RingKernel, grid, dim3(kThreads), RingLds(r.nphi, r.tables.P),
                      stream, a);
   SL3_HIP(hipGetLastError());
