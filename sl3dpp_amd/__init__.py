@@ -13,6 +13,15 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.path.join(_HERE, "libsl3dpp_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "sl3dpp_amd.h")
 
+# One HIP runtime per process: torch bundles its own libamdhip64 and HSA
+# runtime, and a process that loads this library first would then hold two
+# (the library's from /opt/rocm and torch's), which do not share devices.
+# Loading torch first lets the library bind to the runtime already loaded.
+try:
+    import torch  # noqa: F401
+except ImportError:  # no torch: the library's own runtime from /opt/rocm
+    torch = None
+
 if not os.path.exists(LIBRARY_PATH):
     raise ImportError(
         f"{LIBRARY_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)"
