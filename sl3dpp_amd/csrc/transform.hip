@@ -468,6 +468,7 @@ __global__ void __launch_bounds__(kThreads) RingKernel(RingArgs a) {
   const size_t ringBase = (static_cast<size_t>(node) * a.ntheta + th0) * nphi;
   const int count = rings * nphi;
   LoadTables(sh, a, P);
+  __syncthreads();  // every twiddle in LDS before any lane reads one
 
   // 1. the ring values
   if (a.op == kRingSynthesise || a.op == kRingPass || a.op == kRingFinish ||
