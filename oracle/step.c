@@ -6,6 +6,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "oracle.h"
 
@@ -513,6 +514,12 @@ static double relative_change(const double* next, const double* now, int64_t n) 
 
 /* ---- the step -------------------------------------------------------------- */
 
+static double now_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+}
+
 int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
               const double* amp, const sl3o_options* o, sl3o_info* info) {
   const int L = p->lmax;
@@ -521,6 +528,8 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
   const size_t G = fs * p->nodes;
   const int64_t S = sl3o_spectral_doubles(L, p->nodes);
   int status = 0;
+  double clock0 = now_ms();
+  info->setup_ms = info->start_ms = info->passes_ms = info->finish_ms = 0.0;
   if (dt == 0.0) scheme = 1;
   task_t* tasks = (task_t*)calloc(ntasks, sizeof(task_t));
   for (int ti = 0; ti < ntasks; ++ti) {
@@ -544,6 +553,8 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
     memcpy(t->unit, p->loads + p->load_off[l], sizeof(double) * t->ndof);
     solve(t, t->unit, 1);
   }
+  info->setup_ms = now_ms() - clock0;
+  clock0 = now_ms();
   {
     double* s = (double*)calloc(S, sizeof(double));
     double* sNext = (double*)calloc(S, sizeof(double));
@@ -621,6 +632,8 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
     info->converged = 0;
     info->mixed_passes = 0;
     g_residual_count = 0;
+    info->start_ms = now_ms() - clock0;
+    clock0 = now_ms();
     for (int it = 0; it < o->max_iterations; ++it) {
       if (it > 0) {
         synthesise_tensor(p, s, grid);
@@ -664,6 +677,8 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
       }
     }
     sl3o_mixer_destroy(mixer);
+    info->passes_ms = now_ms() - clock0;
+    clock0 = now_ms();
     /* Finish (:1900-1914) */
     synthesise_tensor(p, s, grid);
     for (size_t x = 0; x < G; ++x) {
@@ -680,6 +695,7 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
     st->has_strain = 1;
     st->has_coef = 1;
     st->t += dt;
+    info->finish_ms = now_ms() - clock0;
     for (int b = 0; b < 3; ++b) {
       free(grid[b]);
       if (d0[b] != st->strain[b]) free(d0[b]);

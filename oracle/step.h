@@ -57,6 +57,10 @@ typedef struct {
   double residual;
   int converged;
   int mixed_passes;
+  /* wall-clock milliseconds: the factorisations, tables and unit solves
+   * (once per dt and scheme in the reference, which caches them), StartStep,
+   * the fixed point's passes, Finish */
+  double setup_ms, start_ms, passes_ms, finish_ms;
 } sl3o_info;
 
 /* The state (caller-owned arrays): m and strain grid components (c0, c1

@@ -38,7 +38,8 @@ class Options(ctypes.Structure):
 
 class Info(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("residual", D), ("converged", ctypes.c_int),
-                ("mixed_passes", ctypes.c_int)]
+                ("mixed_passes", ctypes.c_int), ("setup_ms", D), ("start_ms", D),
+                ("passes_ms", D), ("finish_ms", D)]
 
 
 class OracleState(ctypes.Structure):
