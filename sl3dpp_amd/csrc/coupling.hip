@@ -2,14 +2,12 @@
 // for gfx950, FP64 (all HBM-bound):
 //  * the memory force (GridCoupling::SpheroidalForce / ToroidalForce with
 //    Gather and ForceRowsFromComponents, src/GridCoupling.cpp:613-809,
-//    include/sl3dpp/SpinComponents.hpp:124-145), written straight into the
-//    solves' scaled right-hand sides (DegreeOperator::BeginSolve,
-//    src/DegreeOperator.cpp:861-886);
+//    include/sl3dpp/SpinComponents.hpp:124-145), written as rows of the
+//    memory dofs, the right-hand sides of the memory response;
 //  * strain recovery (GridCoupling::AddStrainOfSolution and Scatter,
 //    src/GridCoupling.cpp:595-689; DeviatoricComponents,
-//    SpinComponents.hpp:53-72) from the unscaled solutions plus the load
-//    channel (FinishSolve, AddUnitChannels: src/DegreeOperator.cpp:888-901,
-//    src/Viscoelastic3D.cpp:1029-1073);
+//    SpinComponents.hpp:53-72) from the response on the memory dofs plus
+//    the load channel (AddUnitChannels, src/Viscoelastic3D.cpp:1029-1073);
 //  * RelativeChange (src/Viscoelastic3D.cpp:1636-1669) and the Anderson
 //    sweeps (include/sl3dpp/detail/AndersonMixer.hpp:123-218), reduced in a
 //    fixed block order so that results do not depend on scheduling.
@@ -102,7 +100,7 @@ constexpr int kChunkOrders = 16;
 // whose top node the chunk's bottom node shares) staged in LDS.
 __global__ void __launch_bounds__(kThreads)
     ForceKernel(const DegreeTask* tasks, RadialGeometry g, SpectralView w,
-                const double* scale, const int* dofs, double* rhs) {
+                const int* dofs, double* force) {
   constexpr int maxPoints = (kChunkElements + 1) * kMaxPoints;
   __shared__ Cplx rowsL[maxPoints * kChunkOrders][4];
   const DegreeTask t = tasks[blockIdx.y];
@@ -122,8 +120,8 @@ __global__ void __launch_bounds__(kThreads)
   const int gnEnd = e1 * nm + (e1 == elemEnd ? 1 : 0);
   const int owned = gnEnd - gnFirst;
   const int* dofTable = dofs + t.dofOff;
-  const double* sc = scale + t.scaleOff;
-  double* b = rhs + t.rhsOff;
+  double* b = force + t.fOff;
+  const size_t ld = t.colsPad;
   for (int mc = 0; mc <= l; mc += kChunkOrders) {
     const int mCount = min(kChunkOrders, l + 1 - mc);
     // stage 1: rows of (point, m)
@@ -189,14 +187,12 @@ __global__ void __launch_bounds__(kThreads)
       const int* d = dofTable + ((elem - g.elem0) * g.ngll + i) * 2;
       const int dU = d[0], dV = tor ? -1 : d[1];
       if (dU >= 0) {
-        const double s = dU == t.pinned ? 0.0 : sc[dU];
-        b[static_cast<size_t>(dU) * t.cols + m] = U.re * s;
-        b[static_cast<size_t>(dU) * t.cols + l + 1 + m] = U.im * s;
+        b[static_cast<size_t>(dU) * ld + m] = U.re;
+        b[static_cast<size_t>(dU) * ld + l + 1 + m] = U.im;
       }
       if (dV >= 0) {
-        const double s = sc[dV];
-        b[static_cast<size_t>(dV) * t.cols + m] = V.re * s;
-        b[static_cast<size_t>(dV) * t.cols + l + 1 + m] = V.im * s;
+        b[static_cast<size_t>(dV) * ld + m] = V.re;
+        b[static_cast<size_t>(dV) * ld + l + 1 + m] = V.im;
       }
     }
     __syncthreads();
@@ -206,9 +202,8 @@ __global__ void __launch_bounds__(kThreads)
 // One thread per (degree, order, memory node), node fastest.
 __global__ void __launch_bounds__(kThreads)
     StrainKernel(const DegreeTask* sph, const DegreeTask* tor, int lmax,
-                 RadialGeometry g, const double* scale, const int* dofs,
-                 const double* rhs, const double* units, const double* amp,
-                 SpectralView out) {
+                 RadialGeometry g, const int* dofs, const double* response,
+                 const double* units, const double* amp, SpectralView out) {
   const int pair = blockIdx.y;
   int l = static_cast<int>((sqrt(8.0 * pair + 1.0) - 1.0) * 0.5);
   while (l * (l + 1) / 2 > pair) --l;
@@ -231,24 +226,23 @@ __global__ void __launch_bounds__(kThreads)
     for (int i = 0; i < g.ngll; ++i) {
       const int* d = dofs + ts.dofOff + ((elem - g.elem0) * g.ngll + i) * 2;
       Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
-      const double* bs = rhs + ts.rhsOff;
-      const double* ss = scale + ts.scaleOff;
+      const double* bs = response + ts.fOff;
       if (d[0] >= 0) {
-        const size_t at = static_cast<size_t>(d[0]) * ts.cols;
-        Ui = ss[d[0]] * C(bs[at + m], bs[at + l + 1 + m]);
-        if (ts.unitOff >= 0) Ui = Ui + units[ts.unitOff + d[0]] * a;
+        const size_t at = static_cast<size_t>(d[0]) * ts.colsPad;
+        Ui = C(bs[at + m], bs[at + l + 1 + m]);
+        if (ts.unitOff >= 0) Ui = Ui + units[ts.dmOff + d[0]] * a;
       }
       if (d[1] >= 0) {
-        const size_t at = static_cast<size_t>(d[1]) * ts.cols;
-        Vi = ss[d[1]] * C(bs[at + m], bs[at + l + 1 + m]);
-        if (ts.unitOff >= 0) Vi = Vi + units[ts.unitOff + d[1]] * a;
+        const size_t at = static_cast<size_t>(d[1]) * ts.colsPad;
+        Vi = C(bs[at + m], bs[at + l + 1 + m]);
+        if (ts.unitOff >= 0) Vi = Vi + units[ts.dmOff + d[1]] * a;
       }
       if (tor1) {
         const int dw = dofs[tt.dofOff + ((elem - g.elem0) * g.ngll + i) * 2];
         if (dw >= 0) {
-          const size_t at = static_cast<size_t>(dw) * tt.cols;
-          const double* bt = rhs + tt.rhsOff;
-          Wi = scale[tt.scaleOff + dw] * C(bt[at + m], bt[at + l + 1 + m]);
+          const size_t at = static_cast<size_t>(dw) * tt.colsPad;
+          const double* bt = response + tt.fOff;
+          Wi = C(bt[at + m], bt[at + l + 1 + m]);
         }
       }
       const double v = g.value[q * g.ngll + i];
@@ -489,27 +483,26 @@ void LaunchCopy(double* dst, const double* src, size_t count, hipStream_t stream
 
 void LaunchForce(const DegreeTask* tasks_dev, int taskCount, int lmax,
                  const RadialGeometry& g, const SpectralView& weighted,
-                 const double* scale, const int* dofs, double* rhs,
-                 hipStream_t stream) {
+                 const int* dofs, double* force, hipStream_t stream) {
   (void)lmax;
   if (taskCount == 0 || g.elements == 0) return;
   if (g.np > kMaxPoints) throw ArgumentError("at most 6 memory points per element");
   const dim3 grid(CeilDiv(g.elements, kChunkElements), taskCount);
   ProfileScope prof497("force", stream);
   hipLaunchKernelGGL(ForceKernel, grid, dim3(kThreads), 0, stream, tasks_dev, g,
-                     weighted, scale, dofs, rhs);
+                     weighted, dofs, force);
   SL3_HIP(hipGetLastError());
 }
 
 void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev,
-                  int lmax, const RadialGeometry& g, const double* scale,
-                  const int* dofs, const double* rhs, const double* units,
-                  const double* amp, SpectralView out, hipStream_t stream) {
+                  int lmax, const RadialGeometry& g, const int* dofs,
+                  const double* response, const double* units, const double* amp,
+                  SpectralView out, hipStream_t stream) {
   const int pairs = (lmax + 1) * (lmax + 2) / 2;
   const dim3 grid(CeilDiv(g.nodes, kThreads), pairs);
   ProfileScope prof509("strain", stream);
   hipLaunchKernelGGL(StrainKernel, grid, dim3(kThreads), 0, stream, sphTasks_dev,
-                     torTasks_dev, lmax, g, scale, dofs, rhs, units, amp, out);
+                     torTasks_dev, lmax, g, dofs, response, units, amp, out);
   SL3_HIP(hipGetLastError());
 }
 

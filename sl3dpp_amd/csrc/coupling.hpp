@@ -23,6 +23,14 @@ struct DegreeTask {
   long long scaleOff = 0;  // ndof doubles
   long long dofOff = 0;    // memory elements x ngll x (U, V) or (W)
   long long unitOff = -1;  // the unit load response (ndof), spheroidal
+  // the memory dofs: the force's rows and the strain's reads
+  int nm = 0;              // count (the pinned dof excluded)
+  int nmPad = 0;           // nm rounded up to 16
+  int colsPad = 0;         // cols rounded up to 16
+  int pad2 = 0;
+  long long dmOff = 0;     // their dof numbers (nm ints), compact unit responses
+  long long gOff = 0;      // the memory response, nmPad^2 doubles
+  long long fOff = 0;      // force and response rows: nmPad x colsPad doubles
 };
 
 // The radial geometry the force and strain maps read (device pointers).
@@ -48,21 +56,22 @@ struct SpectralView {
 void LaunchFill(double* dst, size_t count, double value, hipStream_t stream);
 void LaunchCopy(double* dst, const double* src, size_t count, hipStream_t stream);
 
-// Forces of the analysed tensor onto every task's right-hand sides, scaled
-// for the solve (BeginSolve), pinned dofs zeroed; rows outside the memory
-// elements are left as they are (the caller fills zeros first).
+// Forces of the analysed tensor on every task's memory dofs (rows of
+// nmPad x colsPad at fOff, unscaled: the memory response carries the
+// scaling). dofs: per task, memory element x GLL node x (U, V) or (W, -),
+// the memory-dof row or -1 (none, or the pinned dof).
 void LaunchForce(const DegreeTask* tasks_dev, int taskCount, int lmax,
                  const RadialGeometry& g, const SpectralView& weighted,
-                 const double* scale, const int* dofs, double* rhs,
-                 hipStream_t stream);
+                 const int* dofs, double* force, hipStream_t stream);
 
-// Strain of the solutions into the spectral iterate (every value written):
-// solutions unscaled (FinishSolve) and the load channel's unit responses
-// added with amplitudes amp (complex per (l, m), offset l(l+1)/2 + m).
+// Strain of the responses on the memory dofs into the spectral iterate
+// (every value written), the load channel's unit responses (per memory dof
+// at dmOff) added with amplitudes amp (complex per (l, m), offset
+// l(l+1)/2 + m).
 void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev,
-                  int lmax, const RadialGeometry& g, const double* scale,
-                  const int* dofs, const double* rhs, const double* units,
-                  const double* amp, SpectralView out, hipStream_t stream);
+                  int lmax, const RadialGeometry& g, const int* dofs,
+                  const double* response, const double* units, const double* amp,
+                  SpectralView out, hipStream_t stream);
 
 // max |n|^2, max |n - o|^2 and a non-finite flag over n values, into
 // result[0..2] (device).

@@ -1,0 +1,56 @@
+// The memory response of the degree systems: per degree task, the dense
+// matrix G = D A_s^{-1} D restricted to the dofs of the memory elements
+// (A_s the diagonally scaled band matrix, D its scaling). The fixed point's
+// force lives only on those dofs and strain recovery reads only those, so
+// one pass's band solves (DegreeOperator::Solve with BeginSolve/FinishSolve,
+// src/DegreeOperator.cpp:861-901) become x = G f on the memory dofs: a
+// grouped FP64 GEMM on the matrix cores instead of a latency-bound
+// substitution per right-hand side. G is built once per (scheme, dt) by the
+// band solves themselves, one unit column per memory dof.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "coupling.hpp"
+
+namespace sl3amd {
+
+// One wave's share of the response GEMM: rows 16 rb .. 16 rb + 31 and
+// columns 16 cb .. 16 cb + 63 of one task's output.
+struct ResponseWork {
+  int task = 0;
+  int rb = 0;
+  int cb = 0;
+  int pad = 0;
+};
+
+// Every wave's work over the tasks, the largest contraction first.
+std::vector<ResponseWork> ResponseWorkList(const std::vector<DegreeTask>& tasks);
+
+// z (per task at zOff[task], ndof x nm row-major, zero on entry): column j
+// gets scale[dm[j]] in row dm[j] (BeginSolve of the unit vector).
+void LaunchUnitColumns(const DegreeTask* tasks_dev, int taskCount, int nmMax,
+                       const int* dm, const double* scale, const long long* zOff,
+                       double* z, hipStream_t stream);
+
+// G[i][j] = scale[dm[i]] z[dm[i]][j] (FinishSolve), zero-padded to nmPad and
+// stored in the matrix-core operand order the response GEMM streams.
+void LaunchPackResponse(const DegreeTask* tasks_dev, int taskCount, int nmPadMax,
+                        const int* dm, const double* scale, const long long* zOff,
+                        const double* z, double* g, hipStream_t stream);
+
+// x = G f for every task (f, x: nmPad x colsPad row-major at fOff).
+void LaunchResponse(const DegreeTask* tasks_dev, const ResponseWork* work_dev,
+                    int workCount, const double* g, const double* f, double* x,
+                    hipStream_t stream);
+
+// The full scaled right-hand sides from the memory-dof force (for a band
+// solve of the whole system): rhs[dm[i]][c] = scale[dm[i]] f[i][c]; rows
+// outside the memory dofs are left as they are.
+void LaunchExpandForce(const DegreeTask* tasks_dev, int taskCount, int nmMax,
+                       int colsMax, const int* dm, const double* scale,
+                       const double* f, double* rhs, hipStream_t stream);
+
+}  // namespace sl3amd

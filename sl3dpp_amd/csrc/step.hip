@@ -367,7 +367,9 @@ Viscoelastic3D::Viscoelastic3D(const Earth& earth, int lmax, const double* delta
     fourierA_[b].Resize(coupling_.FourierDoubles(b));
     fourierB_[b].Resize(coupling_.FourierDoubles(b));
   }
-  rhs_.Resize(rhsDoubles_);
+  force_.Resize(std::max<size_t>(fDoubles_, 1));
+  LaunchFill(force_.data(), force_.size(), 0.0, s);  // the padding stays zero
+  response_.Resize(std::max<size_t>(fDoubles_, 1));
   amp_.Resize(static_cast<size_t>(lmax + 1) * (lmax + 2));
   partials_.Resize(1024 * 16);
   result_.Resize(16);
@@ -405,7 +407,11 @@ void Viscoelastic3D::BuildTasks() {
   });
   tasks_.clear();
   dofsHost_.clear();
+  dofsMemHost_.clear();
+  dmHost_.clear();
   rhsDoubles_ = scaleDoubles_ = unitDoubles_ = 0;
+  gDoubles_ = fDoubles_ = 0;
+  nmMax_ = nmPadMax_ = colsMax_ = 0;
   auto add = [&](int l, int kind) {
     DegreeTask t;
     t.l = l;
@@ -442,6 +448,34 @@ void Viscoelastic3D::BuildTasks() {
         }
       }
     }
+    // the memory dofs, in dof order, and the table's memory-dof rows
+    std::vector<int> dm;
+    for (size_t k = static_cast<size_t>(t.dofOff); k < dofsHost_.size(); ++k) {
+      const int d = dofsHost_[k];
+      if (d >= 0 && d != t.pinned) dm.push_back(d);
+    }
+    std::sort(dm.begin(), dm.end());
+    dm.erase(std::unique(dm.begin(), dm.end()), dm.end());
+    for (size_t k = static_cast<size_t>(t.dofOff); k < dofsHost_.size(); ++k) {
+      const int d = dofsHost_[k];
+      int row = -1;
+      if (d >= 0 && d != t.pinned) {
+        row = static_cast<int>(std::lower_bound(dm.begin(), dm.end(), d) - dm.begin());
+      }
+      dofsMemHost_.push_back(row);
+    }
+    t.nm = static_cast<int>(dm.size());
+    t.nmPad = 16 * CeilDiv(t.nm, 16);
+    t.colsPad = 16 * CeilDiv(t.cols, 16);
+    t.dmOff = static_cast<long long>(dmHost_.size());
+    dmHost_.insert(dmHost_.end(), dm.begin(), dm.end());
+    t.gOff = static_cast<long long>(gDoubles_);
+    gDoubles_ += static_cast<size_t>(t.nmPad) * t.nmPad;
+    t.fOff = static_cast<long long>(fDoubles_);
+    fDoubles_ += static_cast<size_t>(t.nmPad) * t.colsPad;
+    nmMax_ = std::max(nmMax_, t.nm);
+    nmPadMax_ = std::max(nmPadMax_, t.nmPad);
+    colsMax_ = std::max(colsMax_, t.cols);
     tasks_.push_back(t);
   };
   for (int l = 0; l <= L; ++l) add(l, 0);
@@ -449,6 +483,11 @@ void Viscoelastic3D::BuildTasks() {
   hipStream_t s = Stream();
   tasksDev_.Upload(tasks_.data(), tasks_.size(), s);
   dofs_.Upload(dofsHost_.data(), dofsHost_.size(), s);
+  dofsMem_.Upload(dofsMemHost_.data(), dofsMemHost_.size(), s);
+  dm_.Upload(dmHost_.data(), std::max<size_t>(dmHost_.size(), 1), s);
+  const auto work = ResponseWorkList(tasks_);
+  work_.Upload(work.data(), work.size(), s);
+  workCount_ = static_cast<int>(work.size());
   SL3_HIP(hipStreamSynchronize(s));
 }
 
@@ -538,6 +577,7 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
                      bands.begin() + chol[k].ab_off + static_cast<long long>(b.kl + 1) * b.n);
       keep.push_back(b);
       f->usedCholesky[cholTask[k]] = 1;
+      f->choleskyTask.push_back(cholTask[k]);
     }
     for (size_t ti = 0; ti < tasks_.size(); ++ti) {
       const auto& t = tasks_[ti];
@@ -588,6 +628,7 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
     }
     f->cholesky = keep;
     f->lu = lu;
+    f->luTask = luTask;
     SL3_HIP(hipStreamSynchronize(s));  // the host staging vectors end here
   }
   f->scale.Upload(scale.data(), scale.size(), s);
@@ -651,6 +692,15 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
     for (int d = 0; d < t.ndof; ++d) units[t.unitOff + d] *= scale[t.scaleOff + d];  // FinishSolve
   }
   f->units.Upload(units.data(), units.size(), s);
+  std::vector<double> unitsMem(std::max<size_t>(dmHost_.size(), 1), 0.0);
+  for (const auto& t : tasks_) {
+    if (t.kind != 0) continue;
+    for (int i = 0; i < t.nm; ++i) {
+      unitsMem[t.dmOff + i] = units[t.unitOff + dmHost_[t.dmOff + i]];
+    }
+  }
+  f->unitsMem.Upload(unitsMem.data(), unitsMem.size(), s);
+  BuildResponse(*f);
   SL3_HIP(hipStreamSynchronize(s));
   flavours_.push_front(std::move(f));
   return *flavours_.front();
@@ -673,18 +723,65 @@ std::unique_ptr<DeviceState> Viscoelastic3D::ZeroState() const {
   return state;
 }
 
-void Viscoelastic3D::SolveAll(const double* weighted, Flavour& f) {
+// The memory response of every task: unit columns on the memory dofs,
+// scaled (BeginSolve), solved with the flavour's factors, unscaled
+// (FinishSolve) on the memory rows and packed for the response GEMM.
+void Viscoelastic3D::BuildResponse(Flavour& f) {
   hipStream_t s = Stream();
-  LaunchFill(rhs_.data(), rhs_.size(), 0.0, s);
-  const SpectralView w = coupling_.View(const_cast<double*>(weighted));
-  LaunchForce(tasksDev_.data(), static_cast<int>(tasks_.size()), coupling_.LMax(),
-              coupling_.Geometry(), w, f.scale.data(), dofs_.data(), rhs_.data(), s);
-  LaunchCholeskySolve(f.choleskyDev.data(), f.choleskyTasks.data(), f.choleskyTaskCount,
-                      f.kdMax, f.factors.data(), rhs_.data(), s);
+  const int count = static_cast<int>(tasks_.size());
+  std::vector<long long> zOff(tasks_.size());
+  long long zTotal = 0;
+  for (size_t ti = 0; ti < tasks_.size(); ++ti) {
+    zOff[ti] = zTotal;
+    zTotal += static_cast<long long>(tasks_[ti].ndof) * tasks_[ti].nm;
+  }
+  DeviceArray<double> z(static_cast<size_t>(std::max<long long>(zTotal, 1)));
+  LaunchFill(z.data(), z.size(), 0.0, s);
+  DeviceArray<long long> zOffDev;
+  zOffDev.Upload(zOff.data(), zOff.size(), s);
+  LaunchUnitColumns(tasksDev_.data(), count, nmMax_, dm_.data(), f.scale.data(),
+                    zOffDev.data(), z.data(), s);
+  auto columns = [&](const std::vector<BandSystem>& systems, const std::vector<int>& task) {
+    std::vector<BandSystem> out;
+    for (size_t k = 0; k < systems.size(); ++k) {
+      const auto& t = tasks_[task[k]];
+      if (t.nm == 0) continue;
+      BandSystem b = systems[k];
+      b.nrhs = t.nm;
+      b.b_off = zOff[task[k]];
+      b.row_stride = t.nm;
+      b.col_stride = 1;
+      out.push_back(b);
+    }
+    return out;
+  };
+  const auto zc = columns(f.cholesky, f.choleskyTask), zl = columns(f.lu, f.luTask);
+  DeviceArray<BandSystem> zcDev, zlDev;
+  zcDev.Upload(zc.data(), zc.size(), s);
+  zlDev.Upload(zl.data(), zl.size(), s);
+  const auto zct = SolveTasks(zc), zlt = SolveTasks(zl);
+  DeviceArray<int2> zctDev, zltDev;
+  zctDev.Upload(zct.data(), zct.size(), s);
+  zltDev.Upload(zlt.data(), zlt.size(), s);
+  LaunchCholeskySolve(zcDev.data(), zctDev.data(), static_cast<int>(zct.size()), f.kdMax,
+                      f.factors.data(), z.data(), s);
   int kvMax = 0;
   for (const auto& b : f.lu) kvMax = std::max(kvMax, b.kl + b.ku);
-  LaunchLuSolve(f.luDev.data(), f.luTasks.data(), f.luTaskCount, kvMax, f.factors.data(),
-                f.pivots.data(), rhs_.data(), s);
+  LaunchLuSolve(zlDev.data(), zltDev.data(), static_cast<int>(zlt.size()), kvMax,
+                f.factors.data(), f.pivots.data(), z.data(), s);
+  f.response.Resize(std::max<size_t>(gDoubles_, 1));
+  LaunchPackResponse(tasksDev_.data(), count, nmPadMax_, dm_.data(), f.scale.data(),
+                     zOffDev.data(), z.data(), f.response.data(), s);
+  SL3_HIP(hipStreamSynchronize(s));  // z and the descriptors end here
+}
+
+void Viscoelastic3D::SolveAll(const double* weighted, Flavour& f) {
+  hipStream_t s = Stream();
+  const SpectralView w = coupling_.View(const_cast<double*>(weighted));
+  LaunchForce(tasksDev_.data(), static_cast<int>(tasks_.size()), coupling_.LMax(),
+              coupling_.Geometry(), w, dofsMem_.data(), force_.data(), s);
+  LaunchResponse(tasksDev_.data(), work_.data(), workCount_, f.response.data(),
+                 force_.data(), response_.data(), s);
 }
 
 void Viscoelastic3D::Amplitudes(const LoadFn& load, double t) {
@@ -739,9 +836,8 @@ void Viscoelastic3D::Pass(const double* s, double* sNext, bool useStartForce, Fl
   SolveAll(analysed_.data(), f);
   const auto* sph = tasksDev_.data();
   const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
-  LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), f.scale.data(),
-               dofs_.data(), rhs_.data(), f.units.data(), amp_.data(),
-               coupling_.View(sNext), st);
+  LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), dofsMem_.data(),
+               response_.data(), f.unitsMem.data(), amp_.data(), coupling_.View(sNext), st);
 }
 
 void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const LoadFn& load,
@@ -763,9 +859,8 @@ void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const LoadFn&
   Amplitudes(load, state.t);
   const auto* sph = tasksDev_.data();
   const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
-  LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), f.scale.data(),
-               dofs_.data(), rhs_.data(), f.units.data(), amp_.data(),
-               coupling_.View(out), s);
+  LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), dofsMem_.data(),
+               response_.data(), f.unitsMem.data(), amp_.data(), coupling_.View(out), s);
 }
 
 StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
@@ -930,8 +1025,20 @@ StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
 
 std::vector<double> Viscoelastic3D::SolutionOfTask(int task) {
   const auto& t = tasks_.at(task);
+  if (flavours_.empty()) throw ArgumentError("no step has been taken");
   Flavour& f = *flavours_.front();
   hipStream_t s = Stream();
+  // the band solves of the last pass's force on every dof
+  rhs_.Resize(std::max<size_t>(rhsDoubles_, 1));
+  LaunchFill(rhs_.data(), rhs_.size(), 0.0, s);
+  LaunchExpandForce(tasksDev_.data(), static_cast<int>(tasks_.size()), nmMax_, colsMax_,
+                    dm_.data(), f.scale.data(), force_.data(), rhs_.data(), s);
+  LaunchCholeskySolve(f.choleskyDev.data(), f.choleskyTasks.data(), f.choleskyTaskCount,
+                      f.kdMax, f.factors.data(), rhs_.data(), s);
+  int kvMax = 0;
+  for (const auto& b : f.lu) kvMax = std::max(kvMax, b.kl + b.ku);
+  LaunchLuSolve(f.luDev.data(), f.luTasks.data(), f.luTaskCount, kvMax, f.factors.data(),
+                f.pivots.data(), rhs_.data(), s);
   std::vector<double> raw(static_cast<size_t>(t.ndof) * t.cols), scale(t.ndof),
       units(t.kind == 0 ? t.ndof : 0), amp(2 * (t.l + 1));
   SL3_HIP(hipMemcpyAsync(raw.data(), rhs_.data() + t.rhsOff, sizeof(double) * raw.size(),

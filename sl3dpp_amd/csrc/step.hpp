@@ -19,6 +19,7 @@
 #include "kernels.hpp"
 #include "model.hpp"
 #include "operators.hpp"
+#include "response.hpp"
 #include "transform.hpp"
 
 namespace sl3amd {
@@ -120,6 +121,10 @@ struct Flavour {
   int choleskyTaskCount = 0, luTaskCount = 0;
   int kdMax = 0;
   std::vector<char> usedCholesky;  // per task
+  std::vector<int> choleskyTask, luTask;  // the task of each solve system
+  // the memory response of every task (G = D A_s^{-1} D on its memory dofs)
+  // and the unit load responses on the memory dofs
+  DeviceArray<double> response, unitsMem;
 };
 
 // Viscoelastic3D::State on the device: one branch's memory tensor, the
@@ -149,8 +154,9 @@ class Viscoelastic3D {
   // null) is the warm start of Converge (:1916-1921).
   StepInfo Step(DeviceState& state, double dt, int scheme, const LoadFn& load,
                 const double* start = nullptr);
-  // The last pass's solutions, unscaled with channels, per task: host
-  // export for the surface outputs and tests.
+  // The last pass's solutions on every dof, unscaled with channels, per
+  // task (band solves of the last pass's force): host export for the
+  // surface outputs and tests.
   std::vector<double> SolutionOfTask(int task);
   const std::vector<DegreeTask>& Tasks() const { return tasks_; }
   Flavour& FlavourOf(int scheme, double dt);
@@ -179,6 +185,7 @@ class Viscoelastic3D {
 
  private:
   void BuildTasks();
+  void BuildResponse(Flavour& f);
   void SolveAll(const double* weighted, Flavour& f);
   void Amplitudes(const LoadFn& load, double t);
   double RelativeChange(const double* next, const double* now);
@@ -194,9 +201,19 @@ class Viscoelastic3D {
   std::vector<int> dofsHost_;
   DeviceArray<int> dofs_;
   size_t rhsDoubles_ = 0, scaleDoubles_ = 0, unitDoubles_ = 0;
+  // the memory dofs: per task their dof numbers (dm), and the dof table
+  // with memory-dof rows in place of dof numbers
+  std::vector<int> dmHost_, dofsMemHost_;
+  DeviceArray<int> dm_, dofsMem_;
+  size_t gDoubles_ = 0, fDoubles_ = 0;
+  int nmMax_ = 0, nmPadMax_ = 0, colsMax_ = 0;
+  DeviceArray<ResponseWork> work_;
+  int workCount_ = 0;
   std::list<std::unique_ptr<Flavour>> flavours_;
   // the open step's buffers
-  DeviceArray<double> weight_, beta_, fixed_, analysed_, rhs_, amp_, s_, sNext_;
+  DeviceArray<double> weight_, beta_, fixed_, analysed_, amp_, s_, sNext_;
+  DeviceArray<double> force_, response_;  // per pass, on the memory dofs
+  DeviceArray<double> rhs_;               // full solves for export only
   DeviceArray<double> fourierA_[3], fourierB_[3];
   DeviceArray<double> partials_, result_;
   DeviceArray<double> history_, xPrev_, fPrev_;
