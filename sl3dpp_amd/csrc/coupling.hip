@@ -340,17 +340,29 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Sums (or maxima) of `width` columns of partials, in block order.
-__global__ void ReducePartialsKernel(const double* partials, int blocks, int width,
-                                     int useMax, double* out) {
-  const int c = threadIdx.x;
-  if (c >= width) return;
+// Sums (or maxima) of `width` columns of partials: one workgroup per
+// column, thread t taking blocks t, t + 256, ..., then a fixed tree, so the
+// grouping of the terms never changes.
+__global__ void __launch_bounds__(kThreads)
+    ReducePartialsKernel(const double* partials, int blocks, int width, int useMax,
+                         double* out) {
+  __shared__ double red[kThreads];
+  const int c = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < blocks; ++b) {
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
     const double v = partials[b * width + c];
     s = useMax ? fmax(s, v) : s + v;
   }
-  out[c] = s;
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double a = red[threadIdx.x], b = red[threadIdx.x + w];
+      red[threadIdx.x] = useMax ? fmax(a, b) : a + b;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
 }
 
 __global__ void AndersonFirstKernel(double* x, const double* gx, double* xPrev,
@@ -513,7 +525,7 @@ void LaunchRelativeChange(const double* next, const double* now, size_t n,
   hipLaunchKernelGGL(RelativeChangeKernel, dim3(partialBlocks), dim3(kThreads), 0,
                      stream, next, now, n / 2, partials);
   SL3_HIP(hipGetLastError());
-  hipLaunchKernelGGL(ReducePartialsKernel, dim3(1), dim3(64), 0, stream, partials,
+  hipLaunchKernelGGL(ReducePartialsKernel, dim3(3), dim3(kThreads), 0, stream, partials,
                      partialBlocks, 3, 1, result);
   SL3_HIP(hipGetLastError());
 }
@@ -532,7 +544,7 @@ void LaunchAndersonSweep(const AndersonSweep& s, size_t n, double* partials,
   hipLaunchKernelGGL(AndersonSweepKernel, dim3(partialBlocks), dim3(kThreads), 0,
                      stream, s, n, partials);
   SL3_HIP(hipGetLastError());
-  hipLaunchKernelGGL(ReducePartialsKernel, dim3(1), dim3(64), 0, stream, partials,
+  hipLaunchKernelGGL(ReducePartialsKernel, dim3(16), dim3(kThreads), 0, stream, partials,
                      partialBlocks, 16, 0, sums);
   SL3_HIP(hipGetLastError());
 }

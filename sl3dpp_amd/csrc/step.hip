@@ -226,6 +226,29 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
   dft_.sinP = sinP_.data();
   dft_.cosN = cosN_.data();
   dft_.sinN = sinN_.data();
+  // the matrix-core operand tables of the P-point transforms
+  dft_.jt = RingMfmaTiles(dft_.P);
+  if (dft_.jt > 0) {
+    const int P = dft_.P, H = (P - 1) / 2, ks = 4 * dft_.jt;
+    std::vector<double> tc(static_cast<size_t>(dft_.jt) * ks * 64, 0.0), ts(tc.size(), 0.0);
+    for (int t = 0; t < dft_.jt; ++t) {
+      for (int kb = 0; kb < ks; ++kb) {
+        for (int lane = 0; lane < 64; ++lane) {
+          const int j = 1 + 16 * t + lane % 16, k = 1 + 4 * kb + lane / 16;
+          if (j > H || k > H) continue;
+          const size_t at = (static_cast<size_t>(t) * ks + kb) * 64 + lane;
+          const int jk = (j * k) % P;
+          tc[at] = std::cos(2.0 * kPi * jk / P);
+          ts[at] = std::sin(2.0 * kPi * jk / P);
+        }
+      }
+    }
+    tabC_.Upload(tc.data(), tc.size(), s);
+    tabS_.Upload(ts.data(), ts.size(), s);
+    dft_.tabC = tabC_.data();
+    dft_.tabS = tabS_.data();
+    SL3_HIP(hipStreamSynchronize(s));  // the host tables are freed next
+  }
   if (RingLds(nphi, dft_.P) > 160 * 1024) {
     throw ArgumentError("grid too large for the ring kernel's LDS staging");
   }

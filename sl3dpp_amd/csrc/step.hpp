@@ -101,7 +101,7 @@ class Coupling {
   DeviceArray<int> orders_;
   int orderCount_ = 0;
   DeviceArray<int> firstNode_, reach_;
-  DeviceArray<double> wtheta_, cosP_, sinP_, cosN_, sinN_;
+  DeviceArray<double> wtheta_, cosP_, sinP_, cosN_, sinN_, tabC_, tabS_;
   DftTables dft_;
   size_t spectralOffset_[4] = {};
   LegendrePlan plan_;

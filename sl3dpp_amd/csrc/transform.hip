@@ -663,6 +663,11 @@ void LaunchLegendreAnalysis(const LegendrePlan& p, const double* const fourier[3
 }
 
 void LaunchRing(const RingLaunch& r, hipStream_t stream) {
+  if (r.tables.jt > 0 &&
+      RingMfmaLds(r.nphi, r.tables.R, r.tables.P, r.lmax, r.block) <= 160 * 1024) {
+    LaunchRingMfma(r, stream);
+    return;
+  }
   RingArgs a{};
   a.t = r.tables;
   a.lmax = r.lmax;

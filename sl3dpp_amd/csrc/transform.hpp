@@ -27,7 +27,17 @@ struct DftTables {
   const double* sinP = nullptr;
   const double* cosN = nullptr;
   const double* sinN = nullptr;
+  // The matrix-core form (odd P): jt tiles of 16 frequencies; tabC / tabS
+  // hold cos / sin(2 pi jk / P) for j, k = 1..(P-1)/2 (zero beyond) in the
+  // A-operand order of v_mfma_f64_16x16x4f64: tile (t, kb), lane i ->
+  // j = 1 + 16 t + i % 16, k = 1 + 4 kb + i / 16. jt = 0: no such form.
+  int jt = 0;
+  const double* tabC = nullptr;
+  const double* tabS = nullptr;
 };
+
+// The largest jt the matrix-core ring kernel is instantiated for.
+constexpr int kRingMfmaMaxTiles = 4;
 
 enum RingOp {
   kRingSynthesise = 0,        // F -> grid
@@ -65,6 +75,11 @@ struct RingLaunch {
 };
 
 size_t RingLds(int nphi, int P);
+// The matrix-core ring kernel (ring_mfma.hip): its tile count for P (0 if
+// it does not apply), its LDS bytes, its launch.
+int RingMfmaTiles(int P);
+size_t RingMfmaLds(int nphi, int R, int P, int lmax, int block);
+void LaunchRingMfma(const RingLaunch& r, hipStream_t stream);
 void LaunchLegendreSynthesis(const LegendrePlan& p, const double* const spectral[3],
                              double* const fourier[3], const int* reach,
                              hipStream_t stream);
