@@ -69,34 +69,51 @@ __device__ inline Geometry MakeGeometry(const RingLaunch& a) {
   return g;
 }
 
-// The fold of the lane's sequence at frequency k (complex, both parts).
-// Fs: the group's staged coefficients [ring][m - mlo].
+// z times i^n (n = 0..3).
+__device__ inline double2 IPow(double2 z, int n) {
+  double a = (n & 1) ? -z.y : z.x;
+  double b = (n & 1) ? z.x : z.y;
+  if (n & 2) {
+    a = -a;
+    b = -b;
+  }
+  return make_double2(a, b);
+}
+
+// The fold of the lane's sequence at frequency k (complex, both parts):
+// with m = k + q P and nphi = R P, e^{2 pi i r m / nphi} =
+// e^{2 pi i r k / nphi} (e^{2 pi i / R})^{r q}, the second factor a power
+// of i, so one complex product per fold. Fs: the group's staged
+// coefficients [ring][m - mlo]; cosN / sinN: cos / sin(2 pi n / nphi).
 template <bool REAL>
 __device__ inline double2 Fold(const Geometry& g, const double2* Fs, const double* cosN,
                                const double* sinN, int ringA, int r, int k) {
-  double2 h = make_double2(0.0, 0.0);
-  int m = k;
-  while (m - g.P >= -g.L) m -= g.P;
-  for (; m <= g.L; m += g.P) {
+  double2 acc = make_double2(0.0, 0.0);
+  int m = k, q = 0;
+  while (m - g.P >= -g.L) {
+    m -= g.P;
+    --q;
+  }
+  const int step4 = 4 / g.R;
+  for (; m <= g.L; m += g.P, ++q) {
     double2 z;
     if (REAL) {
       // Z_m = F1_m + i F2_m, F_{-m} = conj(F_m), the real part at m = 0
       const int am = m < 0 ? -m : m;
       const double2 f1 = Fs[ringA * g.orders + am];
       const double2 f2 = Fs[(ringA + 1) * g.orders + am];
-      const double s = m < 0 ? -1.0 : 1.0;
-      const double i1 = m == 0 ? 0.0 : s * f1.y, i2 = m == 0 ? 0.0 : s * f2.y;
+      const double sg = m < 0 ? -1.0 : 1.0;
+      const double i1 = m == 0 ? 0.0 : sg * f1.y, i2 = m == 0 ? 0.0 : sg * f2.y;
       z = make_double2(f1.x - i2, i1 + f2.x);
     } else {
       z = Fs[ringA * g.orders + (m - g.mlo)];
     }
-    int idx = (r * m) % g.nphi;
-    if (idx < 0) idx += g.nphi;
-    const double c = cosN[idx], sn = sinN[idx];
-    h.x += z.x * c - z.y * sn;
-    h.y += z.x * sn + z.y * c;
+    const double2 t = IPow(z, (r * q * step4) & 3);
+    acc.x += t.x;
+    acc.y += t.y;
   }
-  return h;
+  const double c = cosN[r * k], sn = sinN[r * k];
+  return make_double2(acc.x * c - acc.y * sn, acc.x * sn + acc.y * c);
 }
 
 // Per-lane indexing: the lane's column is sequence s = col / 2, part p.
@@ -131,8 +148,9 @@ __device__ inline Lane MakeLane(const Geometry& g, int th0, int ntheta) {
 }
 
 // The forward sub-DFTs of y (the lane's part at j, at P - j, at 0) on the
-// matrix cores, staged as G[seq][k] in LDS.
-template <int JT>
+// matrix cores, staged as G[seq][k] in LDS. SYNC: the stage overlays the
+// operand tables, so every wave first finishes its products.
+template <int JT, bool SYNC>
 __device__ inline void ForwardToStage(const Geometry& g, const Lane& ln,
                                       const double* tabC, const double* tabS,
                                       const double (&yj)[4 * JT], const double (&yq)[4 * JT],
@@ -153,15 +171,17 @@ __device__ inline void ForwardToStage(const Geometry& g, const Lane& ln,
     C[t] = Acc4{0.0, 0.0, 0.0, 0.0};
     S[t] = Acc4{0.0, 0.0, 0.0, 0.0};
   }
+  // 2 JT independent accumulation chains, interleaved
 #pragma unroll
-  for (int t = 0; t < JT; ++t) {
+  for (int kb = 0; kb < KS; ++kb) {
 #pragma unroll
-    for (int kb = 0; kb < KS; ++kb) {
+    for (int t = 0; t < JT; ++t) {
       const int at = (t * KS + kb) * kWave + ln.lane;
       C[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tabC[at], av[kb], C[t], 0, 0, 0);
       S[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tabS[at], bv[kb], S[t], 0, 0, 0);
     }
   }
+  if constexpr (SYNC) __syncthreads();
   // G(k) = y0 + C'_k - i S'_k, G(P - k) = y0 + C'_k + i S'_k
   const double sg = ln.p == 0 ? 1.0 : -1.0;
   double* row = Gd + static_cast<size_t>(ln.seqGroup) * g.P * 2 + ln.p;
@@ -180,63 +200,88 @@ __device__ inline void ForwardToStage(const Geometry& g, const Lane& ln,
   }
 }
 
-// Combine the staged G into F_m for every ring of the group.
+// Combine the staged G into F_m for every ring of the group: per
+// frequency k < P, u_r = G_r(k) e^{-2 pi i r k / nphi}, then every order
+// m = k + q P is the R-point sum of u_r i^{-(4/R) r q}; consecutive
+// threads take consecutive rings.
 template <bool REAL>
 __device__ inline void CombineFromStage(const Geometry& g, const RingLaunch& a,
                                         const double* cosN, const double* sinN,
                                         const double2* Gs, int node, int th0,
                                         double* out) {
   const double scale = 2.0 * 3.14159265358979323846 / g.nphi;
-  const int mCount = g.L - g.mlo + 1;
-  const int tasks = mCount * g.ringsPerGroup;
+  const int tasks = g.P * g.ringsPerGroup;
+  const int step4 = 4 / g.R;
   for (int e = threadIdx.x; e < tasks; e += blockDim.x) {
     const int ring = e % g.ringsPerGroup;
-    const int mo = e / g.ringsPerGroup;
-    const int m = g.mlo + mo;
+    const int k = e / g.ringsPerGroup;
     const int th = th0 + ring;
     if (th >= a.ntheta) continue;
     const int wave = ring / g.ringsPerWave, gl = ring % g.ringsPerWave;
-    double2 res;
+    const double w = scale * a.wtheta[th];
+    auto at = [&](int mo) {
+      return out + ((static_cast<size_t>(mo) * a.nodes + node) * a.ntheta + th) * 2;
+    };
     if (REAL) {
       const int base = wave * kSeq + (gl / 2) * g.R;
-      int kp = m % g.P;
-      int km = (g.P - kp) % g.P;
-      double2 zp = make_double2(0.0, 0.0), zm = make_double2(0.0, 0.0);
+      const int kk = (g.P - k) % g.P;
+      double2 u[4], v[4];
       for (int r = 0; r < g.R; ++r) {
-        const int idx = (r * m) % g.nphi;
-        const double c = cosN[idx], sn = sinN[idx];
-        const double2 gp = Gs[(base + r) * g.P + kp];
-        const double2 gm = Gs[(base + r) * g.P + km];
-        // e^{-i a} G(m), e^{+i a} G(-m)
-        zp.x += gp.x * c + gp.y * sn;
-        zp.y += gp.y * c - gp.x * sn;
-        zm.x += gm.x * c - gm.y * sn;
-        zm.y += gm.y * c + gm.x * sn;
+        const double2 gp = Gs[(base + r) * g.P + k];
+        const double2 gm = Gs[(base + r) * g.P + kk];
+        const double c = cosN[r * k], sn = sinN[r * k];
+        const double c2 = cosN[r * kk], s2 = sinN[r * kk];
+        u[r] = make_double2(gp.x * c + gp.y * sn, gp.y * c - gp.x * sn);
+        v[r] = make_double2(gm.x * c2 + gm.y * s2, gm.y * c2 - gm.x * s2);
       }
-      if (gl % 2 == 0) {
-        res = make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y - zm.y));
-      } else {
-        // (Z_m - conj(Z_{-m})) / 2i
-        res = make_double2(0.5 * (zp.y + zm.y), -0.5 * (zp.x - zm.x));
+      // m = k + q P >= 0; -m = kk + q' P with q' = -q (k = 0) or -(q + 1)
+      for (int m = k, q = 0; m <= g.L; m += g.P, ++q) {
+        const int qm = k == 0 ? -q : -(q + 1);
+        double2 zp = make_double2(0.0, 0.0), zm = make_double2(0.0, 0.0);
+        for (int r = 0; r < g.R; ++r) {
+          const double2 tp = IPow(u[r], (-r * q * step4) & 3);
+          const double2 tm = IPow(v[r], (-r * qm * step4) & 3);
+          zp.x += tp.x;
+          zp.y += tp.y;
+          zm.x += tm.x;
+          zm.y += tm.y;
+        }
+        double2 res;
+        if (gl % 2 == 0) {
+          res = make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y - zm.y));
+        } else {
+          // (Z_m - conj(Z_{-m})) / 2i
+          res = make_double2(0.5 * (zp.y + zm.y), -0.5 * (zp.x - zm.x));
+        }
+        double* dst = at(m);
+        dst[0] = res.x * w;
+        dst[1] = res.y * w;
       }
     } else {
       const int base = wave * kSeq + gl * g.R;
-      int k = m % g.P;
-      if (k < 0) k += g.P;
-      res = make_double2(0.0, 0.0);
+      double2 u[4];
       for (int r = 0; r < g.R; ++r) {
-        int idx = (r * m) % g.nphi;
-        if (idx < 0) idx += g.nphi;
-        const double c = cosN[idx], sn = sinN[idx];
         const double2 gv = Gs[(base + r) * g.P + k];
-        res.x += gv.x * c + gv.y * sn;
-        res.y += gv.y * c - gv.x * sn;
+        const double c = cosN[r * k], sn = sinN[r * k];
+        u[r] = make_double2(gv.x * c + gv.y * sn, gv.y * c - gv.x * sn);
+      }
+      int m = k, q = 0;
+      while (m - g.P >= -g.L) {
+        m -= g.P;
+        --q;
+      }
+      for (; m <= g.L; m += g.P, ++q) {
+        double2 res = make_double2(0.0, 0.0);
+        for (int r = 0; r < g.R; ++r) {
+          const double2 t = IPow(u[r], (-r * q * step4) & 3);
+          res.x += t.x;
+          res.y += t.y;
+        }
+        double* dst = at(m - g.mlo);
+        dst[0] = res.x * w;
+        dst[1] = res.y * w;
       }
     }
-    const double w = scale * a.wtheta[th];
-    double* dst = out + ((static_cast<size_t>(mo) * a.nodes + node) * a.ntheta + th) * 2;
-    dst[0] = res.x * w;
-    dst[1] = res.y * w;
   }
 }
 
@@ -253,15 +298,26 @@ __global__ void __launch_bounds__(kThreads) RingMfmaKernel(RingLaunch a) {
   const int node = blockIdx.y;
   const int th0 = blockIdx.x * g.ringsPerGroup;
   const Lane ln = MakeLane<REAL>(g, th0, a.ntheta);
-  double* cosN = lds;
-  double* sinN = lds + g.nphi;
-  double* stageD = lds + 2 * g.nphi;
+  const double* cosN = a.tables.cosN;  // small, cache-resident
+  const double* sinN = a.tables.sinN;
+  // LDS: the operand tables (all classes but the step start, which keeps
+  // them in the cache), then the staged coefficients; the staged G of the
+  // forward transforms overlays both once they are spent.
+  constexpr bool kTabLds = CLASS != kClassStart;
+  constexpr int kTab = JT * KS * kWave;
+  double* stageD = lds + (kTabLds ? 2 * kTab : 0);
   double2* stage = reinterpret_cast<double2*>(stageD);
+  double* Gd = lds;
+  const double2* Gs = reinterpret_cast<const double2*>(lds);
   const double* tabC = a.tables.tabC;
   const double* tabS = a.tables.tabS;
-  for (int i = threadIdx.x; i < g.nphi; i += blockDim.x) {
-    cosN[i] = a.tables.cosN[i];
-    sinN[i] = a.tables.sinN[i];
+  if constexpr (kTabLds) {
+    for (int i = threadIdx.x; i < kTab; i += blockDim.x) {
+      lds[i] = a.tables.tabC[i];
+      lds[kTab + i] = a.tables.tabS[i];
+    }
+    tabC = lds;
+    tabS = lds + kTab;
   }
   const bool synth = a.op == kRingSynthesise || a.op == kRingPass ||
                      a.op == kRingFinish || a.op == kRingStartFromSpectral;
@@ -285,6 +341,21 @@ __global__ void __launch_bounds__(kThreads) RingMfmaKernel(RingLaunch a) {
   double xj[KS], xq[KS], x0 = 0.0;
   const int th = th0 + ln.ring;
   const size_t ringBase = (static_cast<size_t>(node) * a.ntheta + (ln.valid ? th : 0)) * g.nphi;
+  // The lane's points: j = 1 + 16 t + 4 q + quad (v = 4 t + q), P - j, and
+  // 0. Where a lane has no point (padding, a ring past the last) the index
+  // is its ring's point 0, always in bounds: loads are unconditional and
+  // issue back to back, only stores are predicated.
+  const size_t p0 = ringBase + ln.r;
+  size_t pj[KS], pq[KS];
+  bool lv[KS];
+#pragma unroll
+  for (int v = 0; v < KS; ++v) {
+    const int j = 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad;
+    lv[v] = ln.valid && j <= g.H;
+    pj[v] = lv[v] ? ringBase + g.R * j + ln.r : p0;
+    pq[v] = lv[v] ? ringBase + g.R * (g.P - j) + ln.r : p0;
+  }
+  auto element = [&](size_t pt) -> size_t { return REAL ? pt : 2 * pt + ln.p; };
   if (synth) {
     double av[KS], bv[KS];
     double sum = 0.0;
@@ -311,9 +382,9 @@ __global__ void __launch_bounds__(kThreads) RingMfmaKernel(RingLaunch a) {
       S[t] = Acc4{0.0, 0.0, 0.0, 0.0};
     }
 #pragma unroll
-    for (int t = 0; t < JT; ++t) {
+    for (int kb = 0; kb < KS; ++kb) {
 #pragma unroll
-      for (int kb = 0; kb < KS; ++kb) {
+      for (int t = 0; t < JT; ++t) {
         const int at = (t * KS + kb) * kWave + ln.lane;
         C[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tabC[at], av[kb], C[t], 0, 0, 0);
         S[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tabS[at], bv[kb], S[t], 0, 0, 0);
@@ -334,60 +405,54 @@ __global__ void __launch_bounds__(kThreads) RingMfmaKernel(RingLaunch a) {
     // a stored ring (complex: interleaved parts; real: the lane's own ring)
 #pragma unroll
     for (int v = 0; v < KS; ++v) {
-      const int j = 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad;
-      xj[v] = 0.0;
-      xq[v] = 0.0;
-      if (ln.valid && j <= g.H) {
-        const size_t pj = ringBase + g.R * j + ln.r, pq = ringBase + g.R * (g.P - j) + ln.r;
-        if (REAL) {
-          xj[v] = a.gridIn[pj];
-          xq[v] = a.gridIn[pq];
-        } else {
-          xj[v] = a.gridIn[2 * pj + ln.p];
-          xq[v] = a.gridIn[2 * pq + ln.p];
-        }
-      }
+      const double vj = a.gridIn[element(pj[v])], vq = a.gridIn[element(pq[v])];
+      xj[v] = lv[v] ? vj : 0.0;
+      xq[v] = lv[v] ? vq : 0.0;
     }
-    if (ln.valid) {
-      const size_t p0 = ringBase + ln.r;
-      x0 = REAL ? a.gridIn[p0] : a.gridIn[2 * p0 + ln.p];
-    }
+    const double v0 = a.gridIn[element(p0)];
+    x0 = ln.valid ? v0 : 0.0;
   }
   __syncthreads();  // the staged coefficients are read; the stage is reused
 
   // the pointwise operation at the lane's points
-  auto point = [&](int v) -> size_t {  // v < 0: the point j = 0
-    if (v < 0) return ringBase + ln.r;
-    const int j = 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad;
-    return ringBase + g.R * j + ln.r;
-  };
-  auto pointQ = [&](int v) -> size_t {
-    const int j = 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad;
-    return ringBase + g.R * (g.P - j) + ln.r;
-  };
-  auto live = [&](int v) -> bool {
-    const int j = 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad;
-    return ln.valid && j <= g.H;
-  };
   const bool own0 = ln.valid && ln.quad == 0;  // the lane that stores point 0
-  auto element = [&](size_t pt) -> size_t { return REAL ? pt : 2 * pt + ln.p; };
 
   if constexpr (CLASS == kClassSynth) {
-    const bool finish = a.op == kRingFinish;
-    auto put = [&](size_t pt, double d) {
-      a.gridOut[element(pt)] = d;
-      if (finish) a.memOut[element(pt)] += a.beta[pt] * d;
-    };
-    if (own0) put(point(-1), x0);
+    if (a.op == kRingFinish) {
+      // d on the grid and m+ = m~ + beta d in place
+      double bj[KS], bq[KS], mj[KS], mq[KS];
 #pragma unroll
-    for (int v = 0; v < KS; ++v) {
-      if (live(v)) {
-        put(point(v), xj[v]);
-        put(pointQ(v), xq[v]);
+      for (int v = 0; v < KS; ++v) {
+        bj[v] = a.beta[pj[v]];
+        bq[v] = a.beta[pq[v]];
+        mj[v] = a.memOut[element(pj[v])];
+        mq[v] = a.memOut[element(pq[v])];
+      }
+      const double b0 = a.beta[p0], m0 = a.memOut[element(p0)];
+      if (own0) {
+        a.gridOut[element(p0)] = x0;
+        a.memOut[element(p0)] = m0 + b0 * x0;
+      }
+#pragma unroll
+      for (int v = 0; v < KS; ++v) {
+        if (lv[v]) {
+          a.gridOut[element(pj[v])] = xj[v];
+          a.gridOut[element(pq[v])] = xq[v];
+          a.memOut[element(pj[v])] = mj[v] + bj[v] * xj[v];
+          a.memOut[element(pq[v])] = mq[v] + bq[v] * xq[v];
+        }
+      }
+    } else {
+      if (own0) a.gridOut[element(p0)] = x0;
+#pragma unroll
+      for (int v = 0; v < KS; ++v) {
+        if (lv[v]) {
+          a.gridOut[element(pj[v])] = xj[v];
+          a.gridOut[element(pq[v])] = xq[v];
+        }
       }
     }
   } else if constexpr (CLASS == kClassStart) {
-    double* Gd = stageD;
     // src/Viscoelastic3D.cpp:1283-1332, one branch: the weights of
     // h = dt itau(x), m~ = e m + alpha d0, beta, W = mu (beta - beta_ref),
     // the fixed force mu m~; then W d0 (:1361-1365). x holds d0.
@@ -400,9 +465,20 @@ __global__ void __launch_bounds__(kThreads) RingMfmaKernel(RingLaunch a) {
     };
     const double refBeta =
         a.scheme == 1 ? 1.0 - 1.0 / (1.0 + href) : 1.0 - phi1(href);
+    // every point's itau and m first (unconditional loads, issued together)
+    double hj[KS], hq[KS], mInJ[KS], mInQ[KS];
+#pragma unroll
+    for (int v = 0; v < KS; ++v) {
+      hj[v] = a.itau[pj[v]];
+      hq[v] = a.itau[pq[v]];
+      mInJ[v] = a.memIn[element(pj[v])];
+      mInQ[v] = a.memIn[element(pq[v])];
+    }
+    const double h0 = a.itau[p0], mIn0 = a.memIn[element(p0)];
     // returns W d0 and writes m~, (REAL) beta and W; mt gets mu m~
-    auto start = [&](size_t pt, double d0, bool store, double& mt) -> double {
-      const double h = a.dt * a.itau[pt];
+    auto start = [&](size_t pt, double itau, double mIn, double d0, bool store,
+                     double& mt) -> double {
+      const double h = a.dt * itau;
       double ew, aw, bw;
       if (a.scheme == 1) {
         const double w = 1.0 / (1.0 + h);
@@ -416,7 +492,7 @@ __global__ void __launch_bounds__(kThreads) RingMfmaKernel(RingLaunch a) {
         aw = p - e;
         bw = 1.0 - p;
       }
-      const double mNew = ew * a.memIn[element(pt)] + aw * d0;
+      const double mNew = ew * mIn + aw * d0;
       if (store) {
         a.memOut[element(pt)] = mNew;
         if (REAL) {
@@ -428,34 +504,39 @@ __global__ void __launch_bounds__(kThreads) RingMfmaKernel(RingLaunch a) {
       return mu * (bw - refBeta) * d0;
     };
     double yj[KS], yq[KS], mj[KS], mq[KS], y0 = 0.0, m0 = 0.0;
-    if (ln.valid) y0 = start(point(-1), x0, ln.quad == 0, m0);
+    y0 = start(p0, h0, mIn0, x0, own0, m0);
+    if (!ln.valid) y0 = m0 = 0.0;
 #pragma unroll
     for (int v = 0; v < KS; ++v) {
-      yj[v] = yq[v] = mj[v] = mq[v] = 0.0;
-      if (live(v)) {
-        yj[v] = start(point(v), xj[v], true, mj[v]);
-        yq[v] = start(pointQ(v), xq[v], true, mq[v]);
-      }
+      yj[v] = start(pj[v], hj[v], mInJ[v], xj[v], lv[v], mj[v]);
+      yq[v] = start(pq[v], hq[v], mInQ[v], xq[v], lv[v], mq[v]);
+      if (!lv[v]) yj[v] = yq[v] = mj[v] = mq[v] = 0.0;
     }
     // the analysis of W d0 (:1361-1376) ...
-    ForwardToStage<JT>(g, ln, tabC, tabS, yj, yq, y0, Gd);
+    ForwardToStage<JT, false>(g, ln, tabC, tabS, yj, yq, y0, Gd);
     __syncthreads();
-    CombineFromStage<REAL>(g, a, cosN, sinN, stage, node, th0, a.fourierOut2);
+    CombineFromStage<REAL>(g, a, cosN, sinN, Gs, node, th0, a.fourierOut2);
     __syncthreads();
     // ... and of the fixed memory force mu m~ (:1355-1356)
-    ForwardToStage<JT>(g, ln, tabC, tabS, mj, mq, m0, Gd);
+    ForwardToStage<JT, false>(g, ln, tabC, tabS, mj, mq, m0, Gd);
     __syncthreads();
-    CombineFromStage<REAL>(g, a, cosN, sinN, stage, node, th0, a.fourierOut);
+    CombineFromStage<REAL>(g, a, cosN, sinN, Gs, node, th0, a.fourierOut);
   } else {
-  double* Gd = stageD;
   // y = factor x (pass and weighted: the point's weight; node scaled: mu)
   double yj[KS], yq[KS], y0 = x0;
   if (a.op == kRingPass || a.op == kRingWeighted) {
-    y0 = ln.valid ? a.weight[point(-1)] * x0 : 0.0;
+    double wj[KS], wq[KS];
 #pragma unroll
     for (int v = 0; v < KS; ++v) {
-      yj[v] = live(v) ? a.weight[point(v)] * xj[v] : 0.0;
-      yq[v] = live(v) ? a.weight[pointQ(v)] * xq[v] : 0.0;
+      wj[v] = a.weight[pj[v]];
+      wq[v] = a.weight[pq[v]];
+    }
+    const double w0 = a.weight[p0];
+    y0 = ln.valid ? w0 * x0 : 0.0;
+#pragma unroll
+    for (int v = 0; v < KS; ++v) {
+      yj[v] = lv[v] ? wj[v] * xj[v] : 0.0;
+      yq[v] = lv[v] ? wq[v] * xq[v] : 0.0;
     }
   } else if (a.op == kRingNodeScaled) {
     const double mu = a.muNode[node];
@@ -472,9 +553,9 @@ __global__ void __launch_bounds__(kThreads) RingMfmaKernel(RingLaunch a) {
       yq[v] = xq[v];
     }
   }
-  ForwardToStage<JT>(g, ln, tabC, tabS, yj, yq, y0, Gd);
+  ForwardToStage<JT, true>(g, ln, tabC, tabS, yj, yq, y0, Gd);
   __syncthreads();
-  CombineFromStage<REAL>(g, a, cosN, sinN, stage, node, th0, a.fourierOut);
+  CombineFromStage<REAL>(g, a, cosN, sinN, Gs, node, th0, a.fourierOut);
   }
 }
 
@@ -514,13 +595,19 @@ int RingMfmaTiles(int P) {
   return jt <= kRingMfmaMaxTiles ? jt : 0;
 }
 
-size_t RingMfmaLds(int nphi, int R, int P, int lmax, int block) {
-  const bool real = block == 2;
-  const int rpg = kWaves * (real ? 2 * kSeq : kSeq) / R;
-  const int orders = real ? lmax + 1 : 2 * lmax + 1;
-  const size_t staged = static_cast<size_t>(rpg) * orders;
-  const size_t g = static_cast<size_t>(kWaves) * kSeq * P;
-  return sizeof(double) * 2 * nphi + 16 * std::max(staged, g);
+size_t RingMfmaLds(const RingLaunch& r) {
+  const bool real = r.block == 2;
+  const int rpg = kWaves * (real ? 2 * kSeq : kSeq) / r.tables.R;
+  const int orders = real ? r.lmax + 1 : 2 * r.lmax + 1;
+  const bool synth = r.op == kRingSynthesise || r.op == kRingPass || r.op == kRingFinish ||
+                     r.op == kRingStartFromSpectral;
+  const bool start = r.op == kRingStart || r.op == kRingStartFromSpectral;
+  const bool analysis = r.op != kRingSynthesise && r.op != kRingFinish;
+  const size_t tables =
+      start ? 0 : 2 * sizeof(double) * (4 * r.tables.jt * r.tables.jt) * kWave;  // 2 x JT x KS x 64
+  const size_t staged = synth ? 16 * static_cast<size_t>(rpg) * orders : 0;
+  const size_t g = analysis ? 16 * static_cast<size_t>(kWaves) * kSeq * r.tables.P : 0;
+  return std::max(tables + staged, g);
 }
 
 void LaunchRingMfma(const RingLaunch& r, hipStream_t stream) {
@@ -528,7 +615,7 @@ void LaunchRingMfma(const RingLaunch& r, hipStream_t stream) {
   const bool real = r.block == 2;
   const int rpg = kWaves * (real ? 2 * kSeq : kSeq) / R;
   const dim3 grid(CeilDiv(r.ntheta, rpg), r.nodes);
-  const size_t lds = RingMfmaLds(r.nphi, R, r.tables.P, r.lmax, r.block);
+  const size_t lds = RingMfmaLds(r);
   ProfileScope prof("ring", stream);
   switch (r.tables.jt) {
     case 1: LaunchJT<1>(r, grid, lds, stream); break;

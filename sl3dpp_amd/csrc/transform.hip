@@ -663,8 +663,7 @@ void LaunchLegendreAnalysis(const LegendrePlan& p, const double* const fourier[3
 }
 
 void LaunchRing(const RingLaunch& r, hipStream_t stream) {
-  if (r.tables.jt > 0 &&
-      RingMfmaLds(r.nphi, r.tables.R, r.tables.P, r.lmax, r.block) <= 160 * 1024) {
+  if (r.tables.jt > 0 && RingMfmaLds(r) <= 160 * 1024) {
     LaunchRingMfma(r, stream);
     return;
   }

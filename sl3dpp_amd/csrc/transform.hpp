@@ -78,7 +78,7 @@ size_t RingLds(int nphi, int P);
 // The matrix-core ring kernel (ring_mfma.hip): its tile count for P (0 if
 // it does not apply), its LDS bytes, its launch.
 int RingMfmaTiles(int P);
-size_t RingMfmaLds(int nphi, int R, int P, int lmax, int block);
+size_t RingMfmaLds(const RingLaunch& r);
 void LaunchRingMfma(const RingLaunch& r, hipStream_t stream);
 void LaunchLegendreSynthesis(const LegendrePlan& p, const double* const spectral[3],
                              double* const fourier[3], const int* reach,
