@@ -175,7 +175,9 @@ typedef struct {
   float start_ms, passes_ms, finish_ms; /* device time of the phases */
 } sl3_step_info;
 
-enum { SL3_EXPONENTIAL_TRAPEZOID = 0, SL3_BACKWARD_EULER = 1 };
+/* Schemes (include/sl3dpp/StepWeights.hpp, StepRecipe.hpp): Sdirk2 runs
+ * two backward Euler stages of length gamma dt, gamma = 1 - 1/sqrt(2). */
+enum { SL3_EXPONENTIAL_TRAPEZOID = 0, SL3_BACKWARD_EULER = 1, SL3_SDIRK2 = 2 };
 
 typedef struct sl3_operator sl3_operator;
 typedef struct sl3_state sl3_state;
@@ -207,7 +209,8 @@ int sl3_state_destroy(sl3_state* state);
 int sl3_state_time(const sl3_state* state, double* t);
 /* Step (src/Viscoelastic3D.cpp:1982-2030), in place on the state.
  * start_dev: a warm start for the fixed point (device layout spectral
- * tensor), or NULL for the step's own start strain. */
+ * tensor), or NULL for the step's own start strain; Sdirk2 takes NULL.
+ * For Sdirk2 the info sums both stages' passes and phase times. */
 int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme,
              sl3_load_fn load, void* user, const double* start_dev,
              sl3_step_info* info);

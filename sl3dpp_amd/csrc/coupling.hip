@@ -292,6 +292,19 @@ __global__ void FillKernel(double* dst, size_t count, double value) {
   }
 }
 
+// StageState (src/Viscoelastic3D.cpp:463-492) for two terms: the sum
+// a x + b y accumulated from zero in the terms' order.
+__global__ void CombineKernel(double* dst, double a, const double* src, double b,
+                              size_t count) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += stride) {
+    const double first = a * dst[i];
+    const double second = b * src[i];
+    dst[i] = first + second;
+  }
+}
+
 __global__ void CopyKernel(double* dst, const double* src, size_t count) {
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -482,6 +495,15 @@ void LaunchFill(double* dst, size_t count, double value, hipStream_t stream) {
   ProfileScope prof475("fill", stream);
   hipLaunchKernelGGL(FillKernel, dim3(GridFor(count)), dim3(kThreads), 0, stream,
                      dst, count, value);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchCombine(double* dst, double a, const double* src, double b, size_t count,
+                   hipStream_t stream) {
+  if (count == 0) return;
+  ProfileScope prof("combine", stream);
+  hipLaunchKernelGGL(CombineKernel, dim3(GridFor(count)), dim3(kThreads), 0, stream, dst, a,
+                     src, b, count);
   SL3_HIP(hipGetLastError());
 }
 

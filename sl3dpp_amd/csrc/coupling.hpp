@@ -55,6 +55,9 @@ struct SpectralView {
 
 void LaunchFill(double* dst, size_t count, double value, hipStream_t stream);
 void LaunchCopy(double* dst, const double* src, size_t count, hipStream_t stream);
+// dst = a dst + b src (a stage's combined state, StageState).
+void LaunchCombine(double* dst, double a, const double* src, double b, size_t count,
+                   hipStream_t stream);
 
 // Forces of the analysed tensor on every task's memory dofs (rows of
 // nmPad x colsPad at fOff, unscaled: the memory response carries the

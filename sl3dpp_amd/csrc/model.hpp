@@ -154,6 +154,9 @@ inline double Phi1(double h, double e) {
   return (1.0 - e) / h;
 }
 
-enum SchemeId { kExponentialTrapezoid = 0, kBackwardEuler = 1 };
+enum SchemeId { kExponentialTrapezoid = 0, kBackwardEuler = 1, kSdirk2 = 2 };
+
+// Sdirk2's stage fraction gamma = 1 - 1/sqrt(2) (include/sl3dpp/StepRecipe.hpp:100).
+constexpr double kSdirk2Gamma = 0.29289321881345247560;
 
 }  // namespace sl3amd

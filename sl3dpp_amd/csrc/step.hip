@@ -888,11 +888,90 @@ void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const LoadFn&
 
 StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
                               const LoadFn& load, const double* start) {
-  if (dt < 0.0) throw ArgumentError("dt must be non-negative");
-  if (scheme != kExponentialTrapezoid && scheme != kBackwardEuler) {
-    throw ArgumentError("the 3D step is implicit: ExponentialTrapezoid or BackwardEuler");
+  if (!(dt >= 0.0)) throw ArgumentError("dt must be non-negative");
+  if (scheme != kExponentialTrapezoid && scheme != kBackwardEuler && scheme != kSdirk2) {
+    throw ArgumentError(
+        "the 3D step is implicit: ExponentialTrapezoid, BackwardEuler or Sdirk2");
   }
-  if (dt == 0.0) scheme = kBackwardEuler;
+  if (dt == 0.0) scheme = kBackwardEuler;  // every scheme's zero-length step
+  if (scheme == kSdirk2) return StepSdirk2(state, dt, load);
+  return StepOnce(state, dt, scheme, load, start);
+}
+
+void Viscoelastic3D::CopyState(const DeviceState& from, DeviceState& to) const {
+  hipStream_t s = Stream();
+  to.t = from.t;
+  for (int b = 0; b < 3; ++b) {
+    to.m[b].Resize(from.m[b].size());
+    LaunchCopy(to.m[b].data(), from.m[b].data(), from.m[b].size(), s);
+    if (from.hasStrain) {
+      to.strain[b].Resize(from.strain[b].size());
+      LaunchCopy(to.strain[b].data(), from.strain[b].data(), from.strain[b].size(), s);
+    }
+  }
+  to.hasStrain = from.hasStrain;
+  if (from.hasCoefficients) {
+    to.coefficients.Resize(from.coefficients.size());
+    LaunchCopy(to.coefficients.data(), from.coefficients.data(), from.coefficients.size(), s);
+  }
+  to.hasCoefficients = from.hasCoefficients;
+}
+
+// Sdirk2 by its recipe (src/StepRecipe.cpp:25-32 with StepByRecipe,
+// include/sl3dpp/StepRecipe.hpp:164-192, and StageState,
+// src/Viscoelastic3D.cpp:463-492): Y1 = BE(gamma h) from the state; the
+// second stage starts from m = (1 - r) m_n + r m(Y1), r = (1 - gamma) /
+// gamma, with Y1's strain and coefficients, at t_n + (1 - gamma) h, and is
+// BE(gamma h) again. The passes add up; the last stage's residual stands.
+StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const LoadFn& load) {
+  hipStream_t s = Stream();
+  const double tn = state.t;
+  const double length = kSdirk2Gamma * dt;
+  const double r = (1.0 - kSdirk2Gamma) / kSdirk2Gamma;
+  if (!stage_) stage_ = std::make_unique<DeviceState>();
+  DeviceState& y1 = *stage_;
+  CopyState(state, y1);
+  const StepInfo first = StepOnce(y1, length, kBackwardEuler, load, nullptr);
+  const auto firstHistory = residualHistory;
+  const Timing firstTiming = lastTiming;
+  for (int b = 0; b < 3; ++b) {
+    LaunchCombine(state.m[b].data(), 1.0 - r, y1.m[b].data(), r, state.m[b].size(), s);
+  }
+  DeviceState staged;
+  staged.t = tn + (1.0 - kSdirk2Gamma) * dt;
+  for (int b = 0; b < 3; ++b) {
+    staged.m[b] = std::move(state.m[b]);
+    staged.strain[b] = std::move(y1.strain[b]);
+  }
+  staged.hasStrain = y1.hasStrain;
+  staged.coefficients = std::move(y1.coefficients);
+  staged.hasCoefficients = y1.hasCoefficients;
+  const StepInfo second = StepOnce(staged, length, kBackwardEuler, load, nullptr);
+  // the stepped stage becomes the state; Y1's buffers return to the cache
+  for (int b = 0; b < 3; ++b) {
+    state.m[b] = std::move(staged.m[b]);
+    y1.strain[b] = std::move(state.strain[b]);
+    state.strain[b] = std::move(staged.strain[b]);
+  }
+  y1.coefficients = std::move(state.coefficients);
+  state.coefficients = std::move(staged.coefficients);
+  state.hasStrain = staged.hasStrain;
+  state.hasCoefficients = staged.hasCoefficients;
+  state.t = tn + dt;
+  StepInfo info;
+  info.iterations = first.iterations + second.iterations;
+  info.mixedPasses = first.mixedPasses + second.mixedPasses;
+  info.residual = second.residual;
+  info.converged = first.converged && second.converged;
+  residualHistory.insert(residualHistory.begin(), firstHistory.begin(), firstHistory.end());
+  lastTiming.startStep += firstTiming.startStep;
+  lastTiming.passes += firstTiming.passes;
+  lastTiming.finish += firstTiming.finish;
+  return info;
+}
+
+StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
+                                  const LoadFn& load, const double* start) {
   hipStream_t s = Stream();
   const size_t S = coupling_.SpectralDoubles();
   double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};

@@ -149,9 +149,11 @@ class Viscoelastic3D {
   StepOptions& MutableOptions() { return options_; }
   std::unique_ptr<DeviceState> ZeroState() const;
   void SetDelta(const double* delta_dev);
-  // Step (src/Viscoelastic3D.cpp:1982-2030) for ExponentialTrapezoid or
-  // BackwardEuler, in place on the state. `start` (device spectral, or
-  // null) is the warm start of Converge (:1916-1921).
+  // Step (src/Viscoelastic3D.cpp:1982-2030) for ExponentialTrapezoid,
+  // BackwardEuler or Sdirk2 (two backward Euler stages by the recipe of
+  // src/StepRecipe.cpp:25-32), in place on the state. `start` (device
+  // spectral, or null) is the warm start of Converge (:1916-1921); a
+  // multi-stage step takes none.
   StepInfo Step(DeviceState& state, double dt, int scheme, const LoadFn& load,
                 const double* start = nullptr);
   // The last pass's solutions on every dof, unscaled with channels, per
@@ -184,6 +186,10 @@ class Viscoelastic3D {
   void Pass(const double* s, double* sNext, bool useStartForce, Flavour& f);
 
  private:
+  StepInfo StepOnce(DeviceState& state, double dt, int scheme, const LoadFn& load,
+                    const double* start);
+  StepInfo StepSdirk2(DeviceState& state, double dt, const LoadFn& load);
+  void CopyState(const DeviceState& from, DeviceState& to) const;
   void BuildTasks();
   void BuildResponse(Flavour& f);
   void SolveAll(const double* weighted, Flavour& f);
@@ -217,6 +223,7 @@ class Viscoelastic3D {
   DeviceArray<double> fourierA_[3], fourierB_[3];
   DeviceArray<double> partials_, result_;
   DeviceArray<double> history_, xPrev_, fPrev_;
+  std::unique_ptr<DeviceState> stage_;  // Sdirk2's first stage
   hipEvent_t events_[4] = {};
 };
 

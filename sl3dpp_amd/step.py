@@ -20,6 +20,8 @@ c_int64_p = ctypes.POINTER(ctypes.c_int64)
 YEAR = 365.25 * 24.0 * 3600.0
 EXPONENTIAL_TRAPEZOID = 0
 BACKWARD_EULER = 1
+SDIRK2 = 2
+SDIRK2_GAMMA = 0.29289321881345247560  # 1 - 1/sqrt(2)
 
 MAX_LAYERS = 8
 
@@ -120,9 +122,9 @@ class Earth:
         self.handle = ctypes.c_void_p()
         check(lib.sl3_earth_create(ctypes.byref(self.spec), ctypes.byref(self.handle)))
 
-    def __del__(self):
+    def __del__(self, _lib=lib):  # the default keeps the library through shutdown
         if getattr(self, "handle", None):
-            lib.sl3_earth_destroy(self.handle)
+            _lib.sl3_earth_destroy(self.handle)
             self.handle = None
 
     def scales(self):
@@ -233,9 +235,9 @@ class State:
         self.handle = ctypes.c_void_p()
         check(lib.sl3_state_zero(op.handle, ctypes.byref(self.handle)))
 
-    def __del__(self):
+    def __del__(self, _lib=lib):  # the default keeps the library through shutdown
         if getattr(self, "handle", None):
-            lib.sl3_state_destroy(self.handle)
+            _lib.sl3_state_destroy(self.handle)
             self.handle = None
 
     @property
@@ -283,9 +285,9 @@ class Viscoelastic3D:
                                         self.theta.ctypes.data_as(c_double_p),
                                         self.wtheta.ctypes.data_as(c_double_p)))
 
-    def __del__(self):
+    def __del__(self, _lib=lib):  # the default keeps the library through shutdown
         if getattr(self, "handle", None):
-            lib.sl3_operator_destroy(self.handle)
+            _lib.sl3_operator_destroy(self.handle)
             self.handle = None
 
     def set_options(self, **overrides):

@@ -9,7 +9,9 @@ the lateral field, the load):
  * whole converged steps with lateral viscosity, adaptive Anderson mixing
    included: the residual history pass by pass, and the state to 1e-8
    relative at rtol 1e-10;
- * the zero-lateral limit closing in two passes."""
+ * the zero-lateral limit closing in two passes;
+ * Sdirk2 steps against the oracle's two backward Euler stages combined by
+   the reference's recipe."""
 
 import ctypes
 
@@ -151,3 +153,44 @@ def test_zero_lateral_step_closes_in_two_passes(earth, oracle):
         assert info.converged == 1
         assert info.iterations <= 3
         assert rel(c, rc) < 1e-11
+
+
+def oracle_sdirk2(ref, load, lmax, tn, dt, **kw):
+    """Sdirk2 by its recipe on the oracle (src/StepRecipe.cpp:25-32,
+    include/sl3dpp/StepRecipe.hpp:164-192, src/Viscoelastic3D.cpp:463-492):
+    BE(gamma dt) to Y1; m <- (1 - r) m_n + r m(Y1) with Y1's strain and
+    coefficients; BE(gamma dt) again. ref is built for BE at gamma dt."""
+    g = st.SDIRK2_GAMMA
+    r = (1.0 - g) / g
+    m0 = [m.copy() for m in ref.m]
+    first = ref.step(g * dt, st.BACKWARD_EULER, amplitudes(load, lmax, tn + g * dt), **kw)
+    for b in range(3):
+        ref.m[b][:] = (1.0 - r) * m0[b] + r * ref.m[b]
+    ref.state.t = tn + (1.0 - g) * dt
+    second = ref.step(g * dt, st.BACKWARD_EULER, amplitudes(load, lmax, tn + dt), **kw)
+    return first, second
+
+
+def test_sdirk2_steps_match_oracle(earth, oracle):
+    lmax = 12
+    L, T, M = earth.scales()
+    dt = 100 * st.YEAR / T
+    kw = dict(max_iterations=80, rtol=1e-11, anderson_memory=6, anderson_rate=0.5)
+    opts = st.StepOptions.default(**kw)
+    op = make_operator(earth, lmax, 0.7, opts)
+    itau = op.export("itau")
+    ref = OracleStep(oracle, earth, lmax, st.BACKWARD_EULER, st.SDIRK2_GAMMA * dt, itau,
+                     theta=op.theta, wtheta=op.wtheta)
+    load, _ = st.zonal_load_rate(lmax, earth)
+    state = op.zero_state()
+    for k in range(2):
+        print(f"[step] sdirk2 lmax {lmax} step {k}", flush=True)
+        info = op.step(state, dt, st.SDIRK2, load)
+        first, second = oracle_sdirk2(ref, load, lmax, k * dt, dt, **kw)
+        assert info.converged == 1 and first.converged == 1 and second.converged == 1
+        assert abs(info.iterations - (first.iterations + second.iterations)) <= 2
+        assert abs(state.t - (k + 1) * dt) <= 1e-12 * dt * (k + 1)
+        assert rel(state.export("coefficients"), ref.coef) < 1e-8
+        for b in range(3):
+            assert rel(state.export(f"m{b}"), ref.m[b]) < 1e-8
+            assert rel(state.export(f"strain{b}"), ref.strain[b]) < 1e-8
