@@ -29,6 +29,16 @@ sys.path.insert(0, ROOT)
 METRIC = "3D lateral-viscosity steps/s at lmax128, tol 1e-8: learned+batched vs plain"
 PEAK_HBM = 8.0e12       # B/s
 PEAK_FP64 = 78.6e12     # flop/s, MI355X dense FP64 (vector and matrix)
+# HBM bytes per launch (read + written), measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE on this build at lmax 128, scaled by a 1 GiB device copy (FETCH_SIZE
+# x 2.0, WRITE_SIZE x 1.0); profiles/step_lmax128_hbm_traffic.md. Only valid for
+# the default workload.
+MEASURED_TRAFFIC = {
+    "ring": 9.247e8,
+    "response": 6.483e8,
+    "legendre_analysis": 1.471e9,
+    "legendre_synthesis": 1.255e9,
+}
 
 
 def parse():
@@ -40,6 +50,8 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run device copies of 1 GiB (the HBM byte counters' scale)")
     return ap.parse_args()
 
 
@@ -47,39 +59,40 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def work_models(op, lmax):
-    """Algorithmic work per launch of the step's kernels (see DESIGN.md):
-    flops for the transforms (only the coefficients the degree truncation
-    keeps), HBM bytes for the vector sweeps."""
-    import numpy as np
-
-    tasks = op.export("tasks").reshape(-1, 4)
+def work_models(op, lmax, passes):
+    """Algorithmic work per launch of the step's kernels (DESIGN.md section 4):
+    flops for the matrix-core and FMA kernels, HBM bytes for the sweeps.
+    passes: the timed steps' pass counts (the ring kernel's launches mix
+    passes, step starts and finishes)."""
+    tasks = op.export("tasks").reshape(-1, 5)
     first_node = {int(t[0]): int(t[3]) for t in tasks if int(t[1]) == 0}
     nodes, nt, nph = op.nodes, op.ntheta, op.nphi
-    active = 0  # (block, m, l, node) coefficients kept
-    for b, spin in enumerate([-2, -1, 0]):
+    active = 0  # (block, m, l, node) coefficients kept by the degree truncation
+    for spin in (-2, -1, 0):
         mlo = 0 if spin == 0 else -lmax
         for m in range(mlo, lmax + 1):
             for l in range(max(abs(m), abs(spin)), lmax + 1):
                 active += nodes - first_node[l]
-    legendre_flops = 4.0 * nt * active  # complex coefficient x real table, 2 flops each
+    legendre_flops = 4.0 * nt * active  # real table x complex coefficient, FMA = 2
+    # longitude transforms: R sequences of P points per ring (spin 0: two
+    # real rings per sequence); a direction costs 8 H^2 flops per sequence
     R = 4 if nph % 4 == 0 else 2
     P = nph // R
     H = (P - 1) // 2
-    orders = [2 * lmax + 1, 2 * lmax + 1, lmax + 1]
-    ring_flops = 0.0
-    for b in range(3):
-        per_dir = R * (12.0 * H * H) + R * orders[b] * 8.0
-        ring_flops += nodes * nt * (2 * per_dir + 2.0 * nph)
-    ring_flops /= 3.0  # per launch (one block per launch)
+    seq = nodes * nt * R * (1.0 + 1.0 + 0.5)  # the three blocks
+    directions = sum(2 * k + 3 for k in passes)  # pass: 2; start: 2; finish: 1
+    launches = sum(3 * (k + 2) for k in passes)
+    ring_flops = seq * 8.0 * H * H * directions / launches
+    # the memory response: x = G f per degree task
+    response_flops = sum(2.0 * t[4] * t[4] * 2 * (t[0] + 1) for t in tasks)
     S = op.spectral_doubles * 8.0
     return {
         "legendre_synthesis": ("mfma", legendre_flops),
         "legendre_analysis": ("mfma", legendre_flops),
         "ring": ("mfma", ring_flops),
-        # the mix sweep: g and history (m = 6) read, x written
-        "anderson": ("hbm", None),
+        "response": ("mfma", response_flops),
         "residual": ("hbm", 2.0 * S),
+        "anderson": ("hbm", None),
     }
 
 
@@ -138,6 +151,15 @@ def main():
     import sl3dpp_amd.step as st
 
     assert torch.cuda.is_available(), "bench.py needs the HIP device"
+    if args.copy_calibration:
+        # a plain copy of known size: 1 GiB read and 1 GiB written per launch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 copies of 1 GiB")
+        return 0
     lmax = args.lmax
     earth = st.Earth(st.three_region_earth(70, 56, 2, 6))
     length, tscale, mass = earth.scales()
@@ -181,23 +203,24 @@ def main():
     steps_per_s = args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     # dominant kernel and its roofline
-    models = work_models(op, lmax)
+    models = work_models(op, lmax, passes)
     top = max(kernels.items(), key=lambda kv: kv[1][0])
     name, (total_ms, launches) = top
     roofline = None
     bound, work = models.get(name, ("hbm", None))
+    traffic = MEASURED_TRAFFIC.get(name) if lmax == 128 else None
     avg_s = total_ms / max(launches, 1) / 1e3
     if work is not None and avg_s > 0:
         achieved = work / avg_s
         if bound == "mfma":
             roofline = {"bound": "mfma", "kernel": name, "achieved": achieved / 1e12,
                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                        "frac": achieved / PEAK_FP64, "traffic": None,
+                        "frac": achieved / PEAK_FP64, "traffic": traffic,
                         "work_per_launch": work, "avg_launch_ms": avg_s * 1e3}
         else:
             roofline = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9,
                         "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                        "frac": achieved / PEAK_HBM, "traffic": None,
+                        "frac": achieved / PEAK_HBM, "traffic": traffic,
                         "work_per_launch": work, "avg_launch_ms": avg_s * 1e3}
     breakdown = {k: {"total_ms": round(v[0], 3), "launches": v[1]} for k, v in kernels.items()}
     cpu = None

@@ -223,7 +223,8 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme,
  *   operator: "weight" "beta" "itau" (grid fields), "fixed" "analysed"
  *             (device layout spectral), "solution:<task>" (the last pass's
  *             solution of a task, ndof x (l+1) complex column-major),
- *             "tasks" (per task l, kind, ndof), "residuals" (last step).
+ *             "tasks" (per task: l, kind, ndof, first memory node, memory
+ *             dofs), "residuals" (last step).
  * dst NULL returns the size in doubles through *count. */
 int sl3_state_export(const sl3_operator* op, const sl3_state* state,
                      const char* name, double* dst, int64_t* count);

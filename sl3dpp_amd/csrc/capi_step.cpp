@@ -413,6 +413,7 @@ int sl3_operator_export(sl3_operator* op, const char* name, double* dst, int64_t
         t.push_back(task.kind);
         t.push_back(task.ndof);
         t.push_back(task.firstNode);
+        t.push_back(task.nm);
       }
       return ExportHost(t, dst, count);
     }
