@@ -31,6 +31,7 @@ namespace {
 constexpr int kTile = 64;
 constexpr int kK = 16;
 constexpr int kThreads = 256;
+constexpr int kPer = kK * kTile / kThreads;  // staged elements per thread
 
 // z index of the launch -> (block, m)
 struct OrderRef {
@@ -80,19 +81,34 @@ __global__ void __launch_bounds__(kThreads) LegendreSynthesisKernel(LegendreArgs
   const double* S = a.spectral[b];
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   double acc[4][4] = {};
-  for (int k0 = lmin; k0 <= lhi; k0 += kK) {
-    // A tile: As[k][theta] = T[(row0 + k0 + k) * ntheta + t0 + theta]
-    for (int e = threadIdx.x; e < kK * kTile; e += kThreads) {
+  // Each thread stages kPer elements of A and of B per K chunk; the next
+  // chunk's loads are in flight while this one is multiplied. Loads are
+  // unconditional from clamped indices (they issue back to back), the
+  // padding is zeroed by selects.
+  double pa[kPer], pb[kPer];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = threadIdx.x + kThreads * q;
       const int k = e / kTile, t = e % kTile;
-      const int l = k0 + k, th = t0 + t;
-      As[k][t] = (l <= lhi && th < a.ntheta)
-                     ? T[static_cast<size_t>(row0 + l) * a.ntheta + th]
-                     : 0.0;
-      const int c = c0 + t;
-      Bs[k][t] = (l <= lhi && c < cols) ? S[static_cast<size_t>(row0 + l) * cols + c]
-                                        : 0.0;
+      const int l = k0 + k, th = t0 + t, c = c0 + t;
+      const int lc = min(l, lhi);
+      const double va = T[static_cast<size_t>(row0 + lc) * a.ntheta + min(th, a.ntheta - 1)];
+      const double vb = S[static_cast<size_t>(row0 + lc) * cols + min(c, cols - 1)];
+      pa[q] = (l <= lhi && th < a.ntheta) ? va : 0.0;
+      pb[q] = (l <= lhi && c < cols) ? vb : 0.0;
+    }
+  };
+  load(lmin);
+  for (int k0 = lmin; k0 <= lhi; k0 += kK) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = threadIdx.x + kThreads * q;
+      As[e / kTile][e % kTile] = pa[q];
+      Bs[e / kTile][e % kTile] = pb[q];
     }
     __syncthreads();
+    if (k0 + kK <= lhi) load(k0 + kK);
 #pragma unroll
     for (int k = 0; k < kK; ++k) {
       double av[4], bv[4];
