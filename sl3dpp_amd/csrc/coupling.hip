@@ -378,43 +378,70 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0) out[c] = red[0];
 }
 
-__global__ void AndersonFirstKernel(double* x, const double* gx, double* xPrev,
-                                    double* fPrev, size_t n) {
-  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += stride) {
-    xPrev[i] = x[i];
-    fPrev[i] = gx[i] - x[i];
-    x[i] = gx[i];
+// The run of compact index c (compact[r] <= c < compact[r + 1]).
+__device__ inline int FindRun(const ActiveRuns& a, long long c) {
+  int lo = 0, hi = a.runs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (a.compact[mid] <= c) {
+      lo = mid;
+    } else {
+      hi = mid;
+    }
+  }
+  return lo;
+}
+
+// Walks the compact range [lo, hi) with stride blockDim.x, calling
+// body(c, j) with j the entry's device-layout index; the run advances
+// monotonically from one binary search.
+template <class Body>
+__device__ inline void ForActive(const ActiveRuns& a, long long lo, long long hi, Body&& body) {
+  long long c = lo + threadIdx.x;
+  if (c >= hi) return;
+  int run = FindRun(a, c);
+  for (; c < hi; c += blockDim.x) {
+    while (c >= a.compact[run + 1]) ++run;
+    body(c, a.full[run] + (c - a.compact[run]));
   }
 }
 
+__global__ void AndersonFirstKernel(const double* x, const double* gx, double* xPrev,
+                                    double* fPrev, ActiveRuns a) {
+  size_t lo, hi;
+  BlockRange(static_cast<size_t>(a.total), lo, hi);
+  ForActive(a, lo, hi, [&](long long c, long long j) {
+    xPrev[c] = x[j];
+    fPrev[c] = gx[j] - x[j];
+  });
+}
+
 __global__ void __launch_bounds__(kThreads)
-    AndersonSweepKernel(AndersonSweep s, size_t n, double* partials) {
+    AndersonSweepKernel(AndersonSweep s, ActiveRuns a, double* partials) {
   __shared__ double red[kThreads][17];
   size_t lo, hi;
-  BlockRange(n, lo, hi);
+  BlockRange(static_cast<size_t>(a.total), lo, hi);
   double col[8] = {}, gam[8] = {};
   double* dFs = s.dF[s.slot];
   double* dGs = s.dG[s.slot];
-  for (size_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+  ForActive(a, lo, hi, [&](long long c, long long j) {
     const double xs = s.x[j];
     const double f = s.gx[j] - xs;
-    const double dx = xs - s.xPrev[j];
-    const double df = f - s.fPrev[j];
-    dFs[j] = df;
-    dGs[j] = dx + df;
-    s.xPrev[j] = xs;
-    s.fPrev[j] = f;
+    const double dx = xs - s.xPrev[c];
+    const double df = f - s.fPrev[c];
+    dFs[c] = df;
+    dGs[c] = dx + df;
+    s.xPrev[c] = xs;
+    s.fPrev[c] = f;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       if (i < s.m) {
-        const double di = i == s.slot ? df : s.dF[i][j];
+        const double di = i == s.slot ? df : s.dF[i][c];
         col[i] += di * df;
         gam[i] += di * f;
       }
     }
-  }
+  });
   for (int i = 0; i < 8; ++i) {
     red[threadIdx.x][i] = col[i];
     red[threadIdx.x][8 + i] = gam[i];
@@ -433,14 +460,14 @@ struct Gamma {
   double v[8];
 };
 
-__global__ void AndersonMixKernel(AndersonSweep s, Gamma gamma, size_t n) {
-  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  for (size_t j = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
-       j += stride) {
+__global__ void AndersonMixKernel(AndersonSweep s, Gamma gamma, ActiveRuns a) {
+  size_t lo, hi;
+  BlockRange(static_cast<size_t>(a.total), lo, hi);
+  ForActive(a, lo, hi, [&](long long c, long long j) {
     double value = s.gx[j];
-    for (int i = 0; i < s.m; ++i) value -= gamma.v[i] * s.dG[i][j];
+    for (int i = 0; i < s.m; ++i) value -= gamma.v[i] * s.dG[i][c];
     s.x[j] = value;
-  }
+  });
 }
 
 __device__ inline void RefIndexToLm(int spin, long long idx, int& l, int& m) {
@@ -553,31 +580,32 @@ void LaunchRelativeChange(const double* next, const double* now, size_t n,
 }
 
 void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPrev,
-                         size_t n, hipStream_t stream) {
-  ProfileScope prof529("anderson", stream);
-  hipLaunchKernelGGL(AndersonFirstKernel, dim3(GridFor(n)), dim3(kThreads), 0,
-                     stream, x, gx, xPrev, fPrev, n);
+                         const ActiveRuns& runs, size_t n, hipStream_t stream) {
+  ProfileScope prof("anderson", stream);
+  hipLaunchKernelGGL(AndersonFirstKernel, dim3(1024), dim3(kThreads), 0, stream, x, gx,
+                     xPrev, fPrev, runs);
+  SL3_HIP(hipGetLastError());
+  hipLaunchKernelGGL(CopyKernel, dim3(GridFor(n)), dim3(kThreads), 0, stream, x, gx, n);
   SL3_HIP(hipGetLastError());
 }
 
-void LaunchAndersonSweep(const AndersonSweep& s, size_t n, double* partials,
+void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double* partials,
                          int partialBlocks, double* sums, hipStream_t stream) {
-  ProfileScope prof537("anderson", stream);
+  ProfileScope prof("anderson", stream);
   hipLaunchKernelGGL(AndersonSweepKernel, dim3(partialBlocks), dim3(kThreads), 0,
-                     stream, s, n, partials);
+                     stream, s, runs, partials);
   SL3_HIP(hipGetLastError());
   hipLaunchKernelGGL(ReducePartialsKernel, dim3(16), dim3(kThreads), 0, stream, partials,
                      partialBlocks, 16, 0, sums);
   SL3_HIP(hipGetLastError());
 }
 
-void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, size_t n,
+void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,
                        hipStream_t stream) {
   Gamma g{};
   for (int i = 0; i < s.m && i < 8; ++i) g.v[i] = gamma[i];
-  ProfileScope prof550("anderson", stream);
-  hipLaunchKernelGGL(AndersonMixKernel, dim3(GridFor(n)), dim3(kThreads), 0, stream,
-                     s, g, n);
+  ProfileScope prof("anderson", stream);
+  hipLaunchKernelGGL(AndersonMixKernel, dim3(2048), dim3(kThreads), 0, stream, s, g, runs);
   SL3_HIP(hipGetLastError());
 }
 

@@ -82,7 +82,20 @@ void LaunchRelativeChange(const double* next, const double* now, size_t n,
                           double* partials, int partialBlocks, double* result,
                           hipStream_t stream);
 
-// Anderson (AndersonMixer.hpp:123-218) on a flat vector of n doubles.
+// The active entries of a spectral tensor: per row (all blocks, in memory
+// order) the run of nodes at or above the row degree's first memory node;
+// the others are exact zeros of every iterate. full[r]: the run's first
+// double in the device layout; compact[r]: its offset in the compact
+// vector (runs + 1 entries, the last the total).
+struct ActiveRuns {
+  const long long* full = nullptr;
+  const long long* compact = nullptr;
+  int runs = 0;
+  long long total = 0;
+};
+
+// Anderson (AndersonMixer.hpp:123-218) over the active entries: x and gx
+// in the device layout, the history (xPrev, fPrev, dF, dG) compact.
 struct AndersonSweep {
   double* x;            // in: iterate; (second sweep) out: mixed
   const double* gx;     // the pass's image
@@ -93,11 +106,14 @@ struct AndersonSweep {
   int m;                // active history columns
   int slot;             // the slot being written
 };
+// The first mixed pass: xPrev = x, fPrev = gx - x (compact), then x = gx
+// (every entry, n doubles).
 void LaunchAndersonFirst(double* x, const double* gx, double* xPrev,
-                         double* fPrev, size_t n, hipStream_t stream);
-void LaunchAndersonSweep(const AndersonSweep& s, size_t n, double* partials,
+                         double* fPrev, const ActiveRuns& runs, size_t n,
+                         hipStream_t stream);
+void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double* partials,
                          int partialBlocks, double* sums, hipStream_t stream);
-void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, size_t n,
+void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,
                        hipStream_t stream);
 
 // Reference layout (per node, l-major, SurfaceGrid::SpinIndex / Index)

@@ -226,6 +226,34 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
   dft_.sinP = sinP_.data();
   dft_.cosN = cosN_.data();
   dft_.sinN = sinN_.data();
+  // the active runs of a spectral tensor: per row, in memory order, the
+  // nodes at or above the row degree's first memory node
+  {
+    std::vector<long long> full, compact;
+    long long total = 0;
+    const int nodes = geometry_.nodes;
+    for (int b = 0; b < 3; ++b) {
+      const auto& lay = grid_.Layout(b);
+      for (int m = lay.mlo; m <= lmax; ++m) {
+        for (int l = lay.LMin(m); l <= lmax; ++l) {
+          const int row = lay.Row(l, m);
+          const int fn = std::min(firstNodeHost_[l], nodes);
+          full.push_back(static_cast<long long>(spectralOffset_[b]) +
+                         2LL * (static_cast<long long>(row) * nodes + fn));
+          compact.push_back(total);
+          total += 2LL * (nodes - fn);
+        }
+      }
+    }
+    compact.push_back(total);
+    runFull_.Upload(full.data(), full.size(), s);
+    runCompact_.Upload(compact.data(), compact.size(), s);
+    runs_.full = runFull_.data();
+    runs_.compact = runCompact_.data();
+    runs_.runs = static_cast<int>(full.size());
+    runs_.total = total;
+    SL3_HIP(hipStreamSynchronize(s));  // the host tables are freed next
+  }
   // the matrix-core operand tables of the P-point transforms
   dft_.jt = RingMfmaTiles(dft_.P);
   if (dft_.jt > 0) {
@@ -1030,10 +1058,12 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
   bool mixerStarted = false;
   int count = 0, slot = 0;
   std::vector<double> gram(static_cast<size_t>(std::max(memory, 1)) * std::max(memory, 1), 0.0);
-  if (memory > 0 && history_.size() != 2 * static_cast<size_t>(memory) * S) {
-    history_.Resize(2 * static_cast<size_t>(memory) * S);
-    xPrev_.Resize(S);
-    fPrev_.Resize(S);
+  const ActiveRuns& runs = coupling_.Runs();
+  const size_t C = static_cast<size_t>(runs.total);  // the compact history's length
+  if (memory > 0 && history_.size() != 2 * static_cast<size_t>(memory) * C) {
+    history_.Resize(2 * static_cast<size_t>(memory) * C);
+    xPrev_.Resize(C);
+    fPrev_.Resize(C);
   }
   double previous = 0.0;
   for (int it = 0; it < options_.maxIterations; ++it) {
@@ -1061,7 +1091,8 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
     previous = residual;
     if (mixing) {
       if (!mixerStarted) {
-        LaunchAndersonFirst(s_.data(), sNext_.data(), xPrev_.data(), fPrev_.data(), S, s);
+        LaunchAndersonFirst(s_.data(), sNext_.data(), xPrev_.data(), fPrev_.data(), runs, S,
+                            s);
         mixerStarted = true;
       } else {
         const int m = std::min(count + 1, memory);
@@ -1071,12 +1102,12 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
         sw.xPrev = xPrev_.data();
         sw.fPrev = fPrev_.data();
         for (int i = 0; i < memory && i < 8; ++i) {
-          sw.dF[i] = history_.data() + static_cast<size_t>(i) * S;
-          sw.dG[i] = history_.data() + static_cast<size_t>(memory + i) * S;
+          sw.dF[i] = history_.data() + static_cast<size_t>(i) * C;
+          sw.dG[i] = history_.data() + static_cast<size_t>(memory + i) * C;
         }
         sw.m = m;
         sw.slot = slot;
-        LaunchAndersonSweep(sw, S, partials_.data(), 1024, result_.data(), s);
+        LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(), s);
         double sums[16];
         SL3_HIP(hipMemcpyAsync(sums, result_.data(), sizeof(sums), hipMemcpyDeviceToHost, s));
         SL3_HIP(hipStreamSynchronize(s));
@@ -1090,7 +1121,7 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
         for (int i = 0; i < m; ++i)
           for (int j = 0; j < m; ++j) g[i * m + j] = gram[i * memory + j];
         GramLeastSquares(m, g, sums + 8, gamma);
-        LaunchAndersonMix(sw, gamma, S, s);
+        LaunchAndersonMix(sw, gamma, runs, s);
       }
       ++info.mixedPasses;
     } else {

@@ -70,6 +70,8 @@ class Coupling {
   int FirstElementOfDegree(int l) const { return firstElement_[l]; }
   SpectralView View(double* spectral) const;
   const int* RowOffsetDevice(int block) const { return rowOffset_[block].data(); }
+  // The active (row, node) runs of a spectral tensor (degree truncation).
+  const ActiveRuns& Runs() const { return runs_; }
 
   // Batched transforms of all three blocks over every node.
   void SynthesiseToFourier(const double* spectral, double* const fourier[3],
@@ -105,6 +107,8 @@ class Coupling {
   DftTables dft_;
   size_t spectralOffset_[4] = {};
   LegendrePlan plan_;
+  DeviceArray<long long> runFull_, runCompact_;
+  ActiveRuns runs_;
 };
 
 // The factorised systems of one (scheme, dt): Viscoelastic3D::Flavour
