@@ -44,6 +44,24 @@ typedef double Acc4 __attribute__((ext_vector_type(4)));
 
 __device__ inline double Shfl(double v, int mask) { return __shfl_xor(v, mask, kWave); }
 
+// Lane exchanges inside a quad (DPP quad_perm: xor 1 = [1,0,3,2], xor 2 =
+// [2,3,0,1]) and inside 32 lanes (ds_swizzle bit mode, xor 4), without an
+// address register or a round trip through LDS memory.
+template <int CTRL>
+__device__ inline double QuadPerm(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double Xor1(double v) { return QuadPerm<0xB1>(v); }
+__device__ inline double Xor2(double v) { return QuadPerm<0x4E>(v); }
+__device__ inline double Xor4(double v) {
+  constexpr int kPattern = 0x1F | (4 << 10);  // and 0x1f, or 0, xor 4
+  const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), kPattern);
+  const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), kPattern);
+  return __hiloint2double(hi, lo);
+}
+
 // The sum over the four lanes of a column (lane, lane ^ 16, ^ 32, ^ 48),
 // the same value in each of them.
 __device__ inline double ColumnSum(double v) {
@@ -129,16 +147,16 @@ __device__ inline double2 Gather(const Geometry& g, const double2* Fs, int slot,
 // x[n1 + P n2].
 __device__ inline double InverseButterfly(const Geometry& g, const Lane& ln, double y,
                                           double c, double s) {
-  const double v = c * y + (ln.p == 0 ? -s : s) * Shfl(y, 1);
+  const double v = c * y + (ln.p == 0 ? -s : s) * Xor1(y);
   if (g.R == 4) {
-    const double vp = Shfl(v, 4);
+    const double vp = Xor4(v);
     double e = ln.a == 0 ? v + vp : vp - v;
-    const double eo = Shfl(e, 1);
+    const double eo = Xor1(e);
     if (ln.a == 1 && ln.b == 1) e = ln.p == 0 ? -eo : eo;  // x i
-    const double ep = Shfl(e, 2);
+    const double ep = Xor2(e);
     return ln.b == 0 ? e + ep : ep - e;
   }
-  const double vp = Shfl(v, 2);
+  const double vp = Xor2(v);
   return ln.b == 0 ? v + vp : vp - v;
 }
 
@@ -148,17 +166,17 @@ __device__ inline double ForwardButterfly(const Geometry& g, const Lane& ln, dou
                                           double c, double s) {
   double w;
   if (g.R == 4) {
-    const double yp = Shfl(y, 2);
+    const double yp = Xor2(y);
     double d = ln.b == 0 ? y + yp : yp - y;
-    const double dn = Shfl(d, 1);
+    const double dn = Xor1(d);
     if (ln.a == 1 && ln.b == 1) d = ln.p == 0 ? dn : -dn;  // x (-i)
-    const double dp = Shfl(d, 4);
+    const double dp = Xor4(d);
     w = ln.a == 0 ? d + dp : dp - d;
   } else {
-    const double yp = Shfl(y, 2);
+    const double yp = Xor2(y);
     w = ln.b == 0 ? y + yp : yp - y;
   }
-  return c * w + (ln.p == 0 ? s : -s) * Shfl(w, 1);
+  return c * w + (ln.p == 0 ? s : -s) * Xor1(w);
 }
 
 // The P-point inverse transforms of the lane's sequence on the matrix
@@ -205,7 +223,7 @@ __device__ inline void InverseSequences(const Geometry& g, const Lane& ln, const
   for (int t = 0; t < JT; ++t) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double sx = Shfl(S[t][q], 1);
+      const double sx = Xor1(S[t][q]);
       yj[4 * t + q] = h0 + C[t][q] + sg * sx;
       yq[4 * t + q] = h0 + C[t][q] - sg * sx;
     }
@@ -257,7 +275,7 @@ __device__ inline void ForwardSequences(const Geometry& g, const Lane& ln, const
   for (int t = 0; t < JT; ++t) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double sx = Shfl(S[t][q], 1);
+      const double sx = Xor1(S[t][q]);
       const int k = 1 + 16 * t + 4 * q + ln.quad;
       if (k <= g.H) {
         put(k, u0 + C[t][q] + sg * sx);
