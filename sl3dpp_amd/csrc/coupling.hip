@@ -199,17 +199,17 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// One thread per (degree, order, memory node), node fastest.
-__global__ void __launch_bounds__(kThreads)
+// One workgroup per (degree, order), one thread per memory node.
+__global__ void __launch_bounds__(1024)
     StrainKernel(const DegreeTask* sph, const DegreeTask* tor, int lmax,
                  RadialGeometry g, const int* dofs, const double* response,
                  const double* units, const double* amp, SpectralView out) {
-  const int pair = blockIdx.y;
+  const int pair = blockIdx.x;
   int l = static_cast<int>((sqrt(8.0 * pair + 1.0) - 1.0) * 0.5);
   while (l * (l + 1) / 2 > pair) --l;
   while ((l + 1) * (l + 2) / 2 <= pair) ++l;
   const int m = pair - l * (l + 1) / 2;
-  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  const int node = threadIdx.x;
   if (node >= g.nodes) return;
   const DegreeTask ts = sph[l];
   Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
@@ -226,23 +226,23 @@ __global__ void __launch_bounds__(kThreads)
     for (int i = 0; i < g.ngll; ++i) {
       const int* d = dofs + ts.dofOff + ((elem - g.elem0) * g.ngll + i) * 2;
       Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
-      const double* bs = response + ts.fOff;
+      // the response columns m (Re) and l + 1 + m (Im), column-major
+      const double* re = response + ts.fOff + static_cast<size_t>(m) * ts.nmPad;
+      const double* im = response + ts.fOff + static_cast<size_t>(l + 1 + m) * ts.nmPad;
       if (d[0] >= 0) {
-        const size_t at = static_cast<size_t>(d[0]) * ts.colsPad;
-        Ui = C(bs[at + m], bs[at + l + 1 + m]);
+        Ui = C(re[d[0]], im[d[0]]);
         if (ts.unitOff >= 0) Ui = Ui + units[ts.dmOff + d[0]] * a;
       }
       if (d[1] >= 0) {
-        const size_t at = static_cast<size_t>(d[1]) * ts.colsPad;
-        Vi = C(bs[at + m], bs[at + l + 1 + m]);
+        Vi = C(re[d[1]], im[d[1]]);
         if (ts.unitOff >= 0) Vi = Vi + units[ts.dmOff + d[1]] * a;
       }
       if (tor1) {
         const int dw = dofs[tt.dofOff + ((elem - g.elem0) * g.ngll + i) * 2];
         if (dw >= 0) {
-          const size_t at = static_cast<size_t>(dw) * tt.colsPad;
           const double* bt = response + tt.fOff;
-          Wi = C(bt[at + m], bt[at + l + 1 + m]);
+          Wi = C(bt[static_cast<size_t>(m) * tt.nmPad + dw],
+                 bt[static_cast<size_t>(l + 1 + m) * tt.nmPad + dw]);
         }
       }
       const double v = g.value[q * g.ngll + i];
@@ -560,9 +560,10 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
                   const double* response, const double* units, const double* amp,
                   SpectralView out, hipStream_t stream) {
   const int pairs = (lmax + 1) * (lmax + 2) / 2;
-  const dim3 grid(CeilDiv(g.nodes, kThreads), pairs);
+  if (g.nodes > 1024) throw ArgumentError("at most 1024 memory nodes");
+  const int threads = 64 * CeilDiv(g.nodes, 64);
   ProfileScope prof509("strain", stream);
-  hipLaunchKernelGGL(StrainKernel, grid, dim3(kThreads), 0, stream, sphTasks_dev,
+  hipLaunchKernelGGL(StrainKernel, dim3(pairs), dim3(threads), 0, stream, sphTasks_dev,
                      torTasks_dev, lmax, g, dofs, response, units, amp, out);
   SL3_HIP(hipGetLastError());
 }

@@ -57,7 +57,9 @@ __global__ void __launch_bounds__(kThreads)
 
 // One wave: a 32 x 64 block of one task's x (2 x 4 matrix-core tiles),
 // accumulated over the whole contraction; blocks past the padded edge are
-// skipped (wave-uniform branches).
+// skipped (wave-uniform branches). x is written column by column (order
+// major), the layout strain recovery reads with neighbouring nodes in
+// neighbouring lanes.
 __global__ void __launch_bounds__(kThreads)
     ResponseKernel(const DegreeTask* tasks, const ResponseWork* work, int count,
                    const double* g, const double* f, double* x) {
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(kThreads)
       for (int j = 0; j < 4; ++j) {
         const long long row = 16 * (w.rb + r) + 4 * j + lane / 16;
         const long long col = 16 * (w.cb + c) + lane % 16;
-        x[t.fOff + row * ld + col] = acc[r][c][j];
+        x[t.fOff + col * t.nmPad + row] = acc[r][c][j];  // column-major
       }
     }
   }

@@ -41,7 +41,8 @@ void LaunchPackResponse(const DegreeTask* tasks_dev, int taskCount, int nmPadMax
                         const int* dm, const double* scale, const long long* zOff,
                         const double* z, double* g, hipStream_t stream);
 
-// x = G f for every task (f, x: nmPad x colsPad row-major at fOff).
+// x = G f for every task (f: nmPad x colsPad row-major at fOff; x the
+// same size at fOff, column-major: x[col][row]).
 void LaunchResponse(const DegreeTask* tasks_dev, const ResponseWork* work_dev,
                     int workCount, const double* g, const double* f, double* x,
                     hipStream_t stream);
