@@ -13,6 +13,8 @@
 //    fixed block order so that results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "common.hpp"
 #include "coupling.hpp"
 
@@ -93,21 +95,26 @@ __device__ inline void ForceRows(int l, const Cplx A[5], double r, Cplx rows[6])
 
 constexpr int kChunkElements = 4;
 constexpr int kMaxPoints = 6;  // memory points per element the LDS holds
-constexpr int kChunkOrders = 16;
+constexpr int kChunkOrders = 8;
 
-// One workgroup: one task, a chunk of memory elements, every order m in
-// chunks; the rows of every point of the chunk (and of the element below,
-// whose top node the chunk's bottom node shares) staged in LDS.
+// One workgroup: a chunk of memory elements, one degree l and kChunkOrders
+// orders, for the spheroidal and the toroidal system together (one read of
+// the analysed tensor). Stage 1 forms the force rows of every point of the
+// chunk (and of the element below, whose top node the chunk's bottom node
+// shares) in LDS; stage 2 sums each owned GLL node's rows.
 __global__ void __launch_bounds__(kThreads)
-    ForceKernel(const DegreeTask* tasks, RadialGeometry g, SpectralView w,
-                const int* dofs, double* force) {
+    ForceKernel(const DegreeTask* tasks, RadialGeometry g, SpectralView w, const int* dofs,
+                const int2* work, double* force) {
   constexpr int maxPoints = (kChunkElements + 1) * kMaxPoints;
-  __shared__ Cplx rowsL[maxPoints * kChunkOrders][4];
-  const DegreeTask t = tasks[blockIdx.y];
-  const int l = t.l;
-  const bool tor = t.kind == 1;
+  __shared__ Cplx rowsL[maxPoints * kChunkOrders][6];
+  const int2 wk = work[blockIdx.y];
+  const int l = wk.x, mc = wk.y;
+  const int L = w.lmax;
+  const DegreeTask ts = tasks[l];
+  const bool hasTor = l >= 1;
+  const DegreeTask tt = hasTor ? tasks[L + l] : ts;  // toroidal l at L + 1 + (l - 1)
   const int elemEnd = g.elem0 + g.elements;
-  const int eLo = max(g.elem0, t.firstElement);
+  const int eLo = max(g.elem0, ts.firstElement);
   const int e0 = g.elem0 + blockIdx.x * kChunkElements;
   const int e1 = min(e0 + kChunkElements, elemEnd);
   if (e1 <= eLo) return;
@@ -119,83 +126,74 @@ __global__ void __launch_bounds__(kThreads)
   const int gnFirst = max(e0, eLo) * nm;
   const int gnEnd = e1 * nm + (e1 == elemEnd ? 1 : 0);
   const int owned = gnEnd - gnFirst;
-  const int* dofTable = dofs + t.dofOff;
-  double* b = force + t.fOff;
-  const size_t ld = t.colsPad;
-  for (int mc = 0; mc <= l; mc += kChunkOrders) {
-    const int mCount = min(kChunkOrders, l + 1 - mc);
-    // stage 1: rows of (point, m)
-    for (int e = threadIdx.x; e < points * mCount; e += blockDim.x) {
-      const int p = e % points, mi = e / points;
-      const int elem = eStart + p / g.np, q = p % g.np;
-      const int node = (elem - g.elem0) * g.np + q;
-      Cplx A[5], rows[6];
-      Gather(w, l, mc + mi, node, g.nodes, A);
-      ForceRows(l, A, g.radius[node], rows);
-      Cplx* dst = rowsL[mi * maxPoints + p];
-      if (tor) {
-        dst[0] = rows[2];
-        dst[1] = rows[5];
-      } else {
-        dst[0] = rows[0];
-        dst[1] = rows[1];
-        dst[2] = rows[3];
-        dst[3] = rows[4];
+  const int mCount = min(kChunkOrders, l + 1 - mc);
+  // stage 1: rows of (point, m)
+  for (int e = threadIdx.x; e < points * mCount; e += blockDim.x) {
+    const int p = e % points, mi = e / points;
+    const int elem = eStart + p / g.np, q = p % g.np;
+    const int node = (elem - g.elem0) * g.np + q;
+    Cplx A[5], rows[6];
+    Gather(w, l, mc + mi, node, g.nodes, A);
+    ForceRows(l, A, g.radius[node], rows);
+    Cplx* dst = rowsL[mi * maxPoints + p];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst[k] = rows[k];
+  }
+  __syncthreads();
+  // stage 2: each owned node's sums, node fastest
+  double* bs = force + ts.fOff;
+  double* bt = force + tt.fOff;
+  const size_t lds = ts.colsPad, ldt = tt.colsPad;
+  for (int e = threadIdx.x; e < owned * mCount; e += blockDim.x) {
+    const int gl = e % owned, mi = e / owned;
+    const int gn = gnFirst + gl;
+    const int m = mc + mi;
+    int elems[2], locals[2], count = 0;
+    const int ea = gn / nm, ia = gn % nm;
+    if (ia == 0 && ea - 1 >= eStart && ea - 1 < e1) {
+      elems[count] = ea - 1;
+      locals[count++] = nm;
+    }
+    if (ea >= eStart && ea < e1) {
+      elems[count] = ea;
+      locals[count++] = ia;
+    }
+    Cplx U = C(0.0, 0.0), V = C(0.0, 0.0), W = C(0.0, 0.0);
+    for (int k = 0; k < count; ++k) {
+      const int elem = elems[k], i = locals[k];
+      const double jac = g.jac[elem - g.elem0];
+      for (int q = 0; q < g.np; ++q) {
+        const int p = (elem - eStart) * g.np + q;
+        const int node = (elem - g.elem0) * g.np + q;
+        const double r = g.radius[node];
+        const double point = jac * g.weight[q] * r;
+        const double v = point * g.value[q * g.ngll + i];
+        const double deriv = g.weight[q] * g.deriv[q * g.ngll + i] * r;
+        const Cplx* rw = rowsL[mi * maxPoints + p];
+        U = U + v * rw[0] + deriv * rw[3];
+        V = V + v * rw[1] + deriv * rw[4];
+        W = W + v * rw[2] + deriv * rw[5];
       }
     }
-    __syncthreads();
-    // stage 2: each owned node's sums, m fastest
-    for (int e = threadIdx.x; e < owned * mCount; e += blockDim.x) {
-      const int mi = e % mCount, gl = e / mCount;
-      const int gn = gnFirst + gl;
-      const int m = mc + mi;
-      int elems[2], locals[2], count = 0;
-      const int ea = gn / nm, ia = gn % nm;
-      if (ia == 0 && ea - 1 >= eStart && ea - 1 < e1) {
-        elems[count] = ea - 1;
-        locals[count++] = nm;
-      }
-      if (ea >= eStart && ea < e1) {
-        elems[count] = ea;
-        locals[count++] = ia;
-      }
-      Cplx U = C(0.0, 0.0), V = C(0.0, 0.0);
-      for (int k = 0; k < count; ++k) {
-        const int elem = elems[k], i = locals[k];
-        const double jac = g.jac[elem - g.elem0];
-        for (int q = 0; q < g.np; ++q) {
-          const int p = (elem - eStart) * g.np + q;
-          const int node = (elem - g.elem0) * g.np + q;
-          const double r = g.radius[node];
-          const double point = jac * g.weight[q] * r;
-          const double v = point * g.value[q * g.ngll + i];
-          const double deriv = g.weight[q] * g.deriv[q * g.ngll + i] * r;
-          const Cplx* rw = rowsL[mi * maxPoints + p];
-          if (tor) {
-            U = U + v * rw[0];
-            U = U + deriv * rw[1];
-          } else {
-            U = U + v * rw[0];
-            V = V + v * rw[1];
-            U = U + deriv * rw[2];
-            V = V + deriv * rw[3];
-          }
-        }
-      }
-      const int elem = count > 0 ? elems[count - 1] : ea;
-      const int i = count > 0 ? locals[count - 1] : ia;
-      const int* d = dofTable + ((elem - g.elem0) * g.ngll + i) * 2;
-      const int dU = d[0], dV = tor ? -1 : d[1];
-      if (dU >= 0) {
-        b[static_cast<size_t>(dU) * ld + m] = U.re;
-        b[static_cast<size_t>(dU) * ld + l + 1 + m] = U.im;
-      }
-      if (dV >= 0) {
-        b[static_cast<size_t>(dV) * ld + m] = V.re;
-        b[static_cast<size_t>(dV) * ld + l + 1 + m] = V.im;
+    const int elem = count > 0 ? elems[count - 1] : ea;
+    const int i = count > 0 ? locals[count - 1] : ia;
+    const int at = ((elem - g.elem0) * g.ngll + i) * 2;
+    const int dU = dofs[ts.dofOff + at], dV = dofs[ts.dofOff + at + 1];
+    if (dU >= 0) {
+      bs[static_cast<size_t>(dU) * lds + m] = U.re;
+      bs[static_cast<size_t>(dU) * lds + l + 1 + m] = U.im;
+    }
+    if (dV >= 0) {
+      bs[static_cast<size_t>(dV) * lds + m] = V.re;
+      bs[static_cast<size_t>(dV) * lds + l + 1 + m] = V.im;
+    }
+    if (hasTor) {
+      const int dW = dofs[tt.dofOff + at];
+      if (dW >= 0) {
+        bt[static_cast<size_t>(dW) * ldt + m] = W.re;
+        bt[static_cast<size_t>(dW) * ldt + l + 1 + m] = W.im;
       }
     }
-    __syncthreads();
   }
 }
 
@@ -542,16 +540,23 @@ void LaunchCopy(double* dst, const double* src, size_t count, hipStream_t stream
   SL3_HIP(hipGetLastError());
 }
 
-void LaunchForce(const DegreeTask* tasks_dev, int taskCount, int lmax,
-                 const RadialGeometry& g, const SpectralView& weighted,
-                 const int* dofs, double* force, hipStream_t stream) {
-  (void)lmax;
-  if (taskCount == 0 || g.elements == 0) return;
+std::vector<int2> ForceWork(int lmax) {
+  std::vector<int2> work;
+  for (int l = 0; l <= lmax; ++l) {
+    for (int m = 0; m <= l; m += kChunkOrders) work.push_back(make_int2(l, m));
+  }
+  return work;
+}
+
+void LaunchForce(const DegreeTask* tasks_dev, const RadialGeometry& g,
+                 const SpectralView& weighted, const int* dofs, const int2* work_dev,
+                 int workCount, double* force, hipStream_t stream) {
+  if (workCount == 0 || g.elements == 0) return;
   if (g.np > kMaxPoints) throw ArgumentError("at most 6 memory points per element");
-  const dim3 grid(CeilDiv(g.elements, kChunkElements), taskCount);
-  ProfileScope prof497("force", stream);
-  hipLaunchKernelGGL(ForceKernel, grid, dim3(kThreads), 0, stream, tasks_dev, g,
-                     weighted, dofs, force);
+  const dim3 grid(CeilDiv(g.elements, kChunkElements), workCount);
+  ProfileScope prof("force", stream);
+  hipLaunchKernelGGL(ForceKernel, grid, dim3(kThreads), 0, stream, tasks_dev, g, weighted,
+                     dofs, work_dev, force);
   SL3_HIP(hipGetLastError());
 }
 

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace sl3amd {
 
@@ -61,11 +62,14 @@ void LaunchCombine(double* dst, double a, const double* src, double b, size_t co
 
 // Forces of the analysed tensor on every task's memory dofs (rows of
 // nmPad x colsPad at fOff, unscaled: the memory response carries the
-// scaling). dofs: per task, memory element x GLL node x (U, V) or (W, -),
-// the memory-dof row or -1 (none, or the pinned dof).
-void LaunchForce(const DegreeTask* tasks_dev, int taskCount, int lmax,
-                 const RadialGeometry& g, const SpectralView& weighted,
-                 const int* dofs, double* force, hipStream_t stream);
+// scaling). tasks: spheroidal l = 0..L, then toroidal l = 1..L. dofs: per
+// task, memory element x GLL node x (U, V) or (W, -), the memory-dof row or
+// -1 (none, or the pinned dof). work: the (degree, first order) chunks of
+// ForceWork(lmax).
+std::vector<int2> ForceWork(int lmax);
+void LaunchForce(const DegreeTask* tasks_dev, const RadialGeometry& g,
+                 const SpectralView& weighted, const int* dofs, const int2* work_dev,
+                 int workCount, double* force, hipStream_t stream);
 
 // Strain of the responses on the memory dofs into the spectral iterate
 // (every value written), the load channel's unit responses (per memory dof

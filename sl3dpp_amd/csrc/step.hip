@@ -539,6 +539,9 @@ void Viscoelastic3D::BuildTasks() {
   const auto work = ResponseWorkList(tasks_);
   work_.Upload(work.data(), work.size(), s);
   workCount_ = static_cast<int>(work.size());
+  const auto forceWork = ForceWork(L);
+  forceWork_.Upload(forceWork.data(), forceWork.size(), s);
+  forceWorkCount_ = static_cast<int>(forceWork.size());
   SL3_HIP(hipStreamSynchronize(s));
 }
 
@@ -829,8 +832,8 @@ void Viscoelastic3D::BuildResponse(Flavour& f) {
 void Viscoelastic3D::SolveAll(const double* weighted, Flavour& f) {
   hipStream_t s = Stream();
   const SpectralView w = coupling_.View(const_cast<double*>(weighted));
-  LaunchForce(tasksDev_.data(), static_cast<int>(tasks_.size()), coupling_.LMax(),
-              coupling_.Geometry(), w, dofsMem_.data(), force_.data(), s);
+  LaunchForce(tasksDev_.data(), coupling_.Geometry(), w, dofsMem_.data(), forceWork_.data(),
+              forceWorkCount_, force_.data(), s);
   LaunchResponse(tasksDev_.data(), work_.data(), workCount_, f.response.data(),
                  force_.data(), response_.data(), s);
 }

@@ -219,6 +219,8 @@ class Viscoelastic3D {
   int nmMax_ = 0, nmPadMax_ = 0, colsMax_ = 0;
   DeviceArray<ResponseWork> work_;
   int workCount_ = 0;
+  DeviceArray<int2> forceWork_;  // the force kernel's (degree, order chunk) list
+  int forceWorkCount_ = 0;
   std::list<std::unique_ptr<Flavour>> flavours_;
   // the open step's buffers
   DeviceArray<double> weight_, beta_, fixed_, analysed_, amp_, s_, sNext_;
