@@ -42,6 +42,35 @@ def run_smoke():
     assert err < 1e-12, err
     print(f"smoke: band factor+solve n={n} kd={kd} nrhs={nrhs} rel err {err:.2e}",
           flush=True)
-    step_smoke = getattr(sl3dpp_amd, "step_smoke", None)
-    if step_smoke is not None:
-        step_smoke()
+    step_check(oracle)
+
+
+def step_check(oracle):
+    """One lateral-viscosity step at lmax 8 on a small Earth (two passes of
+    the fixed point: StartStep, transforms, product, force, memory response,
+    strain, Anderson start, Finish) against the oracle's step."""
+    import sys
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import sl3dpp_amd.step as st
+    from step_problem import OracleStep, amplitudes, small_earth_spec
+
+    lmax = 8
+    earth = st.Earth(small_earth_spec())
+    L, T, M = earth.scales()
+    dt = 100 * st.YEAR / T
+    kw = dict(max_iterations=2, rtol=1e-14, anderson_memory=6, anderson_rate=0.5)
+    op = st.Viscoelastic3D(earth, lmax, None, st.StepOptions.default(**kw))
+    op.set_delta(op.delta_field(st.delta_coefficients(6, op.nodes, seed=5, rms=0.7, lmax=lmax)))
+    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.export("itau"),
+                     theta=op.theta, wtheta=op.wtheta)
+    load, _ = st.zonal_load_rate(lmax, earth)
+    state = op.zero_state()
+    info = op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    rinfo = ref.step(dt, st.EXPONENTIAL_TRAPEZOID, amplitudes(load, lmax, dt), **kw)
+    c, rc = state.export("coefficients"), ref.coef
+    err = np.max(np.abs(c - rc)) / np.max(np.abs(rc))
+    assert info.iterations == rinfo.iterations == 2, (info.iterations, rinfo.iterations)
+    assert err < 1e-11, err
+    print(f"smoke: 3D step lmax {lmax}, {info.iterations} passes, strain coefficients "
+          f"rel err {err:.2e} against the oracle", flush=True)
