@@ -1,11 +1,13 @@
 """The HIP 3D step against the oracle on identical inputs (the host set-up,
 the lateral field, the load):
  * the batched GSH transforms (SurfaceGrid's Synthesise/ExpandSpinBatch) for
-   spins -2, -1, 0 against the oracle's direct sums, node by node (1e-12);
+   spins -2, -1, 0 against the oracle's direct sums, node by node (1e-12),
+   on grids that take each longitude kernel (nphi = 4 P and 2 P with P odd,
+   and P even);
  * one pass of the step (Converge capped at one pass: StartStep's sweep and
-   analyses, the force, the band solves, the channel, strain recovery,
-   Finish) at lmax 8 and at lmax 32, where the degree truncation cuts the
-   deeper memory nodes (1e-11 relative);
+   analyses, the force, the memory response, the channel, strain recovery,
+   Finish) on the same grids up to lmax 32, where the degree truncation cuts
+   the deeper memory nodes (1e-11 relative);
  * whole converged steps with lateral viscosity, adaptive Anderson mixing
    included: the residual history pass by pass, and the state to 1e-8
    relative at rtol 1e-10;
@@ -48,7 +50,10 @@ def make_operator(earth, lmax, delta_scale, options=None, seed=11):
     return op
 
 
-@pytest.mark.parametrize("lmax", [8, 32])
+# lmax 8, 32: nphi = 4 P, P odd (the matrix-core ring kernel with R = 4);
+# lmax 10: nphi = 34 = 2 x 17 (R = 2); lmax 12: nphi = 40, P = 10 even (the
+# generic ring kernel).
+@pytest.mark.parametrize("lmax", [8, 10, 12, 32])
 @pytest.mark.parametrize("spin", [-2, -1, 0])
 def test_gsh_transforms_match_oracle(earth, oracle, lmax, spin):
     import torch
@@ -119,7 +124,7 @@ def run_pair(earth, oracle, lmax, delta_scale, steps, max_iterations, rtol, hist
     return out
 
 
-@pytest.mark.parametrize("lmax", [8, 32])
+@pytest.mark.parametrize("lmax", [8, 10, 12, 32])
 def test_one_pass_matches_oracle(earth, oracle, lmax):
     results = run_pair(earth, oracle, lmax, 0.7, steps=2, max_iterations=1, rtol=1e-14)
     for info, rinfo, c, rc, m, rm, d, rd in results:
