@@ -390,53 +390,63 @@ __device__ inline int FindRun(const ActiveRuns& a, long long c) {
   return lo;
 }
 
-// Walks the compact range [lo, hi) with stride blockDim.x, calling
-// body(c, j) with j the entry's device-layout index; the run advances
-// monotonically from one binary search.
+// Walks the compact range with stride blockDim.x in complex pairs (every
+// run has even length and offset), calling body(c, j) with c and j the
+// pair's first double in the compact vector and in the device layout; the
+// run advances monotonically from one binary search. The workgroup's pairs
+// are a fixed slice of all of them, so reductions group the same terms on
+// every run.
 template <class Body>
-__device__ inline void ForActive(const ActiveRuns& a, long long lo, long long hi, Body&& body) {
-  long long c = lo + threadIdx.x;
-  if (c >= hi) return;
-  int run = FindRun(a, c);
-  for (; c < hi; c += blockDim.x) {
+__device__ inline void ForActivePairs(const ActiveRuns& a, Body&& body) {
+  size_t lo, hi;
+  BlockRange(static_cast<size_t>(a.total / 2), lo, hi);
+  long long p = static_cast<long long>(lo) + threadIdx.x;
+  if (p >= static_cast<long long>(hi)) return;
+  int run = FindRun(a, 2 * p);
+  for (; p < static_cast<long long>(hi); p += blockDim.x) {
+    const long long c = 2 * p;
     while (c >= a.compact[run + 1]) ++run;
     body(c, a.full[run] + (c - a.compact[run]));
   }
 }
 
+__device__ inline double2 Load2(const double* v, long long i) {
+  return *reinterpret_cast<const double2*>(v + i);
+}
+__device__ inline void Store2(double* v, long long i, double2 x) {
+  *reinterpret_cast<double2*>(v + i) = x;
+}
+
 __global__ void AndersonFirstKernel(const double* x, const double* gx, double* xPrev,
                                     double* fPrev, ActiveRuns a) {
-  size_t lo, hi;
-  BlockRange(static_cast<size_t>(a.total), lo, hi);
-  ForActive(a, lo, hi, [&](long long c, long long j) {
-    xPrev[c] = x[j];
-    fPrev[c] = gx[j] - x[j];
+  ForActivePairs(a, [&](long long c, long long j) {
+    const double2 xv = Load2(x, j), gv = Load2(gx, j);
+    Store2(xPrev, c, xv);
+    Store2(fPrev, c, make_double2(gv.x - xv.x, gv.y - xv.y));
   });
 }
 
 __global__ void __launch_bounds__(kThreads)
     AndersonSweepKernel(AndersonSweep s, ActiveRuns a, double* partials) {
   __shared__ double red[kThreads][17];
-  size_t lo, hi;
-  BlockRange(static_cast<size_t>(a.total), lo, hi);
   double col[8] = {}, gam[8] = {};
   double* dFs = s.dF[s.slot];
   double* dGs = s.dG[s.slot];
-  ForActive(a, lo, hi, [&](long long c, long long j) {
-    const double xs = s.x[j];
-    const double f = s.gx[j] - xs;
-    const double dx = xs - s.xPrev[c];
-    const double df = f - s.fPrev[c];
-    dFs[c] = df;
-    dGs[c] = dx + df;
-    s.xPrev[c] = xs;
-    s.fPrev[c] = f;
+  ForActivePairs(a, [&](long long c, long long j) {
+    const double2 xs = Load2(s.x, j), gx = Load2(s.gx, j);
+    const double2 xp = Load2(s.xPrev, c), fp = Load2(s.fPrev, c);
+    const double2 f = make_double2(gx.x - xs.x, gx.y - xs.y);
+    const double2 df = make_double2(f.x - fp.x, f.y - fp.y);
+    Store2(dFs, c, df);
+    Store2(dGs, c, make_double2(xs.x - xp.x + df.x, xs.y - xp.y + df.y));
+    Store2(s.xPrev, c, xs);
+    Store2(s.fPrev, c, f);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       if (i < s.m) {
-        const double di = i == s.slot ? df : s.dF[i][c];
-        col[i] += di * df;
-        gam[i] += di * f;
+        const double2 di = i == s.slot ? df : Load2(s.dF[i], c);
+        col[i] += di.x * df.x + di.y * df.y;
+        gam[i] += di.x * f.x + di.y * f.y;
       }
     }
   });
@@ -454,17 +464,51 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x < 16) partials[blockIdx.x * 16 + threadIdx.x] = red[0][threadIdx.x];
 }
 
+// RelativeChange over the active entries (the others are zero in both).
+__global__ void __launch_bounds__(kThreads)
+    RelativeChangeActiveKernel(const double* next, const double* now, ActiveRuns a,
+                               double* partials) {
+  __shared__ double s1[kThreads], s2[kThreads], s3[kThreads];
+  double scale = 0.0, diff = 0.0, bad = 0.0;
+  ForActivePairs(a, [&](long long, long long j) {
+    const double2 n = Load2(next, j), o = Load2(now, j);
+    const double dr = n.x - o.x, di = n.y - o.y;
+    if (!(isfinite(n.x) && isfinite(n.y))) bad = 1.0;
+    scale = fmax(scale, n.x * n.x + n.y * n.y);
+    diff = fmax(diff, dr * dr + di * di);
+  });
+  s1[threadIdx.x] = scale;
+  s2[threadIdx.x] = diff;
+  s3[threadIdx.x] = bad;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      s3[threadIdx.x] = fmax(s3[threadIdx.x], s3[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x] = s1[0];
+    partials[3 * blockIdx.x + 1] = s2[0];
+    partials[3 * blockIdx.x + 2] = s3[0];
+  }
+}
+
 struct Gamma {
   double v[8];
 };
 
 __global__ void AndersonMixKernel(AndersonSweep s, Gamma gamma, ActiveRuns a) {
-  size_t lo, hi;
-  BlockRange(static_cast<size_t>(a.total), lo, hi);
-  ForActive(a, lo, hi, [&](long long c, long long j) {
-    double value = s.gx[j];
-    for (int i = 0; i < s.m; ++i) value -= gamma.v[i] * s.dG[i][c];
-    s.x[j] = value;
+  ForActivePairs(a, [&](long long c, long long j) {
+    double2 value = Load2(s.gx, j);
+    for (int i = 0; i < s.m; ++i) {
+      const double2 d = Load2(s.dG[i], c);
+      value.x -= gamma.v[i] * d.x;
+      value.y -= gamma.v[i] * d.y;
+    }
+    Store2(s.x, j, value);
   });
 }
 
@@ -579,6 +623,18 @@ void LaunchRelativeChange(const double* next, const double* now, size_t n,
   ProfileScope prof518("residual", stream);
   hipLaunchKernelGGL(RelativeChangeKernel, dim3(partialBlocks), dim3(kThreads), 0,
                      stream, next, now, n / 2, partials);
+  SL3_HIP(hipGetLastError());
+  hipLaunchKernelGGL(ReducePartialsKernel, dim3(3), dim3(kThreads), 0, stream, partials,
+                     partialBlocks, 3, 1, result);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchRelativeChangeActive(const double* next, const double* now, const ActiveRuns& runs,
+                                double* partials, int partialBlocks, double* result,
+                                hipStream_t stream) {
+  ProfileScope prof("residual", stream);
+  hipLaunchKernelGGL(RelativeChangeActiveKernel, dim3(partialBlocks), dim3(kThreads), 0,
+                     stream, next, now, runs, partials);
   SL3_HIP(hipGetLastError());
   hipLaunchKernelGGL(ReducePartialsKernel, dim3(3), dim3(kThreads), 0, stream, partials,
                      partialBlocks, 3, 1, result);

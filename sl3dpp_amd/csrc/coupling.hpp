@@ -110,6 +110,12 @@ struct AndersonSweep {
   int m;                // active history columns
   int slot;             // the slot being written
 };
+// RelativeChange over the active entries only (exact when the inactive
+// entries of both iterates are zero).
+void LaunchRelativeChangeActive(const double* next, const double* now, const ActiveRuns& runs,
+                                double* partials, int partialBlocks, double* result,
+                                hipStream_t stream);
+
 // The first mixed pass: xPrev = x, fPrev = gx - x (compact), then x = gx
 // (every entry, n doubles).
 void LaunchAndersonFirst(double* x, const double* gx, double* xPrev,

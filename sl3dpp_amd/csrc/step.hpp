@@ -198,7 +198,9 @@ class Viscoelastic3D {
   void BuildResponse(Flavour& f);
   void SolveAll(const double* weighted, Flavour& f);
   void Amplitudes(const LoadFn& load, double t);
-  double RelativeChange(const double* next, const double* now);
+  // RelativeChange (src/Viscoelastic3D.cpp:1636-1669): over every entry,
+  // or over the active ones (where the inactive ones are zero in both).
+  double RelativeChange(const double* next, const double* now, bool everyEntry);
   void ElasticStrainCoefficients(DeviceState& state, const LoadFn& load, double* out);
 
   const Earth* earth_;
