@@ -145,12 +145,16 @@ int sl3_earth_export(const sl3_earth* earth, const char* name, double* dst,
  * (dt = 0 the elastic operators'; dt < 0 none). Returns ndof, kd, the first
  * element, the pinned dof (or -1); fills, where non-NULL, the unscaled
  * upper band ((kd+1) x ndof), the dof map (per global mesh node: U, V, phi
- * for spheroidal, W for toroidal; -1 absent) and the unit load vector
- * (spheroidal, ndof). */
+ * for spheroidal, W for toroidal; -1 absent), the unit load vector
+ * (spheroidal, ndof; DegreeOperator::Load, src/DegreeOperator.cpp:960-970)
+ * and the unit tide vector (spheroidal, ndof; the force of the potential
+ * (r/a)^p, p = 2 at l = 0 and l above: DegreeOperator::Tide,
+ * src/DegreeOperator.cpp:999-1038, as AddUnitChannels takes it,
+ * src/Viscoelastic3D.cpp:1049). */
 int sl3_earth_degree_system(const sl3_earth* earth, int l, int kind, int scheme,
                             double dt, double eps, int* ndof, int* kd,
                             int* first_element, int* pinned, double* band,
-                            int* dofs, double* load);
+                            int* dofs, double* load, double* tide);
 
 /* ===================================================================== *
  * Viscoelastic3D (include/sl3dpp/Viscoelastic3D.hpp:89-675): the 3D      *
@@ -201,19 +205,24 @@ int sl3_operator_geometry(const sl3_operator* op, double* radii, double* theta,
                           double* wtheta);
 
 /* LoadCallable3D: write degree l's m >= 0 columns at time t as (re, im)
- * pairs into columns (2(l+1) doubles) and return 1, or return 0 for none. */
+ * pairs into columns (2(l+1) doubles) and return 1, or return 0 for none.
+ * The same type serves the load and the tide channel. */
 typedef int (*sl3_load_fn)(int l, double t, double* columns, void* user);
 
 int sl3_state_zero(sl3_operator* op, sl3_state** out);
 int sl3_state_destroy(sl3_state* state);
 int sl3_state_time(const sl3_state* state, double* t);
-/* Step (src/Viscoelastic3D.cpp:1982-2030), in place on the state.
- * start_dev: a warm start for the fixed point (device layout spectral
- * tensor), or NULL for the step's own start strain; Sdirk2 takes NULL.
- * For Sdirk2 the info sums both stages' passes and phase times. */
+/* Step (src/Viscoelastic3D.cpp:1982-2030; Viscoelastic3D::Step(state, dt,
+ * scheme, load, tide), include/sl3dpp/Viscoelastic3D.hpp:407-410), in place
+ * on the state. load, tide: the channels (NULL for none), both called with
+ * user; a tide whose degree-1 columns are not zero is an error
+ * (src/Viscoelastic3D.cpp:142-154). start_dev: a warm start for the fixed
+ * point (device layout spectral tensor), or NULL for the step's own start
+ * strain; Sdirk2 takes NULL. For Sdirk2 the info sums both stages' passes
+ * and phase times. */
 int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme,
-             sl3_load_fn load, void* user, const double* start_dev,
-             sl3_step_info* info);
+             sl3_load_fn load, sl3_load_fn tide, void* user,
+             const double* start_dev, sl3_step_info* info);
 
 /* Host copies for checks and outputs. Names:
  *   state:    "coefficients" (the strain coefficients, reference l-major

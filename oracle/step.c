@@ -286,7 +286,7 @@ typedef struct {
   double* scale;
   double* factor;
   int* piv;
-  double* unit;
+  double* unit; /* spheroidal: the unit load response, then the unit tide's */
   const int* dofs;
 } task_t;
 
@@ -354,7 +354,8 @@ static void solve(const task_t* t, double* x, int nrhs) {
 /* ---- force (src/GridCoupling.cpp:691-809) and solves --------------------- */
 
 static void force_and_solve(const sl3o_problem* p, task_t* tasks, int ntasks,
-                            const double* weighted, const double* amp, double** solutions) {
+                            const double* weighted, const double* amp, const double* tamp,
+                            double** solutions) {
 #pragma omp parallel for schedule(dynamic)
   for (int ti = 0; ti < ntasks; ++ti) {
     task_t* t = &tasks[ti];
@@ -422,11 +423,15 @@ static void force_and_solve(const sl3o_problem* p, task_t* tasks, int ntasks,
     for (int c = 0; c < cols; ++c) {
       for (int d = 0; d < nd; ++d) {
         double re = x[(size_t)c * nd + d], im = x[(size_t)(cols + c) * nd + d];
-        /* AddUnitChannels (:1029-1073) */
+        /* AddUnitChannels (:1029-1073): load column 0, tide column 1 */
+        const size_t at = (size_t)l * (l + 1) / 2 + c;
         if (t->kind == 0 && amp != NULL) {
-          const size_t at = (size_t)l * (l + 1) / 2 + c;
           re += amp[2 * at] * t->unit[d];
           im += amp[2 * at + 1] * t->unit[d];
+        }
+        if (t->kind == 0 && tamp != NULL) {
+          re += tamp[2 * at] * t->unit[nd + d];
+          im += tamp[2 * at + 1] * t->unit[nd + d];
         }
         solutions[ti][2 * ((size_t)c * nd + d)] = re;
         solutions[ti][2 * ((size_t)c * nd + d) + 1] = im;
@@ -521,7 +526,8 @@ static double now_ms(void) {
 }
 
 int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
-              const double* amp, const sl3o_options* o, sl3o_info* info) {
+              const double* amp, const double* tamp, const sl3o_options* o,
+              sl3o_info* info) {
   const int L = p->lmax;
   const int ntasks = 2 * L + 1;
   const size_t fs = (size_t)p->ntheta * p->nphi;
@@ -531,6 +537,7 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
   double clock0 = now_ms();
   info->setup_ms = info->start_ms = info->passes_ms = info->finish_ms = 0.0;
   if (dt == 0.0) scheme = 1;
+  if (tamp != NULL && p->tides == NULL) return 2;
   task_t* tasks = (task_t*)calloc(ntasks, sizeof(task_t));
   for (int ti = 0; ti < ntasks; ++ti) {
     task_t* t = &tasks[ti];
@@ -549,9 +556,13 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
   g_tables = &tables;
   for (int l = 0; l <= L; ++l) {
     task_t* t = &tasks[l];
-    t->unit = (double*)malloc(sizeof(double) * t->ndof);
+    /* the unit load and tide solved together (:1042-1055) */
+    t->unit = (double*)calloc((size_t)2 * t->ndof, sizeof(double));
     memcpy(t->unit, p->loads + p->load_off[l], sizeof(double) * t->ndof);
-    solve(t, t->unit, 1);
+    if (p->tides != NULL) {
+      memcpy(t->unit + t->ndof, p->tides + p->tide_off[l], sizeof(double) * t->ndof);
+    }
+    solve(t, t->unit, 2);
   }
   info->setup_ms = now_ms() - clock0;
   clock0 = now_ms();
@@ -647,7 +658,7 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
         analyse_tensor(p, grid, analysed);
         for (int64_t i = 0; i < S; ++i) analysed[i] += fixedS[i];
       }
-      force_and_solve(p, tasks, ntasks, analysed, amp, sol);
+      force_and_solve(p, tasks, ntasks, analysed, amp, tamp, sol);
       strain_of(p, tasks, sol, sNext);
       const double residual = relative_change(sNext, s, S);
       info->iterations = it + 1;

@@ -43,6 +43,8 @@ typedef struct {
   const int* dofs;
   const int64_t* load_off;  /* spheroidal tasks' unit loads (ndof) */
   const double* loads;
+  const int64_t* tide_off;  /* spheroidal tasks' unit tides (ndof), or NULL */
+  const double* tides;
 } sl3o_problem;
 
 typedef struct {
@@ -80,12 +82,14 @@ typedef struct {
 /* Size in doubles of the reference-layout spectral tensor. */
 int64_t sl3o_spectral_doubles(int lmax, int nodes);
 
-/* One step in place; amp: the load channel's columns at the step's end
- * time, complex per (l, m) at l(l+1)/2 + m (NULL for none). scheme 0 the
- * exponential trapezoid, 1 backward Euler. Returns 0, or non-zero when a
- * factorisation fails. */
+/* One step in place; amp, tamp: the load and tide channels' columns at
+ * the step's end time, complex per (l, m) at l(l+1)/2 + m (NULL for none;
+ * a tide needs the problem's unit tides). scheme 0 the exponential
+ * trapezoid, 1 backward Euler. Returns 0, non-zero when a factorisation
+ * fails, 2 for a tide without unit tides. */
 int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
-              const double* amp, const sl3o_options* o, sl3o_info* info);
+              const double* amp, const double* tamp, const sl3o_options* o,
+              sl3o_info* info);
 
 /* The residual history of the last sl3o_step (count passes). */
 int sl3o_last_residuals(double* out, int capacity);

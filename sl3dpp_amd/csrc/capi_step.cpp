@@ -197,7 +197,7 @@ int sl3_earth_export(const sl3_earth* earth, const char* name, double* dst,
 int sl3_earth_degree_system(const sl3_earth* earth, int l, int kind, int scheme,
                             double dt, double eps, int* ndof, int* kd,
                             int* first_element, int* pinned, double* band,
-                            int* dofs, double* load) {
+                            int* dofs, double* load, double* tide) {
   return Guard([&] {
     Check(earth != nullptr, "null earth");
     Check(kind == 0 || kind == 1, "kind 0 (spheroidal) or 1 (toroidal)");
@@ -224,6 +224,10 @@ int sl3_earth_degree_system(const sl3_earth* earth, int l, int kind, int scheme,
       if (load) {
         const auto b = op.Load();
         std::memcpy(load, b.data(), sizeof(double) * b.size());
+      }
+      if (tide) {
+        const auto b = op.Tide(l == 0 ? 2 : -1);
+        std::memcpy(tide, b.data(), sizeof(double) * b.size());
       }
     } else {
       ToroidalOperator op(e.reference.nodal, l, eps, psp);
@@ -342,16 +346,20 @@ int sl3_state_time(const sl3_state* state, double* t) {
 }
 
 int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme, sl3_load_fn load,
-             void* user, const double* start_dev, sl3_step_info* info) {
+             sl3_load_fn tide, void* user, const double* start_dev, sl3_step_info* info) {
   return Guard([&] {
     Check(op != nullptr && state != nullptr, "null argument");
-    LoadFn fn;
-    if (load) {
-      fn = [load, user](int l, double t, double* columns) {
-        return load(l, t, columns, user) != 0;
-      };
-    }
-    const auto r = op->op->Step(*state->state, dt, scheme, fn, start_dev);
+    auto wrap = [user](sl3_load_fn fn) {
+      LoadFn out;
+      if (fn) {
+        out = [fn, user](int l, double t, double* columns) {
+          return fn(l, t, columns, user) != 0;
+        };
+      }
+      return out;
+    };
+    const Channels channels{wrap(load), wrap(tide)};
+    const auto r = op->op->Step(*state->state, dt, scheme, channels, start_dev);
     if (info) {
       info->iterations = r.iterations;
       info->residual = r.residual;

@@ -201,7 +201,8 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(1024)
     StrainKernel(const DegreeTask* sph, const DegreeTask* tor, int lmax,
                  RadialGeometry g, const int* dofs, const double* response,
-                 const double* units, const double* amp, SpectralView out) {
+                 const double* units, const double* amp, const double* tideUnits,
+                 const double* tideAmp, SpectralView out) {
   const int pair = blockIdx.x;
   int l = static_cast<int>((sqrt(8.0 * pair + 1.0) - 1.0) * 0.5);
   while (l * (l + 1) / 2 > pair) --l;
@@ -218,6 +219,8 @@ __global__ void __launch_bounds__(1024)
     Cplx U = C(0, 0), V = C(0, 0), W = C(0, 0), rdU = C(0, 0), rdV = C(0, 0),
          rdW = C(0, 0);
     const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
+    const Cplx at = tideAmp ? C(tideAmp[2 * pair], tideAmp[2 * pair + 1]) : C(0.0, 0.0);
+    const bool tide = tideAmp != nullptr;
     const bool tor1 = l >= 1;
     DegreeTask tt;
     if (tor1) tt = tor[l - 1];
@@ -230,10 +233,12 @@ __global__ void __launch_bounds__(1024)
       if (d[0] >= 0) {
         Ui = C(re[d[0]], im[d[0]]);
         if (ts.unitOff >= 0) Ui = Ui + units[ts.dmOff + d[0]] * a;
+        if (tide) Ui = Ui + tideUnits[ts.dmOff + d[0]] * at;
       }
       if (d[1] >= 0) {
         Vi = C(re[d[1]], im[d[1]]);
         if (ts.unitOff >= 0) Vi = Vi + units[ts.dmOff + d[1]] * a;
+        if (tide) Vi = Vi + tideUnits[ts.dmOff + d[1]] * at;
       }
       if (tor1) {
         const int dw = dofs[tt.dofOff + ((elem - g.elem0) * g.ngll + i) * 2];
@@ -607,13 +612,15 @@ void LaunchForce(const DegreeTask* tasks_dev, const RadialGeometry& g,
 void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev,
                   int lmax, const RadialGeometry& g, const int* dofs,
                   const double* response, const double* units, const double* amp,
-                  SpectralView out, hipStream_t stream) {
+                  const double* tideUnits, const double* tideAmp, SpectralView out,
+                  hipStream_t stream) {
   const int pairs = (lmax + 1) * (lmax + 2) / 2;
   if (g.nodes > 1024) throw ArgumentError("at most 1024 memory nodes");
   const int threads = 64 * CeilDiv(g.nodes, 64);
   ProfileScope prof509("strain", stream);
   hipLaunchKernelGGL(StrainKernel, dim3(pairs), dim3(threads), 0, stream, sphTasks_dev,
-                     torTasks_dev, lmax, g, dofs, response, units, amp, out);
+                     torTasks_dev, lmax, g, dofs, response, units, amp, tideUnits, tideAmp,
+                     out);
   SL3_HIP(hipGetLastError());
 }
 

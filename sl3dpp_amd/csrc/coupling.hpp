@@ -23,7 +23,7 @@ struct DegreeTask {
   long long rhsOff = 0;    // [dof][cols] row-major
   long long scaleOff = 0;  // ndof doubles
   long long dofOff = 0;    // memory elements x ngll x (U, V) or (W)
-  long long unitOff = -1;  // the unit load response (ndof), spheroidal
+  long long unitOff = -1;  // the unit load, then tide, responses (2 ndof), spheroidal
   // the memory dofs: the force's rows and the strain's reads
   int nm = 0;              // count (the pinned dof excluded)
   int nmPad = 0;           // nm rounded up to 16
@@ -72,13 +72,15 @@ void LaunchForce(const DegreeTask* tasks_dev, const RadialGeometry& g,
                  int workCount, double* force, hipStream_t stream);
 
 // Strain of the responses on the memory dofs into the spectral iterate
-// (every value written), the load channel's unit responses (per memory dof
-// at dmOff) added with amplitudes amp (complex per (l, m), offset
-// l(l+1)/2 + m).
+// (every value written), the channels' unit responses (per memory dof at
+// dmOff) added with their amplitudes (complex per (l, m), offset
+// l(l+1)/2 + m): the load's from units and amp, the tide's from tideUnits
+// and tideAmp (tideAmp null: no tide).
 void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev,
                   int lmax, const RadialGeometry& g, const int* dofs,
                   const double* response, const double* units, const double* amp,
-                  SpectralView out, hipStream_t stream);
+                  const double* tideUnits, const double* tideAmp, SpectralView out,
+                  hipStream_t stream);
 
 // max |n|^2, max |n - o|^2 and a non-finite flag over n values, into
 // result[0..2] (device).

@@ -304,6 +304,56 @@ std::vector<double> DegreeOperator::Load() const {
   return b;
 }
 
+// The force of an external potential psi = (r/a)^p on the degree system
+// (src/DegreeOperator.cpp:999-1038): minus the gravitational couplings of
+// the assembly applied to psi. Solid nodes take the U row (rho r^2 psi')
+// and the V row (l(l+1) rho r psi); fluid nodes the stratification on the
+// potential; each fluid-solid interface node the point coupling
+// sign rho_F r^2 psi. p = 0 is no force.
+std::vector<double> DegreeOperator::Tide(int power) const {
+  const int p = power < 0 ? l_ : power;
+  std::vector<double> b(ndof_, 0.0);
+  if (p == 0) return b;
+  const auto& mat = *material_;
+  const auto& mesh = *mat.mesh;
+  const int n = mesh.NGll();
+  const double a = mat.radius;
+  const double k2 = l_ * (l_ + 1.0);
+  const auto& w = mesh.Weights();
+  for (int e = firstElement_; e < mesh.ElementCount(); ++e) {
+    const bool fluid = mat.IsFluid(e) && l_ > 0;
+    const double jac = mesh.Jacobian(e);
+    for (int q = 0; q < n; ++q) {
+      const size_t at = static_cast<size_t>(e) * n + q;
+      const int gn = mesh.GlobalIndex(e, q);
+      const double r = mesh.Radius(e, q);
+      const double wq = w[q] * jac;
+      const double psi = std::pow(r / a, p);
+      if (fluid) {
+        b[Dof(gn, Component::Phi)] -= wq * Stratification(r, mat.drho[at], mat.g[at]) * psi;
+        continue;
+      }
+      const double dpsi = (p / a) * std::pow(r / a, p - 1);
+      const int iU = Dof(gn, Component::U);
+      if (iU >= 0) b[iU] -= wq * mat.rho[at] * r * r * dpsi;
+      if (l_ > 0) b[Dof(gn, Component::V)] -= wq * k2 * mat.rho[at] * r * psi;
+    }
+  }
+  if (l_ > 0) {
+    for (int e = firstElement_; e + 1 < mesh.ElementCount(); ++e) {
+      if (mat.IsFluid(e) == mat.IsFluid(e + 1)) continue;
+      const int gn = mesh.GlobalIndex(e, n - 1);
+      const double rt = mesh.Radius(e, n - 1);
+      const bool below = mat.IsFluid(e);  // the fluid side is the lower element
+      const double rhoF = below ? mat.rho[static_cast<size_t>(e) * n + (n - 1)]
+                                : mat.rho[static_cast<size_t>(e + 1) * n];
+      const double sign = below ? 1.0 : -1.0;
+      b[Dof(gn, Component::U)] -= sign * rhoF * rt * rt * std::pow(rt / a, p);
+    }
+  }
+  return b;
+}
+
 static std::vector<double> ApplyUpperBand(const std::vector<double>& band,
                                           int ndof, int kd,
                                           const std::vector<double>& x,

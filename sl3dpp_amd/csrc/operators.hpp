@@ -47,6 +47,9 @@ class DegreeOperator {
   // The unit surface load's force, LoadPart::Both
   // (src/DegreeOperator.cpp:960-970).
   std::vector<double> Load() const;
+  // The force of the external potential (r/a)^p, p = power, or p = l for a
+  // negative power (src/DegreeOperator.cpp:999-1038).
+  std::vector<double> Tide(int power) const;
   // A x with the unscaled band, nrhs column-major columns
   // (src/DegreeOperator.cpp:480-505).
   std::vector<double> Apply(const std::vector<double>& x, int nrhs) const;

@@ -421,7 +421,8 @@ Viscoelastic3D::Viscoelastic3D(const Earth& earth, int lmax, const double* delta
   force_.Resize(std::max<size_t>(fDoubles_, 1));
   LaunchFill(force_.data(), force_.size(), 0.0, s);  // the padding stays zero
   response_.Resize(std::max<size_t>(fDoubles_, 1));
-  amp_.Resize(static_cast<size_t>(lmax + 1) * (lmax + 2));
+  ampDoubles_ = static_cast<size_t>(lmax + 1) * (lmax + 2);
+  amp_.Resize(2 * ampDoubles_);
   partials_.Resize(1024 * 16);
   result_.Resize(16);
   for (auto& e : events_) SL3_HIP(hipEventCreate(&e));
@@ -481,7 +482,7 @@ void Viscoelastic3D::BuildTasks() {
     scaleDoubles_ += t.ndof;
     if (kind == 0) {
       t.unitOff = static_cast<long long>(unitDoubles_);
-      unitDoubles_ += t.ndof;
+      unitDoubles_ += 2 * static_cast<size_t>(t.ndof);
     }
     t.dofOff = static_cast<long long>(dofsHost_.size());
     for (int e = geo.elem0; e < geo.elem0 + geo.elements; ++e) {
@@ -695,14 +696,17 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
   f->choleskyTaskCount = static_cast<int>(ct.size());
   f->luTaskCount = static_cast<int>(lt.size());
 
-  // The unit load responses (Viscoelastic3D::AddUnitChannels' cache,
-  // src/Viscoelastic3D.cpp:1042-1055): Load() solved with this flavour.
+  // The unit load and tide responses (Viscoelastic3D::AddUnitChannels'
+  // cache, src/Viscoelastic3D.cpp:1042-1055): Load() and Tide(2 at l = 0,
+  // else l) solved together with this flavour.
   std::vector<double> unitRhs(unitDoubles_, 0.0);
   for (const auto& t : tasks_) {
     if (t.kind != 0) continue;
     const auto load = sph[t.l]->Load();
-    for (int d = 0; d < t.ndof; ++d) {
-      unitRhs[t.unitOff + d] = load[d] * scale[t.scaleOff + d];  // BeginSolve
+    const auto tide = sph[t.l]->Tide(t.l == 0 ? 2 : -1);
+    for (int d = 0; d < t.ndof; ++d) {  // BeginSolve
+      unitRhs[t.unitOff + d] = load[d] * scale[t.scaleOff + d];
+      unitRhs[t.unitOff + t.ndof + d] = tide[d] * scale[t.scaleOff + d];
     }
   }
   DeviceArray<double> unitDev;
@@ -715,7 +719,7 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
         if (tasks_[k].rhsOff == b.b_off) ti = static_cast<int>(k);
       }
       if (ti < 0 || tasks_[ti].kind != 0) continue;
-      b.nrhs = 1;
+      b.nrhs = 2;
       b.b_off = tasks_[ti].unitOff;
       b.row_stride = 1;
       b.col_stride = b.n;
@@ -743,14 +747,20 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
   SL3_HIP(hipStreamSynchronize(s));
   for (const auto& t : tasks_) {
     if (t.kind != 0) continue;
-    for (int d = 0; d < t.ndof; ++d) units[t.unitOff + d] *= scale[t.scaleOff + d];  // FinishSolve
+    for (int d = 0; d < t.ndof; ++d) {  // FinishSolve
+      units[t.unitOff + d] *= scale[t.scaleOff + d];
+      units[t.unitOff + t.ndof + d] *= scale[t.scaleOff + d];
+    }
   }
   f->units.Upload(units.data(), units.size(), s);
-  std::vector<double> unitsMem(std::max<size_t>(dmHost_.size(), 1), 0.0);
+  const size_t memDofs = dmHost_.size();
+  std::vector<double> unitsMem(std::max<size_t>(2 * memDofs, 1), 0.0);
   for (const auto& t : tasks_) {
     if (t.kind != 0) continue;
     for (int i = 0; i < t.nm; ++i) {
-      unitsMem[t.dmOff + i] = units[t.unitOff + dmHost_[t.dmOff + i]];
+      const int d = dmHost_[t.dmOff + i];
+      unitsMem[t.dmOff + i] = units[t.unitOff + d];
+      unitsMem[memDofs + t.dmOff + i] = units[t.unitOff + t.ndof + d];
     }
   }
   f->unitsMem.Upload(unitsMem.data(), unitsMem.size(), s);
@@ -838,21 +848,45 @@ void Viscoelastic3D::SolveAll(const double* weighted, Flavour& f) {
                  force_.data(), response_.data(), s);
 }
 
-void Viscoelastic3D::Amplitudes(const LoadFn& load, double t) {
-  const int L = coupling_.LMax();
-  std::vector<double> amp(static_cast<size_t>(L + 1) * (L + 2), 0.0);
-  if (load) {
-    std::vector<double> cols(2 * (L + 1));
-    for (int l = 0; l <= L; ++l) {
-      std::fill(cols.begin(), cols.end(), 0.0);
-      if (!load(l, t, cols.data())) continue;
-      for (int m = 0; m <= l; ++m) {
-        const size_t at = static_cast<size_t>(l) * (l + 1) / 2 + m;
-        amp[2 * at] = cols[2 * m];
-        amp[2 * at + 1] = cols[2 * m + 1];
-      }
+// TideColumns' check (src/Viscoelastic3D.cpp:142-154) on degree 1's columns.
+static void RejectDegreeOneTide(const double* cols) {
+  for (int k = 0; k < 4; ++k) {
+    if (cols[k] != 0.0) {
+      throw ArgumentError(
+          "a tide channel has no degree-1 part: tides are taken relative to the "
+          "centre of mass, its degree-1 columns must vanish");
     }
   }
+}
+
+void Viscoelastic3D::CheckTide(const Channels& channels, double t) const {
+  if (!channels.tide || coupling_.LMax() < 1) return;
+  double cols[4] = {0.0, 0.0, 0.0, 0.0};
+  if (channels.tide(1, t, cols)) RejectDegreeOneTide(cols);
+}
+
+// The columns of both channels at t (AddUnitChannels' amplitudes,
+// src/Viscoelastic3D.cpp:1058-1071); a tide with a nonzero degree-1 column
+// is rejected as TideColumns does (:142-154).
+void Viscoelastic3D::Amplitudes(const Channels& channels, double t) {
+  const int L = coupling_.LMax();
+  std::vector<double> amp(2 * ampDoubles_, 0.0);
+  std::vector<double> cols(2 * (L + 1));
+  auto gather = [&](const LoadFn& fn, double* into, bool tide) {
+    for (int l = 0; l <= L; ++l) {
+      std::fill(cols.begin(), cols.end(), 0.0);
+      if (!fn(l, t, cols.data())) continue;
+      if (tide && l == 1) RejectDegreeOneTide(cols.data());
+      for (int m = 0; m <= l; ++m) {
+        const size_t at = static_cast<size_t>(l) * (l + 1) / 2 + m;
+        into[2 * at] = cols[2 * m];
+        into[2 * at + 1] = cols[2 * m + 1];
+      }
+    }
+  };
+  if (channels.load) gather(channels.load, amp.data(), false);
+  hasTide_ = static_cast<bool>(channels.tide);
+  if (hasTide_) gather(channels.tide, amp.data() + ampDoubles_, true);
   hipStream_t s = Stream();
   SL3_HIP(hipMemcpyAsync(amp_.data(), amp.data(), sizeof(double) * amp.size(),
                          hipMemcpyHostToDevice, s));
@@ -896,10 +930,11 @@ void Viscoelastic3D::Pass(const double* s, double* sNext, bool useStartForce, Fl
   const auto* sph = tasksDev_.data();
   const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
   LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), dofsMem_.data(),
-               response_.data(), f.unitsMem.data(), amp_.data(), coupling_.View(sNext), st);
+               response_.data(), f.unitsMem.data(), amp_.data(),
+               f.unitsMem.data() + dmHost_.size(), TideAmplitudes(), coupling_.View(sNext), st);
 }
 
-void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const LoadFn& load,
+void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const Channels& channels,
                                                double* out) {
   hipStream_t s = Stream();
   Flavour& f = FlavourOf(kBackwardEuler, 0.0);
@@ -915,23 +950,24 @@ void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const LoadFn&
   }
   coupling_.AnalyseFromFourier(B, analysed_.data(), nullptr, true, s);
   SolveAll(analysed_.data(), f);
-  Amplitudes(load, state.t);
+  Amplitudes(channels, state.t);
   const auto* sph = tasksDev_.data();
   const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
   LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), dofsMem_.data(),
-               response_.data(), f.unitsMem.data(), amp_.data(), coupling_.View(out), s);
+               response_.data(), f.unitsMem.data(), amp_.data(),
+               f.unitsMem.data() + dmHost_.size(), TideAmplitudes(), coupling_.View(out), s);
 }
 
 StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
-                              const LoadFn& load, const double* start) {
+                              const Channels& channels, const double* start) {
   if (!(dt >= 0.0)) throw ArgumentError("dt must be non-negative");
   if (scheme != kExponentialTrapezoid && scheme != kBackwardEuler && scheme != kSdirk2) {
     throw ArgumentError(
         "the 3D step is implicit: ExponentialTrapezoid, BackwardEuler or Sdirk2");
   }
   if (dt == 0.0) scheme = kBackwardEuler;  // every scheme's zero-length step
-  if (scheme == kSdirk2) return StepSdirk2(state, dt, load);
-  return StepOnce(state, dt, scheme, load, start);
+  if (scheme == kSdirk2) return StepSdirk2(state, dt, channels);
+  return StepOnce(state, dt, scheme, channels, start);
 }
 
 void Viscoelastic3D::CopyState(const DeviceState& from, DeviceState& to) const {
@@ -959,15 +995,18 @@ void Viscoelastic3D::CopyState(const DeviceState& from, DeviceState& to) const {
 // second stage starts from m = (1 - r) m_n + r m(Y1), r = (1 - gamma) /
 // gamma, with Y1's strain and coefficients, at t_n + (1 - gamma) h, and is
 // BE(gamma h) again. The passes add up; the last stage's residual stands.
-StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const LoadFn& load) {
+StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const Channels& channels) {
   hipStream_t s = Stream();
   const double tn = state.t;
   const double length = kSdirk2Gamma * dt;
+  // both stages' tides checked before the state changes
+  CheckTide(channels, tn + length);
+  CheckTide(channels, tn + dt);
   const double r = (1.0 - kSdirk2Gamma) / kSdirk2Gamma;
   if (!stage_) stage_ = std::make_unique<DeviceState>();
   DeviceState& y1 = *stage_;
   CopyState(state, y1);
-  const StepInfo first = StepOnce(y1, length, kBackwardEuler, load, nullptr);
+  const StepInfo first = StepOnce(y1, length, kBackwardEuler, channels, nullptr);
   const auto firstHistory = residualHistory;
   const Timing firstTiming = lastTiming;
   for (int b = 0; b < 3; ++b) {
@@ -982,7 +1021,7 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const LoadFn&
   staged.hasStrain = y1.hasStrain;
   staged.coefficients = std::move(y1.coefficients);
   staged.hasCoefficients = y1.hasCoefficients;
-  const StepInfo second = StepOnce(staged, length, kBackwardEuler, load, nullptr);
+  const StepInfo second = StepOnce(staged, length, kBackwardEuler, channels, nullptr);
   // the stepped stage becomes the state; Y1's buffers return to the cache
   for (int b = 0; b < 3; ++b) {
     state.m[b] = std::move(staged.m[b]);
@@ -1007,7 +1046,7 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const LoadFn&
 }
 
 StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
-                                  const LoadFn& load, const double* start) {
+                                  const Channels& channels, const double* start) {
   hipStream_t s = Stream();
   const size_t S = coupling_.SpectralDoubles();
   double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
@@ -1024,8 +1063,11 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
                              state.strain[2].data()};
     coupling_.Analyse(grid, s_.data(), B, s);
   } else {
-    ElasticStrainCoefficients(state, load, s_.data());
+    ElasticStrainCoefficients(state, channels, s_.data());
   }
+  // the channels at the end time, gathered (and a degree-1 tide rejected)
+  // before the sweep rewrites the state's memory
+  Amplitudes(channels, tEnd);
   // StartStep's sweep (:1283-1376)
   if (!state.hasStrain) coupling_.SynthesiseToFourier(s_.data(), A, false, s);
   for (int b = 0; b < 3; ++b) {
@@ -1049,7 +1091,6 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
   }
   coupling_.AnalyseFromFourier(B, fixed_.data(), nullptr, true, s);
   coupling_.AnalyseFromFourier(A, analysed_.data(), fixed_.data(), true, s);
-  Amplitudes(load, tEnd);
   SL3_HIP(hipEventRecord(events_[1], s));
 
   // Converge (:1916-1974)
@@ -1184,17 +1225,22 @@ std::vector<double> Viscoelastic3D::SolutionOfTask(int task) {
   LaunchLuSolve(f.luDev.data(), f.luTasks.data(), f.luTaskCount, kvMax, f.factors.data(),
                 f.pivots.data(), rhs_.data(), s);
   std::vector<double> raw(static_cast<size_t>(t.ndof) * t.cols), scale(t.ndof),
-      units(t.kind == 0 ? t.ndof : 0), amp(2 * (t.l + 1));
+      units(t.kind == 0 ? 2 * t.ndof : 0), amp(2 * (t.l + 1)), tideAmp(2 * (t.l + 1), 0.0);
   SL3_HIP(hipMemcpyAsync(raw.data(), rhs_.data() + t.rhsOff, sizeof(double) * raw.size(),
                          hipMemcpyDeviceToHost, s));
   SL3_HIP(hipMemcpyAsync(scale.data(), f.scale.data() + t.scaleOff,
                          sizeof(double) * t.ndof, hipMemcpyDeviceToHost, s));
   if (t.kind == 0) {
     SL3_HIP(hipMemcpyAsync(units.data(), f.units.data() + t.unitOff,
-                           sizeof(double) * t.ndof, hipMemcpyDeviceToHost, s));
+                           sizeof(double) * units.size(), hipMemcpyDeviceToHost, s));
     SL3_HIP(hipMemcpyAsync(amp.data(),
                            amp_.data() + static_cast<size_t>(t.l) * (t.l + 1),
                            sizeof(double) * amp.size(), hipMemcpyDeviceToHost, s));
+    if (hasTide_) {
+      SL3_HIP(hipMemcpyAsync(tideAmp.data(),
+                             TideAmplitudes() + static_cast<size_t>(t.l) * (t.l + 1),
+                             sizeof(double) * tideAmp.size(), hipMemcpyDeviceToHost, s));
+    }
   }
   SL3_HIP(hipStreamSynchronize(s));
   // the reference's Solutions layout: ndof x (l + 1) complex, column-major
@@ -1204,8 +1250,8 @@ std::vector<double> Viscoelastic3D::SolutionOfTask(int task) {
       double re = raw[static_cast<size_t>(d) * t.cols + m] * scale[d];
       double im = raw[static_cast<size_t>(d) * t.cols + t.l + 1 + m] * scale[d];
       if (t.kind == 0) {
-        re += amp[2 * m] * units[d];
-        im += amp[2 * m + 1] * units[d];
+        re += amp[2 * m] * units[d] + tideAmp[2 * m] * units[t.ndof + d];
+        im += amp[2 * m + 1] * units[d] + tideAmp[2 * m + 1] * units[t.ndof + d];
       }
       out[2 * (static_cast<size_t>(m) * t.ndof + d)] = re;
       out[2 * (static_cast<size_t>(m) * t.ndof + d) + 1] = im;

@@ -48,6 +48,14 @@ struct StepInfo {
 // or false for none.
 using LoadFn = std::function<bool(int l, double t, double* columns)>;
 
+// The step's channels (Viscoelastic3D::Step's load and tide, either may be
+// empty). The tide's degree-1 columns must vanish (src/Viscoelastic3D.cpp:
+// 138-154).
+struct Channels {
+  LoadFn load;
+  LoadFn tide;
+};
+
 // The device grid coupling (GridCoupling + SurfaceGrid): memory nodes,
 // Wigner tables, DFT tables, layouts.
 class Coupling {
@@ -117,7 +125,8 @@ struct Flavour {
   int scheme = 0;
   double dt = 0.0;
   std::vector<double> pointShear;
-  DeviceArray<double> scale, units, factors;
+  DeviceArray<double> scale, factors;
+  DeviceArray<double> units;  // per spheroidal task at unitOff: load, tide (ndof each)
   DeviceArray<int> pivots;
   std::vector<BandSystem> cholesky, lu;  // the solve batches
   DeviceArray<BandSystem> choleskyDev, luDev;
@@ -127,7 +136,8 @@ struct Flavour {
   std::vector<char> usedCholesky;  // per task
   std::vector<int> choleskyTask, luTask;  // the task of each solve system
   // the memory response of every task (G = D A_s^{-1} D on its memory dofs)
-  // and the unit load responses on the memory dofs
+  // and the unit load and tide responses on the memory dofs (load for every
+  // memory dof, then tide for every memory dof)
   DeviceArray<double> response, unitsMem;
 };
 
@@ -158,7 +168,7 @@ class Viscoelastic3D {
   // src/StepRecipe.cpp:25-32), in place on the state. `start` (device
   // spectral, or null) is the warm start of Converge (:1916-1921); a
   // multi-stage step takes none.
-  StepInfo Step(DeviceState& state, double dt, int scheme, const LoadFn& load,
+  StepInfo Step(DeviceState& state, double dt, int scheme, const Channels& channels,
                 const double* start = nullptr);
   // The last pass's solutions on every dof, unscaled with channels, per
   // task (band solves of the last pass's force): host export for the
@@ -190,18 +200,25 @@ class Viscoelastic3D {
   void Pass(const double* s, double* sNext, bool useStartForce, Flavour& f);
 
  private:
-  StepInfo StepOnce(DeviceState& state, double dt, int scheme, const LoadFn& load,
+  StepInfo StepOnce(DeviceState& state, double dt, int scheme, const Channels& channels,
                     const double* start);
-  StepInfo StepSdirk2(DeviceState& state, double dt, const LoadFn& load);
+  StepInfo StepSdirk2(DeviceState& state, double dt, const Channels& channels);
   void CopyState(const DeviceState& from, DeviceState& to) const;
   void BuildTasks();
   void BuildResponse(Flavour& f);
   void SolveAll(const double* weighted, Flavour& f);
-  void Amplitudes(const LoadFn& load, double t);
+  // The channels' columns at t into amp_ (load, then tide); hasTide_ says
+  // whether the tide half is in use.
+  void Amplitudes(const Channels& channels, double t);
+  // Throws when the tide's degree-1 columns at t are not zero.
+  void CheckTide(const Channels& channels, double t) const;
   // RelativeChange (src/Viscoelastic3D.cpp:1636-1669): over every entry,
   // or over the active ones (where the inactive ones are zero in both).
   double RelativeChange(const double* next, const double* now, bool everyEntry);
-  void ElasticStrainCoefficients(DeviceState& state, const LoadFn& load, double* out);
+  void ElasticStrainCoefficients(DeviceState& state, const Channels& channels, double* out);
+  const double* TideAmplitudes() const {
+    return hasTide_ ? amp_.data() + ampDoubles_ : nullptr;
+  }
 
   const Earth* earth_;
   StepOptions options_;
@@ -226,6 +243,8 @@ class Viscoelastic3D {
   std::list<std::unique_ptr<Flavour>> flavours_;
   // the open step's buffers
   DeviceArray<double> weight_, beta_, fixed_, analysed_, amp_, s_, sNext_;
+  size_t ampDoubles_ = 0;  // one channel's amplitudes, (L + 1)(L + 2)
+  bool hasTide_ = false;
   DeviceArray<double> force_, response_;  // per pass, on the memory dofs
   DeviceArray<double> rhs_;               // full solves for export only
   DeviceArray<double> fourierA_[3], fourierB_[3];

@@ -149,26 +149,30 @@ class Earth:
     def degree_system(self, l, kind, scheme=EXPONENTIAL_TRAPEZOID, dt=-1.0, eps=1e-8):
         """The assembled degree-l system: ndof, kd, first element, pinned
         dof, unscaled upper band ((ndof, kd+1): row j is column j of the
-        band), dof map and unit load."""
+        band), dof map, unit load and unit tide (the potential (r/a)^p,
+        p = 2 at l = 0, else l)."""
         ndof, kd, first, pinned = (ctypes.c_int() for _ in range(4))
         check(lib.sl3_earth_degree_system(self.handle, l, kind, scheme, ctypes.c_double(dt),
                                           ctypes.c_double(eps), ctypes.byref(ndof),
                                           ctypes.byref(kd), ctypes.byref(first),
-                                          ctypes.byref(pinned), None, None, None))
+                                          ctypes.byref(pinned), None, None, None, None))
         elements, ngll = self.mesh()
         nodes = elements * (ngll - 1) + 1
         band = np.zeros((ndof.value, kd.value + 1))
         dofs = np.zeros(nodes * (3 if kind == 0 else 1), dtype=np.int32)
         load = np.zeros(ndof.value)
+        tide = np.zeros(ndof.value)
         check(lib.sl3_earth_degree_system(
             self.handle, l, kind, scheme, ctypes.c_double(dt), ctypes.c_double(eps),
             ctypes.byref(ndof), ctypes.byref(kd), ctypes.byref(first), ctypes.byref(pinned),
             band.ctypes.data_as(c_double_p), dofs.ctypes.data_as(c_int_p),
-            load.ctypes.data_as(c_double_p) if kind == 0 else None))
+            load.ctypes.data_as(c_double_p) if kind == 0 else None,
+            tide.ctypes.data_as(c_double_p) if kind == 0 else None))
         return dict(ndof=ndof.value, kd=kd.value, first_element=first.value,
                     pinned=pinned.value, band=band,
                     dofs=dofs.reshape(nodes, 3) if kind == 0 else dofs,
-                    load=load if kind == 0 else None)
+                    load=load if kind == 0 else None,
+                    tide=tide if kind == 0 else None)
 
 
 def zonal_load_rate(lmax, earth):
@@ -305,12 +309,13 @@ class Viscoelastic3D:
     def zero_state(self):
         return State(self)
 
-    def step(self, state, dt, scheme=EXPONENTIAL_TRAPEZOID, load=None, start=None):
-        """Step (src/Viscoelastic3D.cpp:1982-2030) in place on the state."""
+    def step(self, state, dt, scheme=EXPONENTIAL_TRAPEZOID, load=None, start=None, tide=None):
+        """Step (src/Viscoelastic3D.cpp:1982-2030) in place on the state;
+        load and tide are LOAD_FN channels or None."""
         info = StepInfo()
         ptr = ctypes.c_void_p(start.data_ptr()) if start is not None else None
         check(lib.sl3_step(self.handle, state.handle, ctypes.c_double(dt), scheme,
-                           load, None, ptr, ctypes.byref(info)))
+                           load, tide, None, ptr, ctypes.byref(info)))
         return info
 
     def export(self, name):

@@ -27,7 +27,7 @@ class Problem(ctypes.Structure):
         ("mu_node", DP), ("itau_ref_node", DP), ("itau", DP),
         ("ndof", IP), ("kd", IP), ("first_element", IP), ("pinned", IP),
         ("band_off", I64P), ("bands", DP), ("dof_off", I64P), ("dofs", IP),
-        ("load_off", I64P), ("loads", DP),
+        ("load_off", I64P), ("loads", DP), ("tide_off", I64P), ("tides", DP),
     ]
 
 
@@ -85,7 +85,7 @@ class OracleStep:
         jac = earth.export("jac")[first: first + count].copy()
         tasks = [(l, 0) for l in range(lmax + 1)] + [(l, 1) for l in range(1, lmax + 1)]
         ndof, kd, fe, pinned = [], [], [], []
-        bands, band_off, dofs, dof_off, loads, load_off = [], [], [], [], [], []
+        bands, band_off, dofs, dof_off, loads, load_off, tides = [], [], [], [], [], [], []
         bo = do = lo = 0
         self.systems = []
         for (l, kind) in tasks:
@@ -113,6 +113,7 @@ class OracleStep:
             do += len(table)
             if kind == 0:
                 loads.append(s["load"])
+                tides.append(s["tide"])
                 load_off.append(lo)
                 lo += s["ndof"]
         self.arrays = dict(
@@ -126,6 +127,7 @@ class OracleStep:
             band_off=np.array(band_off, dtype=np.int64), bands=np.concatenate(bands),
             dof_off=np.array(dof_off, dtype=np.int64), dofs=np.concatenate(dofs),
             load_off=np.array(load_off, dtype=np.int64), loads=np.concatenate(loads),
+            tides=np.concatenate(tides),
         )
         a = self.arrays
         p = Problem()
@@ -135,13 +137,13 @@ class OracleStep:
         for name, key in [("radius", "radius"), ("jac", "jac"), ("pweight", "pweight"),
                           ("pvalue", "pvalue"), ("pderiv", "pderiv"), ("mu_node", "mu"),
                           ("itau_ref_node", "itau_ref"), ("itau", "itau"),
-                          ("bands", "bands"), ("loads", "loads")]:
+                          ("bands", "bands"), ("loads", "loads"), ("tides", "tides")]:
             setattr(p, name, a[key].ctypes.data_as(DP))
         for name, key in [("ndof", "ndof"), ("kd", "kd"), ("first_element", "fe"),
                           ("pinned", "pinned"), ("dofs", "dofs")]:
             setattr(p, name, a[key].ctypes.data_as(IP))
         for name, key in [("band_off", "band_off"), ("dof_off", "dof_off"),
-                          ("load_off", "load_off")]:
+                          ("load_off", "load_off"), ("tide_off", "load_off")]:
             setattr(p, name, a[key].ctypes.data_as(I64P))
         self.problem = p
         self.lmax, self.nodes, self.ntheta, self.nphi = lmax, nodes, nt, nph
@@ -161,19 +163,20 @@ class OracleStep:
         self.state.has_coef = 0
 
     def step(self, dt, scheme, amp, max_iterations=40, rtol=1e-11, damping=1.0,
-             anderson_memory=5, anderson_rate=0.5):
+             anderson_memory=5, anderson_rate=0.5, tamp=None):
         o = Options(max_iterations, rtol, damping, anderson_memory, anderson_rate)
         info = Info()
         amp_ptr = amp.ctypes.data_as(DP) if amp is not None else None
+        tamp_ptr = tamp.ctypes.data_as(DP) if tamp is not None else None
         status = self.oracle.sl3o_step(ctypes.byref(self.problem), ctypes.byref(self.state),
-                                       D(dt), scheme, amp_ptr, ctypes.byref(o),
+                                       D(dt), scheme, amp_ptr, tamp_ptr, ctypes.byref(o),
                                        ctypes.byref(info))
         assert status == 0, "oracle factorisation failed"
         return info
 
 
 def amplitudes(load_fn, lmax, t):
-    """The load channel's columns at t, complex per (l, m) at l(l+1)/2 + m."""
+    """A channel's columns at t, complex per (l, m) at l(l+1)/2 + m."""
     amp = np.zeros(2 * (lmax + 1) * (lmax + 2) // 2)
     cols = (D * (2 * (lmax + 1)))()
     for l in range(lmax + 1):
@@ -187,3 +190,35 @@ def amplitudes(load_fn, lmax, t):
 
 def oracle_library():
     return ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+
+
+def tidal_rate(lmax, earth, scale=1.0):
+    """A tide channel growing linearly from zero at t = 0, so the zero
+    state is its elastic start: psi_lm(t) = rate_lm t, with degree 0 and
+    degrees 2..lmax, orders 0 and 2 (the centrifugal and semidiurnal
+    shapes), about 3 m^2 s^-2 per 30 years at degree 2 falling by e per
+    three degrees; degree 1 stays zero (a tide carries none). In the
+    non-dimensional units; returns a LOAD_FN for the C ABI."""
+    from sl3dpp_amd.step import LOAD_FN, YEAR
+
+    length, time, _ = earth.scales()
+    potential = (length / time) ** 2
+    rate = {0: (0.4, 0.0, None)}
+    for l in range(2, lmax + 1):
+        r = 3.0 * math.exp(-(l - 2) / 3.0) / (30.0 * YEAR) * scale
+        rate[l] = (r, 0.0, (0.6 * r, -0.3 * r))
+
+    def tide(l, t, columns, user):
+        if l not in rate:
+            return 0
+        for k in range(2 * (l + 1)):
+            columns[k] = 0.0
+        f = (t * time) / potential
+        r0, _, r2 = rate[l]
+        columns[0] = r0 * f
+        if r2 is not None and l >= 2:
+            columns[4] = r2[0] * f
+            columns[5] = r2[1] * f
+        return 1
+
+    return LOAD_FN(tide)
