@@ -28,6 +28,28 @@ int sl3o_last_residuals(double* out, int capacity) {
   return g_residual_count;
 }
 
+/* The last pass's solutions of the last step (StepResult::solutions), kept
+ * until the next step. */
+static double** g_solutions = NULL;
+static int64_t* g_solution_size = NULL;
+static int g_solution_tasks = 0;
+
+static void keep_solutions(double** sol, int64_t* size, int ntasks) {
+  for (int ti = 0; ti < g_solution_tasks; ++ti) free(g_solutions[ti]);
+  free(g_solutions);
+  free(g_solution_size);
+  g_solutions = sol;
+  g_solution_size = size;
+  g_solution_tasks = ntasks;
+}
+
+int64_t sl3o_last_solution(int task, double* out, int64_t capacity) {
+  if (task < 0 || task >= g_solution_tasks) return -1;
+  const int64_t n = g_solution_size[task];
+  if (out != NULL) memcpy(out, g_solutions[task], sizeof(double) * (n < capacity ? n : capacity));
+  return n;
+}
+
 /* ---- layouts (include/sl3dpp/SurfaceGrid.hpp:116-118, 191-200) ---------- */
 
 static int64_t block_size(int b, int lmax) {
@@ -711,8 +733,11 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
       free(grid[b]);
       if (d0[b] != st->strain[b]) free(d0[b]);
     }
-    for (int ti = 0; ti < ntasks; ++ti) free(sol[ti]);
-    free(sol);
+    {
+      int64_t* size = (int64_t*)malloc(sizeof(int64_t) * ntasks);
+      for (int ti = 0; ti < ntasks; ++ti) size[ti] = (int64_t)2 * tasks[ti].ndof * (tasks[ti].l + 1);
+      keep_solutions(sol, size, ntasks);
+    }
     free(beta);
     free(weight);
     free(s);

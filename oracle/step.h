@@ -94,6 +94,12 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
 /* The residual history of the last sl3o_step (count passes). */
 int sl3o_last_residuals(double* out, int capacity);
 
+/* The last pass's solutions of the last sl3o_step for one task (spheroidal
+ * l = 0..lmax, then toroidal 1..lmax): ndof x (l+1) complex, column-major,
+ * unscaled, with the channels. Returns the size in doubles (out NULL for
+ * the size only), or -1 for an unknown task. */
+int64_t sl3o_last_solution(int task, double* out, int64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1052,6 +1052,7 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
   double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
   double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
   Flavour& f = FlavourOf(scheme, dt);
+  lastStep_ = {scheme, dt};
   const double tEnd = state.t + dt;
 
   SL3_HIP(hipEventRecord(events_[0], s));
@@ -1210,8 +1211,10 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
 
 std::vector<double> Viscoelastic3D::SolutionOfTask(int task) {
   const auto& t = tasks_.at(task);
-  if (flavours_.empty()) throw ArgumentError("no step has been taken");
-  Flavour& f = *flavours_.front();
+  if (lastStep_.scheme < 0) throw ArgumentError("no step has been taken");
+  // the factors of the last step's own flavour (a first step's elastic
+  // start uses another one after it)
+  Flavour& f = FlavourOf(lastStep_.scheme, lastStep_.dt);
   hipStream_t s = Stream();
   // the band solves of the last pass's force on every dof
   rhs_.Resize(std::max<size_t>(rhsDoubles_, 1));

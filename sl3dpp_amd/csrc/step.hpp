@@ -244,6 +244,10 @@ class Viscoelastic3D {
   // the open step's buffers
   DeviceArray<double> weight_, beta_, fixed_, analysed_, amp_, s_, sNext_;
   size_t ampDoubles_ = 0;  // one channel's amplitudes, (L + 1)(L + 2)
+  struct {
+    int scheme = -1;  // none yet
+    double dt = 0.0;
+  } lastStep_;  // the flavour of the last pass, for the solution export
   bool hasTide_ = false;
   DeviceArray<double> force_, response_;  // per pass, on the memory dofs
   DeviceArray<double> rhs_;               // full solves for export only

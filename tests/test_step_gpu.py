@@ -270,3 +270,41 @@ def test_degree_one_tide_is_rejected_before_the_state_changes(earth, scheme):
     assert state.t == t0
     for b in range(3):
         assert np.array_equal(state.export(f"m{b}"), before[b])
+
+
+def oracle_solution(oracle, task):
+    oracle.sl3o_last_solution.restype = ctypes.c_int64
+    n = oracle.sl3o_last_solution(task, None, 0)
+    assert n > 0
+    out = np.zeros(n)
+    oracle.sl3o_last_solution(task, out.ctypes.data_as(DP), n)
+    return out
+
+
+def test_step_solutions_match_oracle(earth, oracle):
+    """Step's solutions (StepResult::solutions: the last pass's per-degree
+    solutions with the channels, ndof x (l+1) complex) for spheroidal and
+    toroidal tasks, after a first step from the zero state (whose elastic
+    start uses the elastic flavour) and after a second step."""
+    lmax = 10
+    L, T, M = earth.scales()
+    dt = 100 * st.YEAR / T
+    opts = st.StepOptions.default(max_iterations=1, rtol=1e-14)
+    op = make_operator(earth, lmax, 0.7, opts)
+    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.export("itau"),
+                     theta=op.theta, wtheta=op.wtheta)
+    load, _ = st.zonal_load_rate(lmax, earth)
+    tide = tidal_rate(lmax, earth)
+    state = op.zero_state()
+    tasks = [0, 1, 2, lmax, lmax + 1, lmax + 2, 2 * lmax]  # sph 0, 1, 2, L; tor 1, 2, L
+    for k in range(2):
+        print(f"[step] solutions lmax {lmax} step {k}", flush=True)
+        op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load, tide=tide)
+        t = (k + 1) * dt
+        ref.step(dt, st.EXPONENTIAL_TRAPEZOID, amplitudes(load, lmax, t), max_iterations=1,
+                 rtol=1e-14, tamp=amplitudes(tide, lmax, t))
+        for task in tasks:
+            got = op.export(f"solution:{task}")
+            want = oracle_solution(oracle, task)
+            assert got.shape == want.shape
+            assert rel(got, want) < 1e-10, task
