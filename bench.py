@@ -75,14 +75,17 @@ def work_models(op, lmax, passes):
                 active += nodes - first_node[l]
     legendre_flops = 4.0 * nt * active  # real table x complex coefficient, FMA = 2
     # longitude transforms: R sequences of P points per ring (spin 0: two
-    # real rings per sequence); a direction costs 8 H^2 flops per sequence
+    # real rings per sequence); a direction costs 8 H kmax flops per sequence
+    # (the symmetric split's cos and sin products over the H frequencies and
+    # the kmax = floor((L + R - 1) / R) that the band limit leaves nonzero)
     R = 4 if nph % 4 == 0 else 2
     P = nph // R
     H = (P - 1) // 2
+    kmax = min(H, (lmax + R - 1) // R)
     seq = nodes * nt * R * (1.0 + 1.0 + 0.5)  # the three blocks
     directions = sum(2 * k + 3 for k in passes)  # pass: 2; start: 2; finish: 1
     launches = sum(3 * (k + 2) for k in passes)
-    ring_flops = seq * 8.0 * H * H * directions / launches
+    ring_flops = seq * 8.0 * H * kmax * directions / launches
     # the memory response: x = G f per degree task
     response_flops = sum(2.0 * t[4] * t[4] * 2 * (t[0] + 1) for t in tasks)
     S = op.spectral_doubles * 8.0

@@ -24,6 +24,12 @@
 // transform's accumulators are the forward transform's operands and the
 // pointwise product between them never leaves the registers. The spin-0
 // block packs two real rings into one complex sequence.
+//
+// The band limit |m| <= L leaves h_k = h_{P-k} = 0 for k > kmax =
+// floor((L + R - 1) / R): the inverse contracts over k <= kmax only and the
+// forward computes only the row tiles that hold orders it stores. Grids
+// whose H fills the 16 JT rows exactly (FULL) drop the per-point padding
+// checks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,6 +78,7 @@ __device__ inline double ColumnSum(double v) {
 
 struct Geometry {
   int R, P, H, nphi, L, mlo, orders;
+  int kIn, tOut;  // the band limit: contraction blocks of the inverse, row tiles of the forward
   int ringsPerWave, ringsPerGroup;  // rings of the grid
   int zOrders;                      // 2 L + 1: the staged output, m = -L..L
 };
@@ -89,6 +96,9 @@ __device__ inline Geometry MakeGeometry(const RingLaunch& a) {
   g.ringsPerWave = (REAL ? 2 * kSeq : kSeq) / g.R;
   g.ringsPerGroup = kWaves * g.ringsPerWave;
   g.zOrders = 2 * g.L + 1;
+  const int kmax = min(g.H, (g.L + g.R - 1) / g.R);
+  g.kIn = (kmax + 3) / 4;
+  g.tOut = (kmax + 15) / 16;
   return g;
 }
 
@@ -181,7 +191,7 @@ __device__ inline double ForwardButterfly(const Geometry& g, const Lane& ln, dou
 
 // The P-point inverse transforms of the lane's sequence on the matrix
 // cores, from the staged coefficients: Y at j and P - j, and at 0.
-template <int JT, bool REAL>
+template <int JT, bool REAL, bool FULL>
 __device__ inline void InverseSequences(const Geometry& g, const Lane& ln, const double2* Fs,
                                         const double* tabC, const double* tabS,
                                         double (&yj)[4 * JT], double (&yq)[4 * JT], double& y0) {
@@ -190,10 +200,14 @@ __device__ inline void InverseSequences(const Geometry& g, const Lane& ln, const
   double sum = 0.0;
 #pragma unroll
   for (int kb = 0; kb < KS; ++kb) {
-    const int k = min(1 + 4 * kb + ln.quad, g.H);
+    if (kb >= g.kIn) {
+      av[kb] = bv[kb] = 0.0;
+      continue;
+    }
+    const int k = FULL ? 1 + 4 * kb + ln.quad : min(1 + 4 * kb + ln.quad, g.H);
     const double2 zk = Gather<REAL>(g, Fs, ln.slot, SignedOrder(g, g.R * k + ln.m2));
     const double2 zq = Gather<REAL>(g, Fs, ln.slot, SignedOrder(g, g.R * (g.P - k) + ln.m2));
-    const bool live = 1 + 4 * kb + ln.quad <= g.H;
+    const bool live = FULL || 1 + 4 * kb + ln.quad <= g.H;
     av[kb] = live ? (ln.p == 0 ? zk.x + zq.x : zk.y + zq.y) : 0.0;
     bv[kb] = live ? (ln.p == 0 ? zk.x - zq.x : zk.y - zq.y) : 0.0;
     sum += av[kb];
@@ -210,6 +224,7 @@ __device__ inline void InverseSequences(const Geometry& g, const Lane& ln, const
   // 2 JT independent accumulation chains, interleaved
 #pragma unroll
   for (int kb = 0; kb < KS; ++kb) {
+    if (kb >= g.kIn) break;
 #pragma unroll
     for (int t = 0; t < JT; ++t) {
       const int at = (t * KS + kb) * kWave + ln.lane;
@@ -233,7 +248,7 @@ __device__ inline void InverseSequences(const Geometry& g, const Lane& ln, const
 // The forward transforms of the lane's sequence (u at j, P - j and 0,
 // after the forward butterflies) on the matrix cores; each output G(k) is
 // the order R k + m2, scattered to the staged tile Zd[slot][m + L].
-template <int JT>
+template <int JT, bool FULL>
 __device__ inline void ForwardSequences(const Geometry& g, const Lane& ln, const double* tabC,
                                         const double* tabS, const double (&uj)[4 * JT],
                                         const double (&uq)[4 * JT], double u0, double* Zd) {
@@ -257,6 +272,7 @@ __device__ inline void ForwardSequences(const Geometry& g, const Lane& ln, const
   for (int kb = 0; kb < KS; ++kb) {
 #pragma unroll
     for (int t = 0; t < JT; ++t) {
+      if (t >= g.tOut) break;
       const int at = (t * KS + kb) * kWave + ln.lane;
       C[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tabC[at], av[kb], C[t], 0, 0, 0);
       S[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tabS[at], bv[kb], S[t], 0, 0, 0);
@@ -273,11 +289,12 @@ __device__ inline void ForwardSequences(const Geometry& g, const Lane& ln, const
   const double sg = ln.p == 0 ? 1.0 : -1.0;
 #pragma unroll
   for (int t = 0; t < JT; ++t) {
+    if (t >= g.tOut) break;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const double sx = Xor1(S[t][q]);
       const int k = 1 + 16 * t + 4 * q + ln.quad;
-      if (k <= g.H) {
+      if (FULL || k <= g.H) {
         put(k, u0 + C[t][q] + sg * sx);
         put(g.P - k, u0 + C[t][q] - sg * sx);
       }
@@ -321,7 +338,7 @@ __device__ inline void CopyOut(const Geometry& g, const RingLaunch& a, const dou
 // weighted, node scaled, pass), the step start (two analyses).
 enum { kClassSynth = 0, kClassOne = 1, kClassStart = 2 };
 
-template <int JT, bool REAL, int CLASS>
+template <int JT, bool REAL, int CLASS, bool FULL>
 __device__ inline void RingBody(const RingLaunch& a) {
   constexpr int KS = 4 * JT;
   extern __shared__ double lds[];
@@ -329,6 +346,37 @@ __device__ inline void RingBody(const RingLaunch& a) {
   const int node = blockIdx.y;
   const int th0 = blockIdx.x * g.ringsPerGroup;
   const Lane ln = MakeLane<REAL>(g, th0, a.ntheta);
+  // The lane's points: n1 = j = 1 + 16 t + 4 q + quad (v = 4 t + q), P - j
+  // and 0, each at n1 + P n2. Where a lane has no point (padding, a ring
+  // past the last) the index is its ring's point P n2, always in bounds:
+  // loads are unconditional and issue back to back, only stores are
+  // predicated. For spin 0 a sequence's two parts are two rings: the lane
+  // holds its own ring's value at every point.
+  const int th = th0 + ln.ring;
+  const size_t ringBase = (static_cast<size_t>(node) * a.ntheta + (ln.valid ? th : 0)) * g.nphi;
+  const size_t p0 = ringBase + static_cast<size_t>(g.P) * ln.n2;
+  auto jOf = [&](int v) { return 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad; };
+  auto lv = [&](int v) { return ln.valid && (FULL || jOf(v) <= g.H); };
+  auto pj = [&](int v) -> size_t { return FULL || lv(v) ? p0 + jOf(v) : p0; };
+  auto pq = [&](int v) -> size_t { return FULL || lv(v) ? p0 + (g.P - jOf(v)) : p0; };
+  auto element = [&](size_t pt) -> size_t { return REAL ? pt : 2 * pt + ln.p; };
+  // the twiddle index m2 n1 at n1 = j (clamped to H) and P - j
+  auto twj = [&](int v) { return ln.m2 * (FULL ? jOf(v) : min(jOf(v), g.H)); };
+  auto twq = [&](int v) { return ln.m2 * (g.P - (FULL ? jOf(v) : min(jOf(v), g.H))); };
+  const bool own0 = ln.valid && ln.quad == 0;  // the lane that stores point P n2
+
+  // the pass's and the weighted analysis' weights first: they stream in
+  // under the staging and the inverse transforms
+  const bool weighted = CLASS == kClassOne && (a.op == kRingPass || a.op == kRingWeighted);
+  double wj[KS], wq[KS], w0 = 0.0;
+  if (weighted) {
+#pragma unroll
+    for (int v = 0; v < KS; ++v) {
+      wj[v] = a.weight[pj(v)];
+      wq[v] = a.weight[pq(v)];
+    }
+    w0 = a.weight[p0];
+  }
   // LDS: the twiddles cos / sin(2 pi n / nphi), the operand tables (all
   // classes but the step start, which keeps them in the cache), then one
   // tile that holds the staged input coefficients and, after them, the
@@ -371,30 +419,12 @@ __device__ inline void RingBody(const RingLaunch& a) {
   }
   __syncthreads();
 
-  // The lane's points: n1 = j = 1 + 16 t + 4 q + quad (v = 4 t + q), P - j
-  // and 0, each at n1 + P n2. Where a lane has no point (padding, a ring
-  // past the last) the index is its ring's point P n2, always in bounds:
-  // loads are unconditional and issue back to back, only stores are
-  // predicated. For spin 0 a sequence's two parts are two rings: the lane
-  // holds its own ring's value at every point.
-  const int th = th0 + ln.ring;
-  const size_t ringBase = (static_cast<size_t>(node) * a.ntheta + (ln.valid ? th : 0)) * g.nphi;
-  const size_t p0 = ringBase + static_cast<size_t>(g.P) * ln.n2;
-  auto jOf = [&](int v) { return 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad; };
-  auto lv = [&](int v) { return ln.valid && jOf(v) <= g.H; };
-  auto pj = [&](int v) -> size_t { return lv(v) ? p0 + jOf(v) : p0; };
-  auto pq = [&](int v) -> size_t { return lv(v) ? p0 + (g.P - jOf(v)) : p0; };
-  auto element = [&](size_t pt) -> size_t { return REAL ? pt : 2 * pt + ln.p; };
-  // the twiddle index m2 n1 at n1 = j (clamped to H) and P - j
-  auto twj = [&](int v) { return ln.m2 * min(jOf(v), g.H); };
-  auto twq = [&](int v) { return ln.m2 * (g.P - min(jOf(v), g.H)); };
-  const bool own0 = ln.valid && ln.quad == 0;  // the lane that stores point P n2
 
   // x at the lane's points
   double xj[KS], xq[KS], x0 = 0.0;
   if (synth) {
     double yj[KS], yq[KS], y0;
-    InverseSequences<JT, REAL>(g, ln, stage, tabC, tabS, yj, yq, y0);
+    InverseSequences<JT, REAL, FULL>(g, ln, stage, tabC, tabS, yj, yq, y0);
 #pragma unroll
     for (int v = 0; v < KS; ++v) {
       xj[v] = InverseButterfly(g, ln, yj[v], cosN[twj(v)], sinN[twj(v)]);
@@ -425,7 +455,7 @@ __device__ inline void RingBody(const RingLaunch& a) {
       uq[v] = ForwardButterfly(g, ln, yq[v], cosN[twq(v)], sinN[twq(v)]);
     }
     const double u0 = ForwardButterfly(g, ln, y0, 1.0, 0.0);
-    ForwardSequences<JT>(g, ln, tabC, tabS, uj, uq, u0, Zd);
+    ForwardSequences<JT, FULL>(g, ln, tabC, tabS, uj, uq, u0, Zd);
     __syncthreads();
     CopyOut<REAL>(g, a, Zs, node, th0, out);
   };
@@ -533,14 +563,7 @@ __device__ inline void RingBody(const RingLaunch& a) {
   } else {
     // y = factor x (pass and weighted: the point's weight; node scaled: mu)
     double yj[KS], yq[KS], y0 = x0;
-    if (a.op == kRingPass || a.op == kRingWeighted) {
-      double wj[KS], wq[KS];
-#pragma unroll
-      for (int v = 0; v < KS; ++v) {
-        wj[v] = a.weight[pj(v)];
-        wq[v] = a.weight[pq(v)];
-      }
-      const double w0 = a.weight[p0];
+    if (weighted) {
       y0 = ln.valid ? w0 * x0 : 0.0;
 #pragma unroll
       for (int v = 0; v < KS; ++v) {
@@ -569,34 +592,45 @@ __device__ inline void RingBody(const RingLaunch& a) {
 // The per-pass classes run two waves per SIMD (two workgroups per CU, as
 // the LDS tile allows): a register budget of 256 per lane. The step start,
 // once per step, keeps its larger live set unspilled at one wave.
-template <int JT, bool REAL, int CLASS>
+template <int JT, bool REAL, int CLASS, bool FULL>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
     RingMfmaKernel(RingLaunch a) {
-  RingBody<JT, REAL, CLASS>(a);
+  RingBody<JT, REAL, CLASS, FULL>(a);
 }
 
-template <int JT, bool REAL>
+template <int JT, bool REAL, bool FULL>
 __global__ void __launch_bounds__(kThreads) RingMfmaStartKernel(RingLaunch a) {
-  RingBody<JT, REAL, kClassStart>(a);
+  RingBody<JT, REAL, kClassStart, FULL>(a);
 }
 
-template <int JT, int CLASS>
+template <int JT, int CLASS, bool FULL>
 void LaunchClass(const RingLaunch& r, dim3 grid, size_t lds, hipStream_t stream) {
   if constexpr (CLASS == kClassStart) {
     if (r.block == 2) {
-      hipLaunchKernelGGL((RingMfmaStartKernel<JT, true>), grid, dim3(kThreads), lds, stream, r);
+      hipLaunchKernelGGL((RingMfmaStartKernel<JT, true, FULL>), grid, dim3(kThreads), lds,
+                         stream, r);
     } else {
-      hipLaunchKernelGGL((RingMfmaStartKernel<JT, false>), grid, dim3(kThreads), lds, stream,
-                         r);
+      hipLaunchKernelGGL((RingMfmaStartKernel<JT, false, FULL>), grid, dim3(kThreads), lds,
+                         stream, r);
     }
   } else {
     if (r.block == 2) {
-      hipLaunchKernelGGL((RingMfmaKernel<JT, true, CLASS>), grid, dim3(kThreads), lds, stream,
-                         r);
+      hipLaunchKernelGGL((RingMfmaKernel<JT, true, CLASS, FULL>), grid, dim3(kThreads), lds,
+                         stream, r);
     } else {
-      hipLaunchKernelGGL((RingMfmaKernel<JT, false, CLASS>), grid, dim3(kThreads), lds,
+      hipLaunchKernelGGL((RingMfmaKernel<JT, false, CLASS, FULL>), grid, dim3(kThreads), lds,
                          stream, r);
     }
+  }
+}
+
+template <int JT, int CLASS>
+void LaunchFull(const RingLaunch& r, dim3 grid, size_t lds, hipStream_t stream) {
+  // every one of the 16 JT rows is a frequency: no padding checks
+  if ((r.tables.P - 1) / 2 == 16 * JT) {
+    LaunchClass<JT, CLASS, true>(r, grid, lds, stream);
+  } else {
+    LaunchClass<JT, CLASS, false>(r, grid, lds, stream);
   }
 }
 
@@ -605,14 +639,14 @@ void LaunchJT(const RingLaunch& r, dim3 grid, size_t lds, hipStream_t stream) {
   switch (r.op) {
     case kRingSynthesise:
     case kRingFinish:
-      LaunchClass<JT, kClassSynth>(r, grid, lds, stream);
+      LaunchFull<JT, kClassSynth>(r, grid, lds, stream);
       break;
     case kRingStart:
     case kRingStartFromSpectral:
-      LaunchClass<JT, kClassStart>(r, grid, lds, stream);
+      LaunchFull<JT, kClassStart>(r, grid, lds, stream);
       break;
     default:
-      LaunchClass<JT, kClassOne>(r, grid, lds, stream);
+      LaunchFull<JT, kClassOne>(r, grid, lds, stream);
   }
 }
 
