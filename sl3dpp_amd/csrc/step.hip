@@ -315,6 +315,7 @@ void Coupling::AnalyseFromFourier(const double* const fourier[3], double* spectr
     for (int b = 0; b < 3; ++b) adds[b] = addend + spectralOffset_[b];
   }
   LaunchLegendreAnalysis(plan_, fourier, blocks, addend ? adds : nullptr,
+                         truncated ? reach_.data() : nullptr,
                          truncated ? firstNode_.data() : nullptr, s);
 }
 
@@ -347,7 +348,7 @@ void Coupling::AnalyseBlockFromFourier(int block, const double* fourier,
   double* blocks[3] = {};
   blocks[block] = spectral;
   fouriers[block] = fourier;
-  LaunchLegendreAnalysis(p, fouriers, blocks, nullptr, nullptr, s);
+  LaunchLegendreAnalysis(p, fouriers, blocks, nullptr, nullptr, nullptr, s);
 }
 
 void Coupling::Ring(RingLaunch r, hipStream_t s) const {

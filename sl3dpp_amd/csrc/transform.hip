@@ -5,13 +5,8 @@
 // with the quadrature weights (SurfaceGrid.hpp:141-252 is the contract;
 // the reference runs GSHTrans loops on the CPU).
 //
-// Legendre stage: per order m a dense real GEMM, [ntheta x K_m] times
-// [K_m x 2 nodes] (complex coefficients as interleaved real columns),
-// 64 x 64 output tiles per 256-thread workgroup, 4 x 4 per thread, staged
-// through LDS in K chunks of 16. With a per-node degree reach (the degree
-// truncation of the radial systems), a tile's K range stops at the largest
-// degree its nodes carry, and analysis tiles whose nodes all lie below a
-// degree's truncation are written as zeros: both only skip exact zeros.
+// Legendre stage: per order m a real GEMM on the matrix cores
+// (legendre_mfma.hip).
 //
 // Longitude stage: nphi = R P (R = 4 or 2), decimation in time: the R
 // P-point DFTs of x_{R n + r} by the symmetric split (pairs n, P - n give a
@@ -26,199 +21,11 @@
 
 namespace sl3amd {
 
-namespace {
-
-constexpr int kTile = 64;
-constexpr int kK = 16;
-constexpr int kThreads = 256;
-constexpr int kPer = kK * kTile / kThreads;  // staged elements per thread
-
-// z index of the launch -> (block, m)
-struct OrderRef {
-  int block;
-  int m;
-};
-
-struct LegendreArgs {
-  const double* table[3];
-  const int* rowOffset[3];  // device, per m - mlo (+ total)
-  const double* spectral[3];
-  double* spectralOut[3];
-  const double* addend[3];  // analysis epilogue: out = acc + addend (or null)
-  double* fourier[3];
-  const double* fourierIn[3];
-  int mlo[3];
-  int spin[3];
-  const OrderRef* orders;  // per z
-  const int* reach;        // per node: the largest degree it carries (or null)
-  const int* firstNode;    // per degree: its first node (or null)
-  int lmax, nodes, ntheta;
-};
-
-__device__ inline int LMinOf(int m, int spin) {
-  const int am = m < 0 ? -m : m, an = spin < 0 ? -spin : spin;
-  return am > an ? am : an;
-}
-
-// C[theta][col] = sum_l T[(m,l)][theta] S[(m,l)][col], written to the
-// Fourier layout F[((m - mlo) nodes + node) ntheta + theta] (re, im).
-__global__ void __launch_bounds__(kThreads) LegendreSynthesisKernel(LegendreArgs a) {
-  __shared__ double As[kK][kTile + 1];
-  __shared__ double Bs[kK][kTile + 1];
-  const OrderRef ord = a.orders[blockIdx.z];
-  const int b = ord.block, m = ord.m;
-  const int lmin = LMinOf(m, a.spin[b]);
-  const int cols = 2 * a.nodes;
-  const int c0 = blockIdx.x * kTile, t0 = blockIdx.y * kTile;
-  if (lmin > a.lmax) return;
-  int lhi = a.lmax;
-  if (a.reach) {
-    const int lastNode = min(a.nodes - 1, (c0 + kTile - 1) / 2);
-    lhi = min(lhi, a.reach[lastNode]);
-  }
-  const int row0 = a.rowOffset[b][m - a.mlo[b]] - lmin;  // row of (m, l) = row0 + l
-  const double* T = a.table[b];
-  const double* S = a.spectral[b];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[4][4] = {};
-  // Each thread stages kPer elements of A and of B per K chunk; the next
-  // chunk's loads are in flight while this one is multiplied. Loads are
-  // unconditional from clamped indices (they issue back to back), the
-  // padding is zeroed by selects.
-  double pa[kPer], pb[kPer];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = threadIdx.x + kThreads * q;
-      const int k = e / kTile, t = e % kTile;
-      const int l = k0 + k, th = t0 + t, c = c0 + t;
-      const int lc = min(l, lhi);
-      const double va = T[static_cast<size_t>(row0 + lc) * a.ntheta + min(th, a.ntheta - 1)];
-      const double vb = S[static_cast<size_t>(row0 + lc) * cols + min(c, cols - 1)];
-      pa[q] = (l <= lhi && th < a.ntheta) ? va : 0.0;
-      pb[q] = (l <= lhi && c < cols) ? vb : 0.0;
-    }
-  };
-  load(lmin);
-  for (int k0 = lmin; k0 <= lhi; k0 += kK) {
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = threadIdx.x + kThreads * q;
-      As[e / kTile][e % kTile] = pa[q];
-      Bs[e / kTile][e % kTile] = pb[q];
-    }
-    __syncthreads();
-    if (k0 + kK <= lhi) load(k0 + kK);
-#pragma unroll
-    for (int k = 0; k < kK; ++k) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[k][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[k][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
-    }
-    __syncthreads();
-  }
-  double* F = a.fourier[b];
-  const size_t mi = static_cast<size_t>(m - a.mlo[b]);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int th = t0 + ty + 16 * i;
-    if (th >= a.ntheta) continue;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = c0 + tx + 16 * j;
-      if (c >= cols) continue;
-      const int node = c >> 1, part = c & 1;
-      F[((mi * a.nodes + node) * a.ntheta + th) * 2 + part] = acc[i][j];
-    }
-  }
-}
-
-// C[(m,l)][col] = sum_theta T[(m,l)][theta] F[(m, node, theta)].part
-// (+ addend), zero where the tile's nodes lie below the degree's first node.
-__global__ void __launch_bounds__(kThreads) LegendreAnalysisKernel(LegendreArgs a) {
-  __shared__ double As[kK][kTile + 1];
-  __shared__ double Bs[kK][kTile + 1];
-  const OrderRef ord = a.orders[blockIdx.z];
-  const int b = ord.block, m = ord.m;
-  const int lmin = LMinOf(m, a.spin[b]);
-  const int cols = 2 * a.nodes;
-  const int c0 = blockIdx.x * kTile;
-  const int l0 = lmin + blockIdx.y * kTile;
-  if (l0 > a.lmax) return;
-  const int row0 = a.rowOffset[b][m - a.mlo[b]] - lmin;
-  const int lastNode = min(a.nodes - 1, (c0 + kTile - 1) / 2);
-  double* out = a.spectralOut[b];
-  const double* add = a.addend[b];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  // Whole tile below every degree's truncation: zeros.
-  const bool dead = a.firstNode && a.firstNode[l0] > lastNode;
-  double acc[4][4] = {};
-  if (!dead) {
-    const double* T = a.table[b];
-    const double* F = a.fourierIn[b];
-    const size_t mi = static_cast<size_t>(m - a.mlo[b]);
-    for (int k0 = 0; k0 < a.ntheta; k0 += kK) {
-      for (int e = threadIdx.x; e < kK * kTile; e += kThreads) {
-        // A: As[k][l] with theta = k0 + k fastest in memory
-        const int lt = e / kK, k = e % kK;
-        const int l = l0 + lt, th = k0 + k;
-        As[k][lt] = (l <= a.lmax && th < a.ntheta)
-                        ? T[static_cast<size_t>(row0 + l) * a.ntheta + th]
-                        : 0.0;
-        // B: Bs[k][c], (theta, part) pairs contiguous per node
-        const int nodeLocal = e / (2 * kK), rem = e % (2 * kK);
-        const int kk = rem >> 1, part = rem & 1;
-        const int node = c0 / 2 + nodeLocal, thb = k0 + kk;
-        const int c = 2 * nodeLocal + part;
-        Bs[kk][c] = (node < a.nodes && thb < a.ntheta)
-                        ? F[((mi * a.nodes + node) * a.ntheta + thb) * 2 + part]
-                        : 0.0;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < kK; ++k) {
-        double av[4], bv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) av[i] = As[k][ty + 16 * i];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) bv[j] = Bs[k][tx + 16 * j];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
-      }
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int l = l0 + ty + 16 * i;
-    if (l > a.lmax) continue;
-    const size_t row = static_cast<size_t>(row0 + l);
-    const int fn = a.firstNode ? a.firstNode[l] : 0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = c0 + tx + 16 * j;
-      if (c >= cols) continue;
-      double v = acc[i][j];
-      if ((c >> 1) < fn) v = 0.0;
-      if (add) v += add[row * cols + c];
-      out[row * cols + c] = v;
-    }
-  }
-}
-
-}  // namespace
-
 // ---- longitude DFT ---------------------------------------------------------
 
 namespace {
+
+constexpr int kThreads = 256;
 
 struct RingArgs {
   DftTables t;
@@ -634,48 +441,15 @@ size_t RingLds(int nphi, int P) {
 void LaunchLegendreSynthesis(const LegendrePlan& p, const double* const spectral[3],
                              double* const fourier[3], const int* reach,
                              hipStream_t stream) {
-  LegendreArgs a{};
-  for (int b = 0; b < 3; ++b) {
-    a.table[b] = p.table[b];
-    a.rowOffset[b] = p.rowOffset[b];
-    a.spectral[b] = spectral[b];
-    a.fourier[b] = fourier[b];
-    a.mlo[b] = p.mlo[b];
-    a.spin[b] = p.spin[b];
-  }
-  a.orders = reinterpret_cast<const OrderRef*>(p.orders);
-  a.reach = reach;
-  a.lmax = p.lmax;
-  a.nodes = p.nodes;
-  a.ntheta = p.ntheta;
-  const dim3 grid(CeilDiv(2 * p.nodes, kTile), CeilDiv(p.ntheta, kTile), p.orderCount);
-  ProfileScope prof634("legendre_synthesis", stream);
-  hipLaunchKernelGGL(LegendreSynthesisKernel, grid, dim3(kThreads), 0, stream, a);
-  SL3_HIP(hipGetLastError());
+  ProfileScope prof("legendre_synthesis", stream);
+  LaunchLegendreSynthesisMfma(p, spectral, fourier, reach, stream);
 }
 
 void LaunchLegendreAnalysis(const LegendrePlan& p, const double* const fourier[3],
                             double* const spectral[3], const double* const addend[3],
-                            const int* firstNode, hipStream_t stream) {
-  LegendreArgs a{};
-  for (int b = 0; b < 3; ++b) {
-    a.table[b] = p.table[b];
-    a.rowOffset[b] = p.rowOffset[b];
-    a.fourierIn[b] = fourier[b];
-    a.spectralOut[b] = spectral[b];
-    a.addend[b] = addend ? addend[b] : nullptr;
-    a.mlo[b] = p.mlo[b];
-    a.spin[b] = p.spin[b];
-  }
-  a.orders = reinterpret_cast<const OrderRef*>(p.orders);
-  a.firstNode = firstNode;
-  a.lmax = p.lmax;
-  a.nodes = p.nodes;
-  a.ntheta = p.ntheta;
-  const dim3 grid(CeilDiv(2 * p.nodes, kTile), CeilDiv(p.lmax + 1, kTile), p.orderCount);
-  ProfileScope prof658("legendre_analysis", stream);
-  hipLaunchKernelGGL(LegendreAnalysisKernel, grid, dim3(kThreads), 0, stream, a);
-  SL3_HIP(hipGetLastError());
+                            const int* reach, const int* firstNode, hipStream_t stream) {
+  ProfileScope prof("legendre_analysis", stream);
+  LaunchLegendreAnalysisMfma(p, fourier, spectral, addend, reach, firstNode, stream);
 }
 
 void LaunchRing(const RingLaunch& r, hipStream_t stream) {

@@ -83,9 +83,18 @@ void LaunchRingMfma(const RingLaunch& r, hipStream_t stream);
 void LaunchLegendreSynthesis(const LegendrePlan& p, const double* const spectral[3],
                              double* const fourier[3], const int* reach,
                              hipStream_t stream);
+// reach (per node, the largest degree it carries) and firstNode (per
+// degree, its first node) give the degree truncation, or null for none.
 void LaunchLegendreAnalysis(const LegendrePlan& p, const double* const fourier[3],
                             double* const spectral[3], const double* const addend[3],
-                            const int* firstNode, hipStream_t stream);
+                            const int* reach, const int* firstNode, hipStream_t stream);
+// The matrix-core Legendre stage (legendre_mfma.hip).
+void LaunchLegendreSynthesisMfma(const LegendrePlan& p, const double* const spectral[3],
+                                 double* const fourier[3], const int* reach,
+                                 hipStream_t stream);
+void LaunchLegendreAnalysisMfma(const LegendrePlan& p, const double* const fourier[3],
+                                double* const spectral[3], const double* const addend[3],
+                                const int* reach, const int* firstNode, hipStream_t stream);
 void LaunchRing(const RingLaunch& r, hipStream_t stream);
 
 }  // namespace sl3amd

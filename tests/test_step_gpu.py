@@ -3,11 +3,11 @@ the lateral field, the load):
  * the batched GSH transforms (SurfaceGrid's Synthesise/ExpandSpinBatch) for
    spins -2, -1, 0 against the oracle's direct sums, node by node (1e-12),
    on grids that take each longitude kernel (nphi = 4 P and 2 P with P odd,
-   and P even);
+   and P even) and each blocking of the Legendre stage, up to lmax 140;
  * one pass of the step (Converge capped at one pass: StartStep's sweep and
    analyses, the force, the memory response, the channel, strain recovery,
-   Finish) on the same grids up to lmax 32, where the degree truncation cuts
-   the deeper memory nodes (1e-11 relative);
+   Finish) on the same grids up to lmax 128, where the degree truncation
+   cuts the deeper memory nodes (1e-11 relative);
  * whole converged steps with lateral viscosity, adaptive Anderson mixing
    included: the residual history pass by pass, and the state to 1e-8
    relative at rtol 1e-10;
@@ -55,8 +55,11 @@ def make_operator(earth, lmax, delta_scale, options=None, seed=11):
 
 # lmax 8, 32: nphi = 4 P, P odd (the matrix-core ring kernel with R = 4);
 # lmax 10: nphi = 34 = 2 x 17 (R = 2); lmax 12: nphi = 40, P = 10 even (the
-# generic ring kernel).
-@pytest.mark.parametrize("lmax", [8, 10, 12, 32])
+# generic ring kernel); lmax 64: two 16-row tiles per P-point transform and
+# two 64-degree blocks in the Legendre analysis; lmax 140: ntheta 212, two
+# colatitude blocks in the Legendre synthesis (and P = 106, the generic
+# ring kernel).
+@pytest.mark.parametrize("lmax", [8, 10, 12, 32, 64, 140])
 @pytest.mark.parametrize("spin", [-2, -1, 0])
 def test_gsh_transforms_match_oracle(earth, oracle, lmax, spin):
     import torch
@@ -130,7 +133,9 @@ def run_pair(earth, oracle, lmax, delta_scale, steps, max_iterations, rtol, hist
     return out
 
 
-@pytest.mark.parametrize("lmax", [8, 10, 12, 32])
+# lmax 64: the two-tile ring kernel; lmax 128: three tiles whose rows are
+# all frequencies (H = 48), the workload's grid
+@pytest.mark.parametrize("lmax", [8, 10, 12, 32, 64, 128])
 def test_one_pass_matches_oracle(earth, oracle, lmax):
     results = run_pair(earth, oracle, lmax, 0.7, steps=2, max_iterations=1, rtol=1e-14)
     for info, rinfo, c, rc, m, rm, d, rd in results:
