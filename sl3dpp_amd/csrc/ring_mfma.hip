@@ -308,28 +308,38 @@ __device__ inline void ForwardSequences(const Geometry& g, const Lane& ln, const
 template <bool REAL>
 __device__ inline void CopyOut(const Geometry& g, const RingLaunch& a, const double2* Zs,
                                int node, int th0, double* out) {
+  constexpr int kBatch = 9;
   const double scale = 2.0 * 3.14159265358979323846 / g.nphi;
   const int mCount = g.L - g.mlo + 1;
-  const int tasks = mCount * g.ringsPerGroup;
-  for (int e = threadIdx.x; e < tasks; e += blockDim.x) {
-    const int ring = e % g.ringsPerGroup;
-    const int mo = e / g.ringsPerGroup;
-    const int th = th0 + ring;
-    if (th >= a.ntheta) continue;
-    const double w = scale * a.wtheta[th];
-    double2 res;
-    if (REAL) {
-      const int pair = ring / 2;
-      const double2 zp = Zs[pair * g.zOrders + (mo + g.L)];
-      const double2 zm = Zs[pair * g.zOrders + (-mo + g.L)];
-      res = ring % 2 == 0 ? make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y - zm.y))
-                          : make_double2(0.5 * (zp.y + zm.y), -0.5 * (zp.x - zm.x));
-    } else {
-      res = Zs[ring * g.zOrders + mo];
+  // element e = thread + 256 i: ring e % rings (fixed per thread), order e / rings
+  const int rpg = g.ringsPerGroup;
+  const int ring = static_cast<int>(threadIdx.x) % rpg;
+  const int moStep = kThreads / rpg;
+  const int th = th0 + ring;
+  if (th >= a.ntheta) return;
+  const double w = scale * a.wtheta[th];
+  double2* dst = reinterpret_cast<double2*>(out) + static_cast<size_t>(node) * a.ntheta + th;
+  const size_t mStride = static_cast<size_t>(a.nodes) * a.ntheta;
+  for (int mo0 = static_cast<int>(threadIdx.x) / rpg; mo0 < mCount; mo0 += kBatch * moStep) {
+    double2 res[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int mo = min(mo0 + u * moStep, mCount - 1);
+      if (REAL) {
+        const int pair = ring / 2;
+        const double2 zp = Zs[pair * g.zOrders + (mo + g.L)];
+        const double2 zm = Zs[pair * g.zOrders + (-mo + g.L)];
+        res[u] = ring % 2 == 0 ? make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y - zm.y))
+                               : make_double2(0.5 * (zp.y + zm.y), -0.5 * (zp.x - zm.x));
+      } else {
+        res[u] = Zs[ring * g.zOrders + mo];
+      }
     }
-    double2* dst = reinterpret_cast<double2*>(out) +
-                   (static_cast<size_t>(mo) * a.nodes + node) * a.ntheta + th;
-    *dst = make_double2(res.x * w, res.y * w);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int mo = mo0 + u * moStep;
+      if (mo < mCount) dst[mo * mStride] = make_double2(res[u].x * w, res[u].y * w);
+    }
   }
 }
 
@@ -390,14 +400,37 @@ __device__ inline void RingBody(const RingLaunch& a) {
   double2* stage = reinterpret_cast<double2*>(stageD);
   const double* tabC = a.tables.tabC;
   const double* tabS = a.tables.tabS;
-  for (int i = threadIdx.x; i < g.nphi; i += blockDim.x) {
-    cosN[i] = a.tables.cosN[i];
-    sinN[i] = a.tables.sinN[i];
+  // Staging: every copy issues all of a thread's loads before its stores,
+  // so their latencies overlap instead of adding up.
+  for (int i0 = 0; i0 < g.nphi; i0 += 2 * kThreads) {
+    double c[2], sn[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = i0 + threadIdx.x + u * kThreads;
+      c[u] = i < g.nphi ? a.tables.cosN[i] : 0.0;
+      sn[u] = i < g.nphi ? a.tables.sinN[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = i0 + threadIdx.x + u * kThreads;
+      if (i < g.nphi) {
+        cosN[i] = c[u];
+        sinN[i] = sn[u];
+      }
+    }
   }
   if constexpr (kTabLds) {
-    for (int i = threadIdx.x; i < kTab; i += blockDim.x) {
-      tables[i] = a.tables.tabC[i];
-      tables[kTab + i] = a.tables.tabS[i];
+    constexpr int kPerThread = kTab / kThreads;  // JT^2: the tables hold 256 JT^2 values
+    double tc[kPerThread], ts[kPerThread];
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      tc[u] = a.tables.tabC[threadIdx.x + u * kThreads];
+      ts[u] = a.tables.tabS[threadIdx.x + u * kThreads];
+    }
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      tables[threadIdx.x + u * kThreads] = tc[u];
+      tables[kTab + threadIdx.x + u * kThreads] = ts[u];
     }
     tabC = tables;
     tabS = tables + kTab;
@@ -405,16 +438,31 @@ __device__ inline void RingBody(const RingLaunch& a) {
   const bool synth = a.op == kRingSynthesise || a.op == kRingPass ||
                      a.op == kRingFinish || a.op == kRingStartFromSpectral;
   if (synth) {
-    const int count = g.orders * g.ringsPerGroup;
-    for (int e = threadIdx.x; e < count; e += blockDim.x) {
-      const int ring = e % g.ringsPerGroup, mo = e / g.ringsPerGroup;
-      const int th = th0 + ring;
-      double2 v = make_double2(0.0, 0.0);
-      if (th < a.ntheta) {
-        v = reinterpret_cast<const double2*>(
-            a.fourierIn)[(static_cast<size_t>(mo) * a.nodes + node) * a.ntheta + th];
+    // element e = thread + 256 i: ring e % rings (the same for all of a
+    // thread's elements, rings dividing 256), stored order e / rings
+    constexpr int kBatch = 9;
+    const int rpg = g.ringsPerGroup;
+    const int count = g.orders * rpg;
+    const int ring = static_cast<int>(threadIdx.x) % rpg;
+    const int moStep = kThreads / rpg;
+    const int th = th0 + ring;
+    const double2* F = reinterpret_cast<const double2*>(a.fourierIn);
+    for (int e0 = 0; e0 < count; e0 += kBatch * kThreads) {
+      const int mo0 = (e0 + static_cast<int>(threadIdx.x)) / rpg;
+      double2 v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int mo = mo0 + u * moStep;
+        v[u] = make_double2(0.0, 0.0);
+        if (mo < g.orders && th < a.ntheta) {
+          v[u] = F[(static_cast<size_t>(mo) * a.nodes + node) * a.ntheta + th];
+        }
       }
-      stage[ring * g.orders + mo] = v;
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int mo = mo0 + u * moStep;
+        if (mo < g.orders) stage[ring * g.orders + mo] = v[u];
+      }
     }
   }
   __syncthreads();
