@@ -76,11 +76,23 @@ __device__ inline double ColumnSum(double v) {
   return v;
 }
 
+// The staged rows' zero padding beyond +-lmax: the inverse reads orders up
+// to R 4 kIn + R - 1 away from order 0.
+__host__ __device__ inline int RowPad(int lmax, int R, int P) {
+  const int H = (P - 1) / 2;
+  const int kmax = H < (lmax + R - 1) / R ? H : (lmax + R - 1) / R;
+  const int kIn = (kmax + 3) / 4;
+  const int pad = R * 4 * kIn + R - 1 - lmax;
+  return pad > 0 ? pad : 0;
+}
+
 struct Geometry {
   int R, P, H, nphi, L, mlo, orders;
   int kIn, tOut;  // the band limit: contraction blocks of the inverse, row tiles of the forward
   int ringsPerWave, ringsPerGroup;  // rings of the grid
-  int zOrders;                      // 2 L + 1: the staged output, m = -L..L
+  // The staged rows (one per sequence slot: a ring, or a spin-0 ring pair):
+  // orders -(L + pad)..L + pad at L + pad + m, zero beyond +-L.
+  int rows, pad, stride;
 };
 
 template <bool REAL>
@@ -95,7 +107,9 @@ __device__ inline Geometry MakeGeometry(const RingLaunch& a) {
   g.orders = g.L - g.mlo + 1;
   g.ringsPerWave = (REAL ? 2 * kSeq : kSeq) / g.R;
   g.ringsPerGroup = kWaves * g.ringsPerWave;
-  g.zOrders = 2 * g.L + 1;
+  g.rows = kWaves * kSeq / g.R;
+  g.pad = RowPad(g.L, g.R, g.P);
+  g.stride = 2 * (g.L + g.pad) + 1;
   const int kmax = min(g.H, (g.L + g.R - 1) / g.R);
   g.kIn = (kmax + 3) / 4;
   g.tOut = (kmax + 15) / 16;
@@ -109,6 +123,11 @@ struct Lane {
   int slot;    // the sequence's ring (spin 0: ring pair) within the group
   int ring;    // the lane's own grid ring within the group
   bool valid;  // that ring exists
+  // signs that turn the butterflies' role selects into fused multiply-adds:
+  // sP -1 for the real part; sA, sB +1 for the sum half of the R-point
+  // stages, -1 for the difference; cI1, cI2 the x i of the (a, b) = (1, 1)
+  // lanes (keep the value, take the swapped part)
+  double sP, sA, sB, cI1, cI2;
 };
 
 template <bool REAL>
@@ -127,29 +146,13 @@ __device__ inline Lane MakeLane(const Geometry& g, int th0, int ntheta) {
   l.slot = wave * (kSeq / g.R) + l.s / g.R;
   l.ring = REAL ? 2 * l.slot + l.p : l.slot;
   l.valid = th0 + l.ring < ntheta;
+  l.sP = l.p == 0 ? -1.0 : 1.0;
+  l.sA = l.a == 0 ? 1.0 : -1.0;
+  l.sB = l.b == 0 ? 1.0 : -1.0;
+  const bool rot = g.R == 4 && l.a == 1 && l.b == 1;
+  l.cI1 = rot ? 0.0 : 1.0;
+  l.cI2 = rot ? l.sP : 0.0;
   return l;
-}
-
-// The signed order of m = R k + m2 (mod nphi).
-__device__ inline int SignedOrder(const Geometry& g, int mm) {
-  return mm <= g.nphi / 2 ? mm : mm - g.nphi;
-}
-
-// The complex sequence value at order m of a slot: the staged coefficient
-// (spin 0: Z_m = F1_m + i F2_m of the ring pair, with F_{-m} = conj(F_m)
-// and the real part at m = 0); zero beyond lmax.
-template <bool REAL>
-__device__ inline double2 Gather(const Geometry& g, const double2* Fs, int slot, int m) {
-  if (m < -g.L || m > g.L) return make_double2(0.0, 0.0);
-  if (REAL) {
-    const int am = m < 0 ? -m : m;
-    const double2 f1 = Fs[(2 * slot) * g.orders + am];
-    const double2 f2 = Fs[(2 * slot + 1) * g.orders + am];
-    const double sg = m < 0 ? -1.0 : 1.0;
-    const double i1 = m == 0 ? 0.0 : sg * f1.y, i2 = m == 0 ? 0.0 : sg * f2.y;
-    return make_double2(f1.x - i2, i1 + f2.x);
-  }
-  return Fs[slot * g.orders + (m - g.mlo)];
 }
 
 // Synthesis at n1: V = e^{2 pi i m2 n1 / nphi} Y, then the R-point inverse
@@ -157,17 +160,13 @@ __device__ inline double2 Gather(const Geometry& g, const double2* Fs, int slot,
 // x[n1 + P n2].
 __device__ inline double InverseButterfly(const Geometry& g, const Lane& ln, double y,
                                           double c, double s) {
-  const double v = c * y + (ln.p == 0 ? -s : s) * Xor1(y);
+  const double v = fma(c, y, (ln.sP * s) * Xor1(y));
   if (g.R == 4) {
-    const double vp = Xor4(v);
-    double e = ln.a == 0 ? v + vp : vp - v;
-    const double eo = Xor1(e);
-    if (ln.a == 1 && ln.b == 1) e = ln.p == 0 ? -eo : eo;  // x i
-    const double ep = Xor2(e);
-    return ln.b == 0 ? e + ep : ep - e;
+    const double e0 = fma(ln.sA, v, Xor4(v));
+    const double e = fma(ln.cI2, Xor1(e0), ln.cI1 * e0);  // x i on the (1, 1) lanes
+    return fma(ln.sB, e, Xor2(e));
   }
-  const double vp = Xor2(v);
-  return ln.b == 0 ? v + vp : vp - v;
+  return fma(ln.sB, v, Xor2(v));
 }
 
 // Analysis at n1: the R-point forward transform across the lanes of the
@@ -176,44 +175,42 @@ __device__ inline double ForwardButterfly(const Geometry& g, const Lane& ln, dou
                                           double c, double s) {
   double w;
   if (g.R == 4) {
-    const double yp = Xor2(y);
-    double d = ln.b == 0 ? y + yp : yp - y;
-    const double dn = Xor1(d);
-    if (ln.a == 1 && ln.b == 1) d = ln.p == 0 ? dn : -dn;  // x (-i)
-    const double dp = Xor4(d);
-    w = ln.a == 0 ? d + dp : dp - d;
+    const double d0 = fma(ln.sB, y, Xor2(y));
+    const double d = fma(-ln.cI2, Xor1(d0), ln.cI1 * d0);  // x (-i) on the (1, 1) lanes
+    w = fma(ln.sA, d, Xor4(d));
   } else {
-    const double yp = Xor2(y);
-    w = ln.b == 0 ? y + yp : yp - y;
+    w = fma(ln.sB, y, Xor2(y));
   }
-  return c * w + (ln.p == 0 ? s : -s) * Xor1(w);
+  return fma(c, w, (-ln.sP * s) * Xor1(w));
 }
 
 // The P-point inverse transforms of the lane's sequence on the matrix
-// cores, from the staged coefficients: Y at j and P - j, and at 0.
+// cores, from its staged row (the lane's part, order m2 at offset 0): Y at j
+// and P - j, and at 0. The sequence's element k is the order m2 + R k, its
+// element P - k the order m2 - R k: reads 2 R k doubles up and down the
+// row, zero beyond +-L (and so for k past kmax or H).
 template <int JT, bool REAL, bool FULL>
-__device__ inline void InverseSequences(const Geometry& g, const Lane& ln, const double2* Fs,
+__device__ inline void InverseSequences(const Geometry& g, const Lane& ln, const double* row,
                                         const double* tabC, const double* tabS,
                                         double (&yj)[4 * JT], double (&yq)[4 * JT], double& y0) {
   constexpr int KS = 4 * JT;
   double av[KS], bv[KS];
   double sum = 0.0;
+  const int R = g.R;
+  const double* up = row + 2 * R * (1 + ln.quad);  // k = 1 + 4 kb + quad
+  const double* dn = row - 2 * R * (1 + ln.quad);
 #pragma unroll
   for (int kb = 0; kb < KS; ++kb) {
     if (kb >= g.kIn) {
       av[kb] = bv[kb] = 0.0;
       continue;
     }
-    const int k = FULL ? 1 + 4 * kb + ln.quad : min(1 + 4 * kb + ln.quad, g.H);
-    const double2 zk = Gather<REAL>(g, Fs, ln.slot, SignedOrder(g, g.R * k + ln.m2));
-    const double2 zq = Gather<REAL>(g, Fs, ln.slot, SignedOrder(g, g.R * (g.P - k) + ln.m2));
-    const bool live = FULL || 1 + 4 * kb + ln.quad <= g.H;
-    av[kb] = live ? (ln.p == 0 ? zk.x + zq.x : zk.y + zq.y) : 0.0;
-    bv[kb] = live ? (ln.p == 0 ? zk.x - zq.x : zk.y - zq.y) : 0.0;
+    const double zk = up[8 * R * kb], zq = dn[-8 * R * kb];
+    av[kb] = zk + zq;
+    bv[kb] = zk - zq;
     sum += av[kb];
   }
-  const double2 z0 = Gather<REAL>(g, Fs, ln.slot, SignedOrder(g, ln.m2));
-  const double h0 = ln.p == 0 ? z0.x : z0.y;
+  const double h0 = row[0];
   y0 = h0 + ColumnSum(sum);
   Acc4 C[JT], S[JT];
 #pragma unroll
@@ -247,11 +244,12 @@ __device__ inline void InverseSequences(const Geometry& g, const Lane& ln, const
 
 // The forward transforms of the lane's sequence (u at j, P - j and 0,
 // after the forward butterflies) on the matrix cores; each output G(k) is
-// the order R k + m2, scattered to the staged tile Zd[slot][m + L].
+// the order m2 + R k (G(P - k): m2 - R k), written to the lane's part of its
+// staged row (order m2 at offset 0) where |order| <= L.
 template <int JT, bool FULL>
 __device__ inline void ForwardSequences(const Geometry& g, const Lane& ln, const double* tabC,
                                         const double* tabS, const double (&uj)[4 * JT],
-                                        const double (&uq)[4 * JT], double u0, double* Zd) {
+                                        const double (&uq)[4 * JT], double u0, double* row) {
   constexpr int KS = 4 * JT;
   double av[KS], bv[KS];
   double sum = 0.0;
@@ -279,25 +277,22 @@ __device__ inline void ForwardSequences(const Geometry& g, const Lane& ln, const
     }
   }
   // G(k) = u0 + C'_k - i S'_k, G(P - k) = u0 + C'_k + i S'_k
-  auto put = [&](int k, double value) {
-    const int m = SignedOrder(g, g.R * k + ln.m2);
-    if (m >= -g.L && m <= g.L) {
-      Zd[(static_cast<size_t>(ln.slot) * g.zOrders + (m + g.L)) * 2 + ln.p] = value;
-    }
-  };
-  if (ln.quad == 0) put(0, g0);
+  if (ln.quad == 0) row[0] = g0;
+  const int R = g.R;
+  double* up = row + 2 * R * (1 + ln.quad);
+  double* dn = row - 2 * R * (1 + ln.quad);
+  const int reach = g.L - R * (1 + ln.quad);  // R (16 t + 4 q) -+ m2 may not pass it
   const double sg = ln.p == 0 ? 1.0 : -1.0;
 #pragma unroll
   for (int t = 0; t < JT; ++t) {
     if (t >= g.tOut) break;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double sx = Xor1(S[t][q]);
-      const int k = 1 + 16 * t + 4 * q + ln.quad;
-      if (FULL || k <= g.H) {
-        put(k, u0 + C[t][q] + sg * sx);
-        put(g.P - k, u0 + C[t][q] - sg * sx);
-      }
+      const int off = R * (16 * t + 4 * q);
+      const double sx = sg * Xor1(S[t][q]);
+      const double uc = u0 + C[t][q];
+      if (off + ln.m2 <= reach) up[2 * off] = uc + sx;
+      if (off - ln.m2 <= reach) dn[-2 * off] = uc - sx;
     }
   }
 }
@@ -326,13 +321,13 @@ __device__ inline void CopyOut(const Geometry& g, const RingLaunch& a, const dou
     for (int u = 0; u < kBatch; ++u) {
       const int mo = min(mo0 + u * moStep, mCount - 1);
       if (REAL) {
-        const int pair = ring / 2;
-        const double2 zp = Zs[pair * g.zOrders + (mo + g.L)];
-        const double2 zm = Zs[pair * g.zOrders + (-mo + g.L)];
+        const double2* row = Zs + (ring / 2) * g.stride + (g.L + g.pad);
+        const double2 zp = row[mo];
+        const double2 zm = row[-mo];
         res[u] = ring % 2 == 0 ? make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y - zm.y))
                                : make_double2(0.5 * (zp.y + zm.y), -0.5 * (zp.x - zm.x));
       } else {
-        res[u] = Zs[ring * g.zOrders + mo];
+        res[u] = Zs[ring * g.stride + g.pad + mo];  // order mo - L at L + pad + order
       }
     }
 #pragma unroll
@@ -398,6 +393,8 @@ __device__ inline void RingBody(const RingLaunch& a) {
   double* tables = lds + 2 * g.nphi;
   double* stageD = tables + (kTabLds ? 2 * kTab : 0);
   double2* stage = reinterpret_cast<double2*>(stageD);
+  // the lane's part of its staged row, order m2 at offset 0
+  double* row = stageD + 2 * (ln.slot * g.stride + g.L + g.pad + ln.m2) + ln.p;
   const double* tabC = a.tables.tabC;
   const double* tabS = a.tables.tabS;
   // Staging: every copy issues all of a thread's loads before its stores,
@@ -461,8 +458,33 @@ __device__ inline void RingBody(const RingLaunch& a) {
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const int mo = mo0 + u * moStep;
-        if (mo < g.orders) stage[ring * g.orders + mo] = v[u];
+        if (REAL) {
+          // the ring pair's sequence Z_m = F1_m + i F2_m at m and -m
+          // (F_{-m} = conj F_m, the imaginary part at m = 0 dropped); the
+          // partner ring is the neighbouring lane, active whenever this is
+          const double ox = Xor1(v[u].x), oy = Xor1(v[u].y);
+          const bool even = ring % 2 == 0;
+          const double2 f1 = even ? v[u] : make_double2(ox, oy);
+          const double2 f2 = even ? make_double2(ox, oy) : v[u];
+          double2* row = stage + (ring / 2) * g.stride + (g.L + g.pad);
+          if (mo < g.orders) {
+            if (mo == 0) {
+              if (even) row[0] = make_double2(f1.x, f2.x);
+            } else if (even) {
+              row[mo] = make_double2(f1.x - f2.y, f1.y + f2.x);
+            } else {
+              row[-mo] = make_double2(f1.x + f2.y, f2.x - f1.y);
+            }
+          }
+        } else if (mo < g.orders) {
+          stage[ring * g.stride + g.pad + mo] = v[u];  // order mo - L at L + pad + order
+        }
       }
+    }
+    // the rows' padding beyond +-L is zero
+    for (int i = threadIdx.x; i < 2 * g.pad * g.rows; i += kThreads) {
+      const int slot = i / (2 * g.pad), k = i % (2 * g.pad);
+      stage[slot * g.stride + (k < g.pad ? k : 2 * g.L + 1 + k)] = make_double2(0.0, 0.0);
     }
   }
   __syncthreads();
@@ -472,7 +494,7 @@ __device__ inline void RingBody(const RingLaunch& a) {
   double xj[KS], xq[KS], x0 = 0.0;
   if (synth) {
     double yj[KS], yq[KS], y0;
-    InverseSequences<JT, REAL, FULL>(g, ln, stage, tabC, tabS, yj, yq, y0);
+    InverseSequences<JT, REAL, FULL>(g, ln, row, tabC, tabS, yj, yq, y0);
 #pragma unroll
     for (int v = 0; v < KS; ++v) {
       xj[v] = InverseButterfly(g, ln, yj[v], cosN[twj(v)], sinN[twj(v)]);
@@ -492,7 +514,6 @@ __device__ inline void RingBody(const RingLaunch& a) {
   }
   __syncthreads();  // the staged coefficients are read; the tile is reused
 
-  double* Zd = stageD;
   const double2* Zs = stage;
   // y at the lane's points -> forward butterflies -> forward transforms
   auto forward = [&](const double (&yj)[KS], const double (&yq)[KS], double y0, double* out) {
@@ -503,7 +524,7 @@ __device__ inline void RingBody(const RingLaunch& a) {
       uq[v] = ForwardButterfly(g, ln, yq[v], cosN[twq(v)], sinN[twq(v)]);
     }
     const double u0 = ForwardButterfly(g, ln, y0, 1.0, 0.0);
-    ForwardSequences<JT, FULL>(g, ln, tabC, tabS, uj, uq, u0, Zd);
+    ForwardSequences<JT, FULL>(g, ln, tabC, tabS, uj, uq, u0, row);
     __syncthreads();
     CopyOut<REAL>(g, a, Zs, node, th0, out);
   };
@@ -708,20 +729,15 @@ int RingMfmaTiles(int P) {
 }
 
 size_t RingMfmaLds(const RingLaunch& r) {
-  const bool real = r.block == 2;
-  const int rings = kWaves * (real ? 2 * kSeq : kSeq) / r.tables.R;
   const int slots = kWaves * kSeq / r.tables.R;
-  const int orders = real ? r.lmax + 1 : 2 * r.lmax + 1;
-  const bool synth = r.op == kRingSynthesise || r.op == kRingPass || r.op == kRingFinish ||
-                     r.op == kRingStartFromSpectral;
   const bool start = r.op == kRingStart || r.op == kRingStartFromSpectral;
-  const bool analysis = r.op != kRingSynthesise && r.op != kRingFinish;
   // the operand tables: 2 x (JT x KS x 64) doubles
   const size_t tables =
       start ? 0 : 2 * sizeof(double) * (4 * r.tables.jt * r.tables.jt) * kWave;
-  const size_t staged = synth ? 16 * static_cast<size_t>(rings) * orders : 0;
-  const size_t outTile = analysis ? 16 * static_cast<size_t>(slots) * (2 * r.lmax + 1) : 0;
-  return sizeof(double) * 2 * r.nphi + tables + std::max(staged, outTile);
+  // the staged rows (input coefficients, then outputs): slots x padded orders
+  const int stride = 2 * (r.lmax + RowPad(r.lmax, r.tables.R, r.tables.P)) + 1;
+  const size_t rows = 16 * static_cast<size_t>(slots) * stride;
+  return sizeof(double) * 2 * r.nphi + tables + rows;
 }
 
 void LaunchRingMfma(const RingLaunch& r, hipStream_t stream) {
