@@ -6,9 +6,9 @@
 //   A (16 x 4):  A[i % 16][i / 16]
 //   B (4 x 16):  B[i / 16][i % 16]
 //   C (16 x 16): four doubles, C[4 j + i / 16][i % 16] for j = 0..3
-// G is stored tile by tile in exactly the A order, so each wave streams it
-// with one coalesced 512-byte load per instruction; f is row-major with
-// rows padded to 16 columns, so a B operand is four 128-byte row segments.
+// G is stored tile by tile in exactly the A order, so a workgroup stages a
+// chunk of it with coalesced 512-byte runs and each lane reads its operand
+// from LDS without conflicts; f is row-major with rows padded to 16 columns.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -55,59 +55,110 @@ __global__ void __launch_bounds__(kThreads)
   g[t.gOff + idx] = v;
 }
 
-// One wave: a 32 x 64 block of one task's x (2 x 4 matrix-core tiles),
-// accumulated over the whole contraction; blocks past the padded edge are
-// skipped (wave-uniform branches). x is written column by column (order
-// major), the layout strain recovery reads with neighbouring nodes in
-// neighbouring lanes.
+// One workgroup: a 64 x 128 block of one task's x (row tiles rb .. rb + 3,
+// column tiles cb .. cb + 7), the waves 2 x 2 with 2 x 4 matrix-core tiles
+// each. The contraction runs in chunks of 16: each chunk's G tiles (already
+// in operand order) and f rows are staged in LDS and shared by the four
+// waves, the next chunk's loads in flight under the current products.
+// Tiles past the task's padded edge are staged as zeros; x is written
+// column by column (order major), the layout strain recovery reads with
+// neighbouring nodes in neighbouring lanes.
+constexpr int kRowTiles = 4;   // 64 rows per workgroup
+constexpr int kColTiles = 8;   // 128 columns per workgroup
+constexpr int kChunk = 4;      // contraction steps of 4 per staged chunk
+constexpr int kAPer = kRowTiles * kChunk * kWave / kThreads;    // staged A per thread
+constexpr int kBPer = 4 * kChunk * 16 * kColTiles / kThreads;   // staged B per thread
+
 __global__ void __launch_bounds__(kThreads)
-    ResponseKernel(const DegreeTask* tasks, const ResponseWork* work, int count,
-                   const double* g, const double* f, double* x) {
-  const int wave = blockIdx.x * (kThreads / kWave) + threadIdx.x / kWave;
-  if (wave >= count) return;
-  const int lane = threadIdx.x % kWave;
-  const ResponseWork w = work[wave];
+    ResponseKernel(const DegreeTask* tasks, const ResponseWork* work, const double* g,
+                   const double* f, double* x) {
+  __shared__ double As[kRowTiles * kChunk][kWave];              // [tile][lane]
+  __shared__ double Bs[4 * kChunk][16 * kColTiles + 1];         // [k][column]
+  const ResponseWork w = work[blockIdx.x];
   const DegreeTask t = tasks[w.task];
   const int rbCount = t.nmPad / 16, k4 = t.nmPad / 4, cbCount = t.colsPad / 16;
-  const bool row1 = w.rb + 1 < rbCount;
-  const int cols = min(4, cbCount - w.cb);
-  const double* a0 = g + t.gOff + static_cast<long long>(w.rb) * k4 * kWave + lane;
-  const double* a1 = a0 + static_cast<long long>(k4) * kWave;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int wr = wave / 2, wc = wave % 2;  // the wave's 2 row tiles and 4 column tiles
   const long long ld = t.colsPad;
-  const double* b = f + t.fOff + (lane / 16) * ld + 16 * w.cb + lane % 16;
+
+  double pa[kAPer], pb[kBPer];
+  auto load = [&](int kc) {  // contraction steps 4 kc .. 4 kc + 3
+#pragma unroll
+    for (int q = 0; q < kAPer; ++q) {
+      const int e = threadIdx.x + kThreads * q;  // (tile, lane), tile = r kChunk + s
+      const int tile = e / kWave, ln = e % kWave;
+      const int r = tile / kChunk, s = tile % kChunk;
+      pa[q] = 0.0;
+      if (w.rb + r < rbCount) {
+        pa[q] = g[t.gOff + (static_cast<long long>(w.rb + r) * k4 + kChunk * kc + s) * kWave + ln];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kBPer; ++q) {
+      const int e = threadIdx.x + kThreads * q;  // (k, column)
+      const int k = e / (16 * kColTiles), c = e % (16 * kColTiles);
+      pb[q] = 0.0;
+      if (16 * w.cb + c < t.colsPad) {
+        pb[q] = f[t.fOff + (4 * kChunk * kc + k) * ld + 16 * w.cb + c];
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int q = 0; q < kAPer; ++q) {
+      const int e = threadIdx.x + kThreads * q;
+      As[e / kWave][e % kWave] = pa[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kBPer; ++q) {
+      const int e = threadIdx.x + kThreads * q;
+      Bs[e / (16 * kColTiles)][e % (16 * kColTiles)] = pb[q];
+    }
+  };
+
   Acc4 acc[2][4];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = Acc4{0.0, 0.0, 0.0, 0.0};
   }
-#pragma unroll 2
-  for (int kb = 0; kb < k4; ++kb) {
-    const double av0 = a0[static_cast<long long>(kb) * kWave];
-    const double av1 = row1 ? a1[static_cast<long long>(kb) * kWave] : 0.0;
-    double bv[4];
+  const int chunks = k4 / kChunk;  // nmPad is a multiple of 16
+  load(0);
+  for (int kc = 0; kc < chunks; ++kc) {
+    store();
+    __syncthreads();
+    if (kc + 1 < chunks) load(kc + 1);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) bv[c] = c < cols ? b[4 * kb * ld + 16 * c] : 0.0;
+    for (int s = 0; s < kChunk; ++s) {
+      const double a0 = As[(2 * wr) * kChunk + s][lane];
+      const double a1 = As[(2 * wr + 1) * kChunk + s][lane];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (c < cols) {
-        acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv[c], acc[0][c], 0, 0, 0);
-        if (row1) {
-          acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv[c], acc[1][c], 0, 0, 0);
+      for (int c = 0; c < 4; ++c) {
+        // tiles past the task's padded edge are skipped (wave-uniform)
+        if (w.cb + 4 * wc + c < cbCount) {
+          const double b = Bs[4 * s + lane / 16][(4 * wc + c) * 16 + lane % 16];
+          if (w.rb + 2 * wr < rbCount) {
+            acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b, acc[0][c], 0, 0, 0);
+          }
+          if (w.rb + 2 * wr + 1 < rbCount) {
+            acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b, acc[1][c], 0, 0, 0);
+          }
         }
       }
     }
+    __syncthreads();
   }
+  // guards, not early exits: the accumulators keep static indices
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    if (r == 1 && !row1) break;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      if (c >= cols) break;
+      const int rt = w.rb + 2 * wr + r, ct = w.cb + 4 * wc + c;
+      if (rt >= rbCount || ct >= cbCount) continue;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const long long row = 16 * (w.rb + r) + 4 * j + lane / 16;
-        const long long col = 16 * (w.cb + c) + lane % 16;
+        const long long row = 16 * rt + 4 * j + lane / 16;
+        const long long col = 16 * ct + lane % 16;
         x[t.fOff + col * t.nmPad + row] = acc[r][c][j];  // column-major
       }
     }
@@ -132,8 +183,8 @@ std::vector<ResponseWork> ResponseWorkList(const std::vector<DegreeTask>& tasks)
   std::vector<ResponseWork> work;
   for (size_t ti = 0; ti < tasks.size(); ++ti) {
     const auto& t = tasks[ti];
-    for (int rb = 0; rb < t.nmPad / 16; rb += 2) {
-      for (int cb = 0; cb < t.colsPad / 16; cb += 4) {
+    for (int rb = 0; rb < t.nmPad / 16; rb += kRowTiles) {
+      for (int cb = 0; cb < t.colsPad / 16; cb += kColTiles) {
         work.push_back({static_cast<int>(ti), rb, cb, 0});
       }
     }
@@ -169,8 +220,8 @@ void LaunchResponse(const DegreeTask* tasks_dev, const ResponseWork* work_dev,
                     hipStream_t stream) {
   if (workCount == 0) return;
   ProfileScope prof("response", stream);
-  hipLaunchKernelGGL(ResponseKernel, dim3(CeilDiv(workCount, kThreads / kWave)),
-                     dim3(kThreads), 0, stream, tasks_dev, work_dev, workCount, g, f, x);
+  hipLaunchKernelGGL(ResponseKernel, dim3(workCount), dim3(kThreads), 0, stream, tasks_dev,
+                     work_dev, g, f, x);
   SL3_HIP(hipGetLastError());
 }
 

@@ -17,8 +17,8 @@
 
 namespace sl3amd {
 
-// One wave's share of the response GEMM: rows 16 rb .. 16 rb + 31 and
-// columns 16 cb .. 16 cb + 63 of one task's output.
+// One workgroup's share of the response GEMM: rows 16 rb .. 16 rb + 63 and
+// columns 16 cb .. 16 cb + 127 of one task's output.
 struct ResponseWork {
   int task = 0;
   int rb = 0;
@@ -26,7 +26,7 @@ struct ResponseWork {
   int pad = 0;
 };
 
-// Every wave's work over the tasks, the largest contraction first.
+// Every workgroup's work over the tasks, the largest contraction first.
 std::vector<ResponseWork> ResponseWorkList(const std::vector<DegreeTask>& tasks);
 
 // z (per task at zOff[task], ndof x nm row-major, zero on entry): column j
