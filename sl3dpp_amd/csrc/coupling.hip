@@ -431,10 +431,15 @@ __global__ void AndersonFirstKernel(const double* x, const double* gx, double* x
   });
 }
 
+// RESIDUAL: the pass's RelativeChange (max |gx|^2, max |gx - x|^2, a
+// non-finite flag) from the same loads, into residualPartials (3 per block).
+template <bool RESIDUAL>
 __global__ void __launch_bounds__(kThreads)
-    AndersonSweepKernel(AndersonSweep s, ActiveRuns a, double* partials) {
+    AndersonSweepKernel(AndersonSweep s, ActiveRuns a, double* partials,
+                        double* residualPartials) {
   __shared__ double red[kThreads][17];
   double col[8] = {}, gam[8] = {};
+  double scale = 0.0, diff = 0.0, bad = 0.0;
   double* dFs = s.dF[s.slot];
   double* dGs = s.dG[s.slot];
   ForActivePairs(a, [&](long long c, long long j) {
@@ -442,6 +447,11 @@ __global__ void __launch_bounds__(kThreads)
     const double2 xp = Load2(s.xPrev, c), fp = Load2(s.fPrev, c);
     const double2 f = make_double2(gx.x - xs.x, gx.y - xs.y);
     const double2 df = make_double2(f.x - fp.x, f.y - fp.y);
+    if constexpr (RESIDUAL) {
+      if (!(isfinite(gx.x) && isfinite(gx.y))) bad = 1.0;
+      scale = fmax(scale, gx.x * gx.x + gx.y * gx.y);
+      diff = fmax(diff, f.x * f.x + f.y * f.y);
+    }
     Store2(dFs, c, df);
     Store2(dGs, c, make_double2(xs.x - xp.x + df.x, xs.y - xp.y + df.y));
     Store2(s.xPrev, c, xs);
@@ -467,6 +477,22 @@ __global__ void __launch_bounds__(kThreads)
     __syncthreads();
   }
   if (threadIdx.x < 16) partials[blockIdx.x * 16 + threadIdx.x] = red[0][threadIdx.x];
+  if constexpr (RESIDUAL) {
+    __syncthreads();
+    red[threadIdx.x][0] = scale;
+    red[threadIdx.x][1] = diff;
+    red[threadIdx.x][2] = bad;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        for (int i = 0; i < 3; ++i) {
+          red[threadIdx.x][i] = fmax(red[threadIdx.x][i], red[threadIdx.x + w][i]);
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x < 3) residualPartials[blockIdx.x * 3 + threadIdx.x] = red[0][threadIdx.x];
+  }
 }
 
 // RelativeChange over the active entries (the others are zero in both).
@@ -659,14 +685,26 @@ void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPr
 }
 
 void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double* partials,
-                         int partialBlocks, double* sums, hipStream_t stream) {
+                         int partialBlocks, double* sums, double* residual,
+                         hipStream_t stream) {
   ProfileScope prof("anderson", stream);
-  hipLaunchKernelGGL(AndersonSweepKernel, dim3(partialBlocks), dim3(kThreads), 0,
-                     stream, s, runs, partials);
+  double* residualPartials = partials + static_cast<size_t>(partialBlocks) * 16;
+  if (residual) {
+    hipLaunchKernelGGL(AndersonSweepKernel<true>, dim3(partialBlocks), dim3(kThreads), 0,
+                       stream, s, runs, partials, residualPartials);
+  } else {
+    hipLaunchKernelGGL(AndersonSweepKernel<false>, dim3(partialBlocks), dim3(kThreads), 0,
+                       stream, s, runs, partials, nullptr);
+  }
   SL3_HIP(hipGetLastError());
   hipLaunchKernelGGL(ReducePartialsKernel, dim3(16), dim3(kThreads), 0, stream, partials,
                      partialBlocks, 16, 0, sums);
   SL3_HIP(hipGetLastError());
+  if (residual) {
+    hipLaunchKernelGGL(ReducePartialsKernel, dim3(3), dim3(kThreads), 0, stream,
+                       residualPartials, partialBlocks, 3, 1, residual);
+    SL3_HIP(hipGetLastError());
+  }
 }
 
 void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,

@@ -123,8 +123,13 @@ void LaunchRelativeChangeActive(const double* next, const double* now, const Act
 void LaunchAndersonFirst(double* x, const double* gx, double* xPrev,
                          double* fPrev, const ActiveRuns& runs, size_t n,
                          hipStream_t stream);
+// The Anderson sweep (differences, the Gram column and right-hand side into
+// sums[0..15]); with residual non-null also the pass's RelativeChange
+// (max |gx|^2, max |gx - x|^2, a non-finite flag) into residual[0..2].
+// partials holds partialBlocks x 19 doubles.
 void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double* partials,
-                         int partialBlocks, double* sums, hipStream_t stream);
+                         int partialBlocks, double* sums, double* residual,
+                         hipStream_t stream);
 void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,
                        hipStream_t stream);
 

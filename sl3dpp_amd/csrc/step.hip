@@ -424,8 +424,8 @@ Viscoelastic3D::Viscoelastic3D(const Earth& earth, int lmax, const double* delta
   response_.Resize(std::max<size_t>(fDoubles_, 1));
   ampDoubles_ = static_cast<size_t>(lmax + 1) * (lmax + 2);
   amp_.Resize(2 * ampDoubles_);
-  partials_.Resize(1024 * 16);
-  result_.Resize(16);
+  partials_.Resize(1024 * 19);  // the sweep's 16 sums and the residual's 3 maxima per block
+  result_.Resize(19);
   for (auto& e : events_) SL3_HIP(hipEventCreate(&e));
   SL3_HIP(hipStreamSynchronize(s));
 }
@@ -1117,12 +1117,49 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
     fPrev_.Resize(C);
   }
   double previous = 0.0;
+  auto sweepOf = [&](int m) {
+    AndersonSweep sw{};
+    sw.x = s_.data();
+    sw.gx = sNext_.data();
+    sw.xPrev = xPrev_.data();
+    sw.fPrev = fPrev_.data();
+    for (int i = 0; i < memory && i < 8; ++i) {
+      sw.dF[i] = history_.data() + static_cast<size_t>(i) * C;
+      sw.dG[i] = history_.data() + static_cast<size_t>(memory + i) * C;
+    }
+    sw.m = m;
+    sw.slot = slot;
+    return sw;
+  };
   for (int it = 0; it < options_.maxIterations; ++it) {
     Pass(s_.data(), sNext_.data(), it == 0 && !warm, f);
-    // Every pass's image is zero below each degree's truncation; the start
-    // (a warm start, or one analysed from a stored strain grid) may hold
-    // anything there, so the first pass compares every entry.
-    const double residual = RelativeChange(sNext_.data(), s_.data(), it == 0);
+    // Once the mixer runs (and without damping) every pass that does not
+    // converge takes a sweep over the same two iterates: the sweep then also
+    // forms the pass's RelativeChange, one read of the data and one
+    // synchronisation. A sweep on the converging pass only writes history
+    // that the next step starts afresh.
+    const bool fused = mixing && mixerStarted && options_.damping == 1.0;
+    const int m = std::min(count + 1, memory);
+    AndersonSweep sw{};
+    double sums[16];
+    double residual;
+    if (fused) {
+      sw = sweepOf(m);
+      LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(),
+                          result_.data() + 16, s);
+      double r[19];
+      SL3_HIP(hipMemcpyAsync(r, result_.data(), sizeof(r), hipMemcpyDeviceToHost, s));
+      SL3_HIP(hipStreamSynchronize(s));
+      std::copy(r, r + 16, sums);
+      residual = r[18] != 0.0 ? std::numeric_limits<double>::quiet_NaN()
+                 : r[16] > 0.0 ? std::sqrt(r[17]) / std::sqrt(r[16])
+                               : 0.0;
+    } else {
+      // Every pass's image is zero below each degree's truncation; the start
+      // (a warm start, or one analysed from a stored strain grid) may hold
+      // anything there, so the first pass compares every entry.
+      residual = RelativeChange(sNext_.data(), s_.data(), it == 0);
+    }
     info.iterations = it + 1;
     info.residual = residual;
     residualHistory.push_back(residual);
@@ -1149,22 +1186,12 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
                             s);
         mixerStarted = true;
       } else {
-        const int m = std::min(count + 1, memory);
-        AndersonSweep sw{};
-        sw.x = s_.data();
-        sw.gx = sNext_.data();
-        sw.xPrev = xPrev_.data();
-        sw.fPrev = fPrev_.data();
-        for (int i = 0; i < memory && i < 8; ++i) {
-          sw.dF[i] = history_.data() + static_cast<size_t>(i) * C;
-          sw.dG[i] = history_.data() + static_cast<size_t>(memory + i) * C;
+        if (!fused) {
+          sw = sweepOf(m);
+          LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(), nullptr, s);
+          SL3_HIP(hipMemcpyAsync(sums, result_.data(), sizeof(sums), hipMemcpyDeviceToHost, s));
+          SL3_HIP(hipStreamSynchronize(s));
         }
-        sw.m = m;
-        sw.slot = slot;
-        LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(), s);
-        double sums[16];
-        SL3_HIP(hipMemcpyAsync(sums, result_.data(), sizeof(sums), hipMemcpyDeviceToHost, s));
-        SL3_HIP(hipStreamSynchronize(s));
         count = m;
         for (int i = 0; i < m; ++i) {
           gram[slot * memory + i] = sums[i];
