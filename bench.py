@@ -59,11 +59,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def work_models(op, lmax, passes):
-    """Algorithmic work per launch of the step's kernels:
-    flops for the matrix-core and FMA kernels, HBM bytes for the sweeps.
-    passes: the timed steps' pass counts (the ring kernel's launches mix
-    passes, step starts and finishes)."""
+def work_models(op, lmax):
+    """Algorithmic work per launch of the step's kernels: flops for the
+    matrix-core kernels, HBM bytes for the sweeps."""
     tasks = op.export("tasks").reshape(-1, 5)
     first_node = {int(t[0]): int(t[3]) for t in tasks if int(t[1]) == 0}
     nodes, nt, nph = op.nodes, op.ntheta, op.nphi
@@ -83,15 +81,18 @@ def work_models(op, lmax, passes):
     H = (P - 1) // 2
     kmax = min(H, (lmax + R - 1) // R)
     seq = nodes * nt * R * (1.0 + 1.0 + 0.5)  # the three blocks
-    directions = sum(2 * k + 3 for k in passes)  # pass: 2; start: 2; finish: 1
-    launches = sum(3 * (k + 2) for k in passes)
-    ring_flops = seq * 8.0 * H * kmax * directions / launches
+    per_direction = seq * 8.0 * H * kmax
+    # the pass: one launch, three blocks, both directions; the step's start
+    # (two analyses) and finish (one synthesis): three launches each
+    ring_pass_flops = 2.0 * per_direction
+    ring_flops = 3.0 * per_direction / 6.0
     # the memory response: x = G f per degree task
     response_flops = sum(2.0 * t[4] * t[4] * 2 * (t[0] + 1) for t in tasks)
     S = op.spectral_doubles * 8.0
     return {
         "legendre_synthesis": ("mfma", legendre_flops),
         "legendre_analysis": ("mfma", legendre_flops),
+        "ring_pass": ("mfma", ring_pass_flops),
         "ring": ("mfma", ring_flops),
         "response": ("mfma", response_flops),
         "residual": ("hbm", 2.0 * S),
@@ -206,7 +207,7 @@ def main():
     steps_per_s = args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     # dominant kernel and its roofline
-    models = work_models(op, lmax, passes)
+    models = work_models(op, lmax)
     top = max(kernels.items(), key=lambda kv: kv[1][0])
     name, (total_ms, launches) = top
     roofline = None

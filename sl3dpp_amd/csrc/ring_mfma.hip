@@ -343,13 +343,12 @@ __device__ inline void CopyOut(const Geometry& g, const RingLaunch& a, const dou
 // weighted, node scaled, pass), the step start (two analyses).
 enum { kClassSynth = 0, kClassOne = 1, kClassStart = 2 };
 
+// One workgroup: the rings th0 .. th0 + rings - 1 of one node.
 template <int JT, bool REAL, int CLASS, bool FULL>
-__device__ inline void RingBody(const RingLaunch& a) {
+__device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
   constexpr int KS = 4 * JT;
   extern __shared__ double lds[];
   const Geometry g = MakeGeometry<REAL>(a);
-  const int node = blockIdx.y;
-  const int th0 = blockIdx.x * g.ringsPerGroup;
   const Lane ln = MakeLane<REAL>(g, th0, a.ntheta);
   // The lane's points: n1 = j = 1 + 16 t + 4 q + quad (v = 4 t + q), P - j
   // and 0, each at n1 + P n2. Where a lane has no point (padding, a ring
@@ -664,12 +663,42 @@ __device__ inline void RingBody(const RingLaunch& a) {
 template <int JT, bool REAL, int CLASS, bool FULL>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
     RingMfmaKernel(RingLaunch a) {
-  RingBody<JT, REAL, CLASS, FULL>(a);
+  RingBody<JT, REAL, CLASS, FULL>(a, blockIdx.y, blockIdx.x * MakeGeometry<REAL>(a).ringsPerGroup);
 }
 
 template <int JT, bool REAL, bool FULL>
 __global__ void __launch_bounds__(kThreads) RingMfmaStartKernel(RingLaunch a) {
-  RingBody<JT, REAL, kClassStart, FULL>(a);
+  RingBody<JT, REAL, kClassStart, FULL>(a, blockIdx.y,
+                                        blockIdx.x * MakeGeometry<REAL>(a).ringsPerGroup);
+}
+
+// The pass over all three blocks in one launch. A super-tile is one node's
+// 16 rings (R = 4): two 8-ring workgroups of each complex block and one
+// 16-ring workgroup of spin 0, the five numbered onto one XCD (workgroups
+// go to the XCDs round robin) so the rings' weights come from its L2 after
+// the first read; one launch also leaves a single tail instead of three.
+constexpr int kPassGroups = 5;
+
+template <int JT, bool FULL>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+    RingMfmaPassKernel(RingLaunch base, RingBlocks blocks) {
+  const int p = blockIdx.x;
+  const int st = (p / (8 * kPassGroups)) * 8 + p % 8;  // the super-tile
+  const int sub = (p / 8) % kPassGroups;
+  const int groups = (base.ntheta + 15) / 16;
+  if (st >= groups * base.nodes) return;
+  const int node = st / groups, th16 = (st % groups) * 16;
+  RingLaunch a = base;
+  a.block = sub < 2 ? 0 : (sub < 4 ? 1 : 2);
+  a.fourierIn = blocks.in[a.block];
+  a.fourierOut = blocks.out[a.block];
+  if (a.block == 2) {
+    RingBody<JT, true, kClassOne, FULL>(a, node, th16);
+  } else {
+    const int th0 = th16 + 8 * (sub % 2);
+    if (th0 >= a.ntheta) return;
+    RingBody<JT, false, kClassOne, FULL>(a, node, th0);
+  }
 }
 
 template <int JT, int CLASS, bool FULL>
@@ -738,6 +767,38 @@ size_t RingMfmaLds(const RingLaunch& r) {
   const int stride = 2 * (r.lmax + RowPad(r.lmax, r.tables.R, r.tables.P)) + 1;
   const size_t rows = 16 * static_cast<size_t>(slots) * stride;
   return sizeof(double) * 2 * r.nphi + tables + rows;
+}
+
+bool RingPassFusable(const RingLaunch& r) {
+  return r.tables.R == 4 && r.tables.jt > 0 && RingMfmaLds(r) <= 160 * 1024;
+}
+
+void LaunchRingPassMfma(const RingLaunch& r, const RingBlocks& blocks, hipStream_t stream) {
+  if (!RingPassFusable(r)) throw ArgumentError("ring pass: no fused launch for this grid");
+  const int superTiles = CeilDiv(r.ntheta, 16) * r.nodes;
+  const dim3 grid(CeilDiv(superTiles, 8) * 8 * kPassGroups);
+  const size_t lds = RingMfmaLds(r);
+  const bool full = (r.tables.P - 1) / 2 == 16 * r.tables.jt;
+  ProfileScope prof("ring_pass", stream);
+  switch (r.tables.jt) {
+#define SL3_RING_PASS(JT)                                                                   \
+  case JT:                                                                                \
+    if (full) {                                                                           \
+      hipLaunchKernelGGL((RingMfmaPassKernel<JT, true>), grid, dim3(kThreads), lds, stream, \
+                         r, blocks);                                                      \
+    } else {                                                                              \
+      hipLaunchKernelGGL((RingMfmaPassKernel<JT, false>), grid, dim3(kThreads), lds,       \
+                         stream, r, blocks);                                              \
+    }                                                                                     \
+    break;
+    SL3_RING_PASS(1)
+    SL3_RING_PASS(2)
+    SL3_RING_PASS(3)
+    SL3_RING_PASS(4)
+#undef SL3_RING_PASS
+    default: throw ArgumentError("ring pass: no matrix-core tiling for this grid");
+  }
+  SL3_HIP(hipGetLastError());
 }
 
 void LaunchRingMfma(const RingLaunch& r, hipStream_t stream) {

@@ -361,6 +361,34 @@ void Coupling::Ring(RingLaunch r, hipStream_t s) const {
   LaunchRing(r, s);
 }
 
+void Coupling::RingPass(double* const in[3], double* const out[3], const double* weight,
+                        hipStream_t s) const {
+  RingLaunch r;
+  r.tables = dft_;
+  r.lmax = lmax_;
+  r.nodes = geometry_.nodes;
+  r.ntheta = grid_.NTheta();
+  r.nphi = grid_.NPhi();
+  r.wtheta = wtheta_.data();
+  r.op = kRingPass;
+  r.weight = weight;
+  if (RingPassFusable(r)) {
+    RingBlocks blocks;
+    for (int b = 0; b < 3; ++b) {
+      blocks.in[b] = in[b];
+      blocks.out[b] = out[b];
+    }
+    LaunchRingPassMfma(r, blocks, s);
+    return;
+  }
+  for (int b = 0; b < 3; ++b) {
+    r.block = b;
+    r.fourierIn = in[b];
+    r.fourierOut = out[b];
+    LaunchRing(r, s);
+  }
+}
+
 void Coupling::Synthesise(const double* spectral, double* const grid[3],
                           double* const fourier[3], hipStream_t s) const {
   SynthesiseToFourier(spectral, fourier, false, s);
@@ -916,15 +944,7 @@ void Viscoelastic3D::Pass(const double* s, double* sNext, bool useStartForce, Fl
     double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
     double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
     coupling_.SynthesiseToFourier(s, A, true, st);
-    for (int b = 0; b < 3; ++b) {
-      RingLaunch r;
-      r.block = b;
-      r.op = kRingPass;
-      r.fourierIn = A[b];
-      r.fourierOut = B[b];
-      r.weight = weight_.data();
-      coupling_.Ring(r, st);
-    }
+    coupling_.RingPass(A, B, weight_.data(), st);
     coupling_.AnalyseFromFourier(B, analysed_.data(), fixed_.data(), true, st);
   }
   SolveAll(analysed_.data(), f);

@@ -87,6 +87,10 @@ class Coupling {
   void AnalyseFromFourier(const double* const fourier[3], double* spectral,
                           const double* addend, bool truncated, hipStream_t s) const;
   void Ring(RingLaunch r, hipStream_t s) const;  // fills tables and sizes
+  // The pass's longitude stage (Fourier -> ring x weight -> Fourier) over
+  // all three blocks: one launch where the grid allows it.
+  void RingPass(double* const in[3], double* const out[3], const double* weight,
+                hipStream_t s) const;
   // One block's Legendre stage alone (the C-ABI transforms).
   void SynthesiseBlockToFourier(int block, const double* spectral, double* fourier,
                                 hipStream_t s) const;

@@ -74,12 +74,22 @@ struct RingLaunch {
   int scheme = 0;
 };
 
+// The three blocks' Fourier buffers of one fused pass launch.
+struct RingBlocks {
+  const double* in[3] = {};
+  double* out[3] = {};
+};
+
 size_t RingLds(int nphi, int P);
 // The matrix-core ring kernel (ring_mfma.hip): its tile count for P (0 if
 // it does not apply), its LDS bytes, its launch.
 int RingMfmaTiles(int P);
 size_t RingMfmaLds(const RingLaunch& r);
 void LaunchRingMfma(const RingLaunch& r, hipStream_t stream);
+// The pass (op kRingPass) over all three blocks in one launch, for R = 4
+// grids the matrix-core kernel takes; r carries the common fields.
+bool RingPassFusable(const RingLaunch& r);
+void LaunchRingPassMfma(const RingLaunch& r, const RingBlocks& blocks, hipStream_t stream);
 void LaunchLegendreSynthesis(const LegendrePlan& p, const double* const spectral[3],
                              double* const fourier[3], const int* reach,
                              hipStream_t stream);
