@@ -34,10 +34,10 @@ PEAK_FP64 = 78.6e12     # flop/s, MI355X dense FP64 (vector and matrix)
 # x 2.0, WRITE_SIZE x 1.0); profiles/step_lmax128_hbm_traffic.md. Only valid for
 # the default workload.
 MEASURED_TRAFFIC = {
-    "ring": 9.247e8,
-    "response": 6.483e8,
-    "legendre_analysis": 1.471e9,
-    "legendre_synthesis": 1.255e9,
+    "ring_pass": 1.5152e9,
+    "response": 5.458e8,
+    "legendre_analysis": 8.424e8,
+    "legendre_synthesis": 7.061e8,
 }
 
 
