@@ -48,6 +48,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--lmax", type=int, default=128)
     ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--solver", choices=["fixed-point", "cg"], default="fixed-point",
+                    help="how a step closes its system: the fixed point (the default, the "
+                         "metric's) or preconditioned conjugate gradients")
+    ap.add_argument("--solve-rtol", type=float, default=1e-11,
+                    help="the conjugate gradients' relative residual (energy norm)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -168,8 +173,10 @@ def main():
     earth = st.Earth(st.three_region_earth(70, 56, 2, 6))
     length, tscale, mass = earth.scales()
     dt = 100.0 * st.YEAR / tscale
+    cg = args.solver == "cg"
     opts = st.StepOptions.default(max_iterations=200, rtol=args.rtol, anderson_memory=6,
-                                  anderson_rate=0.5)
+                                  anderson_rate=0.5, solve_rtol=args.solve_rtol,
+                                  solver=st.CONJUGATE_GRADIENTS if cg else st.FIXED_POINT)
     log("building the operator ...")
     t0 = time.perf_counter()
     op = st.Viscoelastic3D(earth, lmax, None, opts)
@@ -228,7 +235,7 @@ def main():
                         "work_per_launch": work, "avg_launch_ms": avg_s * 1e3}
     breakdown = {k: {"total_ms": round(v[0], 3), "launches": v[1]} for k, v in kernels.items()}
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not cg:
         cpu = cpu_baseline(op, earth, dt, load, float(np.mean(passes)), args.rtol)
     out = {
         "metric": METRIC,
@@ -247,8 +254,11 @@ def main():
             "workload": ("one 3D lateral-viscosity step, lmax 128, grid 194x388, 128 elements "
                          "(70/56/2) x 6 GLL nodes, 280 Gauss memory nodes, delta GRF "
                          "(1+l/8)^-3 to l=64 rms 0.7 AR(1) 0.9, exponential trapezoid "
-                         "dt 100 yr, zonal load rate, Anderson depth 6, rtol 1e-8"),
-            "variant": "plain GPU fixed point (no learned warm start, one realisation)",
+                         "dt 100 yr, zonal load rate, "
+                         + (f"conjugate gradients, solve_rtol {args.solve_rtol:g}" if cg else
+                            f"Anderson depth 6, rtol {args.rtol:g}")),
+            "variant": ("preconditioned conjugate gradients (one realisation)" if cg else
+                        "plain GPU fixed point (no learned warm start, one realisation)"),
             "lmax": lmax, "grid": [op.ntheta, op.nphi], "memory_nodes": op.nodes,
             "passes_per_step": passes, "mixed_passes": mixed, "converged": converged,
         },

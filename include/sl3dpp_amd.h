@@ -159,8 +159,12 @@ int sl3_earth_degree_system(const sl3_earth* earth, int l, int kind, int scheme,
 /* ===================================================================== *
  * Viscoelastic3D (include/sl3dpp/Viscoelastic3D.hpp:89-675): the 3D      *
  * lateral-viscosity step on the GPU. One Maxwell branch, Gauss memory    *
- * points, the fixed-point step solver with adaptive Anderson mixing.     *
+ * points; the step's system closed by the fixed point with adaptive      *
+ * Anderson mixing, or by preconditioned conjugate gradients.             *
  * ===================================================================== */
+/* Options::StepSolver (include/sl3dpp/Viscoelastic3D.hpp:122-123). */
+enum { SL3_FIXED_POINT = 0, SL3_CONJUGATE_GRADIENTS = 1 };
+
 typedef struct {
   double eps;           /* truncation level, 1e-8 */
   double oversampling;  /* 1.5 */
@@ -169,6 +173,10 @@ typedef struct {
   double damping;       /* 1 */
   int anderson_memory;  /* 5 */
   double anderson_rate; /* 0.5; <= 0 for Anderson from the first pass */
+  int solver;           /* SL3_FIXED_POINT */
+  double solve_rtol;    /* 1e-11: the relative residual in the preconditioned
+                           energy norm the conjugate gradients stop at */
+  int solve_max_iterations; /* 200 */
 } sl3_step_options;
 
 typedef struct {
@@ -177,6 +185,7 @@ typedef struct {
   int converged;
   int mixed_passes;
   float start_ms, passes_ms, finish_ms; /* device time of the phases */
+  int solve_iterations; /* conjugate-gradient iterations (StepInfo::solveIterations) */
 } sl3_step_info;
 
 /* Schemes (include/sl3dpp/StepWeights.hpp, StepRecipe.hpp): Sdirk2 runs
@@ -233,7 +242,8 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme,
  *             (device layout spectral), "solution:<task>" (the last pass's
  *             solution of a task, ndof x (l+1) complex column-major),
  *             "tasks" (per task: l, kind, ndof, first memory node, memory
- *             dofs), "residuals" (last step).
+ *             dofs), "residuals" (last step), "solve_energy" (the last
+ *             conjugate-gradient solve's <K^-1 f, f>, its residuals' scale).
  * dst NULL returns the size in doubles through *count. */
 int sl3_state_export(const sl3_operator* op, const sl3_state* state,
                      const char* name, double* dst, int64_t* count);

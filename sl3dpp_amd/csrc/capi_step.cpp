@@ -42,6 +42,12 @@ StepOptions ToOptions(const sl3_step_options* o) {
   s.damping = o->damping;
   s.andersonMemory = o->anderson_memory;
   s.andersonRate = o->anderson_rate;
+  s.solver = o->solver;
+  s.solveRtol = o->solve_rtol;
+  s.solveMaxIterations = o->solve_max_iterations;
+  Check(s.solver == kFixedPoint || s.solver == kConjugateGradients,
+        "solver: SL3_FIXED_POINT or SL3_CONJUGATE_GRADIENTS");
+  Check(s.solveMaxIterations >= 0, "solve_max_iterations >= 0");
   Check(s.andersonMemory >= 0 && s.andersonMemory <= 8, "anderson memory in [0, 8]");
   Check(s.maxIterations >= 1, "max_iterations >= 1");
   Check(s.eps > 0.0 && s.eps < 1.0, "eps in (0, 1)");
@@ -253,6 +259,9 @@ void sl3_step_options_default(sl3_step_options* o) {
   o->damping = s.damping;
   o->anderson_memory = s.andersonMemory;
   o->anderson_rate = s.andersonRate;
+  o->solver = s.solver;
+  o->solve_rtol = s.solveRtol;
+  o->solve_max_iterations = s.solveMaxIterations;
 }
 
 int sl3_operator_create(const sl3_earth* earth, int lmax, const double* delta_dev,
@@ -368,6 +377,7 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme, sl3_load
       info->start_ms = op->op->lastTiming.startStep;
       info->passes_ms = op->op->lastTiming.passes;
       info->finish_ms = op->op->lastTiming.finish;
+      info->solve_iterations = r.solveIterations;
     }
   });
 }
@@ -412,6 +422,7 @@ int sl3_operator_export(sl3_operator* op, const char* name, double* dst, int64_t
     if (key == "fixed") return Export(v.FixedForce(), c.SpectralDoubles(), dst, count);
     if (key == "analysed") return Export(v.Analysed(), c.SpectralDoubles(), dst, count);
     if (key == "residuals") return ExportHost(v.residualHistory, dst, count);
+    if (key == "solve_energy") return ExportHost(v.solveEnergy, dst, count);
     if (key == "mu_node") return ExportHost(v.MuNode(), dst, count);
     if (key == "itau_ref_node") return ExportHost(v.ItauRefNode(), dst, count);
     if (key == "tasks") {

@@ -572,6 +572,9 @@ void Viscoelastic3D::BuildTasks() {
   const auto forceWork = ForceWork(L);
   forceWork_.Upload(forceWork.data(), forceWork.size(), s);
   forceWorkCount_ = static_cast<int>(forceWork.size());
+  const auto pairWork = PairWorkList(tasks_);
+  pairWork_.Upload(pairWork.data(), std::max<size_t>(pairWork.size(), 1), s);
+  pairWorkCount_ = static_cast<int>(pairWork.size());
   SL3_HIP(hipStreamSynchronize(s));
 }
 
@@ -729,10 +732,13 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
   // cache, src/Viscoelastic3D.cpp:1042-1055): Load() and Tide(2 at l = 0,
   // else l) solved together with this flavour.
   std::vector<double> unitRhs(unitDoubles_, 0.0);
+  std::vector<std::vector<double>> unitForce(L + 1);  // load, then tide, per degree
   for (const auto& t : tasks_) {
     if (t.kind != 0) continue;
     const auto load = sph[t.l]->Load();
     const auto tide = sph[t.l]->Tide(t.l == 0 ? 2 : -1);
+    unitForce[t.l] = load;
+    unitForce[t.l].insert(unitForce[t.l].end(), tide.begin(), tide.end());
     for (int d = 0; d < t.ndof; ++d) {  // BeginSolve
       unitRhs[t.unitOff + d] = load[d] * scale[t.scaleOff + d];
       unitRhs[t.unitOff + t.ndof + d] = tide[d] * scale[t.scaleOff + d];
@@ -782,6 +788,22 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
     }
   }
   f->units.Upload(units.data(), units.size(), s);
+  // the channels' part of the forcing's energy <f, K^-1 f> (the conjugate
+  // gradients' scale): each unit force against each unit response
+  f->channelEnergy.assign(4 * static_cast<size_t>(L + 1), 0.0);
+  for (const auto& t : tasks_) {
+    if (t.kind != 0) continue;
+    const double* u = unitForce[t.l].data();
+    const double* z = units.data() + t.unitOff;
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int d = 0; d < t.ndof; ++d) {
+      e[0] += u[d] * z[d];
+      e[1] += u[d] * z[t.ndof + d];
+      e[2] += u[t.ndof + d] * z[d];
+      e[3] += u[t.ndof + d] * z[t.ndof + d];
+    }
+    std::copy(e, e + 4, f->channelEnergy.begin() + 4 * t.l);
+  }
   const size_t memDofs = dmHost_.size();
   std::vector<double> unitsMem(std::max<size_t>(2 * memDofs, 1), 0.0);
   for (const auto& t : tasks_) {
@@ -916,6 +938,7 @@ void Viscoelastic3D::Amplitudes(const Channels& channels, double t) {
   if (channels.load) gather(channels.load, amp.data(), false);
   hasTide_ = static_cast<bool>(channels.tide);
   if (hasTide_) gather(channels.tide, amp.data() + ampDoubles_, true);
+  ampHost_ = amp;
   hipStream_t s = Stream();
   SL3_HIP(hipMemcpyAsync(amp_.data(), amp.data(), sizeof(double) * amp.size(),
                          hipMemcpyHostToDevice, s));
@@ -1057,6 +1080,7 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const Channel
   StepInfo info;
   info.iterations = first.iterations + second.iterations;
   info.mixedPasses = first.mixedPasses + second.mixedPasses;
+  info.solveIterations = first.solveIterations + second.solveIterations;
   info.residual = second.residual;
   info.converged = first.converged && second.converged;
   residualHistory.insert(residualHistory.begin(), firstHistory.begin(), firstHistory.end());
@@ -1112,14 +1136,17 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
     coupling_.Ring(r, s);
   }
   coupling_.AnalyseFromFourier(B, fixed_.data(), nullptr, true, s);
-  coupling_.AnalyseFromFourier(A, analysed_.data(), fixed_.data(), true, s);
+  const bool cg = options_.solver == kConjugateGradients;
+  // the first pass's start force (the fixed point's only)
+  if (!cg) coupling_.AnalyseFromFourier(A, analysed_.data(), fixed_.data(), true, s);
   SL3_HIP(hipEventRecord(events_[1], s));
 
-  // Converge (:1916-1974)
+  // Converge (:1916-1974), or the conjugate-gradient solve (:2008-2015)
   StepInfo info;
   residualHistory.clear();
+  if (cg) info = CloseByConjugateGradients(f);  // takes no warm start, as the reference's
   bool warm = false;
-  if (start != nullptr) {
+  if (start != nullptr && !cg) {
     LaunchCopy(s_.data(), start, S, s);
     warm = true;
   }
@@ -1151,7 +1178,7 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
     sw.slot = slot;
     return sw;
   };
-  for (int it = 0; it < options_.maxIterations; ++it) {
+  for (int it = 0; it < (cg ? 0 : options_.maxIterations); ++it) {
     Pass(s_.data(), sNext_.data(), it == 0 && !warm, f);
     // Once the mixer runs (and without damping) every pass that does not
     // converge takes a sweep over the same two iterates: the sweep then also
@@ -1254,6 +1281,122 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
   SL3_HIP(hipEventElapsedTime(&lastTiming.startStep, events_[0], events_[1]));
   SL3_HIP(hipEventElapsedTime(&lastTiming.passes, events_[1], events_[2]));
   SL3_HIP(hipEventElapsedTime(&lastTiming.finish, events_[2], events_[3]));
+  return info;
+}
+
+void Viscoelastic3D::LateralForce(const double* response, bool channels, double* force,
+                                  Flavour& f) {
+  hipStream_t s = Stream();
+  const auto* sph = tasksDev_.data();
+  const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
+  // the strain (the unit responses' pointers stay valid: the kernel reads
+  // them times a zero amplitude when there are no channels)
+  LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), dofsMem_.data(), response,
+               f.unitsMem.data(), channels ? amp_.data() : nullptr,
+               f.unitsMem.data() + dmHost_.size(), channels ? TideAmplitudes() : nullptr,
+               coupling_.View(sNext_.data()), s);
+  double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
+  double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
+  coupling_.SynthesiseToFourier(sNext_.data(), A, true, s);
+  coupling_.RingPass(A, B, weight_.data(), s);
+  coupling_.AnalyseFromFourier(B, analysed_.data(), nullptr, true, s);
+  LaunchForce(tasksDev_.data(), coupling_.Geometry(), coupling_.View(analysed_.data()),
+              dofsMem_.data(), forceWork_.data(), forceWorkCount_, force, s);
+}
+
+double Viscoelastic3D::Pair(const double* x, const double* force, const double* g,
+                            const ChannelTerms& ch) {
+  hipStream_t s = Stream();
+  LaunchLateralPair(tasksDev_.data(), pairWork_.data(), pairWorkCount_, x, force, g, ch,
+                    cgPartials_.data(), result_.data(), s);
+  double v = 0.0;
+  SL3_HIP(hipMemcpyAsync(&v, result_.data(), sizeof(v), hipMemcpyDeviceToHost, s));
+  SL3_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+// detail::LateralConjugateGradients (include/sl3dpp/detail/LateralSolve.hpp:
+// 110-183) with no start, on the memory dofs (lateral_cg.hpp): x = x0 +
+// K^-1 Pᵀ c, kept as phi = f_m + c (its force) and y = G phi (its response
+// without the channels), so the last pass's export solves phi as the fixed
+// point's does its force.
+StepInfo Viscoelastic3D::CloseByConjugateGradients(Flavour& f) {
+  hipStream_t s = Stream();
+  const size_t n = std::max<size_t>(fDoubles_, 1);
+  for (auto* v : {&cgPhi_, &cgY_, &cgRho_, &cgGrho_, &cgPi_, &cgGpi_, &cgQ_}) {
+    if (v->size() != n) {
+      v->Resize(n);
+      LaunchFill(v->data(), n, 0.0, s);  // the padding stays zero
+    }
+  }
+  cgPartials_.Resize(std::max(pairWorkCount_, 1));
+  const ChannelTerms none{};
+  auto response = [&](const double* force, double* out) {
+    LaunchResponse(tasksDev_.data(), work_.data(), workCount_, f.response.data(), force, out,
+                   s);
+  };
+  // x0 = K^-1 f (OpenStep::Forces, :1448-1471): the fixed memory force's
+  // response; the channels enter through their unit responses
+  SolveAll(fixed_.data(), f);  // force_ = f_m, response_ = G f_m
+  // <x0, f> = <G f_m + 2 chan, f_m> + <chan force, K^-1 chan force>, K^-1
+  // symmetric (LateralSolve.hpp:118)
+  double chanchan = 0.0;
+  const int L = coupling_.LMax();
+  for (int l = 0; l <= L; ++l) {
+    const double* e = f.channelEnergy.data() + 4 * l;
+    for (int m = 0; m <= l; ++m) {
+      const size_t pair = static_cast<size_t>(l) * (l + 1) / 2 + m;
+      for (int part = 0; part < 2; ++part) {
+        const double a = ampHost_[2 * pair + part];
+        const double b = hasTide_ ? ampHost_[ampDoubles_ + 2 * pair + part] : 0.0;
+        chanchan += (m == 0 ? 1.0 : 2.0) * (a * a * e[0] + a * b * (e[1] + e[2]) + b * b * e[3]);
+      }
+    }
+  }
+  ChannelTerms twice{f.unitsMem.data(), amp_.data(), f.unitsMem.data() + dmHost_.size(),
+                     TideAmplitudes(), 2.0};
+  const double energy = Pair(response_.data(), force_.data(), nullptr, twice) + chanchan;
+  solveEnergy.assign(1, energy);
+  LaunchCopy(cgY_.data(), response_.data(), n, s);
+  // r0 = f - A x0 = Pᵀ L P x0
+  LateralForce(response_.data(), true, cgRho_.data(), f);
+  LaunchCopy(cgPhi_.data(), force_.data(), n, s);
+  response(cgRho_.data(), cgGrho_.data());
+  double rz = Pair(cgGrho_.data(), cgRho_.data(), nullptr, none);
+  auto relative = [&](double v) {
+    return energy > 0.0 ? std::sqrt(std::abs(v) / energy) : std::sqrt(std::abs(v));
+  };
+  StepInfo info;
+  info.residual = relative(rz);
+  residualHistory.push_back(info.residual);
+  LaunchCopy(cgPi_.data(), cgRho_.data(), n, s);
+  LaunchCopy(cgGpi_.data(), cgGrho_.data(), n, s);
+  const double target = options_.solveRtol;
+  for (int it = 0; it < options_.solveMaxIterations; ++it) {
+    if (info.residual <= target || !std::isfinite(info.residual)) break;
+    LateralForce(cgGpi_.data(), false, cgQ_.data(), f);  // q = L G pi
+    const double curvature = Pair(cgGpi_.data(), cgPi_.data(), cgQ_.data(), none);
+    const double alpha = rz / curvature;
+    LaunchCgAdvance(cgPhi_.data(), cgY_.data(), cgRho_.data(), cgPi_.data(), cgGpi_.data(),
+                    cgQ_.data(), alpha, n, s);
+    response(cgRho_.data(), cgGrho_.data());
+    const double next = Pair(cgGrho_.data(), cgRho_.data(), nullptr, none);
+    const double beta = next / rz;
+    rz = next;
+    LaunchCgDirection(cgPi_.data(), cgGpi_.data(), cgRho_.data(), cgGrho_.data(), beta, n, s);
+    info.iterations = it + 1;
+    info.residual = relative(rz);
+    residualHistory.push_back(info.residual);
+  }
+  info.converged = info.residual <= target;
+  info.solveIterations = info.iterations;
+  // the solution: its force for the export's band solves, its strain
+  LaunchCopy(force_.data(), cgPhi_.data(), n, s);
+  LaunchCopy(response_.data(), cgY_.data(), n, s);
+  LaunchStrain(tasksDev_.data(), tasksDev_.data() + (L + 1), L, coupling_.Geometry(),
+               dofsMem_.data(), response_.data(), f.unitsMem.data(), amp_.data(),
+               f.unitsMem.data() + dmHost_.size(), TideAmplitudes(), coupling_.View(s_.data()),
+               s);
   return info;
 }
 

@@ -17,6 +17,7 @@
 #include "coupling.hpp"
 #include "grid.hpp"
 #include "kernels.hpp"
+#include "lateral_cg.hpp"
 #include "model.hpp"
 #include "operators.hpp"
 #include "response.hpp"
@@ -24,8 +25,12 @@
 
 namespace sl3amd {
 
-// Viscoelastic3D::Options (include/sl3dpp/Viscoelastic3D.hpp:89-187), the
-// fixed-point subset.
+// How Step closes a step's linear system (Options::StepSolver,
+// include/sl3dpp/Viscoelastic3D.hpp:122-125).
+enum { kFixedPoint = 0, kConjugateGradients = 1 };
+
+// Viscoelastic3D::Options (include/sl3dpp/Viscoelastic3D.hpp:89-187): the
+// fixed point's controls and the step solver's.
 struct StepOptions {
   double eps = 1e-8;
   double oversampling = 1.5;
@@ -34,6 +39,9 @@ struct StepOptions {
   double damping = 1.0;
   int andersonMemory = 5;
   double andersonRate = 0.5;
+  int solver = kFixedPoint;
+  double solveRtol = 1e-11;
+  int solveMaxIterations = 200;
 };
 
 // Viscoelastic3D::StepInfo (include/sl3dpp/Viscoelastic3D.hpp:357-375).
@@ -42,6 +50,7 @@ struct StepInfo {
   double residual = 0.0;
   bool converged = false;
   int mixedPasses = 0;
+  int solveIterations = 0;  // conjugate-gradient iterations
 };
 
 // LoadCallable3D: the m >= 0 columns of degree l at time t, (re, im) pairs,
@@ -143,6 +152,9 @@ struct Flavour {
   // and the unit load and tide responses on the memory dofs (load for every
   // memory dof, then tide for every memory dof)
   DeviceArray<double> response, unitsMem;
+  // per degree, the unit channel forces against their responses over every
+  // dof: load.zLoad, load.zTide, tide.zLoad, tide.zTide
+  std::vector<double> channelEnergy;
 };
 
 // Viscoelastic3D::State on the device: one branch's memory tensor, the
@@ -184,8 +196,11 @@ class Viscoelastic3D {
   struct Timing {
     float startStep = 0, passes = 0, finish = 0;
   } lastTiming;
-  // Per-pass residuals of the last step.
+  // Per-pass residuals of the last step (conjugate gradients: the relative
+  // residual before each iteration and after the last).
   std::vector<double> residualHistory;
+  // The last conjugate-gradient solve's scale <x0, f> (x0 = K^-1 f).
+  std::vector<double> solveEnergy;
   // The device buffers, for export.
   const double* Weight() const { return weight_.data(); }
   const double* Beta() const { return beta_.data(); }
@@ -207,6 +222,16 @@ class Viscoelastic3D {
   StepInfo StepOnce(DeviceState& state, double dt, int scheme, const Channels& channels,
                     const double* start);
   StepInfo StepSdirk2(DeviceState& state, double dt, const Channels& channels);
+  // OpenStep::Forces + Solve + StrainOf (src/Viscoelastic3D.cpp:2008-2015):
+  // the step's system by conjugate gradients on the memory dofs, its strain
+  // coefficients into s_.
+  StepInfo CloseByConjugateGradients(Flavour& f);
+  // The lateral force of a response on the memory dofs (with the channels'
+  // unit responses added when `channels`): strain, synthesis, the weight,
+  // analysis and the force map, into `force` (the force layout).
+  void LateralForce(const double* response, bool channels, double* force, Flavour& f);
+  // LateralPair of a response and a force (minus g when given), on the host.
+  double Pair(const double* x, const double* force, const double* g, const ChannelTerms& ch);
   void CopyState(const DeviceState& from, DeviceState& to) const;
   void BuildTasks();
   void BuildResponse(Flavour& f);
@@ -244,10 +269,17 @@ class Viscoelastic3D {
   int workCount_ = 0;
   DeviceArray<int2> forceWork_;  // the force kernel's (degree, order chunk) list
   int forceWorkCount_ = 0;
+  DeviceArray<PairWork> pairWork_;  // the pairing's tiles
+  int pairWorkCount_ = 0;
+  // the conjugate gradients' memory-dof vectors (allocated on first use):
+  // the force so far and its response, the residual and its response, the
+  // direction and its response, the direction's lateral force
+  DeviceArray<double> cgPhi_, cgY_, cgRho_, cgGrho_, cgPi_, cgGpi_, cgQ_, cgPartials_;
   std::list<std::unique_ptr<Flavour>> flavours_;
   // the open step's buffers
   DeviceArray<double> weight_, beta_, fixed_, analysed_, amp_, s_, sNext_;
   size_t ampDoubles_ = 0;  // one channel's amplitudes, (L + 1)(L + 2)
+  std::vector<double> ampHost_;  // the last amplitudes gathered (load, then tide)
   struct {
     int scheme = -1;  // none yet
     double dt = 0.0;

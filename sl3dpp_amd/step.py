@@ -22,6 +22,8 @@ EXPONENTIAL_TRAPEZOID = 0
 BACKWARD_EULER = 1
 SDIRK2 = 2
 SDIRK2_GAMMA = 0.29289321881345247560  # 1 - 1/sqrt(2)
+FIXED_POINT = 0  # Options::StepSolver
+CONJUGATE_GRADIENTS = 1
 
 MAX_LAYERS = 8
 
@@ -55,6 +57,9 @@ class StepOptions(ctypes.Structure):
         ("damping", ctypes.c_double),
         ("anderson_memory", ctypes.c_int),
         ("anderson_rate", ctypes.c_double),
+        ("solver", ctypes.c_int),
+        ("solve_rtol", ctypes.c_double),
+        ("solve_max_iterations", ctypes.c_int),
     ]
 
     @classmethod
@@ -77,6 +82,7 @@ class StepInfo(ctypes.Structure):
         ("start_ms", ctypes.c_float),
         ("passes_ms", ctypes.c_float),
         ("finish_ms", ctypes.c_float),
+        ("solve_iterations", ctypes.c_int),
     ]
 
 
