@@ -35,6 +35,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "ring_complex.hpp"
 #include "transform.hpp"
 
 namespace sl3amd {
@@ -673,13 +674,14 @@ __global__ void __launch_bounds__(kThreads) RingMfmaStartKernel(RingLaunch a) {
 }
 
 // The pass over all three blocks in one launch. A super-tile is one node's
-// 16 rings (R = 4): two 8-ring workgroups of each complex block and one
-// 16-ring workgroup of spin 0, the five numbered onto one XCD (workgroups
-// go to the XCDs round robin) so the rings' weights come from its L2 after
-// the first read; one launch also leaves a single tail instead of three.
-constexpr int kPassGroups = 5;
+// 16 rings (R = 4): one workgroup of each complex block with whole complex
+// values per lane (ring_complex.hpp) and one 16-ring workgroup of spin 0 on
+// the body above, the three numbered onto one XCD (workgroups go to the XCDs
+// round robin) so the rings' weights come from its L2 after the first read;
+// one launch also leaves a single tail instead of three.
+constexpr int kPassGroups = 3;
 
-template <int JT, bool FULL>
+template <int JT, bool FULL, int KIN>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
     RingMfmaPassKernel(RingLaunch base, RingBlocks blocks) {
   const int p = blockIdx.x;
@@ -688,16 +690,16 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
   const int groups = (base.ntheta + 15) / 16;
   if (st >= groups * base.nodes) return;
   const int node = st / groups, th16 = (st % groups) * 16;
-  RingLaunch a = base;
-  a.block = sub < 2 ? 0 : (sub < 4 ? 1 : 2);
-  a.fourierIn = blocks.in[a.block];
-  a.fourierOut = blocks.out[a.block];
-  if (a.block == 2) {
-    RingBody<JT, true, kClassOne, FULL>(a, node, th16);
+  if (sub < 2) {
+    complex_lanes::ComplexPassBody<JT, FULL, KIN>(
+        base, reinterpret_cast<const double2*>(blocks.in[sub]),
+        reinterpret_cast<double2*>(blocks.out[sub]), node, th16);
   } else {
-    const int th0 = th16 + 8 * (sub % 2);
-    if (th0 >= a.ntheta) return;
-    RingBody<JT, false, kClassOne, FULL>(a, node, th0);
+    RingLaunch a = base;
+    a.block = 2;
+    a.fourierIn = blocks.in[2];
+    a.fourierOut = blocks.out[2];
+    RingBody<JT, true, kClassOne, FULL>(a, node, th16);
   }
 }
 
@@ -770,34 +772,37 @@ size_t RingMfmaLds(const RingLaunch& r) {
 }
 
 bool RingPassFusable(const RingLaunch& r) {
-  return r.tables.R == 4 && r.tables.jt > 0 && RingMfmaLds(r) <= 160 * 1024;
+  return r.tables.R == 4 && r.tables.jt > 0 && r.lmax >= 3 && RingMfmaLds(r) <= 160 * 1024;
 }
 
 void LaunchRingPassMfma(const RingLaunch& r, const RingBlocks& blocks, hipStream_t stream) {
   if (!RingPassFusable(r)) throw ArgumentError("ring pass: no fused launch for this grid");
   const int superTiles = CeilDiv(r.ntheta, 16) * r.nodes;
   const dim3 grid(CeilDiv(superTiles, 8) * 8 * kPassGroups);
+  // the spin-0 workgroups' staging tile is the larger footprint
   const size_t lds = RingMfmaLds(r);
   const bool full = (r.tables.P - 1) / 2 == 16 * r.tables.jt;
+  const int H = (r.tables.P - 1) / 2;
+  const int kIn = (std::min(H, (r.lmax + 3) / 4) + 3) / 4;
   ProfileScope prof("ring_pass", stream);
-  switch (r.tables.jt) {
-#define SL3_RING_PASS(JT)                                                                   \
-  case JT:                                                                                \
-    if (full) {                                                                           \
-      hipLaunchKernelGGL((RingMfmaPassKernel<JT, true>), grid, dim3(kThreads), lds, stream, \
-                         r, blocks);                                                      \
-    } else {                                                                              \
-      hipLaunchKernelGGL((RingMfmaPassKernel<JT, false>), grid, dim3(kThreads), lds,       \
-                         stream, r, blocks);                                              \
-    }                                                                                     \
-    break;
-    SL3_RING_PASS(1)
-    SL3_RING_PASS(2)
-    SL3_RING_PASS(3)
-    SL3_RING_PASS(4)
-#undef SL3_RING_PASS
-    default: throw ArgumentError("ring pass: no matrix-core tiling for this grid");
+#define SL3_RING_PASS(JT, FULL, KIN)                                                    \
+  hipLaunchKernelGGL((RingMfmaPassKernel<JT, FULL, KIN>), grid, dim3(kThreads), lds,  \
+                     stream, r, blocks)
+  // the workload's grid (lmax 128: P = 97, three full tiles, kmax 32) with
+  // the complex blocks' contraction fixed; every other grid with a run-time
+  // count
+  if (r.tables.jt == 3 && full && kIn == 8) {
+    SL3_RING_PASS(3, true, 8);
+  } else {
+    switch (r.tables.jt) {
+      case 1: if (full) SL3_RING_PASS(1, true, 0); else SL3_RING_PASS(1, false, 0); break;
+      case 2: if (full) SL3_RING_PASS(2, true, 0); else SL3_RING_PASS(2, false, 0); break;
+      case 3: if (full) SL3_RING_PASS(3, true, 0); else SL3_RING_PASS(3, false, 0); break;
+      case 4: if (full) SL3_RING_PASS(4, true, 0); else SL3_RING_PASS(4, false, 0); break;
+      default: throw ArgumentError("ring pass: no matrix-core tiling for this grid");
+    }
   }
+#undef SL3_RING_PASS
   SL3_HIP(hipGetLastError());
 }
 
