@@ -382,11 +382,10 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
     }
     w0 = a.weight[p0];
   }
-  // LDS: the twiddles cos / sin(2 pi n / nphi), the operand tables (all
-  // classes but the step start, which keeps them in the cache), then one
-  // tile that holds the staged input coefficients and, after them, the
+  // LDS: the twiddles cos / sin(2 pi n / nphi), the operand tables, then
+  // one tile that holds the staged input coefficients and, after them, the
   // staged output orders.
-  constexpr bool kTabLds = CLASS != kClassStart;
+  constexpr bool kTabLds = true;
   constexpr int kTab = JT * KS * kWave;
   double* cosN = lds;
   double* sinN = lds + g.nphi;
@@ -516,12 +515,13 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
 
   const double2* Zs = stage;
   // y at the lane's points -> forward butterflies -> forward transforms
-  auto forward = [&](const double (&yj)[KS], const double (&yq)[KS], double y0, double* out) {
+  auto forward = [&](const double (&yj)[KS], const double (&yq)[KS], double y0, double* out,
+                     const double* cosT, const double* sinT) {
     double uj[KS], uq[KS];
 #pragma unroll
     for (int v = 0; v < KS; ++v) {
-      uj[v] = ForwardButterfly(g, ln, yj[v], cosN[twj(v)], sinN[twj(v)]);
-      uq[v] = ForwardButterfly(g, ln, yq[v], cosN[twq(v)], sinN[twq(v)]);
+      uj[v] = ForwardButterfly(g, ln, yj[v], cosT[twj(v)], sinT[twj(v)]);
+      uq[v] = ForwardButterfly(g, ln, yq[v], cosT[twq(v)], sinT[twq(v)]);
     }
     const double u0 = ForwardButterfly(g, ln, y0, 1.0, 0.0);
     ForwardSequences<JT, FULL>(g, ln, tabC, tabS, uj, uq, u0, row);
@@ -577,7 +577,9 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
     };
     const double refBeta =
         a.scheme == 1 ? 1.0 - 1.0 / (1.0 + href) : 1.0 - phi1(href);
-    // every point's itau and m first (unconditional loads, issued together)
+    // every point's itau and m first (unconditional loads, issued together),
+    // after the inverse transforms: held through them they would spill
+    __builtin_amdgcn_sched_barrier(0);
     double hj[KS], hq[KS], mInJ[KS], mInQ[KS];
 #pragma unroll
     for (int v = 0; v < KS; ++v) {
@@ -587,7 +589,7 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
       mInQ[v] = a.memIn[element(pq(v))];
     }
     const double h0 = a.itau[p0], mIn0 = a.memIn[element(p0)];
-    // returns W d0 and writes m~, (spin 0) beta and W; mt gets mu m~
+    // returns W d0 and writes m~, (spin 0) beta and W; mt gets m~
     auto start = [&](size_t pt, double itau, double mIn, double d0, bool store,
                      double& mt) -> double {
       const double h = a.dt * itau;
@@ -612,23 +614,38 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
           a.weightOut[pt] = mu * (bw - refBeta);
         }
       }
-      mt = mu * mNew;
+      mt = mNew;
       return mu * (bw - refBeta) * d0;
     };
-    double yj[KS], yq[KS], mj[KS], mq[KS], y0 = 0.0, m0 = 0.0;
+    double yj[KS], yq[KS], y0 = 0.0, m0 = 0.0, unused;
     y0 = start(p0, h0, mIn0, x0, own0, m0);
     if (!ln.valid) y0 = m0 = 0.0;
+    m0 *= mu;
 #pragma unroll
     for (int v = 0; v < KS; ++v) {
-      yj[v] = start(pj(v), hj[v], mInJ[v], xj[v], lv(v), mj[v]);
-      yq[v] = start(pq(v), hq[v], mInQ[v], xq[v], lv(v), mq[v]);
-      if (!lv(v)) yj[v] = yq[v] = mj[v] = mq[v] = 0.0;
+      yj[v] = start(pj(v), hj[v], mInJ[v], xj[v], lv(v), unused);
+      yq[v] = start(pq(v), hq[v], mInQ[v], xq[v], lv(v), unused);
+      if (!lv(v)) yj[v] = yq[v] = 0.0;
     }
     // the analysis of W d0 (:1361-1376) ...
-    forward(yj, yq, y0, a.fourierOut2);
+    int fresh1 = 0;  // the twiddles read again, not held from the synthesis
+    asm volatile("" : "+v"(fresh1));
+    forward(yj, yq, y0, a.fourierOut2, cosN + fresh1, sinN + fresh1);
     __syncthreads();
-    // ... and of the fixed memory force mu m~ (:1355-1356)
-    forward(mj, mq, m0, a.fourierOut);
+    // ... and of the fixed memory force mu m~ (:1355-1356): m~ at the
+    // lane's points j and P - j read back from the lane's own stores (in the
+    // cache) rather than held in registers through the first analysis; at
+    // P n2, which one lane of the column stores, every lane kept its value.
+    double mj[KS], mq[KS];
+#pragma unroll
+    for (int v = 0; v < KS; ++v) {
+      mj[v] = lv(v) ? mu * a.memOut[element(pj(v))] : 0.0;
+      mq[v] = lv(v) ? mu * a.memOut[element(pq(v))] : 0.0;
+    }
+    // the twiddles read again (an opaque offset), not held from the first
+    int fresh = 0;
+    asm volatile("" : "+v"(fresh));
+    forward(mj, mq, m0, a.fourierOut, cosN + fresh, sinN + fresh);
   } else {
     // y = factor x (pass and weighted: the point's weight; node scaled: mu)
     double yj[KS], yq[KS], y0 = x0;
@@ -654,7 +671,7 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
         yq[v] = xq[v];
       }
     }
-    forward(yj, yq, y0, a.fourierOut);
+    forward(yj, yq, y0, a.fourierOut, cosN, sinN);
   }
 }
 
@@ -668,7 +685,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
 }
 
 template <int JT, bool REAL, bool FULL>
-__global__ void __launch_bounds__(kThreads) RingMfmaStartKernel(RingLaunch a) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+    RingMfmaStartKernel(RingLaunch a) {
   RingBody<JT, REAL, kClassStart, FULL>(a, blockIdx.y,
                                         blockIdx.x * MakeGeometry<REAL>(a).ringsPerGroup);
 }
@@ -761,10 +779,8 @@ int RingMfmaTiles(int P) {
 
 size_t RingMfmaLds(const RingLaunch& r) {
   const int slots = kWaves * kSeq / r.tables.R;
-  const bool start = r.op == kRingStart || r.op == kRingStartFromSpectral;
   // the operand tables: 2 x (JT x KS x 64) doubles
-  const size_t tables =
-      start ? 0 : 2 * sizeof(double) * (4 * r.tables.jt * r.tables.jt) * kWave;
+  const size_t tables = 2 * sizeof(double) * (4 * r.tables.jt * r.tables.jt) * kWave;
   // the staged rows (input coefficients, then outputs): slots x padded orders
   const int stride = 2 * (r.lmax + RowPad(r.lmax, r.tables.R, r.tables.P)) + 1;
   const size_t rows = 16 * static_cast<size_t>(slots) * stride;
