@@ -134,8 +134,11 @@ def run_pair(earth, oracle, lmax, delta_scale, steps, max_iterations, rtol, hist
 
 
 # lmax 64: the two-tile ring kernel; lmax 128: three tiles whose rows are
-# all frequencies (H = 48), the workload's grid
-@pytest.mark.parametrize("lmax", [8, 10, 12, 32, 64, 128])
+# all frequencies (H = 48), the workload's grid, with the complex blocks'
+# contraction fixed at compile time; lmax 96 (P = 73, H = 36) and 160
+# (P = 121, H = 60): three and four tiles with padding rows and a run-time
+# contraction length
+@pytest.mark.parametrize("lmax", [8, 10, 12, 32, 64, 96, 128, 160])
 def test_one_pass_matches_oracle(earth, oracle, lmax):
     results = run_pair(earth, oracle, lmax, 0.7, steps=2, max_iterations=1, rtol=1e-14)
     for info, rinfo, c, rc, m, rm, d, rd in results:
