@@ -4,8 +4,9 @@
  *
  * Plain C: pointers, sizes and status codes; no HIP or torch types. Device
  * pointers are marked `_dev`; everything else is host memory borrowed for
- * the duration of the call. Calls run on the library's stream (see
- * sl3_synchronize) and are asynchronous unless they return host data.
+ * the duration of the call. Calls run on the library's stream of the calling
+ * thread (see sl3_synchronize) and are asynchronous unless they return host
+ * data.
  *
  * Status codes: 0 success; < 0 argument or runtime failure (message from
  * sl3_last_error); > 0 a numerical failure the reference reports by
@@ -34,7 +35,8 @@ enum {
 const char* sl3_last_error(void);
 /* Library version string. */
 const char* sl3_version(void);
-/* Waits for every call issued so far on the library's stream. */
+/* Waits for every call issued so far on the calling thread's library
+ * stream (one per host thread). */
 int sl3_synchronize(void);
 
 /* Kernel timing with device events on the library's stream (off by

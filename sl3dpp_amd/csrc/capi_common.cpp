@@ -19,14 +19,14 @@ thread_local std::string lastError;
 
 void SetLastError(const std::string& message) { lastError = message; }
 
-// One stream for the library's work on the current device, at the default
-// priority (created on first use).
+// The library's stream: one per host thread, at the default priority
+// (created on the thread's first call, kept until the process ends). Every
+// entry point finishes its work on it before returning, so a handle made
+// on one thread can be used on another, and operators stepped from
+// different threads (independent realisations) run concurrently.
 hipStream_t Stream() {
-  static std::once_flag once;
-  static hipStream_t stream = nullptr;
-  std::call_once(once, [] {
-    SL3_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  });
+  thread_local hipStream_t stream = nullptr;
+  if (stream == nullptr) SL3_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   return stream;
 }
 
