@@ -675,9 +675,9 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
   }
 }
 
-// The per-pass classes run two waves per SIMD (two workgroups per CU, as
-// the LDS tile allows): a register budget of 256 per lane. The step start,
-// once per step, keeps its larger live set unspilled at one wave.
+// Every class runs two waves per SIMD (two workgroups per CU, as the LDS
+// tile allows): a register budget of 256 per lane. The step start fits it
+// by reading m~ back for its second analysis instead of holding it.
 template <int JT, bool REAL, int CLASS, bool FULL>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
     RingMfmaKernel(RingLaunch a) {
