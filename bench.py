@@ -153,9 +153,9 @@ def cpu_baseline(op, earth, dt, load, passes_per_step, rtol):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the step shards by radius across GPUs only by design, see DESIGN.md)",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: the step "
+              "would shard by radius (contiguous memory-node slices per GPU, one separator "
+              "sum per pass), which is not built", file=sys.stderr)
         return 2
     import numpy as np
     import torch
