@@ -217,10 +217,19 @@ int sl3_operator_geometry(const sl3_operator* op, double* radii, double* theta,
 
 /* LoadCallable3D: write degree l's m >= 0 columns at time t as (re, im)
  * pairs into columns (2(l+1) doubles) and return 1, or return 0 for none.
- * The same type serves the load and the tide channel. */
+ * The same type serves the load and the tide channel. A negative return
+ * (the caller's own failure, an exception in the reference's callable)
+ * ends the step with SL3_EINVAL: the channels are gathered before the
+ * state changes, except in Sdirk2's second stage, after which the state
+ * holds the stages' combination. */
 typedef int (*sl3_load_fn)(int l, double t, double* columns, void* user);
 
 int sl3_state_zero(sl3_operator* op, sl3_state** out);
+/* A copy of a state (device memory of its own): Viscoelastic3D::Step
+ * (include/sl3dpp/Viscoelastic3D.hpp:407-410) returns a new State and leaves
+ * its argument unchanged, which a caller keeping both gets by stepping a
+ * clone. */
+int sl3_state_clone(const sl3_operator* op, const sl3_state* state, sl3_state** out);
 int sl3_state_destroy(sl3_state* state);
 int sl3_state_time(const sl3_state* state, double* t);
 /* Step (src/Viscoelastic3D.cpp:1982-2030; Viscoelastic3D::Step(state, dt,

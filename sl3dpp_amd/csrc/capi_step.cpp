@@ -340,6 +340,15 @@ int sl3_state_zero(sl3_operator* op, sl3_state** out) {
   });
 }
 
+int sl3_state_clone(const sl3_operator* op, const sl3_state* state, sl3_state** out) {
+  return Guard([&] {
+    Check(op != nullptr && state != nullptr && out != nullptr, "null argument");
+    auto s = std::make_unique<sl3_state>();
+    s->state = op->op->CloneState(*state->state);
+    *out = s.release();
+  });
+}
+
 int sl3_state_destroy(sl3_state* state) {
   return Guard([&] {
     if (state) SL3_HIP(hipStreamSynchronize(Stream()));
@@ -362,7 +371,12 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme, sl3_load
       LoadFn out;
       if (fn) {
         out = [fn, user](int l, double t, double* columns) {
-          return fn(l, t, columns, user) != 0;
+          const int r = fn(l, t, columns, user);
+          if (r < 0) {
+            throw ArgumentError("a load channel reported a failure at degree " +
+                                std::to_string(l));
+          }
+          return r != 0;
         };
       }
       return out;

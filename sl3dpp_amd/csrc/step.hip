@@ -892,6 +892,7 @@ void Viscoelastic3D::BuildResponse(Flavour& f) {
 
 void Viscoelastic3D::SolveAll(const double* weighted, Flavour& f) {
   hipStream_t s = Stream();
+  rhsCurrent_ = false;
   const SpectralView w = coupling_.View(const_cast<double*>(weighted));
   LaunchForce(tasksDev_.data(), coupling_.Geometry(), w, dofsMem_.data(), forceWork_.data(),
               forceWorkCount_, force_.data(), s);
@@ -1014,6 +1015,13 @@ StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
   return StepOnce(state, dt, scheme, channels, start);
 }
 
+std::unique_ptr<DeviceState> Viscoelastic3D::CloneState(const DeviceState& from) const {
+  auto to = std::make_unique<DeviceState>();
+  CopyState(from, *to);
+  SL3_HIP(hipStreamSynchronize(Stream()));
+  return to;
+}
+
 void Viscoelastic3D::CopyState(const DeviceState& from, DeviceState& to) const {
   hipStream_t s = Stream();
   to.t = from.t;
@@ -1098,6 +1106,7 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
   double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
   Flavour& f = FlavourOf(scheme, dt);
   lastStep_ = {scheme, dt};
+  rhsCurrent_ = false;
   const double tEnd = state.t + dt;
 
   SL3_HIP(hipEventRecord(events_[0], s));
@@ -1392,6 +1401,7 @@ StepInfo Viscoelastic3D::CloseByConjugateGradients(Flavour& f) {
   info.solveIterations = info.iterations;
   // the solution: its force for the export's band solves, its strain
   LaunchCopy(force_.data(), cgPhi_.data(), n, s);
+  rhsCurrent_ = false;
   LaunchCopy(response_.data(), cgY_.data(), n, s);
   LaunchStrain(tasksDev_.data(), tasksDev_.data() + (L + 1), L, coupling_.Geometry(),
                dofsMem_.data(), response_.data(), f.unitsMem.data(), amp_.data(),
@@ -1407,17 +1417,21 @@ std::vector<double> Viscoelastic3D::SolutionOfTask(int task) {
   // start uses another one after it)
   Flavour& f = FlavourOf(lastStep_.scheme, lastStep_.dt);
   hipStream_t s = Stream();
-  // the band solves of the last pass's force on every dof
-  rhs_.Resize(std::max<size_t>(rhsDoubles_, 1));
-  LaunchFill(rhs_.data(), rhs_.size(), 0.0, s);
-  LaunchExpandForce(tasksDev_.data(), static_cast<int>(tasks_.size()), nmMax_, colsMax_,
-                    dm_.data(), f.scale.data(), force_.data(), rhs_.data(), s);
-  LaunchCholeskySolve(f.choleskyDev.data(), f.choleskyTasks.data(), f.choleskyTaskCount,
-                      f.kdMax, f.factors.data(), rhs_.data(), s);
-  int kvMax = 0;
-  for (const auto& b : f.lu) kvMax = std::max(kvMax, b.kl + b.ku);
-  LaunchLuSolve(f.luDev.data(), f.luTasks.data(), f.luTaskCount, kvMax, f.factors.data(),
-                f.pivots.data(), rhs_.data(), s);
+  // the band solves of the last pass's force on every dof, once per force
+  // (every task's export then reads the same solves)
+  if (!rhsCurrent_) {
+    rhs_.Resize(std::max<size_t>(rhsDoubles_, 1));
+    LaunchFill(rhs_.data(), rhs_.size(), 0.0, s);
+    LaunchExpandForce(tasksDev_.data(), static_cast<int>(tasks_.size()), nmMax_, colsMax_,
+                      dm_.data(), f.scale.data(), force_.data(), rhs_.data(), s);
+    LaunchCholeskySolve(f.choleskyDev.data(), f.choleskyTasks.data(), f.choleskyTaskCount,
+                        f.kdMax, f.factors.data(), rhs_.data(), s);
+    int kvMax = 0;
+    for (const auto& b : f.lu) kvMax = std::max(kvMax, b.kl + b.ku);
+    LaunchLuSolve(f.luDev.data(), f.luTasks.data(), f.luTaskCount, kvMax, f.factors.data(),
+                  f.pivots.data(), rhs_.data(), s);
+    rhsCurrent_ = true;
+  }
   std::vector<double> raw(static_cast<size_t>(t.ndof) * t.cols), scale(t.ndof),
       units(t.kind == 0 ? 2 * t.ndof : 0), amp(2 * (t.l + 1)), tideAmp(2 * (t.l + 1), 0.0);
   SL3_HIP(hipMemcpyAsync(raw.data(), rhs_.data() + t.rhsOff, sizeof(double) * raw.size(),

@@ -178,6 +178,9 @@ class Viscoelastic3D {
   const StepOptions& Options() const { return options_; }
   StepOptions& MutableOptions() { return options_; }
   std::unique_ptr<DeviceState> ZeroState() const;
+  // A copy of a state (the reference's Step returns a new state and leaves
+  // its argument as it was).
+  std::unique_ptr<DeviceState> CloneState(const DeviceState& from) const;
   void SetDelta(const double* delta_dev);
   // Step (src/Viscoelastic3D.cpp:1982-2030) for ExponentialTrapezoid,
   // BackwardEuler or Sdirk2 (two backward Euler stages by the recipe of
@@ -287,6 +290,7 @@ class Viscoelastic3D {
   bool hasTide_ = false;
   DeviceArray<double> force_, response_;  // per pass, on the memory dofs
   DeviceArray<double> rhs_;               // full solves for export only
+  bool rhsCurrent_ = false;               // rhs_ holds the solves of force_
   DeviceArray<double> fourierA_[3], fourierB_[3];
   DeviceArray<double> partials_, result_;
   DeviceArray<double> history_, xPrev_, fPrev_;
