@@ -188,6 +188,7 @@ typedef struct {
   int mixed_passes;
   float start_ms, passes_ms, finish_ms; /* device time of the phases */
   int solve_iterations; /* conjugate-gradient iterations (StepInfo::solveIterations) */
+  int warm_fallback;    /* sl3_step_guarded: the warm start was dropped */
 } sl3_step_info;
 
 /* Schemes (include/sl3dpp/StepWeights.hpp, StepRecipe.hpp): Sdirk2 runs
@@ -244,6 +245,19 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme,
              sl3_load_fn load, sl3_load_fn tide, void* user,
              const double* start_dev, sl3_step_info* info);
 
+/* sl3_step with a guard on the warm start (no counterpart in the
+ * reference; the learned warm start's fallback): when the first pass from
+ * start_dev changes it by a relative residual above `guard` (or a non-finite
+ * one), the step drops it and converges from its own start, the dropped pass
+ * counted in info->iterations and info->warm_fallback set. guard 0: no
+ * guard (sl3_step). */
+int sl3_step_guarded(sl3_operator* op, sl3_state* state, double dt, int scheme,
+                     sl3_load_fn load, sl3_load_fn tide, void* user,
+                     const double* start_dev, double guard, sl3_step_info* info);
+/* The state's strain coefficients (device layout, the next step's start),
+ * borrowed until the state is stepped or destroyed; NULL before a step. */
+int sl3_state_spectral(const sl3_state* state, const double** spectral_dev);
+
 /* Host copies for checks and outputs. Names:
  *   state:    "coefficients" (the strain coefficients, reference l-major
  *             layout per node, blocks spin -2, -1, 0 end to end, complex),
@@ -266,6 +280,55 @@ int sl3_spectral_to_reference(const sl3_operator* op, const double* device_dev,
                               double* reference_dev);
 int sl3_spectral_from_reference(const sl3_operator* op, const double* reference_dev,
                                 double* device_dev);
+
+/* ===================================================================== *
+ * The learned warm start (no counterpart in the reference: it feeds the  *
+ * `start` of OpenStep::Converge, src/Viscoelastic3D.cpp:1916-1921).      *
+ * ===================================================================== */
+/* The rows of degree <= cut of a device-layout spectral tensor (spin -2,
+ * -1 and 0 blocks; a row is one (l, m) over every memory node). */
+int sl3_spectral_rows(sl3_operator* op, int cut, int64_t* count);
+/* Per memory node, the rows of degree <= cut as float features:
+ * out[node * ld + col0 + 2 e + c] = ((spectral - base)[row e][node].c -
+ * mean[2 e + c]) / scale[2 e + c], c = re, im. base, mean/scale may be NULL.
+ * Device pointers. */
+int sl3_spectral_gather(sl3_operator* op, const double* spectral_dev, const double* base_dev,
+                        int cut, const float* mean_dev, const float* scale_dev, float* out_dev,
+                        int ld, int col0);
+/* out = base (zero when NULL) with the rows of degree <= cut increased by
+ * in[node * ld + 2 e + c] * scale + mean. Device pointers. */
+int sl3_spectral_scatter(sl3_operator* op, const float* in_dev, int ld, int cut,
+                         const float* mean_dev, const float* scale_dev, const double* base_dev,
+                         double* out_dev);
+
+/* A multilayer perceptron on the BF16 matrix cores (FP32 accumulation):
+ * widths[0] -> ... -> widths[layers], GELU (erf form) between layers, the
+ * last linear; weights W_k (widths[k] x widths[k-1], row-major) and biases
+ * kept in FP32 for Adam and in BF16 for the products. Initialised
+ * U(-1/sqrt(in), 1/sqrt(in)) from `seed`. Training rows per call <= max_rows;
+ * forward takes any number of rows. Device pointers, row-major FP32. */
+typedef struct sl3_mlp sl3_mlp;
+int sl3_mlp_create(int layers, const int* widths, int max_rows, uint64_t seed, sl3_mlp** out);
+int sl3_mlp_destroy(sl3_mlp* mlp);
+int sl3_mlp_forward(sl3_mlp* mlp, int64_t rows, const float* x_dev, float* y_dev);
+/* Gradients of mean((y - target)^2) into the network's gradient buffer;
+ * loss (may be NULL) receives the loss. */
+int sl3_mlp_backward(sl3_mlp* mlp, int rows, const float* x_dev, const float* target_dev,
+                     double* loss);
+/* One Adam step (bias-corrected) from the gradient buffer. */
+int sl3_mlp_adam(sl3_mlp* mlp, double lr, double beta1, double beta2, double eps);
+/* Host copies by name: "weight:k", "bias:k", "grad_weight:k", "grad_bias:k"
+ * (k = 1..layers); dst NULL returns the size through *count. Import sets a
+ * weight or bias. */
+int sl3_mlp_export(sl3_mlp* mlp, const char* name, float* dst, int64_t* count);
+int sl3_mlp_import(sl3_mlp* mlp, const char* name, const float* src, int64_t count);
+/* Column means and scales (population standard deviation, at least
+ * floor x the largest) of a rows x cols matrix. Device pointers. */
+int sl3_column_moments(int64_t rows, int cols, const float* x_dev, float floor, float* mean_dev,
+                       float* scale_dev);
+/* x = (x - mean) / scale per column, in place; inverse != 0: x scale + mean. */
+int sl3_column_standardise(int64_t rows, int cols, float* x_dev, const float* mean_dev,
+                           const float* scale_dev, int inverse);
 
 /* ===================================================================== *
  * Batched generalised spherical-harmonic transforms over the operator's *

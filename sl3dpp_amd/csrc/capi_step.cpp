@@ -2,13 +2,17 @@
 // (include/sl3dpp_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
 #include "coupling.hpp"
 #include "kernels.hpp"
+#include "learned.hpp"
 #include "step.hpp"
 
 struct sl3_earth {
@@ -18,6 +22,10 @@ struct sl3_earth {
 struct sl3_operator {
   const sl3_earth* earth = nullptr;
   std::unique_ptr<sl3amd::Viscoelastic3D> op;
+  // per degree cut: the device-layout offsets of the rows of degree <= cut
+  // (the learned warm start's gather and scatter)
+  std::mutex rowsMutex;
+  std::map<int, std::pair<int, sl3amd::DeviceArray<long long>>> cutRows;
 };
 
 struct sl3_state {
@@ -365,6 +373,12 @@ int sl3_state_time(const sl3_state* state, double* t) {
 
 int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme, sl3_load_fn load,
              sl3_load_fn tide, void* user, const double* start_dev, sl3_step_info* info) {
+  return sl3_step_guarded(op, state, dt, scheme, load, tide, user, start_dev, 0.0, info);
+}
+
+int sl3_step_guarded(sl3_operator* op, sl3_state* state, double dt, int scheme,
+                     sl3_load_fn load, sl3_load_fn tide, void* user, const double* start_dev,
+                     double guard, sl3_step_info* info) {
   return Guard([&] {
     Check(op != nullptr && state != nullptr, "null argument");
     auto wrap = [user](sl3_load_fn fn) {
@@ -382,7 +396,7 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme, sl3_load
       return out;
     };
     const Channels channels{wrap(load), wrap(tide)};
-    const auto r = op->op->Step(*state->state, dt, scheme, channels, start_dev);
+    const auto r = op->op->Step(*state->state, dt, scheme, channels, start_dev, guard);
     if (info) {
       info->iterations = r.iterations;
       info->residual = r.residual;
@@ -392,6 +406,7 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme, sl3_load
       info->passes_ms = op->op->lastTiming.passes;
       info->finish_ms = op->op->lastTiming.finish;
       info->solve_iterations = r.solveIterations;
+      info->warm_fallback = r.warmFallback ? 1 : 0;
     }
   });
 }
@@ -465,6 +480,85 @@ int sl3_spectral_to_reference(const sl3_operator* op, const double* device_dev,
     Check(op && device_dev && reference_dev, "null argument");
     ToReference(op->op->GetCoupling(), device_dev, reference_dev, Stream());
     SL3_HIP(hipStreamSynchronize(Stream()));
+  });
+}
+
+int sl3_state_spectral(const sl3_state* state, const double** spectral_dev) {
+  return Guard([&] {
+    Check(state != nullptr && spectral_dev != nullptr, "null argument");
+    const auto& st = *state->state;
+    *spectral_dev = st.hasCoefficients ? st.coefficients.data() : nullptr;
+  });
+}
+
+namespace {
+
+// The rows of degree <= cut of the device layout, blocks spin -2, -1, 0 in
+// memory order (each row: the nodes' complex values), as double offsets.
+const std::pair<int, DeviceArray<long long>>& CutRows(sl3_operator* op, int cut) {
+  const auto& c = op->op->GetCoupling();
+  Check(cut >= 0 && cut <= c.LMax(), "degree cut must be in 0..lmax");
+  std::lock_guard<std::mutex> lock(op->rowsMutex);
+  auto it = op->cutRows.find(cut);
+  if (it != op->cutRows.end()) return it->second;
+  std::vector<long long> rows;
+  const long long nodes = c.NodeCount();
+  for (int b = 0; b < 3; ++b) {
+    const auto& lay = c.Grid().Layout(b);
+    for (int m = lay.mlo; m <= lay.lmax; ++m)
+      for (int l = lay.LMin(m); l <= std::min(cut, lay.lmax); ++l)
+        rows.push_back(static_cast<long long>(c.SpectralOffset(b)) +
+                       2ll * nodes * lay.Row(l, m));
+  }
+  auto& entry = op->cutRows[cut];
+  entry.first = static_cast<int>(rows.size());
+  entry.second.Upload(rows.data(), rows.size(), Stream());
+  SL3_HIP(hipStreamSynchronize(Stream()));
+  return entry;
+}
+
+}  // namespace
+
+int sl3_spectral_rows(sl3_operator* op, int cut, int64_t* count) {
+  return Guard([&] {
+    Check(op != nullptr && count != nullptr, "null argument");
+    *count = CutRows(op, cut).first;
+  });
+}
+
+int sl3_spectral_gather(sl3_operator* op, const double* spectral_dev, const double* base_dev,
+                        int cut, const float* mean_dev, const float* scale_dev, float* out_dev,
+                        int ld, int col0) {
+  return Guard([&] {
+    Check(op && spectral_dev && out_dev, "null argument");
+    Check((mean_dev == nullptr) == (scale_dev == nullptr), "mean and scale go together");
+    const auto& rows = CutRows(op, cut);
+    Check(col0 >= 0 && ld >= col0 + 2 * rows.first, "ld must hold col0 + 2 rows");
+    LaunchSpectralGather(spectral_dev, base_dev, rows.second.data(), rows.first,
+                         op->op->GetCoupling().NodeCount(), mean_dev, scale_dev, out_dev, ld,
+                         col0, Stream());
+    SL3_HIP(hipStreamSynchronize(Stream()));
+  });
+}
+
+int sl3_spectral_scatter(sl3_operator* op, const float* in_dev, int ld, int cut,
+                         const float* mean_dev, const float* scale_dev, const double* base_dev,
+                         double* out_dev) {
+  return Guard([&] {
+    Check(op && in_dev && out_dev, "null argument");
+    Check((mean_dev == nullptr) == (scale_dev == nullptr), "mean and scale go together");
+    const auto& rows = CutRows(op, cut);
+    Check(ld >= 2 * rows.first, "ld must hold 2 rows");
+    const auto& c = op->op->GetCoupling();
+    hipStream_t s = Stream();
+    if (base_dev != nullptr) {
+      if (base_dev != out_dev) LaunchCopy(out_dev, base_dev, c.SpectralDoubles(), s);
+    } else {
+      LaunchFill(out_dev, c.SpectralDoubles(), 0.0, s);
+    }
+    LaunchSpectralScatterAdd(in_dev, ld, rows.second.data(), rows.first, c.NodeCount(),
+                             mean_dev, scale_dev, out_dev, s);
+    SL3_HIP(hipStreamSynchronize(s));
   });
 }
 

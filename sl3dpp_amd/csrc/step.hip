@@ -1004,15 +1004,17 @@ void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const Channel
 }
 
 StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
-                              const Channels& channels, const double* start) {
+                              const Channels& channels, const double* start,
+                              double warmGuard) {
   if (!(dt >= 0.0)) throw ArgumentError("dt must be non-negative");
+  if (!(warmGuard >= 0.0)) throw ArgumentError("the warm-start guard must be non-negative");
   if (scheme != kExponentialTrapezoid && scheme != kBackwardEuler && scheme != kSdirk2) {
     throw ArgumentError(
         "the 3D step is implicit: ExponentialTrapezoid, BackwardEuler or Sdirk2");
   }
   if (dt == 0.0) scheme = kBackwardEuler;  // every scheme's zero-length step
   if (scheme == kSdirk2) return StepSdirk2(state, dt, channels);
-  return StepOnce(state, dt, scheme, channels, start);
+  return StepOnce(state, dt, scheme, channels, start, warmGuard);
 }
 
 std::unique_ptr<DeviceState> Viscoelastic3D::CloneState(const DeviceState& from) const {
@@ -1058,7 +1060,7 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const Channel
   if (!stage_) stage_ = std::make_unique<DeviceState>();
   DeviceState& y1 = *stage_;
   CopyState(state, y1);
-  const StepInfo first = StepOnce(y1, length, kBackwardEuler, channels, nullptr);
+  const StepInfo first = StepOnce(y1, length, kBackwardEuler, channels, nullptr, 0.0);
   const auto firstHistory = residualHistory;
   const Timing firstTiming = lastTiming;
   for (int b = 0; b < 3; ++b) {
@@ -1073,7 +1075,7 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const Channel
   staged.hasStrain = y1.hasStrain;
   staged.coefficients = std::move(y1.coefficients);
   staged.hasCoefficients = y1.hasCoefficients;
-  const StepInfo second = StepOnce(staged, length, kBackwardEuler, channels, nullptr);
+  const StepInfo second = StepOnce(staged, length, kBackwardEuler, channels, nullptr, 0.0);
   // the stepped stage becomes the state; Y1's buffers return to the cache
   for (int b = 0; b < 3; ++b) {
     state.m[b] = std::move(staged.m[b]);
@@ -1099,7 +1101,8 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const Channel
 }
 
 StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
-                                  const Channels& channels, const double* start) {
+                                  const Channels& channels, const double* start,
+                                  double warmGuard) {
   hipStream_t s = Stream();
   const size_t S = coupling_.SpectralDoubles();
   double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
@@ -1155,10 +1158,17 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
   residualHistory.clear();
   if (cg) info = CloseByConjugateGradients(f);  // takes no warm start, as the reference's
   bool warm = false;
+  const bool guarded = start != nullptr && !cg && warmGuard > 0.0;
+  if (guarded) {
+    // the step's own start, kept in case the warm one is dropped
+    coldStart_.Resize(S);
+    LaunchCopy(coldStart_.data(), s_.data(), S, s);
+  }
   if (start != nullptr && !cg) {
     LaunchCopy(s_.data(), start, S, s);
     warm = true;
   }
+  int first = 0;  // the first pass of the sequence being converged
   const bool adaptive = options_.andersonRate > 0.0;
   const int memory = options_.andersonMemory;
   bool mixing = memory > 0 && !adaptive;
@@ -1214,11 +1224,19 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
       // Every pass's image is zero below each degree's truncation; the start
       // (a warm start, or one analysed from a stored strain grid) may hold
       // anything there, so the first pass compares every entry.
-      residual = RelativeChange(sNext_.data(), s_.data(), it == 0);
+      residual = RelativeChange(sNext_.data(), s_.data(), it == first);
     }
     info.iterations = it + 1;
     info.residual = residual;
     residualHistory.push_back(residual);
+    if (guarded && it == 0 && !(residual <= warmGuard)) {
+      // the warm start's first change exceeds the guard: converge from the
+      // step's own start instead (the pass counts)
+      LaunchCopy(s_.data(), coldStart_.data(), S, s);
+      info.warmFallback = true;
+      first = 1;
+      continue;
+    }
     if (!std::isfinite(residual)) break;
     if (residual <= options_.rtol) {
       info.converged = true;
@@ -1231,7 +1249,7 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
                          s_.data(), options_.damping, S);
       SL3_HIP(hipGetLastError());
     }
-    if (adaptive && !mixing && memory > 0 && it > 0 &&
+    if (adaptive && !mixing && memory > 0 && it > first &&
         residual > options_.andersonRate * previous) {
       mixing = true;
     }

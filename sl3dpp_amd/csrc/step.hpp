@@ -51,6 +51,7 @@ struct StepInfo {
   bool converged = false;
   int mixedPasses = 0;
   int solveIterations = 0;  // conjugate-gradient iterations
+  bool warmFallback = false;  // a guarded warm start was dropped for the step's own
 };
 
 // LoadCallable3D: the m >= 0 columns of degree l at time t, (re, im) pairs,
@@ -186,9 +187,11 @@ class Viscoelastic3D {
   // BackwardEuler or Sdirk2 (two backward Euler stages by the recipe of
   // src/StepRecipe.cpp:25-32), in place on the state. `start` (device
   // spectral, or null) is the warm start of Converge (:1916-1921); a
-  // multi-stage step takes none.
+  // multi-stage step takes none. warmGuard > 0: when the first pass from
+  // `start` changes it by a relative residual above warmGuard (or a
+  // non-finite one), the step drops it and converges from its own start.
   StepInfo Step(DeviceState& state, double dt, int scheme, const Channels& channels,
-                const double* start = nullptr);
+                const double* start = nullptr, double warmGuard = 0.0);
   // The last pass's solutions on every dof, unscaled with channels, per
   // task (band solves of the last pass's force): host export for the
   // surface outputs and tests.
@@ -223,7 +226,7 @@ class Viscoelastic3D {
 
  private:
   StepInfo StepOnce(DeviceState& state, double dt, int scheme, const Channels& channels,
-                    const double* start);
+                    const double* start, double warmGuard);
   StepInfo StepSdirk2(DeviceState& state, double dt, const Channels& channels);
   // OpenStep::Forces + Solve + StrainOf (src/Viscoelastic3D.cpp:2008-2015):
   // the step's system by conjugate gradients on the memory dofs, its strain
@@ -281,6 +284,7 @@ class Viscoelastic3D {
   std::list<std::unique_ptr<Flavour>> flavours_;
   // the open step's buffers
   DeviceArray<double> weight_, beta_, fixed_, analysed_, amp_, s_, sNext_;
+  DeviceArray<double> coldStart_;  // the step's own start under a guarded warm start
   size_t ampDoubles_ = 0;  // one channel's amplitudes, (L + 1)(L + 2)
   std::vector<double> ampHost_;  // the last amplitudes gathered (load, then tide)
   struct {

@@ -83,6 +83,7 @@ class StepInfo(ctypes.Structure):
         ("passes_ms", ctypes.c_float),
         ("finish_ms", ctypes.c_float),
         ("solve_iterations", ctypes.c_int),
+        ("warm_fallback", ctypes.c_int),
     ]
 
 
@@ -315,13 +316,17 @@ class Viscoelastic3D:
     def zero_state(self):
         return State(self)
 
-    def step(self, state, dt, scheme=EXPONENTIAL_TRAPEZOID, load=None, start=None, tide=None):
+    def step(self, state, dt, scheme=EXPONENTIAL_TRAPEZOID, load=None, start=None, tide=None,
+             guard=0.0):
         """Step (src/Viscoelastic3D.cpp:1982-2030) in place on the state;
-        load and tide are LOAD_FN channels or None."""
+        load and tide are LOAD_FN channels or None; start a device-layout
+        warm start (a torch tensor) or None; guard > 0 drops a warm start
+        whose first pass changes it by more than guard."""
         info = StepInfo()
         ptr = ctypes.c_void_p(start.data_ptr()) if start is not None else None
-        check(lib.sl3_step(self.handle, state.handle, ctypes.c_double(dt), scheme,
-                           load, tide, None, ptr, ctypes.byref(info)))
+        check(lib.sl3_step_guarded(self.handle, state.handle, ctypes.c_double(dt), scheme,
+                                   load, tide, None, ptr, ctypes.c_double(guard),
+                                   ctypes.byref(info)))
         return info
 
     def export(self, name):
