@@ -57,6 +57,14 @@ def parse():
                     help="independent realisations (delta seeds) stepped at once, each on its "
                          "own host thread and so its own stream; value is their aggregate "
                          "steps/s")
+    ap.add_argument("--learned-train", type=int, default=64,
+                    help="training steps (delta seeds) of the learned warm start's leg, "
+                         "0 to skip it (the full learned configuration: 4000)")
+    ap.add_argument("--learned-eval", type=int, default=6,
+                    help="held-out delta seeds the learned leg evaluates (full: 200)")
+    ap.add_argument("--learned-epochs", type=int, default=50)
+    ap.add_argument("--extrapolate-order", type=int, default=2,
+                    help="order of the extrapolated-start trajectory leg (0 skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -227,6 +235,7 @@ def main():
     elapsed = time.perf_counter() - t0
     sl3dpp_amd.profile_enable(False)
     kernels = sl3dpp_amd.profile_read()
+    plain_final = state.export("coefficients")
     steps_per_s = args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     # dominant kernel and its roofline
@@ -253,6 +262,12 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and not cg:
         cpu = cpu_baseline(op, earth, dt, load, float(np.mean(passes)), args.rtol)
+    extrapolated = None
+    if args.extrapolate_order > 0 and not cg:
+        extrapolated = extrapolated_leg(op, dt, load, args, steps_per_s, plain_final)
+    learned = None
+    if args.learned_train > 0 and not cg:
+        learned = learned_leg(op, dt, load, args)
     out = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -274,16 +289,107 @@ def main():
                          + (f"conjugate gradients, solve_rtol {args.solve_rtol:g}" if cg else
                             f"Anderson depth 6, rtol {args.rtol:g}")),
             "variant": ("preconditioned conjugate gradients (one realisation)" if cg else
-                        "plain GPU fixed point (no learned warm start, one realisation)"),
+                        "plain GPU fixed point (one realisation); the learned warm start's "
+                        "first steps from rest are under 'learned'"),
             "lmax": lmax, "grid": [op.ntheta, op.nphi], "memory_nodes": op.nodes,
             "passes_per_step": passes, "mixed_passes": mixed, "converged": converged,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extrapolated_start": extrapolated,
+        "learned": learned,
         "kernels": breakdown,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def extrapolated_leg(op, dt, load, args, plain_steps_per_s, plain_final):
+    """The same trajectory (zero state, the same warm-up and timed steps)
+    with each step warm-started from the Lagrange extrapolation in time of
+    the last converged steps' strain coefficients (order 2: the last three),
+    converged to the same tolerance; its final state against the plain
+    trajectory's."""
+    import numpy as np
+    import torch
+
+    import sl3dpp_amd
+    import sl3dpp_amd.step as st
+
+    state = op.zero_state()
+    ws = st.ExtrapolatedStart(op, order=args.extrapolate_order)
+    for k in range(args.warmup):
+        ws.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    passes = []
+    torch.cuda.synchronize()
+    sl3dpp_amd.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        info = ws.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+        passes.append(info.iterations)
+        assert info.converged, "an extrapolated-start step did not converge"
+    sl3dpp_amd.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    final = state.export("coefficients")
+    diff = float(np.max(np.abs(final - plain_final)) / np.max(np.abs(plain_final)))
+    out = {
+        "order": args.extrapolate_order,
+        "steps_per_s": args.steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "passes_per_step": passes,
+        "speedup_vs_plain": (args.steps / elapsed) / plain_steps_per_s,
+        "final_state_vs_plain": diff,
+        "what": ("the timed trajectory again from the zero state, each step started from the "
+                 "extrapolation in time of the last converged strains (Converge's start), "
+                 "converged to the same tolerance; final_state_vs_plain = max |difference| / "
+                 "max |plain| of the final strain coefficients"),
+    }
+    log(f"extrapolated start (order {args.extrapolate_order}): {out['steps_per_s']:.2f} steps/s, "
+        f"passes {passes}, x{out['speedup_vs_plain']:.2f} of plain, final state vs plain {diff:.1e}")
+    return out
+
+
+def learned_leg(op, dt, load, args):
+    """The learned warm start on first steps from rest (the learned
+    configuration's protocol): training triples made by the solver (one
+    converged first step per delta seed 5000.., the delta distribution of
+    the timed workload), the per-node network (delta coefficients to l 32 and
+    their column mean, the radial response to l 8, radius -> the correction
+    to l 32; hidden width 512, four layers, BF16 products, FP32
+    accumulation) trained by Adam at lr 1e-3 on the library's kernels, then
+    held-out seeds 9000.. stepped cold (the plain fixed point from the step's
+    own start), from the radial reference alone, and from the learned start
+    (guarded), each to the same tolerance. Steps/s of the cold and learned
+    steps over the same seeds; the learned time includes inference."""
+    import numpy as np
+
+    import sl3dpp_amd.learned as ln
+    import sl3dpp_amd.step as st
+
+    t0 = time.perf_counter()
+    ws = ln.WarmStart(op, dt, st.EXPONENTIAL_TRAPEZOID, load, lin=32, lrad=8, lcut=32,
+                      width=512, layers=4, max_rows=1024, seed=7, context=True)
+    sample = ln.delta_sampler(op)
+    X, T, infos = ws.make_triples(sample, list(range(5000, 5000 + args.learned_train)), log=log)
+    triples_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ws.fit(X, T, infos, epochs=args.learned_epochs, lr=1e-3, log=log)
+    fit_s = time.perf_counter() - t0
+    del X, T
+    ln.evaluate(ws, sample, [8999], radial=False)  # untimed: first-call set-up
+    records = ln.evaluate(ws, sample, list(range(9000, 9000 + args.learned_eval)), log=log)
+    s = ln.summary(records)
+    s.update({
+        "train_steps": args.learned_train, "epochs": args.learned_epochs,
+        "triples_s": round(triples_s, 2), "fit_s": round(fit_s, 2),
+        "final_train_loss": ws.history[-1], "guard": ws.guard,
+        "network": {"widths": ws.mlp.widths, "rows_per_step": ws.nodes},
+        "protocol": ("first steps from rest, held-out delta seeds; cold = the plain fixed point "
+                     "from the step's own start; radial = from the delta-0 response alone"),
+    })
+    log("learned: " + json.dumps({k: v for k, v in s.items() if k != "network"}))
+    return s
 
 
 def batched(args):

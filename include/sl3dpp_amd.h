@@ -254,6 +254,13 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme,
 int sl3_step_guarded(sl3_operator* op, sl3_state* state, double dt, int scheme,
                      sl3_load_fn load, sl3_load_fn tide, void* user,
                      const double* start_dev, double guard, sl3_step_info* info);
+/* dst = sum_j coefficients[j] sources[j] over a spectral tensor (1 to 4
+ * terms; dst may be one of the sources). With the states' coefficients as
+ * sources this forms the extrapolated warm start of a trajectory's next
+ * step (the `start` of OpenStep::Converge, src/Viscoelastic3D.cpp:
+ * 1916-1921). Device pointers. */
+int sl3_spectral_combine(const sl3_operator* op, int count, const double* coefficients,
+                         const double* const* sources_dev, double* dst_dev);
 /* The state's strain coefficients (device layout, the next step's start),
  * borrowed until the state is stepped or destroyed; NULL before a step. */
 int sl3_state_spectral(const sl3_state* state, const double** spectral_dev);

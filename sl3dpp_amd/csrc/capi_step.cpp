@@ -562,6 +562,23 @@ int sl3_spectral_scatter(sl3_operator* op, const float* in_dev, int ld, int cut,
   });
 }
 
+int sl3_spectral_combine(const sl3_operator* op, int count, const double* coefficients,
+                         const double* const* sources_dev, double* dst_dev) {
+  return Guard([&] {
+    Check(op && coefficients && sources_dev && dst_dev, "null argument");
+    Check(count >= 1 && count <= 4, "combine: 1 to 4 terms");
+    SpectralTerms t;
+    t.count = count;
+    for (int j = 0; j < count; ++j) {
+      Check(sources_dev[j] != nullptr, "null source");
+      t.c[j] = coefficients[j];
+      t.src[j] = sources_dev[j];
+    }
+    LaunchCombineTerms(t, dst_dev, op->op->GetCoupling().SpectralDoubles(), Stream());
+    SL3_HIP(hipStreamSynchronize(Stream()));
+  });
+}
+
 int sl3_spectral_from_reference(const sl3_operator* op, const double* reference_dev,
                                 double* device_dev) {
   return Guard([&] {

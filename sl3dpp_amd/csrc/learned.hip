@@ -296,6 +296,17 @@ __global__ __launch_bounds__(256) void StandardiseKernel(float* __restrict__ x, 
   }
 }
 
+// dst = sum_j c_j src_j (up to four terms; dst may be one of the sources:
+// every element reads its sources before its store).
+__global__ __launch_bounds__(256) void CombineTermsKernel(SpectralTerms t, double* dst,
+                                                          size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) {
+    double v = 0.0;
+    for (int j = 0; j < t.count; ++j) v += t.c[j] * t.src[j][i];
+    dst[i] = v;
+  }
+}
+
 int GridOf(size_t n) {
   return static_cast<int>(std::min<size_t>((n + 255) / 256, 4096));
 }
@@ -563,6 +574,14 @@ void Standardise(float* x, long long rows, int cols, const float* mean, const fl
   if (n == 0) return;
   hipLaunchKernelGGL(StandardiseKernel, dim3(GridOf(n)), dim3(256), 0, s, x, rows, cols, mean,
                      scale, inverse);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchCombineTerms(const SpectralTerms& t, double* dst, size_t n, hipStream_t s) {
+  if (t.count < 1 || t.count > 4) throw ArgumentError("combine: 1 to 4 terms");
+  if (n == 0) return;
+  ProfileScope prof("combine", s);
+  hipLaunchKernelGGL(CombineTermsKernel, dim3(GridOf(n)), dim3(256), 0, s, t, dst, n);
   SL3_HIP(hipGetLastError());
 }
 

@@ -66,6 +66,14 @@ void ColumnMoments(long long rows, int cols, const float* x, float* mean, float*
 void Standardise(float* x, long long rows, int cols, const float* mean, const float* scale,
                  bool inverse, hipStream_t s);
 
+// A linear combination of up to four device vectors.
+struct SpectralTerms {
+  int count = 0;
+  double c[4] = {};
+  const double* src[4] = {};
+};
+void LaunchCombineTerms(const SpectralTerms& t, double* dst, size_t n, hipStream_t s);
+
 // out[node * ld + col0 + 2 e + c] = ((spec - base)[row e][node].c - mean) /
 // scale, c = re, im, for the rows e of `rowOffsets` (the double offset of
 // each selected row in the device layout; a row holds `nodes` complex

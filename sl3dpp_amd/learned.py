@@ -8,7 +8,9 @@ plain step's to that tolerance.
 A multilayer perceptron (hidden layers of `width`, GELU) acts on every memory
 node alike (shared weights, one row per node):
   * input row: the log-viscosity field's spin-0 coefficients of degree
-    <= lin at the node (re, im), the radial-reference response's rows of
+    <= lin at the node (re, im) and (context) their mean over the column of
+    memory nodes, since the degree systems couple every node; the
+    radial-reference response's rows of
     degree <= lrad at the node (the step's converged strain with delta = 0,
     spins -2, -1, 0), and the node's radius;
   * output row: the lateral correction at the node, the converged strain
@@ -176,7 +178,7 @@ class WarmStart:
     fixed load, scheme and dt (the radial reference is theirs)."""
 
     def __init__(self, op, dt, scheme, load, lin=32, lrad=8, lcut=32, width=512, layers=4,
-                 max_rows=1024, seed=7):
+                 max_rows=1024, seed=7, context=True):
         import torch
 
         self.op, self.dt, self.scheme, self.load = op, dt, scheme, load
@@ -185,7 +187,8 @@ class WarmStart:
         self.n_delta = (self.lin + 1) * (self.lin + 2) // 2
         self.rows_rad = spectral_rows(op, self.lrad)
         self.rows_out = spectral_rows(op, self.lcut)
-        self.f_in = 2 * self.n_delta + 2 * self.rows_rad + 1
+        self.context = bool(context)
+        self.f_in = (4 if self.context else 2) * self.n_delta + 2 * self.rows_rad + 1
         self.f_out = 2 * self.rows_out
         self.mlp = Mlp([self.f_in] + [width] * (layers - 1) + [self.f_out], max_rows, seed)
         self.guard = 0.0
@@ -215,7 +218,11 @@ class WarmStart:
         d = np.empty((self.nodes, 2 * self.n_delta), dtype=np.float32)
         d[:, 0::2] = c.real
         d[:, 1::2] = c.imag
-        rows = np.concatenate([d, self.static], axis=1)
+        parts = [d]
+        if self.context:
+            # the column's mean field: the degree systems couple every node
+            parts.append(np.broadcast_to(d.mean(axis=0), d.shape))
+        rows = np.concatenate(parts + [self.static], axis=1)
         return torch.from_numpy(np.ascontiguousarray(rows)).cuda()
 
     def set_delta(self, coefficients):
@@ -368,9 +375,9 @@ def evaluate(ws, coefficient_fn, seeds, log=None, radial=True):
                                         max(np.max(np.abs(cold)), 1e-300))
         out.append(rec)
         if log:
-            log(f"seed {seed}: cold {rec['cold_passes']} passes, "
+            log(f"seed {seed}: cold {rec['cold_passes']} passes {1e3 * rec['cold_s']:.1f} ms, "
                 + (f"radial {rec['radial_passes']}, " if radial else "")
-                + f"learned {rec['learned_passes']} (first residual "
+                + f"learned {rec['learned_passes']} {1e3 * rec['learned_s']:.1f} ms (first residual "
                 f"{rec['learned_first_residual']:.2e}, fallback {rec['fallback']}), network error "
                 f"{rec['error']:.3f}")
     return out

@@ -363,3 +363,87 @@ class Viscoelastic3D:
                             device="cuda")
         self.synthesise(0, coef, field)
         return field
+
+
+lib.sl3_spectral_combine.argtypes = [ctypes.c_void_p, ctypes.c_int, c_double_p,
+                                     ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]
+lib.sl3_state_spectral.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+
+
+def lagrange_extrapolation_weights(times, target):
+    """Weights c_j with p(target) = sum_j c_j p(times[j]) for every
+    polynomial p of degree < len(times) (Lagrange interpolation through the
+    points, evaluated at the target)."""
+    out = []
+    for j, tj in enumerate(times):
+        w = 1.0
+        for k, tk in enumerate(times):
+            if k != j:
+                w *= (target - tk) / (tj - tk)
+        out.append(w)
+    return out
+
+
+class ExtrapolatedStart:
+    """Warm starts along a trajectory. The next step's strain coefficients
+    are extrapolated in time from the last converged steps' (the Lagrange
+    polynomial through up to order + 1 of them, at their times) and handed to
+    the step as the `start` of OpenStep::Converge
+    (src/Viscoelastic3D.cpp:1916-1921). The fixed point then iterates to the
+    same tolerance, so a step's result is the plain step's to that
+    tolerance. With fewer than two recorded steps, and for Sdirk2 (which
+    takes no start), the step uses its own start. guard > 0 drops an
+    extrapolated start whose first pass changes it by more than guard."""
+
+    def __init__(self, op, order=2, guard=0.0):
+        import torch
+
+        if order < 1 or order > 3:
+            raise ValueError("order must be 1, 2 or 3")
+        self.op, self.order, self.guard = op, order, guard
+        self.buffers = [torch.empty(op.spectral_doubles, dtype=torch.float64, device="cuda")
+                        for _ in range(order + 1)]
+        self.start_buffer = torch.empty(op.spectral_doubles, dtype=torch.float64,
+                                        device="cuda")
+        self.recorded = []  # (time, buffer index), oldest first
+        torch.cuda.synchronize()
+
+    def _combine(self, weights, sources, dst):
+        n = len(weights)
+        c = (ctypes.c_double * n)(*weights)
+        src = (ctypes.c_void_p * n)(*sources)
+        check(lib.sl3_spectral_combine(self.op.handle, n, c, src, ctypes.c_void_p(dst.data_ptr())))
+
+    def start(self, target_time):
+        """The extrapolated coefficients at target_time, or None."""
+        if len(self.recorded) < 2:
+            return None
+        times = [t for t, _ in self.recorded]
+        weights = lagrange_extrapolation_weights(times, target_time)
+        sources = [self.buffers[i].data_ptr() for _, i in self.recorded]
+        self._combine(weights, sources, self.start_buffer)
+        return self.start_buffer
+
+    def record(self, state):
+        """Keeps a converged state's coefficients (device to device)."""
+        p = ctypes.c_void_p()
+        check(lib.sl3_state_spectral(state.handle, ctypes.byref(p)))
+        if not p.value:
+            raise ValueError("the state has no strain coefficients yet")
+        used = {i for _, i in self.recorded}
+        if len(self.recorded) == len(self.buffers):
+            free = self.recorded.pop(0)[1]
+        else:
+            free = min(set(range(len(self.buffers))) - used)
+        self._combine([1.0], [p.value], self.buffers[free])
+        self.recorded.append((state.t, free))
+
+    def reset(self):
+        self.recorded = []
+
+    def step(self, state, dt, scheme=EXPONENTIAL_TRAPEZOID, load=None, tide=None):
+        start = None if scheme == SDIRK2 else self.start(state.t + dt)
+        info = self.op.step(state, dt, scheme, load, start=start, tide=tide,
+                            guard=self.guard if start is not None else 0.0)
+        self.record(state)
+        return info
