@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--learned-eval", type=int, default=6,
                     help="held-out delta seeds the learned leg evaluates (full: 200)")
     ap.add_argument("--learned-epochs", type=int, default=50)
-    ap.add_argument("--extrapolate-order", type=int, default=2,
+    ap.add_argument("--extrapolate-order", type=int, default=3,
                     help="order of the extrapolated-start trajectory leg (0 skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
@@ -307,7 +307,7 @@ def main():
 def extrapolated_leg(op, dt, load, args, plain_steps_per_s, plain_final):
     """The same trajectory (zero state, the same warm-up and timed steps)
     with each step warm-started from the Lagrange extrapolation in time of
-    the last converged steps' strain coefficients (order 2: the last three),
+    the last converged steps' strain coefficients (order 3: the last four),
     converged to the same tolerance; its final state against the plain
     trajectory's."""
     import numpy as np

@@ -395,7 +395,7 @@ class ExtrapolatedStart:
     takes no start), the step uses its own start. guard > 0 drops an
     extrapolated start whose first pass changes it by more than guard."""
 
-    def __init__(self, op, order=2, guard=0.0):
+    def __init__(self, op, order=3, guard=0.0):
         import torch
 
         if order < 1 or order > 3:
