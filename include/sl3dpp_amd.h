@@ -261,6 +261,22 @@ int sl3_step_guarded(sl3_operator* op, sl3_state* state, double dt, int scheme,
  * 1916-1921). Device pointers. */
 int sl3_spectral_combine(const sl3_operator* op, int count, const double* coefficients,
                          const double* const* sources_dev, double* dst_dev);
+/* Warm starts along a trajectory (a caller of Converge's `start`,
+ * src/Viscoelastic3D.cpp:1916-1921; no counterpart in the reference's
+ * library): the extrapolator keeps the last order + 1 converged strains of
+ * the states it stepped, and sl3_extrapolated_step starts each step from
+ * their Lagrange extrapolation in time to the step's end (order 1..3; the
+ * step's own start while fewer than two are kept, for Sdirk2, and when the
+ * state is not the last one it stepped, which starts a new history). The
+ * fixed point then converges to the same tolerance. guard as in
+ * sl3_step_guarded. */
+typedef struct sl3_extrapolator sl3_extrapolator;
+int sl3_extrapolator_create(sl3_operator* op, int order, sl3_extrapolator** out);
+int sl3_extrapolator_destroy(sl3_extrapolator* ex);
+int sl3_extrapolator_reset(sl3_extrapolator* ex);
+int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int scheme,
+                          sl3_load_fn load, sl3_load_fn tide, void* user, double guard,
+                          sl3_step_info* info);
 /* The state's strain coefficients (device layout, the next step's start),
  * borrowed until the state is stepped or destroyed; NULL before a step. */
 int sl3_state_spectral(const sl3_state* state, const double** spectral_dev);

@@ -94,6 +94,7 @@ class Viscoelastic3D {
     bool converged = true;
     int mixedPasses = 0;
     int solveIterations = 0;
+    bool warmFallback = false;  // a guarded warm start was dropped
   };
 
   /* LoadCallable3D: degree l's complex m >= 0 columns at time t. */
@@ -186,17 +187,51 @@ class Viscoelastic3D {
   /* The step in place on the state (the boundary's form, no copy). */
   StepInfo StepInPlace(State& state, double dt, Scheme scheme = Scheme::Sdirk2,
                        const LoadCallable3D& load = {}, const LoadCallable3D& tide = {}) const {
-    Channel channels[2] = {{&load, nullptr}, {&tide, nullptr}};
-    sl3_step_info info{};
-    const int status =
-        sl3_step(handle_, state.Handle(), dt, static_cast<int>(scheme), load ? &Trampoline : nullptr,
-                 tide ? &Trampoline2 : nullptr, channels, nullptr, &info);
-    for (const auto& c : channels) {
-      if (c.error) std::rethrow_exception(c.error);
+    return StepInPlace(state, dt, scheme, load, tide, nullptr, 0.0);
+  }
+
+  /* The step from a warm start: Converge's `start` (:1916-1921) as a
+   * device-layout spectral tensor; guard > 0 drops a start whose first pass
+   * changes it by more than guard (StepInfo::warmFallback). */
+  StepInfo StepInPlace(State& state, double dt, Scheme scheme, const LoadCallable3D& load,
+                       const LoadCallable3D& tide, const double* start_dev, double guard) const {
+    return WithChannels(load, tide, [&](sl3_load_fn l, sl3_load_fn t, void* user,
+                                        sl3_step_info* info) {
+      return sl3_step_guarded(handle_, state.Handle(), dt, static_cast<int>(scheme), l, t, user,
+                              start_dev, guard, info);
+    });
+  }
+
+  /* Warm starts along a trajectory: the Lagrange extrapolation in time of
+   * the last order + 1 converged strains of the states stepped through it
+   * (sl3_extrapolator). */
+  class Extrapolator {
+   public:
+    Extrapolator(const Viscoelastic3D& op, int order = 3) {
+      Check(sl3_extrapolator_create(op.Handle(), order, &handle_));
     }
-    Check(status);
-    return StepInfo{info.iterations, info.residual, info.converged != 0, info.mixed_passes,
-                    info.solve_iterations};
+    ~Extrapolator() {
+      if (handle_) sl3_extrapolator_destroy(handle_);
+    }
+    Extrapolator(const Extrapolator&) = delete;
+    Extrapolator& operator=(const Extrapolator&) = delete;
+    void Reset() { Check(sl3_extrapolator_reset(handle_)); }
+    sl3_extrapolator* Handle() const { return handle_; }
+
+   private:
+    sl3_extrapolator* handle_ = nullptr;
+  };
+
+  /* A trajectory's step in place, warm-started by the extrapolator (the
+   * step's own start until it holds two steps of this state's history). */
+  StepInfo StepInPlace(State& state, Extrapolator& ex, double dt, Scheme scheme,
+                       const LoadCallable3D& load = {}, const LoadCallable3D& tide = {},
+                       double guard = 0.0) const {
+    return WithChannels(load, tide, [&](sl3_load_fn l, sl3_load_fn t, void* user,
+                                        sl3_step_info* info) {
+      return sl3_extrapolated_step(ex.Handle(), state.Handle(), dt, static_cast<int>(scheme), l,
+                                   t, user, guard, info);
+    });
   }
 
   /* The last step's solutions on every dof (its last pass's, with the
@@ -240,6 +275,22 @@ class Viscoelastic3D {
     const LoadCallable3D* fn;
     std::exception_ptr error;
   };
+
+  // Runs a step call with the load and tide channels bound to the C
+  // callbacks; a channel's own exception is rethrown, then the status.
+  template <class F>
+  static StepInfo WithChannels(const LoadCallable3D& load, const LoadCallable3D& tide, F&& call) {
+    Channel channels[2] = {{&load, nullptr}, {&tide, nullptr}};
+    sl3_step_info info{};
+    const int status = call(load ? &Trampoline : nullptr, tide ? &Trampoline2 : nullptr,
+                            static_cast<void*>(channels), &info);
+    for (const auto& c : channels) {
+      if (c.error) std::rethrow_exception(c.error);
+    }
+    Check(status);
+    return StepInfo{info.iterations, info.residual, info.converged != 0, info.mixed_passes,
+                    info.solve_iterations, info.warm_fallback != 0};
+  }
 
   static int Call(Channel& c, int l, double t, double* columns) {
     if (c.error) return -1;

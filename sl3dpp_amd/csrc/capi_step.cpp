@@ -32,6 +32,16 @@ struct sl3_state {
   std::unique_ptr<sl3amd::DeviceState> state;
 };
 
+// The converged strain coefficients of a trajectory's last steps (device
+// copies, oldest first) and the buffer their extrapolation is formed in.
+struct sl3_extrapolator {
+  sl3_operator* op = nullptr;
+  int order = 3;
+  std::vector<sl3amd::DeviceArray<double>> buffers;
+  sl3amd::DeviceArray<double> start;
+  std::vector<std::pair<double, int>> recorded;  // (time, buffer)
+};
+
 using namespace sl3amd;
 
 namespace {
@@ -559,6 +569,84 @@ int sl3_spectral_scatter(sl3_operator* op, const float* in_dev, int ld, int cut,
     LaunchSpectralScatterAdd(in_dev, ld, rows.second.data(), rows.first, c.NodeCount(),
                              mean_dev, scale_dev, out_dev, s);
     SL3_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int sl3_extrapolator_create(sl3_operator* op, int order, sl3_extrapolator** out) {
+  return Guard([&] {
+    Check(op != nullptr && out != nullptr, "null argument");
+    Check(order >= 1 && order <= 3, "extrapolation order must be 1, 2 or 3");
+    *out = nullptr;
+    auto ex = std::make_unique<sl3_extrapolator>();
+    ex->op = op;
+    ex->order = order;
+    const size_t S = op->op->GetCoupling().SpectralDoubles();
+    ex->buffers.resize(order + 1);
+    for (auto& b : ex->buffers) b.Resize(S);
+    ex->start.Resize(S);
+    *out = ex.release();
+  });
+}
+
+int sl3_extrapolator_destroy(sl3_extrapolator* ex) {
+  return Guard([&] { delete ex; });
+}
+
+int sl3_extrapolator_reset(sl3_extrapolator* ex) {
+  return Guard([&] {
+    Check(ex != nullptr, "null argument");
+    ex->recorded.clear();
+  });
+}
+
+int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int scheme,
+                          sl3_load_fn load, sl3_load_fn tide, void* user, double guard,
+                          sl3_step_info* info) {
+  const double* start = nullptr;
+  int status = Guard([&] {
+    Check(ex != nullptr && state != nullptr, "null argument");
+    Check(guard >= 0.0, "the warm-start guard must be non-negative");
+    const double t0 = state->state->t;
+    // another state than the last recorded one starts a new trajectory
+    if (!ex->recorded.empty() && ex->recorded.back().first != t0) ex->recorded.clear();
+    if (scheme == SL3_SDIRK2 || !(dt > 0.0) || ex->recorded.size() < 2) return;
+    // Lagrange extrapolation through the recorded times to t0 + dt
+    const double target = t0 + dt;
+    SpectralTerms t;
+    t.count = static_cast<int>(ex->recorded.size());
+    for (int j = 0; j < t.count; ++j) {
+      double w = 1.0;
+      for (int k = 0; k < t.count; ++k)
+        if (k != j)
+          w *= (target - ex->recorded[k].first) / (ex->recorded[j].first - ex->recorded[k].first);
+      t.c[j] = w;
+      t.src[j] = ex->buffers[ex->recorded[j].second].data();
+    }
+    LaunchCombineTerms(t, ex->start.data(), ex->start.size(), Stream());
+    start = ex->start.data();
+  });
+  if (status != SL3_OK) return status;
+  status = sl3_step_guarded(ex->op, state, dt, scheme, load, tide, user, start,
+                            start != nullptr ? guard : 0.0, info);
+  if (status != SL3_OK) return status;
+  return Guard([&] {
+    const auto& st = *state->state;
+    if (!st.hasCoefficients) return;
+    int slot = -1;
+    if (ex->recorded.size() == ex->buffers.size()) {
+      slot = ex->recorded.front().second;
+      ex->recorded.erase(ex->recorded.begin());
+    } else {
+      for (int b = 0; b < static_cast<int>(ex->buffers.size()) && slot < 0; ++b) {
+        bool used = false;
+        for (const auto& r : ex->recorded) used = used || r.second == b;
+        if (!used) slot = b;
+      }
+    }
+    LaunchCopy(ex->buffers[slot].data(), st.coefficients.data(), ex->buffers[slot].size(),
+               Stream());
+    SL3_HIP(hipStreamSynchronize(Stream()));
+    ex->recorded.emplace_back(st.t, slot);
   });
 }
 

@@ -384,66 +384,49 @@ def lagrange_extrapolation_weights(times, target):
     return out
 
 
+def spectral_combine(op, weights, sources, dst):
+    """dst = sum_j weights[j] sources[j] over op's spectral tensor
+    (sl3_spectral_combine; sources are device pointers, dst a tensor)."""
+    n = len(weights)
+    c = (ctypes.c_double * n)(*weights)
+    src = (ctypes.c_void_p * n)(*sources)
+    check(lib.sl3_spectral_combine(op.handle, n, c, src, ctypes.c_void_p(dst.data_ptr())))
+
+
+lib.sl3_extrapolator_create.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+lib.sl3_extrapolator_destroy.argtypes = [ctypes.c_void_p]
+lib.sl3_extrapolator_reset.argtypes = [ctypes.c_void_p]
+
+
 class ExtrapolatedStart:
-    """Warm starts along a trajectory. The next step's strain coefficients
-    are extrapolated in time from the last converged steps' (the Lagrange
-    polynomial through up to order + 1 of them, at their times) and handed to
-    the step as the `start` of OpenStep::Converge
-    (src/Viscoelastic3D.cpp:1916-1921). The fixed point then iterates to the
-    same tolerance, so a step's result is the plain step's to that
-    tolerance. With fewer than two recorded steps, and for Sdirk2 (which
-    takes no start), the step uses its own start. guard > 0 drops an
+    """Warm starts along a trajectory (sl3_extrapolator). The next step's
+    strain coefficients are extrapolated in time from the last converged
+    steps' (the Lagrange polynomial through up to order + 1 of them, at
+    their times) and handed to the step as the `start` of
+    OpenStep::Converge (src/Viscoelastic3D.cpp:1916-1921). The fixed point
+    then iterates to the same tolerance, so a step's result is the plain
+    step's to that tolerance. With fewer than two recorded steps, for Sdirk2
+    (which takes no start), and for a state other than the last one stepped
+    (a new history), the step uses its own start. guard > 0 drops an
     extrapolated start whose first pass changes it by more than guard."""
 
     def __init__(self, op, order=3, guard=0.0):
-        import torch
-
-        if order < 1 or order > 3:
-            raise ValueError("order must be 1, 2 or 3")
         self.op, self.order, self.guard = op, order, guard
-        self.buffers = [torch.empty(op.spectral_doubles, dtype=torch.float64, device="cuda")
-                        for _ in range(order + 1)]
-        self.start_buffer = torch.empty(op.spectral_doubles, dtype=torch.float64,
-                                        device="cuda")
-        self.recorded = []  # (time, buffer index), oldest first
-        torch.cuda.synchronize()
+        self.handle = ctypes.c_void_p()
+        check(lib.sl3_extrapolator_create(op.handle, order, ctypes.byref(self.handle)))
 
-    def _combine(self, weights, sources, dst):
-        n = len(weights)
-        c = (ctypes.c_double * n)(*weights)
-        src = (ctypes.c_void_p * n)(*sources)
-        check(lib.sl3_spectral_combine(self.op.handle, n, c, src, ctypes.c_void_p(dst.data_ptr())))
-
-    def start(self, target_time):
-        """The extrapolated coefficients at target_time, or None."""
-        if len(self.recorded) < 2:
-            return None
-        times = [t for t, _ in self.recorded]
-        weights = lagrange_extrapolation_weights(times, target_time)
-        sources = [self.buffers[i].data_ptr() for _, i in self.recorded]
-        self._combine(weights, sources, self.start_buffer)
-        return self.start_buffer
-
-    def record(self, state):
-        """Keeps a converged state's coefficients (device to device)."""
-        p = ctypes.c_void_p()
-        check(lib.sl3_state_spectral(state.handle, ctypes.byref(p)))
-        if not p.value:
-            raise ValueError("the state has no strain coefficients yet")
-        used = {i for _, i in self.recorded}
-        if len(self.recorded) == len(self.buffers):
-            free = self.recorded.pop(0)[1]
-        else:
-            free = min(set(range(len(self.buffers))) - used)
-        self._combine([1.0], [p.value], self.buffers[free])
-        self.recorded.append((state.t, free))
+    def __del__(self, _lib=lib):  # the default keeps the library through shutdown
+        if getattr(self, "handle", None):
+            _lib.sl3_extrapolator_destroy(self.handle)
+            self.handle = None
 
     def reset(self):
-        self.recorded = []
+        check(lib.sl3_extrapolator_reset(self.handle))
 
     def step(self, state, dt, scheme=EXPONENTIAL_TRAPEZOID, load=None, tide=None):
-        start = None if scheme == SDIRK2 else self.start(state.t + dt)
-        info = self.op.step(state, dt, scheme, load, start=start, tide=tide,
-                            guard=self.guard if start is not None else 0.0)
-        self.record(state)
+        info = StepInfo()
+        check(lib.sl3_extrapolated_step(self.handle, state.handle, ctypes.c_double(dt), scheme,
+                                        load, tide, None, ctypes.c_double(self.guard),
+                                        ctypes.byref(info)))
         return info

@@ -168,6 +168,47 @@ int Gpu() {
       [](int, double) -> std::optional<std::vector<Complex>> { throw LoadFailure{}; };
   Expect(Throws<LoadFailure>([&] { op.Step(r2.state, dt, Scheme::ExponentialTrapezoid, failing); }),
          "a load channel's exception comes out of Step");
+  // a trajectory warm-started by extrapolation reaches the plain
+  // trajectory's states, in fewer passes once it holds a history
+  {
+    auto plain = op.ZeroState();
+    auto warm = op.ZeroState();
+    Viscoelastic3D::Extrapolator ex(op, 2);
+    int plainPasses = 0, warmPasses = 0;
+    double worst = 0.0;
+    for (int k = 0; k < 6; ++k) {
+      const auto a = op.StepInPlace(plain, dt, Scheme::ExponentialTrapezoid, load);
+      const auto b = op.StepInPlace(warm, ex, dt, Scheme::ExponentialTrapezoid, load);
+      if (k >= 3) {
+        plainPasses += a.iterations;
+        warmPasses += b.iterations;
+      }
+      const auto x = op.Export(plain, "coefficients");
+      const auto y = op.Export(warm, "coefficients");
+      double top = 0.0, diff = 0.0;
+      for (size_t i = 0; i < x.size(); ++i) {
+        top = std::max(top, std::abs(x[i]));
+        diff = std::max(diff, std::abs(x[i] - y[i]));
+      }
+      worst = std::max(worst, diff / top);
+    }
+    std::printf("[cpp] extrapolated trajectory: passes %d vs %d, state difference %.2e\n",
+                warmPasses, plainPasses, worst);
+    Expect(worst < 1e-8, "the extrapolated trajectory reaches the plain one's states");
+    Expect(warmPasses < plainPasses, "the extrapolated trajectory takes fewer passes");
+    // a warm start that fails its guard falls back to the step's own start
+    auto q = op.ZeroState();
+    std::vector<double> zeros(static_cast<size_t>(spectral), 0.0);
+    double* bad = nullptr;
+    if (hipMalloc(&bad, sizeof(double) * zeros.size()) != hipSuccess) return 1;
+    for (auto& v : zeros) v = 1.0;
+    if (hipMemcpy(bad, zeros.data(), sizeof(double) * zeros.size(), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return 1;
+    const auto g = op.StepInPlace(q, dt, Scheme::ExponentialTrapezoid, load, {}, bad, 1e-3);
+    Expect(g.warmFallback && g.converged, "a guarded warm start is dropped");
+    (void)hipFree(bad);
+  }
   (void)hipFree(delta_dev);
   return failures;
 }

@@ -53,14 +53,13 @@ def test_combine_against_numpy(trajectory_problem):
     a = rng.normal(size=op.spectral_doubles)
     b = rng.normal(size=op.spectral_doubles)
     c = rng.normal(size=op.spectral_doubles)
-    ws = st.ExtrapolatedStart(op, order=2)
     da, db, dc = (torch.from_numpy(v).cuda() for v in (a, b, c))
     out = torch.empty_like(da)
     torch.cuda.synchronize()
-    ws._combine([3.0, -3.0, 1.0], [da.data_ptr(), db.data_ptr(), dc.data_ptr()], out)
+    st.spectral_combine(op, [3.0, -3.0, 1.0], [da.data_ptr(), db.data_ptr(), dc.data_ptr()], out)
     np.testing.assert_allclose(out.cpu().numpy(), 3 * a - 3 * b + c, rtol=0, atol=1e-14)
     # dst aliasing a source
-    ws._combine([0.5, 2.0], [da.data_ptr(), db.data_ptr()], da)
+    st.spectral_combine(op, [0.5, 2.0], [da.data_ptr(), db.data_ptr()], da)
     np.testing.assert_allclose(da.cpu().numpy(), 0.5 * a + 2 * b, rtol=0, atol=1e-14)
 
 
@@ -86,6 +85,11 @@ def test_extrapolated_trajectory_matches_the_plain_one(trajectory_problem, order
             b = warm_state.export(name)
             assert np.max(np.abs(a - b)) <= 1e-7 * np.max(np.abs(a)), (k, name)
     print(f"order {order}: plain passes {plain_passes}, extrapolated {warm_passes}")
+    # a state other than the last one stepped starts a new history: its
+    # step takes the step's own start, the plain step's pass count
+    other = op.zero_state()
+    info = ws.step(other, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    assert info.iterations == plain_passes[0]
     # the first two steps have no history; afterwards the start is better
     # (on this small Earth the higher orders gain less than at lmax 128,
     # where cubic is the best; the pass counts are checked to order 2)
