@@ -45,6 +45,9 @@ int sl3_synchronize(void);
  * "band_solve", "strain", "anderson", "residual", ...); names lists them,
  * newline-separated. */
 int sl3_profile_enable(int on);
+/* Limits the timing to the named kernels (comma-separated; NULL or "": all),
+ * so that a timed run carries events only where they are read. */
+int sl3_profile_select(const char* kernels);
 int sl3_profile_reset(void);
 int sl3_profile_read(const char* kernel, double* total_ms, int64_t* launches);
 int sl3_profile_names(char* buffer, int64_t capacity);

@@ -76,6 +76,11 @@ def profile_enable(on=True):
     check(lib.sl3_profile_enable(1 if on else 0))
 
 
+def profile_select(names=None):
+    """Times only the named kernels (an iterable of names; None: all)."""
+    check(lib.sl3_profile_select(",".join(names).encode() if names else None))
+
+
 def profile_reset():
     check(lib.sl3_profile_reset())
 

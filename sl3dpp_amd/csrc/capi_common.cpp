@@ -5,6 +5,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,7 @@ hipStream_t Stream() {
 namespace {
 struct Profiler {
   bool enabled = false;
+  std::set<std::string> only;  // the kernels timed (empty: every kernel)
   std::mutex mutex;
   std::vector<hipEvent_t> pool;
   struct Pending {
@@ -79,6 +81,7 @@ ProfileScope::ProfileScope(const char* name, hipStream_t stream)
   auto& p = GetProfiler();
   if (!p.enabled) return;
   std::lock_guard<std::mutex> lock(p.mutex);
+  if (!p.only.empty() && p.only.count(name_) == 0) return;
   slot_ = p.Take();
   SL3_HIP(hipEventRecord(p.pool[slot_], stream_));
 }
@@ -102,6 +105,25 @@ int sl3_profile_enable(int on) {
     std::lock_guard<std::mutex> lock(p.mutex);
     p.Drain();
     p.enabled = on != 0;
+  });
+}
+
+int sl3_profile_select(const char* kernels) {
+  return sl3amd::Guard([&] {
+    auto& p = sl3amd::GetProfiler();
+    std::lock_guard<std::mutex> lock(p.mutex);
+    p.only.clear();
+    if (kernels == nullptr) return;
+    std::string cur;
+    for (const char* c = kernels;; ++c) {
+      if (*c == ',' || *c == ' ' || *c == 0) {
+        if (!cur.empty()) p.only.insert(cur);
+        cur.clear();
+        if (*c == 0) break;
+      } else {
+        cur += *c;
+      }
+    }
   });
 }
 
