@@ -404,7 +404,8 @@ def learned_leg(op, dt, load, args):
         "train_steps": args.learned_train, "epochs": args.learned_epochs,
         "triples_s": round(triples_s, 2), "fit_s": round(fit_s, 2),
         "final_train_loss": ws.history[-1], "guard": ws.guard,
-        "network": {"widths": ws.mlp.widths, "rows_per_step": ws.nodes},
+        "network": {"widths": ws.mlp.widths, "rows_per_step": ws.nodes,
+                    "chunks": [[int(v) for v in t] for t in ws.mlp.chunks()]},
         "protocol": ("first steps from rest, held-out delta seeds; cold = the plain fixed point "
                      "from the step's own start; radial = from the delta-0 response alone"),
     })

@@ -344,10 +344,15 @@ int sl3_mlp_backward(sl3_mlp* mlp, int rows, const float* x_dev, const float* ta
 /* One Adam step (bias-corrected) from the gradient buffer. */
 int sl3_mlp_adam(sl3_mlp* mlp, double lr, double beta1, double beta2, double eps);
 /* Host copies by name: "weight:k", "bias:k", "grad_weight:k", "grad_bias:k"
- * (k = 1..layers); dst NULL returns the size through *count. Import sets a
+ * (k = 1..layers), and "chunks" (the tuner's choices: kind, M, N, K, slice
+ * depth per product shape); dst NULL returns the size through *count. Import sets a
  * weight or bias. */
 int sl3_mlp_export(sl3_mlp* mlp, const char* name, float* dst, int64_t* count);
 int sl3_mlp_import(sl3_mlp* mlp, const char* name, const float* src, int64_t count);
+/* The products' LDS slice depth: 0 (default) times 32 and 128 on a shape's
+ * first use and keeps the faster; 32 or 128 fixes it. Both give the same
+ * bits. */
+int sl3_mlp_set_chunk(sl3_mlp* mlp, int chunk);
 /* Column means and scales (population standard deviation, at least
  * floor x the largest) of a rows x cols matrix. Device pointers. */
 int sl3_column_moments(int64_t rows, int cols, const float* x_dev, float floor, float* mean_dev,

@@ -16,6 +16,15 @@
 #include "kernels.hpp"
 
 namespace sl3amd {
+// What a product's epilogue writes (Bf16GemmKernel's MODE selects).
+struct Epilogue {
+  const float* bias = nullptr;   // per column (kBiasGelu, kBias)
+  float* outF = nullptr;         // FP32 output (kStoreF32, kBias; kBiasGelu: z)
+  uint16_t* outH = nullptr;      // BF16 output (kBiasGelu: gelu(z); kGeluGrad)
+  const float* zPrev = nullptr;  // pre-activations whose GELU derivative scales (kGeluGrad)
+  int ldc = 0;
+};
+
 namespace {
 
 typedef short Bf16x4 __attribute__((ext_vector_type(4)));
@@ -39,38 +48,31 @@ __device__ inline float GeluGrad(float z) {
          z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// C (M x N) = sum_k A(i, k) B(k, j) in 64 x 64 workgroup tiles, 32-deep
-// chunks staged through LDS as [row][k] BF16 (rows of the tile: i for A, j
-// for B, padded to 40 so that 16-byte stores stay aligned). Each of the four
-// waves holds a 32 x 32 quarter as 2 x 2 products of 16 x 16 x 16.
+// C (M x N) = sum_k A(i, k) B(k, j) in 64 x 64 workgroup tiles of four
+// waves, CHUNK-deep slices (32 or 128) staged through LDS as [row][k] BF16
+// (rows of the tile: i for A, j for B, padded by 8 so that 16-byte stores
+// stay aligned). Each wave holds a 32 x 32 quarter as 2 x 2 products of
+// 16 x 16 x 16. Both depths sum every element's products in the same order
+// (16 deep at a time, k ascending), so they give the same bits; deeper
+// slices take fewer load round trips and barriers, shallower ones less LDS.
+// Mlp's tuner picks per shape.
 // Operand layouts of v_mfma_f32_16x16x16bf16_1k (lane l): A(i = l % 16,
 // k = 4 (l / 16) + 0..3), B(k = 4 (l / 16) + 0..3, j = l % 16), and the
 // accumulator C(i = 4 (l / 16) + 0..3, j = l % 16).
 constexpr int kTile = 64;
-constexpr int kChunk = 32;
-constexpr int kLdsRow = 40;
 
 enum { kStoreF32 = 0, kBiasGelu = 1, kBias = 2, kGeluGrad = 3 };
 
-struct Epilogue {
-  const float* bias = nullptr;   // per column (kBiasGelu, kBias)
-  float* outF = nullptr;         // FP32 output (kStoreF32, kBias; kBiasGelu: z)
-  uint16_t* outH = nullptr;      // BF16 output (kBiasGelu: gelu(z); kGeluGrad)
-  const float* zPrev = nullptr;  // pre-activations whose GELU derivative scales (kGeluGrad)
-  int ldc = 0;
-};
 
-// Stages rows o0..o0+63, depth k0..k0+31 of an operand into T[row][k].
-// KCONTIG: element (o, k) at p[o ld + k]; otherwise at p[k ld + o]. vec: ld
-// a multiple of 8 and p 16-byte aligned (8-element loads). Out-of-range
-// elements are zeros.
-template <bool KCONTIG>
-__device__ inline void StageTile(const uint16_t* __restrict__ p, int ld, int outer, int K, int o0,
-                                 int k0, bool vec, uint16_t* T, int tid) {
+// One thread's eight elements of a staged tile (see StageTile).
+template <bool KCONTIG, int CHUNK>
+__device__ inline void StageEight(const uint16_t* __restrict__ p, int ld, int outer, int K, int o0,
+                                  int k0, bool vec, uint16_t* T, int idx) {
+  constexpr int kRow = CHUNK + 8;
   if (KCONTIG) {
-    const int r = tid >> 2, ks = (tid & 3) * 8;
+    const int r = idx / (CHUNK / 8), ks = (idx % (CHUNK / 8)) * 8;
     const int o = o0 + r, k = k0 + ks;
-    uint16_t* dst = T + r * kLdsRow + ks;
+    uint16_t* dst = T + r * kRow + ks;
     if (vec && o < outer && k + 8 <= K) {
       *reinterpret_cast<uint4*>(dst) =
           *reinterpret_cast<const uint4*>(p + static_cast<size_t>(o) * ld + k);
@@ -80,7 +82,7 @@ __device__ inline void StageTile(const uint16_t* __restrict__ p, int ld, int out
         dst[e] = (o < outer && k + e < K) ? p[static_cast<size_t>(o) * ld + k + e] : 0;
     }
   } else {
-    const int kq = tid >> 3, rs = (tid & 7) * 8;
+    const int kq = idx / (kTile / 8), rs = (idx % (kTile / 8)) * 8;
     const int k = k0 + kq, o = o0 + rs;
     uint16_t v[8];
     if (vec && k < K && o + 8 <= outer) {
@@ -92,17 +94,30 @@ __device__ inline void StageTile(const uint16_t* __restrict__ p, int ld, int out
         v[e] = (k < K && o + e < outer) ? p[static_cast<size_t>(k) * ld + o + e] : 0;
     }
 #pragma unroll
-    for (int e = 0; e < 8; ++e) T[(rs + e) * kLdsRow + kq] = v[e];
+    for (int e = 0; e < 8; ++e) T[(rs + e) * kRow + kq] = v[e];
   }
 }
 
-template <bool KA, bool KB, int MODE>
+// Stages rows o0..o0+63, depth k0..k0+CHUNK-1 of an operand into
+// T[row][k], 8 elements per thread and pass (256 threads). KCONTIG: element
+// (o, k) at p[o ld + k]; otherwise at p[k ld + o]. vec: ld a multiple of 8
+// and p 16-byte aligned (8-element loads). Out-of-range elements are zeros.
+template <bool KCONTIG, int CHUNK>
+__device__ inline void StageTile(const uint16_t* __restrict__ p, int ld, int outer, int K, int o0,
+                                 int k0, bool vec, uint16_t* T, int tid) {
+#pragma unroll
+  for (int pass = 0; pass < kTile * CHUNK / (8 * 256); ++pass)
+    StageEight<KCONTIG, CHUNK>(p, ld, outer, K, o0, k0, vec, T, tid + pass * 256);
+}
+
+template <bool KA, bool KB, int MODE, int CHUNK>
 __global__ __launch_bounds__(256) void Bf16GemmKernel(int M, int N, int K,
                                                       const uint16_t* __restrict__ A, int lda,
                                                       bool vecA, const uint16_t* __restrict__ B,
                                                       int ldb, bool vecB, Epilogue e) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[kTile * kLdsRow];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[kTile * kLdsRow];
+  constexpr int kRow = CHUNK + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t As[kTile * kRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[kTile * kRow];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
   const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
@@ -111,19 +126,19 @@ __global__ __launch_bounds__(256) void Bf16GemmKernel(int M, int N, int K,
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = F32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += kChunk) {
-    StageTile<KA>(A, lda, M, K, i0, k0, vecA, As, tid);
-    StageTile<KB>(B, ldb, N, K, j0, k0, vecB, Bs, tid);
+  for (int k0 = 0; k0 < K; k0 += CHUNK) {
+    StageTile<KA, CHUNK>(A, lda, M, K, i0, k0, vecA, As, tid);
+    StageTile<KB, CHUNK>(B, ldb, N, K, j0, k0, vecB, Bs, tid);
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < kChunk; kk += 16) {
+    for (int kk = 0; kk < CHUNK; kk += 16) {
       Bf16x4 a[2], b[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         a[t] = *reinterpret_cast<const Bf16x4*>(
-            &As[(wi + 16 * t + (lane & 15)) * kLdsRow + kk + 4 * (lane >> 4)]);
+            &As[(wi + 16 * t + (lane & 15)) * kRow + kk + 4 * (lane >> 4)]);
         b[t] = *reinterpret_cast<const Bf16x4*>(
-            &Bs[(wj + 16 * t + (lane & 15)) * kLdsRow + kk + 4 * (lane >> 4)]);
+            &Bs[(wj + 16 * t + (lane & 15)) * kRow + kk + 4 * (lane >> 4)]);
       }
 #pragma unroll
       for (int ti = 0; ti < 2; ++ti)
@@ -314,14 +329,19 @@ int GridOf(size_t n) {
 bool Aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 template <bool KA, bool KB, int MODE>
-void Gemm(int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb,
+void Gemm(int chunk, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb,
           const Epilogue& e, hipStream_t s) {
   if (M <= 0 || N <= 0) return;
   const bool vecA = lda % 8 == 0 && Aligned16(A);
   const bool vecB = ldb % 8 == 0 && Aligned16(B);
-  dim3 grid(CeilDiv(N, kTile), CeilDiv(M, kTile));
-  hipLaunchKernelGGL((Bf16GemmKernel<KA, KB, MODE>), grid, dim3(256), 0, s, M, N, K, A, lda,
-                     vecA, B, ldb, vecB, e);
+  const dim3 grid(CeilDiv(N, kTile), CeilDiv(M, kTile));
+  if (chunk == 128) {
+    hipLaunchKernelGGL((Bf16GemmKernel<KA, KB, MODE, 128>), grid, dim3(256), 0, s, M, N, K, A,
+                       lda, vecA, B, ldb, vecB, e);
+  } else {
+    hipLaunchKernelGGL((Bf16GemmKernel<KA, KB, MODE, 32>), grid, dim3(256), 0, s, M, N, K, A,
+                       lda, vecA, B, ldb, vecB, e);
+  }
   SL3_HIP(hipGetLastError());
 }
 
@@ -349,6 +369,59 @@ void ColumnSums(const T* x, long long rows, int cols, const float* center, float
 size_t Round8(size_t n) { return (n + 7) & ~static_cast<size_t>(7); }
 
 }  // namespace
+
+// The slice depth of a GEMM shape: fixed, or on the shape's first use the
+// faster of the two, timed with device events over three launches each
+// after one untimed launch (every launch writes the same outputs from the
+// same inputs, so the trial launches change nothing).
+template <bool KA, bool KB, int MODE>
+void Mlp::Product(int kind, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B,
+                  int ldb, const Epilogue& e, hipStream_t s) {
+  int tile = fixedChunk_;
+  if (tile == 0) {
+    const std::array<int, 4> key{kind, M, N, K};
+    auto it = tiles_.find(key);
+    if (it == tiles_.end()) {
+      float best = 0.0f;
+      int choice = 32;
+      for (int trial : {32, 128}) {
+        Gemm<KA, KB, MODE>(trial, M, N, K, A, lda, B, ldb, e, s);
+        SL3_HIP(hipEventRecord(tuneEvents_[0], s));
+        for (int r = 0; r < 3; ++r) Gemm<KA, KB, MODE>(trial, M, N, K, A, lda, B, ldb, e, s);
+        SL3_HIP(hipEventRecord(tuneEvents_[1], s));
+        SL3_HIP(hipEventSynchronize(tuneEvents_[1]));
+        float ms = 0.0f;
+        SL3_HIP(hipEventElapsedTime(&ms, tuneEvents_[0], tuneEvents_[1]));
+        if (trial == 32 || ms < best) {
+          best = ms;
+          choice = trial;
+        }
+      }
+      it = tiles_.emplace(key, choice).first;
+    }
+    tile = it->second;
+  }
+  Gemm<KA, KB, MODE>(tile, M, N, K, A, lda, B, ldb, e, s);
+}
+
+Mlp::~Mlp() {
+  for (auto& ev : tuneEvents_)
+    if (ev != nullptr) (void)hipEventDestroy(ev);
+}
+
+void Mlp::SetChunk(int chunk) {
+  if (chunk != 0 && chunk != 32 && chunk != 128) throw ArgumentError("chunk must be 0, 32 or 128");
+  fixedChunk_ = chunk;
+}
+
+std::vector<float> Mlp::Chunks() const {
+  std::vector<float> out;
+  for (const auto& [key, tile] : tiles_) {
+    for (int v : key) out.push_back(static_cast<float>(v));
+    out.push_back(static_cast<float>(tile));
+  }
+  return out;
+}
 
 Mlp::Mlp(std::vector<int> widths, int maxRows, uint64_t seed)
     : widths_(std::move(widths)), maxRows_(maxRows) {
@@ -406,6 +479,7 @@ Mlp::Mlp(std::vector<int> widths, int maxRows, uint64_t seed)
   const size_t chunks = (R + 255) / 256;
   partials_.Resize(std::max<size_t>(kReduceBlocks, chunks * maxW));
   partialsHost_.resize(kReduceBlocks);
+  for (auto& ev : tuneEvents_) SL3_HIP(hipEventCreate(&ev));
   SL3_HIP(hipStreamSynchronize(s));
 }
 
@@ -423,13 +497,13 @@ void Mlp::ForwardChunk(int rows, const float* x, float* y, hipStream_t s) {
     if (k < L) {
       e.outF = z_[k].data();
       e.outH = h_[k].data();
-      Gemm<true, true, kBiasGelu>(rows, widths_[k], widths_[k - 1], in, widths_[k - 1], W,
-                                  widths_[k - 1], e, s);
+      Product<true, true, kBiasGelu>(kBiasGelu, rows, widths_[k], widths_[k - 1], in,
+                                     widths_[k - 1], W, widths_[k - 1], e, s);
       in = h_[k].data();
     } else {
       e.outF = y;
-      Gemm<true, true, kBias>(rows, widths_[k], widths_[k - 1], in, widths_[k - 1], W,
-                              widths_[k - 1], e, s);
+      Product<true, true, kBias>(kBias, rows, widths_[k], widths_[k - 1], in, widths_[k - 1],
+                                 W, widths_[k - 1], e, s);
     }
   }
 }
@@ -469,8 +543,8 @@ void Mlp::Backward(int rows, const float* x, const float* target, double* loss, 
     Epilogue ew;
     ew.outF = grads_.data() + wOff_[k];
     ew.ldc = widths_[k - 1];
-    Gemm<false, false, kStoreF32>(widths_[k], widths_[k - 1], rows, cur, widths_[k], in,
-                                  widths_[k - 1], ew, s);
+    Product<false, false, kStoreF32>(kStoreF32, widths_[k], widths_[k - 1], rows, cur,
+                                     widths_[k], in, widths_[k - 1], ew, s);
     // db_k = column sums of dZ_k
     ColumnSums<uint16_t, false>(cur, rows, widths_[k], nullptr, 1.0f, false, partials_.data(),
                                 partials_.size(), grads_.data() + bOff_[k], s);
@@ -480,8 +554,9 @@ void Mlp::Backward(int rows, const float* x, const float* target, double* loss, 
       ed.outH = other;
       ed.zPrev = z_[k - 1].data();
       ed.ldc = widths_[k - 1];
-      Gemm<true, false, kGeluGrad>(rows, widths_[k - 1], widths_[k], cur, widths_[k],
-                                   paramsBf16_.data() + wOff_[k], widths_[k - 1], ed, s);
+      Product<true, false, kGeluGrad>(kGeluGrad, rows, widths_[k - 1], widths_[k], cur,
+                                      widths_[k], paramsBf16_.data() + wOff_[k], widths_[k - 1],
+                                      ed, s);
       std::swap(cur, other);
     }
   }
@@ -656,11 +731,19 @@ int sl3_mlp_adam(sl3_mlp* mlp, double lr, double beta1, double beta2, double eps
   });
 }
 
+int sl3_mlp_set_chunk(sl3_mlp* mlp, int chunk) {
+  return Guard([&] {
+    if (mlp == nullptr) throw ArgumentError("sl3_mlp_set_chunk: null network");
+    mlp->mlp->SetChunk(chunk);
+  });
+}
+
 int sl3_mlp_export(sl3_mlp* mlp, const char* name, float* dst, int64_t* count) {
   return Guard([&] {
     if (mlp == nullptr || name == nullptr || count == nullptr)
       throw ArgumentError("sl3_mlp_export: bad arguments");
-    const auto v = mlp->mlp->Export(name, Stream());
+    const auto v = std::string(name) == "chunks" ? mlp->mlp->Chunks()
+                                                : mlp->mlp->Export(name, Stream());
     if (dst != nullptr) {
       if (*count < static_cast<int64_t>(v.size())) throw ArgumentError("export buffer too small");
       std::copy(v.begin(), v.end(), dst);

@@ -7,7 +7,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -18,9 +21,14 @@ namespace sl3amd {
 // form), the last one linear. Weights W_k (w_k x w_{k-1}, row-major) and
 // biases b_k are held in FP32 (the optimiser's copy) and in BF16 (the matrix
 // cores' operands); products accumulate in FP32.
+struct Epilogue;
+
 class Mlp {
  public:
   Mlp(std::vector<int> widths, int maxRows, uint64_t seed);
+  ~Mlp();
+  Mlp(const Mlp&) = delete;
+  Mlp& operator=(const Mlp&) = delete;
   int Layers() const { return static_cast<int>(widths_.size()) - 1; }
   const std::vector<int>& Widths() const { return widths_; }
   int MaxRows() const { return maxRows_; }
@@ -37,8 +45,16 @@ class Mlp {
   std::vector<float> Export(const std::string& name, hipStream_t s) const;
   void Import(const std::string& name, const float* src, size_t count, hipStream_t s);
   long long Steps() const { return steps_; }
+  // The products' slice depth: 0 picks per shape by timing (the default),
+  // 32 or 128 fixes it. Chunks() lists (kind, M, N, K, depth) per tuned
+  // shape.
+  void SetChunk(int chunk);
+  std::vector<float> Chunks() const;
 
  private:
+  template <bool KA, bool KB, int MODE>
+  void Product(int kind, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B,
+               int ldb, const Epilogue& e, hipStream_t s);
   void ForwardChunk(int rows, const float* x, float* y, hipStream_t s);
   size_t Offset(const std::string& name, size_t* count) const;
   std::vector<int> widths_;
@@ -55,6 +71,9 @@ class Mlp {
   DeviceArray<float> partials_;
   std::vector<float> partialsHost_;
   long long steps_ = 0;
+  int fixedChunk_ = 0;
+  std::map<std::array<int, 4>, int> tiles_;
+  hipEvent_t tuneEvents_[2] = {nullptr, nullptr};
 };
 
 // Column means and scales (the population standard deviation, floored at

@@ -47,6 +47,7 @@ lib.sl3_mlp_adam.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
 lib.sl3_mlp_export.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_float_p,
                                ctypes.POINTER(ctypes.c_int64)]
 lib.sl3_mlp_import.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_float_p, ctypes.c_int64]
+lib.sl3_mlp_set_chunk.argtypes = [ctypes.c_void_p, ctypes.c_int]
 lib.sl3_column_moments.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
                                    ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
 lib.sl3_column_standardise.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
@@ -119,6 +120,15 @@ class Mlp:
         check(lib.sl3_mlp_export(self.handle, name.encode(), out.ctypes.data_as(c_float_p),
                                  ctypes.byref(count)))
         return out
+
+    def set_chunk(self, chunk):
+        """0: per-shape tuning (default); 32 or 128: a fixed slice depth."""
+        check(lib.sl3_mlp_set_chunk(self.handle, chunk))
+
+    def chunks(self):
+        """The tuner's choices: [(kind, M, N, K, depth), ...]."""
+        v = self.export("chunks").astype(int)
+        return [tuple(v[i:i + 5]) for i in range(0, v.size, 5)]
 
     def import_(self, name, values):
         v = np.ascontiguousarray(values, dtype=np.float32).ravel()

@@ -81,6 +81,34 @@ def test_forward_and_gradients_against_oracle(learned, widths, rows):
         assert norm_rel(gb, db[k - 1]) < 4e-2, k
 
 
+def test_slice_depths_give_the_same_bits(learned):
+    """The products staged 32 and 128 deep, and the tuner's per-shape
+    choice, sum in the same order: outputs and gradients agree bit for
+    bit."""
+    import torch
+
+    ln = learned
+    widths, rows = [130, 200, 72, 33], 257
+    results = []
+    for chunk in (32, 128, 0):
+        net, W, b, x, t, xd, td = _network(ln, widths, rows, seed=5)
+        net.set_chunk(chunk)
+        y = torch.empty(rows, widths[-1], dtype=torch.float32, device="cuda")
+        net.forward(xd, y)
+        ln._sync()
+        net.backward(rows, xd, td, want_loss=False)
+        grads = [net.export(f"grad_weight:{k}") for k in range(1, len(widths))]
+        grads += [net.export(f"grad_bias:{k}") for k in range(1, len(widths))]
+        results.append((y.cpu().numpy(), grads))
+        if chunk == 0:
+            chosen = net.chunks()
+            assert chosen and all(c[4] in (32, 128) for c in chosen)
+    for y, grads in results[1:]:
+        np.testing.assert_array_equal(y, results[0][0])
+        for g, g0 in zip(grads, results[0][1]):
+            np.testing.assert_array_equal(g, g0)
+
+
 def test_adam_update_against_oracle(learned):
     ln = learned
     widths = [24, 40, 16]
