@@ -287,6 +287,127 @@ __global__ void __launch_bounds__(1024)
   Store(out, 2, l, m, node, g.nodes, A[2]);
 }
 
+// StrainKernel with the workgroup's operands staged in LDS first: the
+// response columns m and l + 1 + m of the degree's spheroidal and toroidal
+// tasks, the unit load and tide responses, and both dof tables, each read
+// once with consecutive threads on consecutive entries. A thread then reads
+// its node's GLL values from LDS (no dependent global loads) and computes
+// exactly what StrainKernel computes, in the same order. Sizes: capS >=
+// every spheroidal task's memory dofs (at most elements x ngll x 2, each
+// memory dof being some (element, GLL node, component) of the table), capT
+// likewise for the toroidal tasks (elements x ngll).
+__global__ void __launch_bounds__(1024)
+    StrainStagedKernel(const DegreeTask* sph, const DegreeTask* tor, int lmax,
+                       RadialGeometry g, const int* dofs, const double* response,
+                       const double* units, const double* amp, const double* tideUnits,
+                       const double* tideAmp, SpectralView out, int capS, int capT) {
+  extern __shared__ double staged[];
+  double* sRe = staged;
+  double* sIm = sRe + capS;
+  double* sUnit = sIm + capS;
+  double* sTide = sUnit + capS;
+  double* tRe = sTide + capS;
+  double* tIm = tRe + capT;
+  int* sDof = reinterpret_cast<int*>(tIm + capT);
+  const int table = g.elements * g.ngll;
+  int* tDof = sDof + 2 * table;
+  const int pair = blockIdx.x;
+  int l = static_cast<int>((sqrt(8.0 * pair + 1.0) - 1.0) * 0.5);
+  while (l * (l + 1) / 2 > pair) --l;
+  while ((l + 1) * (l + 2) / 2 <= pair) ++l;
+  const int m = pair - l * (l + 1) / 2;
+  const DegreeTask ts = sph[l];
+  const bool tor1 = l >= 1;
+  DegreeTask tt;
+  if (tor1) tt = tor[l - 1];
+  const bool tide = tideAmp != nullptr;
+  const bool unitsOn = ts.unitOff >= 0;
+  {
+    const double* re = response + ts.fOff + static_cast<size_t>(m) * ts.nmPad;
+    const double* im = response + ts.fOff + static_cast<size_t>(l + 1 + m) * ts.nmPad;
+    for (int j = threadIdx.x; j < ts.nm; j += blockDim.x) {
+      sRe[j] = re[j];
+      sIm[j] = im[j];
+      if (unitsOn) sUnit[j] = units[ts.dmOff + j];
+      if (tide) sTide[j] = tideUnits[ts.dmOff + j];
+    }
+    for (int j = threadIdx.x; j < 2 * table; j += blockDim.x) sDof[j] = dofs[ts.dofOff + j];
+    if (tor1) {
+      const double* bt = response + tt.fOff;
+      for (int j = threadIdx.x; j < tt.nm; j += blockDim.x) {
+        tRe[j] = bt[static_cast<size_t>(m) * tt.nmPad + j];
+        tIm[j] = bt[static_cast<size_t>(l + 1 + m) * tt.nmPad + j];
+      }
+      for (int j = threadIdx.x; j < table; j += blockDim.x) tDof[j] = dofs[tt.dofOff + 2 * j];
+    }
+  }
+  __syncthreads();
+  const int node = threadIdx.x;
+  if (node >= g.nodes) return;
+  Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
+  if (node >= ts.firstNode) {
+    const int elem = g.elem0 + node / g.np, q = node % g.np;
+    const double r = g.radius[node];
+    const double jac = g.jac[elem - g.elem0];
+    Cplx U = C(0, 0), V = C(0, 0), W = C(0, 0), rdU = C(0, 0), rdV = C(0, 0),
+         rdW = C(0, 0);
+    const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
+    const Cplx at = tide ? C(tideAmp[2 * pair], tideAmp[2 * pair + 1]) : C(0.0, 0.0);
+    for (int i = 0; i < g.ngll; ++i) {
+      const int slot = (elem - g.elem0) * g.ngll + i;
+      const int d0 = sDof[2 * slot], d1 = sDof[2 * slot + 1];
+      Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
+      if (d0 >= 0) {
+        Ui = C(sRe[d0], sIm[d0]);
+        if (unitsOn) Ui = Ui + sUnit[d0] * a;
+        if (tide) Ui = Ui + sTide[d0] * at;
+      }
+      if (d1 >= 0) {
+        Vi = C(sRe[d1], sIm[d1]);
+        if (unitsOn) Vi = Vi + sUnit[d1] * a;
+        if (tide) Vi = Vi + sTide[d1] * at;
+      }
+      if (tor1) {
+        const int dw = tDof[slot];
+        if (dw >= 0) Wi = C(tRe[dw], tIm[dw]);
+      }
+      const double v = g.value[q * g.ngll + i];
+      U = U + v * Ui;
+      V = V + v * Vi;
+      W = W + v * Wi;
+      const double bb = r * g.deriv[q * g.ngll + i] / jac;
+      rdU = rdU + bb * Ui;
+      rdV = rdV + bb * Vi;
+      rdW = rdW + bb * Wi;
+    }
+    const double zeta2 = l * (l + 1.0);
+    const double zeta = sqrt(zeta2);
+    A[2] = Div(rdU - U + (0.5 * zeta2) * V, 3.0 * r);
+    if (l >= 1) {
+      const Cplx tangential = rdV - V + U;
+      const Cplx twist = W - rdW;
+      const double den = 2.0 * kSqrt2 * r;
+      A[1] = Div(zeta * (tangential + TimesI(twist)), den);
+      A[3] = Div(zeta * (tangential - TimesI(twist)), den);
+    }
+    if (l >= 2) {
+      const double f = 0.5 * zeta * sqrt(zeta2 - 2.0) / r;
+      A[0] = f * (V - TimesI(W));
+      A[4] = f * (V + TimesI(W));
+    }
+  }
+  const double sign = (m & 1) ? -1.0 : 1.0;
+  if (l >= 2) {
+    Store(out, 0, l, m, node, g.nodes, A[0]);
+    if (m > 0) Store(out, 0, l, -m, node, g.nodes, sign * Conj(A[4]));
+  }
+  if (l >= 1) {
+    Store(out, 1, l, m, node, g.nodes, A[1]);
+    if (m > 0) Store(out, 1, l, -m, node, g.nodes, sign * Conj(A[3]));
+  }
+  Store(out, 2, l, m, node, g.nodes, A[2]);
+}
+
 __global__ void FillKernel(double* dst, size_t count, double value) {
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -644,9 +765,20 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
   if (g.nodes > 1024) throw ArgumentError("at most 1024 memory nodes");
   const int threads = 64 * CeilDiv(g.nodes, 64);
   ProfileScope prof509("strain", stream);
-  hipLaunchKernelGGL(StrainKernel, dim3(pairs), dim3(threads), 0, stream, sphTasks_dev,
-                     torTasks_dev, lmax, g, dofs, response, units, amp, tideUnits, tideAmp,
-                     out);
+  // every task's memory dofs fit these bounds (see StrainStagedKernel)
+  const int table = g.elements * g.ngll;
+  const int capS = 2 * table, capT = table;
+  const size_t lds = sizeof(double) * (4 * static_cast<size_t>(capS) + 2 * capT) +
+                     sizeof(int) * (2 * static_cast<size_t>(table) + table);
+  if (lds <= 64 * 1024) {
+    hipLaunchKernelGGL(StrainStagedKernel, dim3(pairs), dim3(threads), lds, stream,
+                       sphTasks_dev, torTasks_dev, lmax, g, dofs, response, units, amp,
+                       tideUnits, tideAmp, out, capS, capT);
+  } else {
+    hipLaunchKernelGGL(StrainKernel, dim3(pairs), dim3(threads), 0, stream, sphTasks_dev,
+                       torTasks_dev, lmax, g, dofs, response, units, amp, tideUnits, tideAmp,
+                       out);
+  }
   SL3_HIP(hipGetLastError());
 }
 
