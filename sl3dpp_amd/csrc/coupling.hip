@@ -140,12 +140,13 @@ __global__ void __launch_bounds__(kThreads)
     for (int k = 0; k < 6; ++k) dst[k] = rows[k];
   }
   __syncthreads();
-  // stage 2: each owned node's sums, node fastest
+  // stage 2: each owned node's sums, order fastest (consecutive threads
+  // write consecutive columns of one force row)
   double* bs = force + ts.fOff;
   double* bt = force + tt.fOff;
   const size_t lds = ts.colsPad, ldt = tt.colsPad;
   for (int e = threadIdx.x; e < owned * mCount; e += blockDim.x) {
-    const int gl = e % owned, mi = e / owned;
+    const int mi = e % mCount, gl = e / mCount;
     const int gn = gnFirst + gl;
     const int m = mc + mi;
     int elems[2], locals[2], count = 0;
