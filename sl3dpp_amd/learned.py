@@ -185,7 +185,10 @@ def scatter(op, values, ld, cut, base, out, mean=None, scale=None):
 
 class WarmStart:
     """The learned warm start of one operator's first steps from rest under a
-    fixed load, scheme and dt (the radial reference is theirs)."""
+    fixed load, scheme and dt (the radial reference is theirs). Computing
+    the radial reference steps the operator with delta = 0 and leaves it
+    there: set the lateral field again (set_delta) before stepping. One
+    thread at a time per instance, like the operator it drives."""
 
     def __init__(self, op, dt, scheme, load, lin=32, lrad=8, lcut=32, width=512, layers=4,
                  max_rows=1024, seed=7, context=True):
