@@ -65,6 +65,11 @@ def parse():
     ap.add_argument("--learned-epochs", type=int, default=50)
     ap.add_argument("--extrapolate-order", type=int, default=3,
                     help="order of the extrapolated-start trajectory leg (0 skips it)")
+    ap.add_argument("--uneven", type=int, default=0,
+                    help="an uneven batch of this many independent realisations (lmax from "
+                         "{32, 64, 96, 128}, meshes of {64, 96, 128} elements x degree 5, seeded), "
+                         "stepped --steps times each by 4 worker threads; value is their "
+                         "aggregate steps/s")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -192,6 +197,8 @@ def main():
         return 0
     if args.realisations > 1:
         return batched(args)
+    if args.uneven > 0:
+        return uneven(args)
     lmax = args.lmax
     earth = st.Earth(st.three_region_earth(70, 56, 2, 6))
     length, tscale, mass = earth.scales()
@@ -411,6 +418,120 @@ def learned_leg(op, dt, load, args):
     })
     log("learned: " + json.dumps({k: v for k, v in s.items() if k != "network"}))
     return s
+
+
+def uneven(args, workers=4, seed=48):
+    """An uneven batch: realisation i has lmax and a radial mesh drawn from
+    the sets above (seed 48), delta seed 3000 + i (rms 0.7, AR(1) 0.9, to
+    l = min(64, lmax)), the benchmark's load, scheme and tolerance. All
+    operators are set up first (untimed); then `workers` host threads (one
+    library stream each) take realisations from a queue and step each
+    args.steps times from rest after args.warmup untimed steps. lmax 192 is
+    left out: its grid (P = 145) takes the generic longitude kernel, whose
+    parity tests run only at small lmax."""
+    import queue
+    import threading
+
+    import numpy as np
+    import torch
+
+    import sl3dpp_amd.step as st
+
+    rng = np.random.default_rng(seed)
+    meshes = {64: (35, 27, 2), 96: (52, 42, 2), 128: (70, 56, 2)}
+    earths = {e: st.Earth(st.three_region_earth(*meshes[e], 6)) for e in meshes}
+    cases = []
+    t0 = time.perf_counter()
+    for i in range(args.uneven):
+        lmax = int(rng.choice([32, 64, 96, 128]))
+        elements = int(rng.choice([64, 96, 128]))
+        earth = earths[elements]
+        _, tscale, _ = earth.scales()
+        dt = 100.0 * st.YEAR / tscale
+        opts = st.StepOptions.default(max_iterations=200, rtol=args.rtol, anderson_memory=6,
+                                      anderson_rate=0.5)
+        op = st.Viscoelastic3D(earth, lmax, None, opts)
+        coef = st.delta_coefficients(min(64, lmax), op.nodes, seed=3000 + i, rms=0.7, ar=0.9,
+                                     lmax=lmax)
+        op.set_delta(op.delta_field(coef))
+        load, _ = st.zonal_load_rate(lmax, earth)
+        state = op.zero_state()
+        for _ in range(args.warmup):
+            op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+        cases.append({"lmax": lmax, "elements": elements, "op": op, "state": state, "dt": dt,
+                      "load": load, "passes": [], "converged": True})
+    torch.cuda.synchronize()
+    log(f"uneven batch: {len(cases)} realisations set up in {time.perf_counter() - t0:.1f} s")
+
+    def run(count, key):
+        """Steps every realisation's state `key` args.steps times on
+        `count` worker threads; returns the elapsed seconds."""
+        work = queue.Queue()
+        for i in range(len(cases)):
+            work.put(i)
+
+        def worker():
+            while True:
+                try:
+                    i = work.get_nowait()
+                except queue.Empty:
+                    return
+                c = cases[i]
+                for _ in range(args.steps):
+                    info = c["op"].step(c[key], c["dt"], st.EXPONENTIAL_TRAPEZOID, c["load"])
+                    if key == "state":
+                        c["passes"].append(info.iterations)
+                    c["converged"] = c["converged"] and bool(info.converged)
+
+        threads = [threading.Thread(target=worker) for _ in range(count)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # the same steps one realisation after another, from copies of the states
+    for c in cases:
+        c["copy"] = c["state"].clone()
+    sequential = run(1, "copy")
+    for c in cases:
+        del c["copy"]
+    elapsed = run(workers, "state")
+    total = len(cases) * args.steps
+    log(f"uneven batch: one at a time {total / sequential:.1f} steps/s, {workers} workers "
+        f"{total / elapsed:.1f} steps/s")
+    assert all(c["converged"] for c in cases), "a realisation's step did not converge"
+    log(f"uneven batch: {total} steps in {elapsed:.2f} s")
+    out = {
+        "metric": METRIC + " (uneven batch of independent realisations)",
+        "value": total / elapsed,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / total,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{len(cases)} independent realisations, lmax from {{32, 64, 96, 128}} "
+                         "and meshes of {64, 96, 128} elements x degree 5 (seed 48), delta "
+                         "seeds 3000.., each stepped on a worker thread's stream (4 workers)"),
+            "realisations": [{"lmax": c["lmax"], "elements": c["elements"],
+                              "passes": c["passes"]} for c in cases],
+            "one_at_a_time_steps_per_s": total / sequential,
+            "speedup_vs_one_at_a_time": sequential / elapsed,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def batched(args):

@@ -241,10 +241,17 @@ def delta_coefficients(lmax_field, nodes, seed=2026, rms=0.7, ar=0.9, lmax=None)
 class State:
     """Viscoelastic3D::State on the device (one Maxwell branch)."""
 
-    def __init__(self, op):
+    def __init__(self, op, _from=None):
         self.op = op
         self.handle = ctypes.c_void_p()
-        check(lib.sl3_state_zero(op.handle, ctypes.byref(self.handle)))
+        if _from is None:
+            check(lib.sl3_state_zero(op.handle, ctypes.byref(self.handle)))
+        else:
+            check(lib.sl3_state_clone(op.handle, _from.handle, ctypes.byref(self.handle)))
+
+    def clone(self):
+        """A device copy of the state (sl3_state_clone)."""
+        return State(self.op, _from=self)
 
     def __del__(self, _lib=lib):  # the default keeps the library through shutdown
         if getattr(self, "handle", None):
