@@ -67,7 +67,8 @@ def parse():
                     help="order of the extrapolated-start trajectory leg (0 skips it)")
     ap.add_argument("--uneven", type=int, default=0,
                     help="an uneven batch of this many independent realisations (lmax from "
-                         "{32, 64, 96, 128}, meshes of {64, 96, 128} elements x degree 5, seeded), "
+                         "{32, 64, 96, 128, 192}, meshes of {64, 96, 128} elements x degree 5, "
+                         "seeded), "
                          "stepped --steps times each by 4 worker threads; value is their "
                          "aggregate steps/s")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -426,9 +427,8 @@ def uneven(args, workers=4, seed=48):
     l = min(64, lmax)), the benchmark's load, scheme and tolerance. All
     operators are set up first (untimed); then `workers` host threads (one
     library stream each) take realisations from a queue and step each
-    args.steps times from rest after args.warmup untimed steps. lmax 192 is
-    left out: its grid (P = 145) takes the generic longitude kernel, whose
-    parity tests run only at small lmax."""
+    args.steps times from rest after args.warmup untimed steps. (At lmax
+    192 the grid, P = 145, takes the generic longitude kernel.)"""
     import queue
     import threading
 
@@ -443,7 +443,7 @@ def uneven(args, workers=4, seed=48):
     cases = []
     t0 = time.perf_counter()
     for i in range(args.uneven):
-        lmax = int(rng.choice([32, 64, 96, 128]))
+        lmax = int(rng.choice([32, 64, 96, 128, 192]))
         elements = int(rng.choice([64, 96, 128]))
         earth = earths[elements]
         _, tscale, _ = earth.scales()
@@ -493,12 +493,17 @@ def uneven(args, workers=4, seed=48):
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    # the same steps one realisation after another, from copies of the states
+    # the same steps one realisation after another, each from a copy of its
+    # state (one copy alive at a time; only the steps are timed)
+    sequential = 0.0
     for c in cases:
-        c["copy"] = c["state"].clone()
-    sequential = run(1, "copy")
-    for c in cases:
-        del c["copy"]
+        copy = c["state"].clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            c["op"].step(copy, c["dt"], st.EXPONENTIAL_TRAPEZOID, c["load"])
+        sequential += time.perf_counter() - t0
+        del copy
     elapsed = run(workers, "state")
     total = len(cases) * args.steps
     log(f"uneven batch: one at a time {total / sequential:.1f} steps/s, {workers} workers "
@@ -519,7 +524,7 @@ def uneven(args, workers=4, seed=48):
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"{len(cases)} independent realisations, lmax from {{32, 64, 96, 128}} "
+            "workload": (f"{len(cases)} independent realisations, lmax from {{32, 64, 96, 128, 192}} "
                          "and meshes of {64, 96, 128} elements x degree 5 (seed 48), delta "
                          "seeds 3000.., each stepped on a worker thread's stream (4 workers)"),
             "realisations": [{"lmax": c["lmax"], "elements": c["elements"],

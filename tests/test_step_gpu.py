@@ -137,9 +137,11 @@ def run_pair(earth, oracle, lmax, delta_scale, steps, max_iterations, rtol, hist
 # all frequencies (H = 48), the workload's grid, with the complex blocks'
 # contraction fixed at compile time; lmax 96 (P = 73, H = 36) and 160
 # (P = 121, H = 60): three and four tiles with padding rows and a run-time
-# contraction length; lmax 2 (R = 2), 3 (P = 3: one frequency) and 4 (P = 4,
-# even): the smallest grids of each kernel
-@pytest.mark.parametrize("lmax", [2, 3, 4, 8, 10, 12, 32, 64, 96, 128, 160])
+# contraction length; lmax 192 (P = 145, H = 72: five tiles, beyond the
+# matrix-core kernels, so the generic longitude kernel at full size; 290
+# colatitudes in two synthesis blocks); lmax 2 (R = 2), 3 (P = 3: one
+# frequency) and 4 (P = 4, even): the smallest grids of each kernel
+@pytest.mark.parametrize("lmax", [2, 3, 4, 8, 10, 12, 32, 64, 96, 128, 160, 192])
 def test_one_pass_matches_oracle(earth, oracle, lmax):
     results = run_pair(earth, oracle, lmax, 0.7, steps=2, max_iterations=1, rtol=1e-14)
     for info, rinfo, c, rc, m, rm, d, rd in results:
