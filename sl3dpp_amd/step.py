@@ -197,8 +197,10 @@ def zonal_load_rate(lmax, earth):
     def load(l, t, columns, user):
         if l < 1 or l > lmax:
             return 0
-        for k in range(2 * (l + 1)):
-            columns[k] = 0.0
+        # the 2 (l + 1) columns zeroed in one call (an element-wise loop
+        # through the pointer costs about 1 ms per step at lmax 128, with
+        # the device idle before the step's start)
+        ctypes.memset(columns, 0, 16 * (l + 1))
         columns[0] = rate[l] * (t * time) / sigma
         return 1
 
