@@ -455,7 +455,34 @@ Viscoelastic3D::Viscoelastic3D(const Earth& earth, int lmax, const double* delta
   partials_.Resize(1024 * 19);  // the sweep's 16 sums and the residual's 3 maxima per block
   result_.Resize(19);
   for (auto& e : events_) SL3_HIP(hipEventCreate(&e));
+  SL3_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 32 * sizeof(double),
+                        hipHostMallocDefault));
+  SL3_HIP(hipEventCreateWithFlags(&readEvent_, hipEventDisableTiming));
   SL3_HIP(hipStreamSynchronize(s));
+}
+
+Viscoelastic3D::~Viscoelastic3D() {
+  for (auto& e : events_)
+    if (e != nullptr) (void)hipEventDestroy(e);
+  if (readEvent_ != nullptr) (void)hipEventDestroy(readEvent_);
+  if (pinned_ != nullptr) (void)hipHostFree(pinned_);
+}
+
+// Copies `count` doubles of a device result to the host through pinned
+// memory and waits by polling an event recorded behind the copy: the poll
+// returns as soon as the copy lands, where a stream synchronisation may
+// sleep until an interrupt wakes it (a pass reads its residual this way, and
+// the device idles until the host has launched the next work).
+void Viscoelastic3D::ReadBack(double* dst, const double* src, int count) {
+  if (count < 0 || count > 32) throw ArgumentError("read-back of at most 32 doubles");
+  hipStream_t s = Stream();
+  SL3_HIP(hipMemcpyAsync(pinned_, src, count * sizeof(double), hipMemcpyDeviceToHost, s));
+  SL3_HIP(hipEventRecord(readEvent_, s));
+  hipError_t status;
+  while ((status = hipEventQuery(readEvent_)) == hipErrorNotReady) {
+  }
+  SL3_HIP(status);
+  std::copy(pinned_, pinned_ + count, dst);
 }
 
 void Viscoelastic3D::SetDelta(const double* delta_dev) {
@@ -956,8 +983,7 @@ double Viscoelastic3D::RelativeChange(const double* next, const double* now, boo
                                result_.data(), s);
   }
   double r[3];
-  SL3_HIP(hipMemcpyAsync(r, result_.data(), sizeof(r), hipMemcpyDeviceToHost, s));
-  SL3_HIP(hipStreamSynchronize(s));
+  ReadBack(r, result_.data(), 3);
   if (r[2] != 0.0) return std::numeric_limits<double>::quiet_NaN();
   return r[0] > 0.0 ? std::sqrt(r[1]) / std::sqrt(r[0]) : 0.0;
 }
@@ -1214,8 +1240,7 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
       LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(),
                           result_.data() + 16, s);
       double r[19];
-      SL3_HIP(hipMemcpyAsync(r, result_.data(), sizeof(r), hipMemcpyDeviceToHost, s));
-      SL3_HIP(hipStreamSynchronize(s));
+      ReadBack(r, result_.data(), 19);
       std::copy(r, r + 16, sums);
       residual = r[18] != 0.0 ? std::numeric_limits<double>::quiet_NaN()
                  : r[16] > 0.0 ? std::sqrt(r[17]) / std::sqrt(r[16])
@@ -1263,8 +1288,7 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
         if (!fused) {
           sw = sweepOf(m);
           LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(), nullptr, s);
-          SL3_HIP(hipMemcpyAsync(sums, result_.data(), sizeof(sums), hipMemcpyDeviceToHost, s));
-          SL3_HIP(hipStreamSynchronize(s));
+          ReadBack(sums, result_.data(), 16);
         }
         count = m;
         for (int i = 0; i < m; ++i) {

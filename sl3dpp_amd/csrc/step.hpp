@@ -175,6 +175,9 @@ class Viscoelastic3D {
   // ([node][theta][phi], device), or null for none.
   Viscoelastic3D(const Earth& earth, int lmax, const double* delta_dev,
                  StepOptions options);
+  ~Viscoelastic3D();
+  Viscoelastic3D(const Viscoelastic3D&) = delete;
+  Viscoelastic3D& operator=(const Viscoelastic3D&) = delete;
   const Coupling& GetCoupling() const { return coupling_; }
   const StepOptions& Options() const { return options_; }
   StepOptions& MutableOptions() { return options_; }
@@ -300,6 +303,11 @@ class Viscoelastic3D {
   DeviceArray<double> history_, xPrev_, fPrev_;
   std::unique_ptr<DeviceState> stage_;  // Sdirk2's first stage
   hipEvent_t events_[4] = {};
+  // a pass's few result doubles come back through pinned memory, the host
+  // polling an event behind the copy (ReadBack)
+  double* pinned_ = nullptr;
+  hipEvent_t readEvent_ = nullptr;
+  void ReadBack(double* dst, const double* src, int count);
 };
 
 }  // namespace sl3amd
