@@ -1361,8 +1361,7 @@ double Viscoelastic3D::Pair(const double* x, const double* force, const double* 
   LaunchLateralPair(tasksDev_.data(), pairWork_.data(), pairWorkCount_, x, force, g, ch,
                     cgPartials_.data(), result_.data(), s);
   double v = 0.0;
-  SL3_HIP(hipMemcpyAsync(&v, result_.data(), sizeof(v), hipMemcpyDeviceToHost, s));
-  SL3_HIP(hipStreamSynchronize(s));
+  ReadBack(&v, result_.data(), 1);
   return v;
 }
 
