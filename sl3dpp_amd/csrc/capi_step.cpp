@@ -633,7 +633,13 @@ int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int
     const auto& st = *state->state;
     if (!st.hasCoefficients) return;
     int slot = -1;
-    if (ex->recorded.size() == ex->buffers.size()) {
+    if (!ex->recorded.empty() && ex->recorded.back().first == st.t) {
+      // a zero-length step ends where the last one did: its strain replaces
+      // that entry (two entries at one time would make the weights divide
+      // by zero)
+      slot = ex->recorded.back().second;
+      ex->recorded.pop_back();
+    } else if (ex->recorded.size() == ex->buffers.size()) {
       slot = ex->recorded.front().second;
       ex->recorded.erase(ex->recorded.begin());
     } else {

@@ -85,6 +85,17 @@ def test_extrapolated_trajectory_matches_the_plain_one(trajectory_problem, order
             b = warm_state.export(name)
             assert np.max(np.abs(a - b)) <= 1e-7 * np.max(np.abs(a)), (k, name)
     print(f"order {order}: plain passes {plain_passes}, extrapolated {warm_passes}")
+    # a zero-length step keeps one history entry per time: the next
+    # extrapolated step is finite and converges to the plain one's state
+    info = ws.step(warm_state, 0.0, st.EXPONENTIAL_TRAPEZOID, load)
+    op.step(plain_state, 0.0, st.EXPONENTIAL_TRAPEZOID, load)
+    assert info.converged
+    info = ws.step(warm_state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    op.step(plain_state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    assert info.converged and np.isfinite(info.residual)
+    a = plain_state.export("coefficients")
+    b = warm_state.export("coefficients")
+    assert np.max(np.abs(a - b)) <= 1e-7 * np.max(np.abs(a))
     # a state other than the last one stepped starts a new history: its
     # step takes the step's own start, the plain step's pass count
     other = op.zero_state()
