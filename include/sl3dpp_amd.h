@@ -270,7 +270,8 @@ int sl3_spectral_combine(const sl3_operator* op, int count, const double* coeffi
  * the states it stepped, and sl3_extrapolated_step starts each step from
  * their Lagrange extrapolation in time to the step's end (order 1..3; the
  * step's own start while fewer than two are kept, for Sdirk2, and when the
- * state is not the last one it stepped, which starts a new history). The
+ * state is not the last one it stepped or has moved since, which starts a
+ * new history). The
  * fixed point then converges to the same tolerance. guard as in
  * sl3_step_guarded. */
 typedef struct sl3_extrapolator sl3_extrapolator;

@@ -40,6 +40,7 @@ struct sl3_extrapolator {
   std::vector<sl3amd::DeviceArray<double>> buffers;
   sl3amd::DeviceArray<double> start;
   std::vector<std::pair<double, int>> recorded;  // (time, buffer)
+  const sl3_state* last = nullptr;                 // the state they are of
 };
 
 using namespace sl3amd;
@@ -596,6 +597,7 @@ int sl3_extrapolator_reset(sl3_extrapolator* ex) {
   return Guard([&] {
     Check(ex != nullptr, "null argument");
     ex->recorded.clear();
+    ex->last = nullptr;
   });
 }
 
@@ -607,8 +609,10 @@ int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int
     Check(ex != nullptr && state != nullptr, "null argument");
     Check(guard >= 0.0, "the warm-start guard must be non-negative");
     const double t0 = state->state->t;
-    // another state than the last recorded one starts a new trajectory
-    if (!ex->recorded.empty() && ex->recorded.back().first != t0) ex->recorded.clear();
+    // another state than the last one stepped, or the same one moved to
+    // another time since, starts a new history
+    if (!ex->recorded.empty() && (ex->last != state || ex->recorded.back().first != t0))
+      ex->recorded.clear();
     if (scheme == SL3_SDIRK2 || !(dt > 0.0) || ex->recorded.size() < 2) return;
     // Lagrange extrapolation through the recorded times to t0 + dt
     const double target = t0 + dt;
@@ -653,6 +657,7 @@ int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int
                Stream());
     SL3_HIP(hipStreamSynchronize(Stream()));
     ex->recorded.emplace_back(st.t, slot);
+    ex->last = state;
   });
 }
 

@@ -97,10 +97,18 @@ def test_extrapolated_trajectory_matches_the_plain_one(trajectory_problem, order
     b = warm_state.export("coefficients")
     assert np.max(np.abs(a - b)) <= 1e-7 * np.max(np.abs(a))
     # a state other than the last one stepped starts a new history: its
-    # step takes the step's own start, the plain step's pass count
+    # step takes the step's own start, the plain step's pass count, even
+    # when it is at the last one's time (a copy of it)
     other = op.zero_state()
     info = ws.step(other, dt, st.EXPONENTIAL_TRAPEZOID, load)
     assert info.iterations == plain_passes[0]
+    ws.step(warm_state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    ws.step(warm_state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    twin = warm_state.clone()
+    plain_twin = warm_state.clone()
+    own = op.step(plain_twin, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    info = ws.step(twin, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    assert info.iterations == own.iterations
     # the first two steps have no history; afterwards the start is better
     # (on this small Earth the higher orders gain less than at lmax 128,
     # where cubic is the best; the pass counts are checked to order 2)
