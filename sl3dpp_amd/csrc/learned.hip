@@ -371,8 +371,8 @@ size_t Round8(size_t n) { return (n + 7) & ~static_cast<size_t>(7); }
 }  // namespace
 
 // The slice depth of a GEMM shape: fixed, or on the shape's first use the
-// faster of the two, timed with device events over three launches each
-// after one untimed launch (every launch writes the same outputs from the
+// faster of the two, timed with device events over ten launches each after
+// one untimed launch (every launch writes the same outputs from the
 // same inputs, so the trial launches change nothing).
 template <bool KA, bool KB, int MODE>
 void Mlp::Product(int kind, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B,
@@ -387,7 +387,7 @@ void Mlp::Product(int kind, int M, int N, int K, const uint16_t* A, int lda, con
       for (int trial : {32, 128}) {
         Gemm<KA, KB, MODE>(trial, M, N, K, A, lda, B, ldb, e, s);
         SL3_HIP(hipEventRecord(tuneEvents_[0], s));
-        for (int r = 0; r < 3; ++r) Gemm<KA, KB, MODE>(trial, M, N, K, A, lda, B, ldb, e, s);
+        for (int r = 0; r < 10; ++r) Gemm<KA, KB, MODE>(trial, M, N, K, A, lda, B, ldb, e, s);
         SL3_HIP(hipEventRecord(tuneEvents_[1], s));
         SL3_HIP(hipEventSynchronize(tuneEvents_[1]));
         float ms = 0.0f;
