@@ -19,7 +19,7 @@
 #include "kernels.hpp"
 #include "lateral_cg.hpp"
 #include "model.hpp"
-#include "operators.hpp"
+#include "degree_systems.hpp"
 #include "response.hpp"
 #include "transform.hpp"
 
