@@ -161,6 +161,79 @@ int sl3_earth_degree_system(const sl3_earth* earth, int l, int kind, int scheme,
                             int* first_element, int* pinned, double* band,
                             int* dofs, double* load, double* tide);
 
+/* An Earth from its nodal material on a given mesh, as a caller holding the
+ * reference's Material with its RadialMesh (include/sl3dpp/Material.hpp,
+ * include/sl3dpp/RadialMesh.hpp) and Rheology::Maxwell's branch
+ * (include/sl3dpp/Rheology.hpp:100-190) passes them. SI; arrays per
+ * (element, GLL node), element-major, unless noted. At most one Maxwell
+ * branch. */
+typedef struct {
+  int elements, ngll;
+  const double* edges;       /* elements + 1 element boundaries (m), increasing */
+  const double* rho;         /* kg m^-3 */
+  const double* drho;        /* d rho / dr, kg m^-4 */
+  const double* gravity;     /* m s^-2 */
+  const double *A, *C, *F, *L, *N; /* transversely isotropic moduli, Pa */
+  const int* fluid;          /* per element */
+  int branches;              /* 0 (elastic) or 1 */
+  const double* branch_mu;   /* the branch's shear, Pa */
+  const double* branch_itau; /* mu / eta, 1/s */
+  double length, density;    /* the scales (m, kg m^-3); 0 for 6371 km and 5515 */
+} sl3_earth_nodal;
+int sl3_earth_create_nodal(const sl3_earth_nodal* spec, sl3_earth** out);
+
+/* DegreeOperator and ToroidalOperator (include/sl3dpp/DegreeOperator.hpp:
+ * 94-384, include/sl3dpp/ToroidalOperator.hpp:58-176): one degree's radial
+ * system, assembled on the host, factorised and solved on the GPU. */
+enum { SL3_SPHEROIDAL = 0, SL3_TOROIDAL = 1 };
+enum { SL3_MATERIAL_ELASTIC = 0, SL3_MATERIAL_LONG_TERM = 1 };
+enum { SL3_POINTS_GAUSS = 0, SL3_POINTS_GLL = 1 };
+/* DegreeOperator::Options (include/sl3dpp/DegreeOperator.hpp:101-131). */
+typedef struct {
+  double eps;       /* graded truncation level, 1e-8 */
+  int gravitating;  /* 1; 0 for the non-gravitating spheroidal system */
+  int material;     /* SL3_MATERIAL_ELASTIC: the unrelaxed material;
+                       SL3_MATERIAL_LONG_TERM: the Maxwell branch's shear
+                       removed (the step's systems) */
+  const double* point_shear; /* NULL, or a shear modulus (Pa) added at the
+                                memory points, (element, point) */
+  int point_set;    /* SL3_POINTS_GAUSS (ngll - 1 points, the step's) or
+                       SL3_POINTS_GLL (the ngll nodes) */
+} sl3_degree_options;
+void sl3_degree_options_default(sl3_degree_options* options);
+typedef struct sl3_degree_operator sl3_degree_operator;
+/* DegreeOperator(material, l, options) / ToroidalOperator(material, l,
+ * options) (src/DegreeOperator.cpp:24-55, src/ToroidalOperator.cpp:16-30);
+ * kind SL3_SPHEROIDAL or SL3_TOROIDAL. */
+int sl3_degree_operator_create(const sl3_earth* earth, int l, int kind,
+                               const sl3_degree_options* options,
+                               sl3_degree_operator** out);
+int sl3_degree_operator_destroy(sl3_degree_operator* op);
+/* DofCount, the band half-width, FirstElement and the pinned dof (-1 for
+ * none). */
+int sl3_degree_operator_shape(const sl3_degree_operator* op, int* ndof, int* kd,
+                              int* first_element, int* pinned);
+/* Host arrays by name: "band" (the unscaled upper band, (kd+1) x ndof),
+ * "scale" (Factorise's diagonal scaling), "dofs" (per global mesh node: U,
+ * V, phi for spheroidal, W for toroidal, -1 where absent, as doubles), and
+ * for the spheroidal system "load", "traction", "traction_tangential"
+ * (DegreeOperator::Load, Traction, TractionTangential,
+ * src/DegreeOperator.cpp:960-997) and "tide:<p>" (Tide(p), p = -1 for l,
+ * :999-1038). dst NULL returns the size through *count. */
+int sl3_degree_operator_export(const sl3_degree_operator* op, const char* name,
+                               double* dst, int64_t* count);
+/* DegreeOperator::Solve (src/DegreeOperator.cpp:861-920): B (ndof x nrhs,
+ * column-major, device) overwritten by X. The factorisation (Cholesky, LU
+ * where the system is not positive definite; src/DegreeOperator.cpp:
+ * 775-852) is made on the first call and kept; used_cholesky (may be NULL)
+ * says which. */
+int sl3_degree_operator_solve(sl3_degree_operator* op, double* b_dev, int nrhs,
+                              int* used_cholesky);
+/* DegreeOperator::Apply (src/DegreeOperator.cpp:946-949): y = A x with the
+ * unscaled system, nrhs column-major columns, host arrays. */
+int sl3_degree_operator_apply(const sl3_degree_operator* op, const double* x,
+                              int nrhs, double* y);
+
 /* ===================================================================== *
  * Viscoelastic3D (include/sl3dpp/Viscoelastic3D.hpp:89-675): the 3D      *
  * lateral-viscosity step on the GPU. One Maxwell branch, Gauss memory    *
@@ -204,7 +277,8 @@ typedef struct sl3_state sl3_state;
 void sl3_step_options_default(sl3_step_options* options);
 /* delta_dev: delta(r, theta, phi) sampled on every memory node's grid,
  * [node][theta][phi] on the device (viscosity eta_ref exp(delta)), or NULL
- * for none (src/Viscoelastic3D.cpp:226-287). The earth must outlive it. */
+ * for none (src/Viscoelastic3D.cpp:226-287). The operator shares the
+ * Earth: the earth handle may be destroyed first. */
 int sl3_operator_create(const sl3_earth* earth, int lmax, const double* delta_dev,
                         const sl3_step_options* options, sl3_operator** out);
 int sl3_operator_destroy(sl3_operator* op);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -15,12 +16,9 @@
 #include "learned.hpp"
 #include "step.hpp"
 
-struct sl3_earth {
-  std::unique_ptr<sl3amd::Earth> earth;
-};
+#include "capi_handles.hpp"
 
 struct sl3_operator {
-  const sl3_earth* earth = nullptr;
   std::unique_ptr<sl3amd::Viscoelastic3D> op;
   // per degree cut: the device-layout offsets of the rows of degree <= cut
   // (the learned warm start's gather and scatter)
@@ -30,6 +28,10 @@ struct sl3_operator {
 
 struct sl3_state {
   std::unique_ptr<sl3amd::DeviceState> state;
+  // the operator whose grid and spectral sizes the state's buffers have:
+  // every entry point taking both checks it before any launch
+  const sl3_operator* owner = nullptr;
+  uint64_t id = 0;  // unique per state made (zero or clone)
 };
 
 // The converged strain coefficients of a trajectory's last steps (device
@@ -40,7 +42,7 @@ struct sl3_extrapolator {
   std::vector<sl3amd::DeviceArray<double>> buffers;
   sl3amd::DeviceArray<double> start;
   std::vector<std::pair<double, int>> recorded;  // (time, buffer)
-  const sl3_state* last = nullptr;                 // the state they are of
+  uint64_t last = 0;  // the id of the state they are of (0: none)
 };
 
 using namespace sl3amd;
@@ -49,6 +51,14 @@ namespace {
 
 void Check(bool ok, const std::string& what) {
   if (!ok) throw ArgumentError(what);
+}
+
+std::atomic<uint64_t> g_nextStateId{1};
+
+// A state's buffers are sized by the operator that made it.
+void CheckOwner(const sl3_operator* op, const sl3_state* state) {
+  Check(op != nullptr && state != nullptr, "null argument");
+  Check(state->owner == op, "the state belongs to another operator");
 }
 
 StepOptions ToOptions(const sl3_step_options* o) {
@@ -289,8 +299,7 @@ int sl3_operator_create(const sl3_earth* earth, int lmax, const double* delta_de
     Check(earth != nullptr && out != nullptr, "null argument");
     Check(lmax >= 2, "lmax >= 2");
     auto op = std::make_unique<sl3_operator>();
-    op->earth = earth;
-    op->op = std::make_unique<Viscoelastic3D>(*earth->earth, lmax, delta_dev,
+    op->op = std::make_unique<Viscoelastic3D>(earth->earth, lmax, delta_dev,
                                               ToOptions(options));
     *out = op.release();
   });
@@ -354,6 +363,8 @@ int sl3_state_zero(sl3_operator* op, sl3_state** out) {
     Check(op != nullptr && out != nullptr, "null argument");
     auto s = std::make_unique<sl3_state>();
     s->state = op->op->ZeroState();
+    s->owner = op;
+    s->id = g_nextStateId++;
     SL3_HIP(hipStreamSynchronize(Stream()));
     *out = s.release();
   });
@@ -361,9 +372,12 @@ int sl3_state_zero(sl3_operator* op, sl3_state** out) {
 
 int sl3_state_clone(const sl3_operator* op, const sl3_state* state, sl3_state** out) {
   return Guard([&] {
-    Check(op != nullptr && state != nullptr && out != nullptr, "null argument");
+    Check(out != nullptr, "null argument");
+    CheckOwner(op, state);
     auto s = std::make_unique<sl3_state>();
     s->state = op->op->CloneState(*state->state);
+    s->owner = op;
+    s->id = g_nextStateId++;
     *out = s.release();
   });
 }
@@ -391,7 +405,7 @@ int sl3_step_guarded(sl3_operator* op, sl3_state* state, double dt, int scheme,
                      sl3_load_fn load, sl3_load_fn tide, void* user, const double* start_dev,
                      double guard, sl3_step_info* info) {
   return Guard([&] {
-    Check(op != nullptr && state != nullptr, "null argument");
+    CheckOwner(op, state);
     auto wrap = [user](sl3_load_fn fn) {
       LoadFn out;
       if (fn) {
@@ -425,7 +439,8 @@ int sl3_step_guarded(sl3_operator* op, sl3_state* state, double dt, int scheme,
 int sl3_state_export(const sl3_operator* op, const sl3_state* state, const char* name,
                      double* dst, int64_t* count) {
   return Guard([&] {
-    Check(op != nullptr && state != nullptr && name != nullptr, "null argument");
+    Check(name != nullptr, "null argument");
+    CheckOwner(op, state);
     const auto& c = op->op->GetCoupling();
     const auto& st = *state->state;
     const std::string key(name);
@@ -597,7 +612,7 @@ int sl3_extrapolator_reset(sl3_extrapolator* ex) {
   return Guard([&] {
     Check(ex != nullptr, "null argument");
     ex->recorded.clear();
-    ex->last = nullptr;
+    ex->last = 0;
   });
 }
 
@@ -607,11 +622,12 @@ int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int
   const double* start = nullptr;
   int status = Guard([&] {
     Check(ex != nullptr && state != nullptr, "null argument");
+    CheckOwner(ex->op, state);
     Check(guard >= 0.0, "the warm-start guard must be non-negative");
     const double t0 = state->state->t;
     // another state than the last one stepped, or the same one moved to
     // another time since, starts a new history
-    if (!ex->recorded.empty() && (ex->last != state || ex->recorded.back().first != t0))
+    if (!ex->recorded.empty() && (ex->last != state->id || ex->recorded.back().first != t0))
       ex->recorded.clear();
     if (scheme == SL3_SDIRK2 || !(dt > 0.0) || ex->recorded.size() < 2) return;
     // Lagrange extrapolation through the recorded times to t0 + dt
@@ -657,7 +673,7 @@ int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int
                Stream());
     SL3_HIP(hipStreamSynchronize(Stream()));
     ex->recorded.emplace_back(st.t, slot);
-    ex->last = state;
+    ex->last = state->id;
   });
 }
 

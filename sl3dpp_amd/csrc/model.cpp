@@ -150,15 +150,11 @@ double MemoryPoints::RadiusAt(const RadialMesh& mesh, int e, int p) const {
   return r;
 }
 
-MemoryPoints MemoryPoints::Gauss(const RadialMesh& mesh) {
+MemoryPoints MemoryPoints::At(const RadialMesh& mesh, const std::vector<double>& x,
+                              const std::vector<double>& w) {
   MemoryPoints out;
   out.ngll = mesh.NGll();
-  out.count = out.ngll - 1;
-  std::vector<double> x, w;
-  GaussLegendre(out.count, x, w);
-  // ascending on [-1, 1]
-  std::reverse(x.begin(), x.end());
-  std::reverse(w.begin(), w.end());
+  out.count = static_cast<int>(x.size());
   out.xi = x;
   out.weight = w;
   const auto& nodes = mesh.Points();
@@ -182,6 +178,19 @@ MemoryPoints MemoryPoints::Gauss(const RadialMesh& mesh) {
   return out;
 }
 
+MemoryPoints MemoryPoints::Gauss(const RadialMesh& mesh) {
+  std::vector<double> x, w;
+  GaussLegendre(mesh.NGll() - 1, x, w);
+  // ascending on [-1, 1]
+  std::reverse(x.begin(), x.end());
+  std::reverse(w.begin(), w.end());
+  return At(mesh, x, w);
+}
+
+MemoryPoints MemoryPoints::Gll(const RadialMesh& mesh) {
+  return At(mesh, mesh.Points(), mesh.Weights());
+}
+
 std::vector<double> Reference1D::PointShearModulus(int scheme, double dt) const {
   if (dt == 0.0) return elasticShear;
   std::vector<double> s(pointMu.size(), 0.0);
@@ -200,8 +209,54 @@ std::vector<double> Reference1D::PointShearModulus(int scheme, double dt) const 
   return s;
 }
 
-std::unique_ptr<Earth> BuildEarth(const EarthSpec& spec) {
-  auto earth = std::make_unique<Earth>();
+namespace {
+
+// The 1D reference's point quantities and the long-term material, from the
+// nodal material and the Maxwell branch.
+void BuildReference(Earth& earth) {
+  const RadialMesh& mesh = *earth.mesh;
+  const int n = mesh.NGll();
+  const int E = mesh.ElementCount();
+  const size_t size = static_cast<size_t>(E) * n;
+  const Material& mat = earth.material;
+  const Rheology& rh = earth.rheology;
+  Reference1D& ref = earth.reference;
+  ref.points = MemoryPoints::Gauss(mesh);
+  const int np = ref.points.count;
+  auto interpolate = [&](const std::vector<double>& nodal) {
+    std::vector<double> out(static_cast<size_t>(E) * np, 0.0);
+    for (int e = 0; e < E; ++e) {
+      for (int p = 0; p < np; ++p) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) {
+          s += ref.points.Value(p, i) * nodal[static_cast<size_t>(e) * n + i];
+        }
+        out[static_cast<size_t>(e) * np + p] = std::max(s, 0.0);
+      }
+    }
+    return out;
+  };
+  ref.pointMu = interpolate(rh.branchMu);
+  ref.pointItau = interpolate(rh.branchItau);
+  // Rheology::LongTermMaterial (src/Rheology.cpp:460-485): each branch's
+  // shear removed through the isotropic perturbation rule.
+  ref.nodal = mat;
+  ref.nodal.mu = rh.muLongTerm;
+  for (size_t i = 0; i < size; ++i) {
+    const double delta = (0.0 - 1.0) * rh.branchMu[i];
+    ref.nodal.A[i] += (4.0 / 3.0) * delta;
+    ref.nodal.C[i] += (4.0 / 3.0) * delta;
+    ref.nodal.F[i] -= (2.0 / 3.0) * delta;
+    ref.nodal.L[i] += delta;
+    ref.nodal.N[i] += delta;
+  }
+  ref.elasticShear = ref.pointMu;
+}
+
+}  // namespace
+
+std::shared_ptr<const Earth> BuildEarth(const EarthSpec& spec) {
+  auto earth = std::make_shared<Earth>();
   earth->spec = spec;
   const Scales& sc = earth->scales;
   const double L = sc.length;
@@ -312,37 +367,82 @@ std::unique_ptr<Earth> BuildEarth(const EarthSpec& spec) {
     }
   }
 
-  Reference1D& ref = earth->reference;
-  ref.points = MemoryPoints::Gauss(mesh);
-  const int np = ref.points.count;
-  auto interpolate = [&](const std::vector<double>& nodal) {
-    std::vector<double> out(static_cast<size_t>(E) * np, 0.0);
-    for (int e = 0; e < E; ++e) {
-      for (int p = 0; p < np; ++p) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) {
-          s += ref.points.Value(p, i) * nodal[static_cast<size_t>(e) * n + i];
-        }
-        out[static_cast<size_t>(e) * np + p] = std::max(s, 0.0);
-      }
-    }
-    return out;
-  };
-  ref.pointMu = interpolate(rh.branchMu);
-  ref.pointItau = interpolate(rh.branchItau);
-  // Rheology::LongTermMaterial (src/Rheology.cpp:460-485): each branch's
-  // shear removed through the isotropic perturbation rule.
-  ref.nodal = mat;
-  ref.nodal.mu = rh.muLongTerm;
-  for (size_t i = 0; i < size; ++i) {
-    const double delta = (0.0 - 1.0) * rh.branchMu[i];
-    ref.nodal.A[i] += (4.0 / 3.0) * delta;
-    ref.nodal.C[i] += (4.0 / 3.0) * delta;
-    ref.nodal.F[i] -= (2.0 / 3.0) * delta;
-    ref.nodal.L[i] += delta;
-    ref.nodal.N[i] += delta;
+  BuildReference(*earth);
+  return earth;
+}
+
+std::shared_ptr<const Earth> BuildEarth(const NodalEarthSpec& spec) {
+  const int n = spec.ngll;
+  const int E = static_cast<int>(spec.edges.size()) - 1;
+  if (E < 1) throw ArgumentError("at least one element");
+  if (n < 2 || n > 16) throw ArgumentError("ngll in [2, 16]");
+  const size_t size = static_cast<size_t>(E) * n;
+  for (const auto* v : {&spec.rho, &spec.drho, &spec.g, &spec.A, &spec.C, &spec.F, &spec.L, &spec.N}) {
+    if (v->size() != size) throw ArgumentError("nodal arrays hold elements x ngll values");
   }
-  ref.elasticShear = ref.pointMu;
+  if (spec.fluid.size() != static_cast<size_t>(E)) throw ArgumentError("one fluid flag per element");
+  const bool maxwell = !spec.branchMu.empty();
+  if (maxwell && (spec.branchMu.size() != size || spec.branchItau.size() != size)) {
+    throw ArgumentError("the Maxwell branch holds elements x ngll values");
+  }
+  if (!(spec.length > 0.0 && spec.density > 0.0)) throw ArgumentError("positive scales");
+  auto earth = std::make_shared<Earth>();
+  Scales& sc = earth->scales;
+  sc.length = spec.length;
+  sc.density = spec.density;
+  const double L = sc.length;
+  std::vector<double> edges(spec.edges);
+  for (auto& r : edges) r /= L;
+  earth->mesh = std::make_shared<RadialMesh>(edges, n);
+  const RadialMesh& mesh = *earth->mesh;
+  Material& mat = earth->material;
+  mat.mesh = earth->mesh;
+  const double gScale = L / (sc.Time() * sc.Time());
+  const double modulus = sc.Modulus();
+  mat.rho.resize(size);
+  mat.drho.resize(size);
+  mat.g.resize(size);
+  mat.A.resize(size);
+  mat.C.resize(size);
+  mat.F.resize(size);
+  mat.L.resize(size);
+  mat.N.resize(size);
+  mat.mu.resize(size);
+  mat.kappa.resize(size);
+  for (size_t i = 0; i < size; ++i) {
+    mat.rho[i] = spec.rho[i] / sc.density;
+    mat.drho[i] = spec.drho[i] * L / sc.density;
+    mat.g[i] = spec.g[i] / gScale;
+    mat.A[i] = spec.A[i] / modulus;
+    mat.C[i] = spec.C[i] / modulus;
+    mat.F[i] = spec.F[i] / modulus;
+    mat.L[i] = spec.L[i] / modulus;
+    mat.N[i] = spec.N[i] / modulus;
+    // the Voigt shear and bulk moduli of the transversely isotropic ones
+    mat.mu[i] = (mat.C[i] + mat.A[i] - 2.0 * mat.F[i] + 5.0 * mat.N[i] + 6.0 * mat.L[i]) / 15.0;
+    mat.kappa[i] = (mat.C[i] + 4.0 * mat.A[i] + 4.0 * mat.F[i] - 4.0 * mat.N[i]) / 9.0;
+  }
+  mat.fluid.assign(spec.fluid.begin(), spec.fluid.end());
+  if (mat.fluid[E - 1]) throw ArgumentError("the surface element must be solid");
+  mat.radius = mesh.Right(E - 1);
+  mat.surfaceGravity = mat.g.back();
+  Rheology& rh = earth->rheology;
+  rh.branchMu.assign(size, 0.0);
+  rh.branchItau.assign(size, 0.0);
+  rh.active.assign(size, 0);
+  if (maxwell) {
+    for (size_t i = 0; i < size; ++i) {
+      rh.branchMu[i] = spec.branchMu[i] / modulus;
+      rh.branchItau[i] = spec.branchItau[i] * sc.Time();
+      if (rh.branchMu[i] > 0.0 && rh.branchItau[i] > 0.0) rh.active[i] = 1;
+    }
+  }
+  for (int e = 0; e < E; ++e) {
+    if (mat.IsFluid(e) && rh.ElementActive(mesh, e)) throw ArgumentError("a fluid element cannot relax");
+  }
+  rh.muLongTerm = mat.mu;
+  for (size_t i = 0; i < size; ++i) rh.muLongTerm[i] -= rh.branchMu[i];
+  BuildReference(*earth);
   return earth;
 }
 

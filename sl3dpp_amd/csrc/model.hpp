@@ -52,6 +52,9 @@ class RadialMesh {
  public:
   RadialMesh(const std::vector<double>& breakpoints,
              const std::vector<int>& elementsPerSpan, int ngll);
+  // One element between each pair of consecutive edges.
+  RadialMesh(const std::vector<double>& edges, int ngll)
+      : RadialMesh(edges, std::vector<int>(edges.empty() ? 0 : edges.size() - 1, 1), ngll) {}
   int ElementCount() const { return static_cast<int>(left_.size()); }
   int NGll() const { return ngll_; }
   int NodeCount() const { return ElementCount() * (ngll_ - 1) + 1; }
@@ -110,7 +113,13 @@ struct MemoryPoints {
   double Value(int p, int i) const { return value[p * ngll + i]; }
   double Derivative(int p, int i) const { return derivative[p * ngll + i]; }
   double RadiusAt(const RadialMesh& mesh, int e, int p) const;
+  // The element's GLL Lagrange basis at the points xi (ascending, in
+  // [-1, 1]) with quadrature weights w.
+  static MemoryPoints At(const RadialMesh& mesh, const std::vector<double>& xi,
+                         const std::vector<double>& w);
   static MemoryPoints Gauss(const RadialMesh& mesh);
+  // The GLL nodes themselves (MemoryPointSet::Gll).
+  static MemoryPoints Gll(const RadialMesh& mesh);
 };
 
 // The 1D reference's quantities at the memory points
@@ -137,7 +146,24 @@ struct Earth {
   Reference1D reference;
 };
 
-std::unique_ptr<Earth> BuildEarth(const EarthSpec& spec);
+std::shared_ptr<const Earth> BuildEarth(const EarthSpec& spec);
+
+// An Earth given by its nodal material on a given mesh, the reference's
+// Material and Rheology::Maxwell as a caller holds them
+// (include/sl3dpp/Material.hpp, include/sl3dpp/Rheology.hpp:100-190). SI
+// throughout; arrays per (element, GLL node) unless noted.
+struct NodalEarthSpec {
+  std::vector<double> edges;  // element boundaries, elements + 1, increasing
+  int ngll = 5;
+  std::vector<double> rho, drho, g;  // kg m^-3, kg m^-4, m s^-2
+  std::vector<double> A, C, F, L, N;  // Pa
+  std::vector<char> fluid;            // per element
+  // At most one Maxwell branch: its shear (Pa) and inverse relaxation time
+  // mu / eta (1/s); empty for an elastic Earth.
+  std::vector<double> branchMu, branchItau;
+  double length = Scales::kRadius, density = Scales::kDensity;  // the scales
+};
+std::shared_ptr<const Earth> BuildEarth(const NodalEarthSpec& spec);
 
 // Gauss-Legendre nodes (descending x) and weights; GLL nodes (ascending)
 // and weights.

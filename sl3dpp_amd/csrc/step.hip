@@ -417,11 +417,12 @@ void Coupling::Analyse(const double* const grid[3], double* spectral,
 
 // ---- Viscoelastic3D ---------------------------------------------------------
 
-Viscoelastic3D::Viscoelastic3D(const Earth& earth, int lmax, const double* delta_dev,
-                               StepOptions options)
-    : earth_(&earth),
+Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
+                               const double* delta_dev, StepOptions options)
+    : earth_(std::move(earthPtr)),
       options_(options),
-      coupling_(earth, lmax, options.oversampling, options.eps) {
+      coupling_(*earth_, lmax, options.oversampling, options.eps) {
+  const Earth& earth = *earth_;
   hipStream_t s = Stream();
   const auto& geo = coupling_.Geometry();
   const int np = geo.np;

@@ -173,7 +173,9 @@ class Viscoelastic3D {
  public:
   // delta_dev: delta(r, theta, phi) sampled on every memory node's grid
   // ([node][theta][phi], device), or null for none.
-  Viscoelastic3D(const Earth& earth, int lmax, const double* delta_dev,
+  // The operator shares the Earth's ownership: the caller's handle may go
+  // first (the reference copies the rheology, src/Viscoelastic3D.cpp:229).
+  Viscoelastic3D(std::shared_ptr<const Earth> earth, int lmax, const double* delta_dev,
                  StepOptions options);
   ~Viscoelastic3D();
   Viscoelastic3D(const Viscoelastic3D&) = delete;
@@ -258,7 +260,7 @@ class Viscoelastic3D {
     return hasTide_ ? amp_.data() + ampDoubles_ : nullptr;
   }
 
-  const Earth* earth_;
+  std::shared_ptr<const Earth> earth_;
   StepOptions options_;
   Coupling coupling_;
   std::vector<double> muNodeHost_, itauRefHost_;

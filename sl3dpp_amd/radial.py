@@ -1,0 +1,341 @@
+"""Python mirror of the reference's radial set-up over the C ABI: an Earth from
+an elastic deck (EarthModel::FromDeck, RadialMesh::Create, Material::Create,
+src/EarthModel.cpp:86-200, src/RadialMesh.cpp:13-200), one degree's radial
+system (DegreeOperator / ToroidalOperator, include/sl3dpp/DegreeOperator.hpp:
+94-384) factorised and solved on the GPU, and the static responses built on
+it: the traction numbers (ComputeTractionNumbers, src/LoveNumbers.cpp:
+463-512) and the load Love numbers of a static elastic solve.
+
+Deck values between rows are joined linearly within a layer; the reference
+joins them by planetmodel's splines, which agree with that on piecewise-
+constant and piecewise-linear decks (the only ones used here)."""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import check, lib
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+G_SI = 6.6743e-11
+SPHEROIDAL, TOROIDAL = 0, 1
+MATERIAL_ELASTIC, MATERIAL_LONG_TERM = 0, 1
+POINTS_GAUSS, POINTS_GLL = 0, 1
+U, V, PHI = 0, 1, 2
+
+
+class NodalSpec(ctypes.Structure):
+    _fields_ = [
+        ("elements", ctypes.c_int),
+        ("ngll", ctypes.c_int),
+        ("edges", c_double_p),
+        ("rho", c_double_p),
+        ("drho", c_double_p),
+        ("gravity", c_double_p),
+        ("A", c_double_p),
+        ("C", c_double_p),
+        ("F", c_double_p),
+        ("L", c_double_p),
+        ("N", c_double_p),
+        ("fluid", c_int_p),
+        ("branches", ctypes.c_int),
+        ("branch_mu", c_double_p),
+        ("branch_itau", c_double_p),
+        ("length", ctypes.c_double),
+        ("density", ctypes.c_double),
+    ]
+
+
+class DegreeOptions(ctypes.Structure):
+    _fields_ = [
+        ("eps", ctypes.c_double),
+        ("gravitating", ctypes.c_int),
+        ("material", ctypes.c_int),
+        ("point_shear", c_double_p),
+        ("point_set", ctypes.c_int),
+    ]
+
+
+lib.sl3_degree_options_default.restype = None
+
+
+def _dp(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+# ---- decks ------------------------------------------------------------------
+
+def read_deck(path):
+    """An elastic deck's rows (radius m, rho, vpv, vsv, vph, vsh, eta), blank
+    lines and '#' comments skipped, split into layers where a radius
+    repeats (a discontinuity). Returns a list of (rows x 7) arrays."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if line:
+                rows.append([float(x) for x in line.split()])
+    data = np.asarray(rows, dtype=np.float64)
+    if data.ndim != 2 or data.shape[1] != 7:
+        raise ValueError(f"{path}: an elastic deck has 7 columns")
+    layers, start = [], 0
+    for i in range(1, len(data)):
+        if data[i, 0] == data[i - 1, 0]:
+            layers.append(data[start:i])
+            start = i
+    layers.append(data[start:])
+    return layers
+
+
+class DeckModel:
+    """EarthModel::FromDeck for an elastic deck: per layer the columns joined
+    linearly, moduli A = rho vph^2, C = rho vpv^2, L = rho vsv^2,
+    N = rho vsh^2, F = eta (A - 2L), gravity from the density's mass
+    integral (SI)."""
+
+    def __init__(self, layers):
+        self.layers = layers
+        self.boundaries = [layers[0][0, 0]] + [ly[-1, 0] for ly in layers]
+        self.radius = self.boundaries[-1]
+        # mass below each layer's bottom: exact for a linear density per row
+        self.mass_below = [0.0]
+        for ly in layers:
+            self.mass_below.append(self.mass_below[-1] + self._mass(ly, ly[0, 0], ly[-1, 0]))
+
+    @staticmethod
+    def _mass(ly, r0, r1):
+        """4 pi int_r0^r1 rho r^2 dr over one layer, rho linear per row."""
+        total = 0.0
+        for k in range(len(ly) - 1):
+            a, b = max(ly[k, 0], r0), min(ly[k + 1, 0], r1)
+            if b <= a:
+                continue
+            x0, x1 = ly[k, 0], ly[k + 1, 0]
+            s = (ly[k + 1, 1] - ly[k, 1]) / (x1 - x0)
+            c = ly[k, 1] - s * x0  # rho = c + s r
+            total += 4.0 * math.pi * (c * (b ** 3 - a ** 3) / 3.0 + s * (b ** 4 - a ** 4) / 4.0)
+        return total
+
+    def layer_of(self, r, above):
+        """The layer holding r; at a boundary the one above or below."""
+        for i in range(len(self.layers)):
+            lo, hi = self.boundaries[i], self.boundaries[i + 1]
+            if (lo <= r < hi) if above else (lo < r <= hi):
+                return i
+        return 0 if r <= self.boundaries[0] else len(self.layers) - 1
+
+    def values(self, layer, r):
+        """(rho, drho/dr, vpv, vsv, vph, vsh, eta) in `layer` at r."""
+        ly = self.layers[layer]
+        k = int(np.clip(np.searchsorted(ly[:, 0], r, side="right") - 1, 0, len(ly) - 2)) \
+            if len(ly) > 1 else 0
+        if len(ly) == 1:
+            row = ly[0]
+            return (row[1], 0.0, *row[2:])
+        x0, x1 = ly[k, 0], ly[k + 1, 0]
+        t = (r - x0) / (x1 - x0)
+        row = ly[k] + t * (ly[k + 1] - ly[k])
+        drho = (ly[k + 1, 1] - ly[k, 1]) / (x1 - x0)
+        return (row[1], drho, *row[2:])
+
+    def gravity(self, layer, r):
+        if r <= 0.0:
+            return 0.0
+        ly = self.layers[layer]
+        m = self.mass_below[layer] + self._mass(ly, ly[0, 0], r)
+        return G_SI * m / (r * r)
+
+    def is_fluid(self, layer):
+        ly = self.layers[layer]
+        return bool(np.all(ly[:, 3] == 0.0) and np.all(ly[:, 5] == 0.0))
+
+
+def mesh_edges(model, lmax=None, drmax=None):
+    """RadialMesh::Create(model, {lmax} or {drmax}): every model boundary an
+    element boundary, each span cut uniformly into elements no wider than
+    drmax = 0.1 a / (lmax + 1), in the model's non-dimensional radii (the
+    surface radius one), returned in metres with each element's layer."""
+    if (lmax is None) == (drmax is None):
+        raise ValueError("give exactly one of lmax and drmax")
+    a = model.radius
+    b = [x / a for x in model.boundaries]
+    step = 0.1 * b[-1] / (lmax + 1) if lmax is not None else drmax / a
+    edges, layer = [b[0]], []
+    for s in range(len(b) - 1):
+        lo, hi = b[s], b[s + 1]
+        nel = max(1, math.ceil((hi - lo) / step))
+        for e in range(1, nel + 1):
+            t = e / nel
+            edges.append(hi if e == nel else (1.0 - t) * lo + t * hi)
+            layer.append(s)
+    return [x * a for x in edges], layer
+
+
+def gll_nodes(n):
+    """GLL nodes on [-1, 1], ascending (numpy's Legendre derivative roots)."""
+    N = n - 1
+    inner = np.polynomial.legendre.Legendre.basis(N).deriv().roots()
+    return np.concatenate([[-1.0], np.sort(inner.real), [1.0]])
+
+
+class Earth:
+    """An Earth handle from nodal arrays (sl3_earth_create_nodal)."""
+
+    def __init__(self, edges, ngll, rho, drho, gravity, A, C, F, L, N, fluid,
+                 branch_mu=None, branch_itau=None, length=0.0, density=0.0):
+        self.ngll = ngll
+        self.edges = np.ascontiguousarray(edges, dtype=np.float64)
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64)
+                      for x in (rho, drho, gravity, A, C, F, L, N)]
+        self._fluid = np.ascontiguousarray(fluid, dtype=np.int32)
+        spec = NodalSpec()
+        spec.elements = len(self.edges) - 1
+        spec.ngll = ngll
+        spec.edges = _dp(self.edges)
+        (spec.rho, spec.drho, spec.gravity, spec.A, spec.C, spec.F, spec.L,
+         spec.N) = [_dp(x) for x in self._keep]
+        spec.fluid = self._fluid.ctypes.data_as(c_int_p)
+        spec.branches = 0
+        if branch_mu is not None:
+            self._branch = [np.ascontiguousarray(branch_mu, dtype=np.float64),
+                            np.ascontiguousarray(branch_itau, dtype=np.float64)]
+            spec.branches = 1
+            spec.branch_mu, spec.branch_itau = [_dp(x) for x in self._branch]
+        spec.length = length
+        spec.density = density
+        h = ctypes.c_void_p()
+        check(lib.sl3_earth_create_nodal(ctypes.byref(spec), ctypes.byref(h)))
+        self.handle = h
+        t, L_, m = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib.sl3_earth_scales(self.handle, ctypes.byref(L_), ctypes.byref(t), ctypes.byref(m)))
+        self.length, self.time, self.mass = L_.value, t.value, m.value
+
+    def __del__(self, _lib=lib):
+        if getattr(self, "handle", None):
+            _lib.sl3_earth_destroy(self.handle)
+            self.handle = None
+
+    @classmethod
+    def from_deck(cls, path, lmax=None, drmax=None, ngll=5):
+        """EarthModel::FromDeck + RadialMesh::Create + Material::Create."""
+        model = DeckModel(read_deck(path))
+        edges, layer = mesh_edges(model, lmax=lmax, drmax=drmax)
+        xi = gll_nodes(ngll)
+        E = len(edges) - 1
+        arrays = {k: np.zeros((E, ngll)) for k in ("rho", "drho", "g", "A", "C", "F", "L", "N")}
+        fluid = np.zeros(E, dtype=np.int32)
+        for e in range(E):
+            k = layer[e]
+            fluid[e] = model.is_fluid(k)
+            lo, hi = edges[e], edges[e + 1]
+            for i in range(ngll):
+                r = lo + (xi[i] + 1.0) * 0.5 * (hi - lo)
+                if i == 0:
+                    r = lo
+                if i == ngll - 1:
+                    r = hi
+                rho, drho, vpv, vsv, vph, vsh, eta = model.values(k, r)
+                A, C = rho * vph ** 2, rho * vpv ** 2
+                L, N = rho * vsv ** 2, rho * vsh ** 2
+                arrays["rho"][e, i], arrays["drho"][e, i] = rho, drho
+                arrays["g"][e, i] = model.gravity(k, r)
+                arrays["A"][e, i], arrays["C"][e, i] = A, C
+                arrays["L"][e, i], arrays["N"][e, i] = L, N
+                arrays["F"][e, i] = eta * (A - 2.0 * L)
+        return cls(edges, ngll, arrays["rho"], arrays["drho"], arrays["g"], arrays["A"],
+                   arrays["C"], arrays["F"], arrays["L"], arrays["N"], fluid,
+                   length=model.radius)
+
+
+class DegreeOperator:
+    """One degree's radial system (sl3_degree_operator_*): DegreeOperator or
+    ToroidalOperator of the reference, solved on the GPU."""
+
+    def __init__(self, earth, l, kind=SPHEROIDAL, gravitating=True, material=MATERIAL_ELASTIC,
+                 eps=1e-8, point_shear=None, point_set=POINTS_GAUSS):
+        self.earth = earth
+        self.l = l
+        self.kind = kind
+        o = DegreeOptions()
+        lib.sl3_degree_options_default(ctypes.byref(o))
+        o.eps = eps
+        o.gravitating = int(gravitating)
+        o.material = material
+        o.point_set = point_set
+        if point_shear is not None:
+            self._shear = np.ascontiguousarray(point_shear, dtype=np.float64)
+            o.point_shear = _dp(self._shear)
+        h = ctypes.c_void_p()
+        check(lib.sl3_degree_operator_create(earth.handle, l, kind, ctypes.byref(o),
+                                             ctypes.byref(h)))
+        self.handle = h
+        n, kd, fe, pin = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib.sl3_degree_operator_shape(self.handle, ctypes.byref(n), ctypes.byref(kd),
+                                            ctypes.byref(fe), ctypes.byref(pin)))
+        self.ndof, self.kd, self.first_element, self.pinned = n.value, kd.value, fe.value, pin.value
+
+    def __del__(self, _lib=lib):
+        if getattr(self, "handle", None):
+            _lib.sl3_degree_operator_destroy(self.handle)
+            self.handle = None
+
+    def export(self, name):
+        count = ctypes.c_int64()
+        check(lib.sl3_degree_operator_export(self.handle, name.encode(), None, ctypes.byref(count)))
+        out = np.empty(count.value)
+        check(lib.sl3_degree_operator_export(self.handle, name.encode(), _dp(out),
+                                             ctypes.byref(count)))
+        return out
+
+    def dof(self, node, component=0):
+        dofs = self.export("dofs")
+        per = 3 if self.kind == SPHEROIDAL else 1
+        return int(dofs[node * per + component])
+
+    def solve(self, b_dev, nrhs):
+        """DegreeOperator::Solve on a device tensor (ndof x nrhs column-major,
+        i.e. a contiguous (nrhs, ndof) float64 tensor), in place."""
+        assert b_dev.is_cuda and b_dev.dtype.is_floating_point and b_dev.is_contiguous()
+        assert b_dev.numel() == self.ndof * nrhs
+        used = ctypes.c_int()
+        check(lib.sl3_degree_operator_solve(self.handle, ctypes.c_void_p(b_dev.data_ptr()), nrhs,
+                                            ctypes.byref(used)))
+        return bool(used.value)
+
+    def apply(self, x, nrhs):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        check(lib.sl3_degree_operator_apply(self.handle, _dp(x), nrhs, _dp(y)))
+        return y
+
+
+def traction_numbers(earth, lmax, gravitating, device="cuda"):
+    """ComputeTractionNumbers (src/LoveNumbers.cpp:463-512): per degree the
+    surface U and V (SI, metres per pascal) of the responses to a unit
+    normal traction (h_u, l_u) and a unit tangential traction (h_v, l_v),
+    each degree's pair solved on the GPU; NaN where the degree is excluded
+    (the non-gravitating degree 1)."""
+    import torch
+
+    h_factor = earth.length ** 2 * earth.time ** 2 / earth.mass  # M^-1 L^2 T^2
+    out = {k: np.full(lmax + 1, np.nan) for k in ("h_u", "l_u", "h_v", "l_v")}
+    nodes = (len(earth.edges) - 1) * (earth.ngll - 1) + 1
+    for l in range(lmax + 1):
+        if l == 1 and not gravitating:
+            continue
+        op = DegreeOperator(earth, l, gravitating=gravitating)
+        rhs = np.concatenate([op.export("traction"), op.export("traction_tangential")])
+        b = torch.tensor(rhs, dtype=torch.float64, device=device)
+        op.solve(b, 2)
+        x = b.cpu().numpy().reshape(2, op.ndof)
+        iu, iv = op.dof(nodes - 1, U), op.dof(nodes - 1, V)
+        val = lambda dof, col: x[col, dof] if dof >= 0 else 0.0  # noqa: E731
+        out["h_u"][l] = val(iu, 0) * h_factor
+        out["l_u"][l] = val(iv, 0) * h_factor
+        out["h_v"][l] = val(iu, 1) * h_factor
+        out["l_v"][l] = val(iv, 1) * h_factor
+    return out
