@@ -137,10 +137,9 @@ def cpu_baseline(op, earth, dt, load, passes_per_step, rtol):
     import sl3dpp_amd.step as st
 
     oracle = oracle_library()
-    itau = op.export("itau")
     lmax = op.lmax
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    p = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, itau,
+    p = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.delta_grid(),
                    theta=op.theta, wtheta=op.wtheta)
     amp = amplitudes(load, lmax, dt)
     cap = 2

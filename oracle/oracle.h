@@ -42,6 +42,8 @@ void sl3o_band_lu_solve(const double* ab, int n, int kl, int ku,
 double sl3o_phi1(double h, double e);
 void sl3o_trapezoid_weights(double h, double* e, double* alpha, double* beta);
 double sl3o_backward_euler_weight(double h);
+void sl3o_lateral_itau(int64_t nodes, int64_t per_node, const double* itau_ref_node,
+                       const double* delta, double* itau);
 
 /* ---- spin algebra: include/sl3dpp/SpinComponents.hpp:53-145 ------------ */
 /* values: U, V, W, rdU, rdV, rdW as (re, im) pairs (12 doubles);

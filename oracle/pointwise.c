@@ -30,6 +30,21 @@ void sl3o_trapezoid_weights(double h, double* e, double* alpha,
 /* include/sl3dpp/StepWeights.hpp:141-144 */
 double sl3o_backward_euler_weight(double h) { return 1.0 / (1.0 + h); }
 
+/* The lateral inverse relaxation time of Viscoelastic3D's constructor
+ * (src/Viscoelastic3D.cpp:250-283): exp(-delta) sampled on each memory
+ * node's grid scales the node's reference inverse time,
+ * itau(x) = itau_ref(node) exp(-delta(x)); per_node values per node, delta
+ * NULL for none. */
+void sl3o_lateral_itau(int64_t nodes, int64_t per_node, const double* itau_ref_node,
+                       const double* delta, double* itau) {
+  for (int64_t node = 0; node < nodes; ++node) {
+    for (int64_t k = 0; k < per_node; ++k) {
+      const int64_t x = node * per_node + k;
+      itau[x] = itau_ref_node[node] * (delta != NULL ? exp(-delta[x]) : 1.0);
+    }
+  }
+}
+
 /* complex helpers on (re, im) pairs */
 typedef struct {
   double re, im;

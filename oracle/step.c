@@ -375,91 +375,82 @@ static void solve(const task_t* t, double* x, int nrhs) {
 
 /* ---- force (src/GridCoupling.cpp:691-809) and solves --------------------- */
 
+/* The memory force of one task (src/GridCoupling.cpp:691-809): complex
+ * columns m = 0..l, ndof each (column-major), into b (zeroed first). */
+static void task_force(const sl3o_problem* p, const task_t* t, const double* weighted, double* b) {
+  const int l = t->l, cols = l + 1, nd = t->ndof, ngll = p->ngll;
+  memset(b, 0, sizeof(double) * 2 * (size_t)nd * cols);
+  const int fn = first_node(p, t->first_element);
+  for (int node = fn; node < p->nodes; ++node) {
+    const int q = node % p->np;
+    const int le = node / p->np;
+    const double r = p->radius[node], jac = p->jac[le];
+    for (int m = 0; m < cols; ++m) {
+      double A[10], rows[12];
+      gather(p, weighted, node, l, m, A);
+      sl3o_force_rows(l, A, r, rows);
+      const double point = jac * p->pweight[q] * r;
+      for (int i = 0; i < ngll; ++i) {
+        const double v = point * p->pvalue[q * ngll + i];
+        const double deriv = p->pweight[q] * p->pderiv[q * ngll + i] * r;
+        const int* d = t->dofs + (le * ngll + i) * 2;
+        /* value rows then derivative rows: U (or W) and V */
+        const int vrow[2] = {t->kind == 1 ? 4 : 0, 2};
+        const int drow[2] = {t->kind == 1 ? 10 : 6, 8};
+        for (int c = 0; c < (t->kind == 1 ? 1 : 2); ++c) {
+          if (d[c] < 0) continue;
+          double* z = b + 2 * ((size_t)m * nd + d[c]);
+          z[0] += v * rows[vrow[c]] + deriv * rows[drow[c]];
+          z[1] += v * rows[vrow[c] + 1] + deriv * rows[drow[c] + 1];
+        }
+      }
+    }
+  }
+}
+
+/* SolveForces (src/Viscoelastic3D.cpp:912-939) of one task's complex force
+ * columns as [Re | Im] real columns, the channels added (AddUnitChannels,
+ * :1029-1073: the load's unit response in column 0 of unit, the tide's in
+ * column 1), into sol (complex, column-major). */
+static void task_solve(const task_t* t, const double* b, const double* amp, const double* tamp,
+                       double* sol) {
+  const int l = t->l, cols = l + 1, nd = t->ndof;
+  double* x = (double*)malloc(sizeof(double) * 2 * nd * cols);
+  for (int c = 0; c < cols; ++c) {
+    for (int d = 0; d < nd; ++d) {
+      x[(size_t)c * nd + d] = b[2 * ((size_t)c * nd + d)];
+      x[(size_t)(cols + c) * nd + d] = b[2 * ((size_t)c * nd + d) + 1];
+    }
+  }
+  solve(t, x, 2 * cols);
+  for (int c = 0; c < cols; ++c) {
+    for (int d = 0; d < nd; ++d) {
+      double re = x[(size_t)c * nd + d], im = x[(size_t)(cols + c) * nd + d];
+      const size_t at = (size_t)l * (l + 1) / 2 + c;
+      if (t->kind == 0 && amp != NULL) {
+        re += amp[2 * at] * t->unit[d];
+        im += amp[2 * at + 1] * t->unit[d];
+      }
+      if (t->kind == 0 && tamp != NULL) {
+        re += tamp[2 * at] * t->unit[nd + d];
+        im += tamp[2 * at + 1] * t->unit[nd + d];
+      }
+      sol[2 * ((size_t)c * nd + d)] = re;
+      sol[2 * ((size_t)c * nd + d) + 1] = im;
+    }
+  }
+  free(x);
+}
+
 static void force_and_solve(const sl3o_problem* p, task_t* tasks, int ntasks,
                             const double* weighted, const double* amp, const double* tamp,
                             double** solutions) {
 #pragma omp parallel for schedule(dynamic)
   for (int ti = 0; ti < ntasks; ++ti) {
     task_t* t = &tasks[ti];
-    const int l = t->l, cols = l + 1, nd = t->ndof, ngll = p->ngll;
-    double* b = (double*)calloc((size_t)2 * nd * cols, sizeof(double)); /* complex, column-major */
-    const int fn = first_node(p, t->first_element);
-    for (int node = fn; node < p->nodes; ++node) {
-      const int e = p->elem0 + node / p->np, q = node % p->np;
-      const int le = e - p->elem0;
-      const double r = p->radius[node], jac = p->jac[le];
-      for (int m = 0; m < cols; ++m) {
-        double A[10], rows[12];
-        gather(p, weighted, node, l, m, A);
-        sl3o_force_rows(l, A, r, rows);
-        const double point = jac * p->pweight[q] * r;
-        for (int i = 0; i < ngll; ++i) {
-          const double v = point * p->pvalue[q * ngll + i];
-          const int* d = t->dofs + (le * ngll + i) * 2;
-          if (t->kind == 1) {
-            if (d[0] >= 0) {
-              b[2 * ((size_t)m * nd + d[0])] += v * rows[4];
-              b[2 * ((size_t)m * nd + d[0]) + 1] += v * rows[5];
-            }
-          } else {
-            if (d[0] >= 0) {
-              b[2 * ((size_t)m * nd + d[0])] += v * rows[0];
-              b[2 * ((size_t)m * nd + d[0]) + 1] += v * rows[1];
-            }
-            if (d[1] >= 0) {
-              b[2 * ((size_t)m * nd + d[1])] += v * rows[2];
-              b[2 * ((size_t)m * nd + d[1]) + 1] += v * rows[3];
-            }
-          }
-        }
-        for (int j = 0; j < ngll; ++j) {
-          const double deriv = p->pweight[q] * p->pderiv[q * ngll + j] * r;
-          const int* d = t->dofs + (le * ngll + j) * 2;
-          if (t->kind == 1) {
-            if (d[0] >= 0) {
-              b[2 * ((size_t)m * nd + d[0])] += deriv * rows[10];
-              b[2 * ((size_t)m * nd + d[0]) + 1] += deriv * rows[11];
-            }
-          } else {
-            if (d[0] >= 0) {
-              b[2 * ((size_t)m * nd + d[0])] += deriv * rows[6];
-              b[2 * ((size_t)m * nd + d[0]) + 1] += deriv * rows[7];
-            }
-            if (d[1] >= 0) {
-              b[2 * ((size_t)m * nd + d[1])] += deriv * rows[8];
-              b[2 * ((size_t)m * nd + d[1]) + 1] += deriv * rows[9];
-            }
-          }
-        }
-      }
-    }
-    /* SolveForces (src/Viscoelastic3D.cpp:912-939): [Re | Im] real columns */
-    double* x = (double*)malloc(sizeof(double) * 2 * nd * cols);
-    for (int c = 0; c < cols; ++c) {
-      for (int d = 0; d < nd; ++d) {
-        x[(size_t)c * nd + d] = b[2 * ((size_t)c * nd + d)];
-        x[(size_t)(cols + c) * nd + d] = b[2 * ((size_t)c * nd + d) + 1];
-      }
-    }
-    solve(t, x, 2 * cols);
-    for (int c = 0; c < cols; ++c) {
-      for (int d = 0; d < nd; ++d) {
-        double re = x[(size_t)c * nd + d], im = x[(size_t)(cols + c) * nd + d];
-        /* AddUnitChannels (:1029-1073): load column 0, tide column 1 */
-        const size_t at = (size_t)l * (l + 1) / 2 + c;
-        if (t->kind == 0 && amp != NULL) {
-          re += amp[2 * at] * t->unit[d];
-          im += amp[2 * at + 1] * t->unit[d];
-        }
-        if (t->kind == 0 && tamp != NULL) {
-          re += tamp[2 * at] * t->unit[nd + d];
-          im += tamp[2 * at + 1] * t->unit[nd + d];
-        }
-        solutions[ti][2 * ((size_t)c * nd + d)] = re;
-        solutions[ti][2 * ((size_t)c * nd + d) + 1] = im;
-      }
-    }
-    free(x);
+    double* b = (double*)malloc(sizeof(double) * 2 * (size_t)t->ndof * (t->l + 1));
+    task_force(p, t, weighted, b);
+    task_solve(t, b, amp, tamp, solutions[ti]);
     free(b);
   }
 }
@@ -756,4 +747,218 @@ done:
   }
   free(tasks);
   return status;
+}
+
+/* ---- the pieces of a pass, one at a time --------------------------------- */
+
+static void tasks_init(const sl3o_problem* p, task_t* tasks, int ntasks) {
+  for (int ti = 0; ti < ntasks; ++ti) {
+    task_t* t = &tasks[ti];
+    memset(t, 0, sizeof(*t));
+    t->kind = ti <= p->lmax ? 0 : 1;
+    t->l = ti <= p->lmax ? ti : ti - p->lmax;
+    t->ndof = p->ndof[ti];
+    t->kd = p->kd[ti];
+    t->first_element = p->first_element[ti];
+    t->pinned = p->pinned[ti];
+    t->dofs = p->dofs + p->dof_off[ti];
+  }
+}
+
+static void tasks_free(task_t* tasks, int ntasks) {
+  for (int ti = 0; ti < ntasks; ++ti) {
+    free(tasks[ti].scale);
+    free(tasks[ti].factor);
+    free(tasks[ti].piv);
+    free(tasks[ti].unit);
+  }
+  free(tasks);
+}
+
+int64_t sl3o_solution_doubles(const sl3o_problem* p) {
+  int64_t n = 0;
+  for (int ti = 0; ti < 2 * p->lmax + 1; ++ti) {
+    const int l = ti <= p->lmax ? ti : ti - p->lmax;
+    n += (int64_t)2 * p->ndof[ti] * (l + 1);
+  }
+  return n;
+}
+
+void sl3o_memory_force(const sl3o_problem* p, const double* weighted, double* forces) {
+  const int ntasks = 2 * p->lmax + 1;
+  task_t* tasks = (task_t*)calloc(ntasks, sizeof(task_t));
+  tasks_init(p, tasks, ntasks);
+  int64_t off = 0;
+  for (int ti = 0; ti < ntasks; ++ti) {
+    task_force(p, &tasks[ti], weighted, forces + off);
+    off += (int64_t)2 * tasks[ti].ndof * (tasks[ti].l + 1);
+  }
+  tasks_free(tasks, ntasks);
+}
+
+void sl3o_strain_of(const sl3o_problem* p, const double* solutions, double* coef) {
+  const int ntasks = 2 * p->lmax + 1;
+  task_t* tasks = (task_t*)calloc(ntasks, sizeof(task_t));
+  tasks_init(p, tasks, ntasks);
+  double** sol = (double**)malloc(sizeof(double*) * ntasks);
+  int64_t off = 0;
+  for (int ti = 0; ti < ntasks; ++ti) {
+    sol[ti] = (double*)solutions + off;
+    off += (int64_t)2 * tasks[ti].ndof * (tasks[ti].l + 1);
+  }
+  strain_of(p, tasks, sol, coef);
+  free(sol);
+  tasks_free(tasks, ntasks);
+}
+
+void sl3o_synthesise_tensor(const sl3o_problem* p, const double* coef, double* const grid[3]) {
+  tables_t tables;
+  tables_build(&tables, p);
+  g_tables = &tables;
+  synthesise_tensor(p, coef, grid);
+  g_tables = NULL;
+  tables_free(&tables);
+}
+
+void sl3o_analyse_tensor(const sl3o_problem* p, double* const grid[3], double* coef) {
+  tables_t tables;
+  tables_build(&tables, p);
+  g_tables = &tables;
+  analyse_tensor(p, grid, coef);
+  g_tables = NULL;
+  tables_free(&tables);
+}
+
+/* ---- the degree-basis stepper --------------------------------------------- */
+/* ViscoelasticOperator's 1D Maxwell step (src/Viscoelastic.cpp:914-1076:
+ * StartStep, SolveElastic, the memory update), one branch, spheroidal
+ * degrees only, at the memory points: the memory is held as the three
+ * deviatoric scalars per (degree, order, point),
+ *   M = V,  R = r U' - U + k2 V / 2,  S = U + r V' - V,
+ * the strain the same scalars of the solution, and the memory force is the
+ * point-shear form mu (k2 (k2 - 2) M M' + 4/3 R R' + k2 S S') with the test
+ * function's scalars M', R', S' (the 1D state's layout:
+ * tests/TestViscoelastic3D.cpp:154-170). No transform is involved: this is
+ * the path the 3D step must reproduce when delta = 0. */
+
+int64_t sl3o_scalar_doubles(const sl3o_problem* p) {
+  int64_t n = 0;
+  for (int l = 0; l <= p->lmax; ++l) n += (int64_t)p->nodes * (l + 1) * 3 * 2;
+  return n;
+}
+
+static int64_t scalar_offset(const sl3o_problem* p, int l) {
+  int64_t n = 0;
+  for (int k = 0; k < l; ++k) n += (int64_t)p->nodes * (k + 1) * 3 * 2;
+  return n;
+}
+
+/* The scalars of basis function i of a node's element at memory point q:
+ * coefficients of (M, R, S) for its U and for its V. */
+static void scalar_rows(const sl3o_problem* p, int q, int le, int i, double k2, double r,
+                        double rowU[3], double rowV[3]) {
+  const double v = p->pvalue[q * p->ngll + i];
+  const double b = r * p->pderiv[q * p->ngll + i] / p->jac[le];
+  rowU[0] = 0.0;
+  rowU[1] = b - v;
+  rowU[2] = v;
+  rowV[0] = v;
+  rowV[1] = 0.5 * k2 * v;
+  rowV[2] = b - v;
+}
+
+int sl3o_step_degree_basis(const sl3o_problem* p, sl3o_state1d* st, double dt, int scheme,
+                           const double* amp, double* solutions) {
+  const int L = p->lmax;
+  task_t* tasks = (task_t*)calloc(L + 1, sizeof(task_t));
+  tasks_init(p, tasks, L + 1);
+  int status = 0;
+  for (int l = 0; l <= L; ++l) {
+    if (factorise(&tasks[l], p->bands + p->band_off[l]) != 0) status = 1;
+  }
+  if (status != 0) {
+    tasks_free(tasks, L + 1);
+    return status;
+  }
+  if (dt == 0.0) scheme = 1;
+  int64_t soff = 0;
+  for (int l = 0; l <= L; ++l) {
+    task_t* t = &tasks[l];
+    const int cols = l + 1, nd = t->ndof, ngll = p->ngll;
+    const double k2 = l * (l + 1.0);
+    const double D[3] = {k2 * (k2 - 2.0), 4.0 / 3.0, k2};
+    double* m = st->m + scalar_offset(p, l);
+    double* d = st->d + scalar_offset(p, l);
+    t->unit = (double*)calloc((size_t)2 * nd, sizeof(double));
+    memcpy(t->unit, p->loads + p->load_off[l], sizeof(double) * nd);
+    solve(t, t->unit, 2);
+    const int fn = first_node(p, t->first_element);
+    double* beta = (double*)calloc(p->nodes, sizeof(double));
+    double* b = (double*)calloc((size_t)2 * nd * cols, sizeof(double));
+    /* StartStep: m~ = e m + alpha d0; the force of mu m~ */
+    for (int node = fn; node < p->nodes; ++node) {
+      const double h = dt * p->itau_ref_node[node];
+      double e, a, bb;
+      if (scheme == 1) {
+        e = sl3o_backward_euler_weight(h);
+        a = 0.0;
+        bb = 1.0 - e;
+      } else {
+        sl3o_trapezoid_weights(h, &e, &a, &bb);
+      }
+      beta[node] = bb;
+      const int q = node % p->np, le = node / p->np;
+      const double r = p->radius[node];
+      const double w = p->jac[le] * p->pweight[q] * p->mu_node[node];
+      for (int c = 0; c < cols; ++c) {
+        double* mm = m + ((int64_t)node * cols + c) * 6;
+        const double* d0 = d + ((int64_t)node * cols + c) * 6;
+        for (int k = 0; k < 6; ++k) mm[k] = e * mm[k] + (st->has_strain ? a * d0[k] : 0.0);
+        for (int i = 0; i < ngll; ++i) {
+          double rowU[3], rowV[3];
+          scalar_rows(p, q, le, i, k2, r, rowU, rowV);
+          const int* dof = t->dofs + (le * ngll + i) * 2;
+          for (int part = 0; part < 2; ++part) {
+            double fu = 0.0, fv = 0.0;
+            for (int k = 0; k < 3; ++k) {
+              fu += rowU[k] * D[k] * mm[2 * k + part];
+              fv += rowV[k] * D[k] * mm[2 * k + part];
+            }
+            if (dof[0] >= 0) b[2 * ((size_t)c * nd + dof[0]) + part] += w * fu;
+            if (dof[1] >= 0) b[2 * ((size_t)c * nd + dof[1]) + part] += w * fv;
+          }
+        }
+      }
+    }
+    double* x = solutions + soff;
+    task_solve(t, b, amp, NULL, x);
+    soff += (int64_t)2 * nd * cols;
+    /* the strain scalars and the memory update m = m~ + beta d */
+    for (int node = fn; node < p->nodes; ++node) {
+      const int q = node % p->np, le = node / p->np;
+      const double r = p->radius[node];
+      for (int c = 0; c < cols; ++c) {
+        double* dd = d + ((int64_t)node * cols + c) * 6;
+        double* mm = m + ((int64_t)node * cols + c) * 6;
+        for (int k = 0; k < 6; ++k) dd[k] = 0.0;
+        for (int i = 0; i < ngll; ++i) {
+          double rowU[3], rowV[3];
+          scalar_rows(p, q, le, i, k2, r, rowU, rowV);
+          const int* dof = t->dofs + (le * ngll + i) * 2;
+          for (int part = 0; part < 2; ++part) {
+            const double U = dof[0] >= 0 ? x[2 * ((size_t)c * nd + dof[0]) + part] : 0.0;
+            const double V = dof[1] >= 0 ? x[2 * ((size_t)c * nd + dof[1]) + part] : 0.0;
+            for (int k = 0; k < 3; ++k) dd[2 * k + part] += rowU[k] * U + rowV[k] * V;
+          }
+        }
+        for (int k = 0; k < 6; ++k) mm[k] += beta[node] * dd[k];
+      }
+    }
+    free(b);
+    free(beta);
+  }
+  st->has_strain = 1;
+  st->t += dt;
+  tasks_free(tasks, L + 1);
+  return 0;
 }

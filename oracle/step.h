@@ -100,6 +100,40 @@ int sl3o_last_residuals(double* out, int capacity);
  * the size only), or -1 for an unknown task. */
 int64_t sl3o_last_solution(int task, double* out, int64_t capacity);
 
+/* The pieces of a pass, one at a time (test infrastructure for the
+ * reference's gates). Solutions and forces: every task's complex columns
+ * (ndof x (l+1), column-major) end to end, spheroidal l = 0..lmax then
+ * toroidal 1..lmax; coefficients in the reference layout (as sl3o_state). */
+int64_t sl3o_solution_doubles(const sl3o_problem* p);
+/* The memory force of a spectral tensor on every task
+ * (src/GridCoupling.cpp:691-809). */
+void sl3o_memory_force(const sl3o_problem* p, const double* weighted, double* forces);
+/* The strain coefficients of the tasks' solutions (src/GridCoupling.cpp:
+ * 635-689). */
+void sl3o_strain_of(const sl3o_problem* p, const double* solutions, double* coef);
+/* Synthesis and analysis of a whole tensor (grids as in sl3o_state). */
+void sl3o_synthesise_tensor(const sl3o_problem* p, const double* coef, double* const grid[3]);
+void sl3o_analyse_tensor(const sl3o_problem* p, double* const grid[3], double* coef);
+
+/* The degree-basis (1D) Maxwell stepper at the memory points, spheroidal
+ * degrees, one branch, no lateral field (ViscoelasticOperator::Step,
+ * src/Viscoelastic.cpp:914-1076). m and d: the memory and the last
+ * strain, three deviatoric scalars (M, R, S) complex per (degree, node,
+ * order m = 0..l), degrees end to end (sl3o_scalar_doubles). The itau of
+ * the problem is not read (only itau_ref_node). */
+typedef struct {
+  double t;
+  double* m;
+  double* d;
+  int has_strain;
+} sl3o_state1d;
+int64_t sl3o_scalar_doubles(const sl3o_problem* p);
+/* One step; solutions: the spheroidal tasks' (l = 0..lmax) solutions end
+ * to end with the load channel (amp as in sl3o_step). Returns 0, or
+ * non-zero when a factorisation fails. */
+int sl3o_step_degree_basis(const sl3o_problem* p, sl3o_state1d* st, double dt, int scheme,
+                           const double* amp, double* solutions);
+
 #ifdef __cplusplus
 }
 #endif

@@ -286,6 +286,7 @@ class Viscoelastic3D:
 
         self.earth = earth
         self.lmax = lmax
+        self.delta = delta
         self.options = options if options is not None else StepOptions.default()
         self.handle = ctypes.c_void_p()
         torch.cuda.synchronize()
@@ -321,6 +322,13 @@ class Viscoelastic3D:
         torch.cuda.synchronize()
         ptr = ctypes.c_void_p(delta.data_ptr()) if delta is not None else None
         check(lib.sl3_operator_set_delta(self.handle, ptr))
+        self.delta = delta
+
+    def delta_grid(self):
+        """A host copy of the lateral field last set ([node][theta][phi]),
+        or None for delta = 0."""
+        d = getattr(self, "delta", None)
+        return None if d is None else d.detach().cpu().numpy().copy()
 
     def zero_state(self):
         return State(self)

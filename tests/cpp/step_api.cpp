@@ -13,6 +13,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
+#include <memory>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -208,6 +209,25 @@ int Gpu() {
     const auto g = op.StepInPlace(q, dt, Scheme::ExponentialTrapezoid, load, {}, bad, 1e-3);
     Expect(g.warmFallback && g.converged, "a guarded warm start is dropped");
     (void)hipFree(bad);
+  }
+  // the operator shares the Earth: destroyed first, stepping with two new
+  // dt values (each a new flavour assembled from the Earth) still works
+  {
+    auto owned = std::make_unique<Earth>(SmallEarth());
+    Viscoelastic3D::Options o2;
+    o2.max_iterations = 40;
+    Viscoelastic3D other(*owned, 4, nullptr, o2);
+    owned.reset();
+    auto s = other.ZeroState();
+    const auto a = other.StepInPlace(s, 0.5 * dt, Scheme::ExponentialTrapezoid, load);
+    const auto b = other.StepInPlace(s, 0.25 * dt, Scheme::BackwardEuler, load);
+    Expect(a.converged && b.converged, "an operator outlives its Earth handle");
+    // a state of one operator is refused by another before any launch
+    Expect(Throws<std::invalid_argument>(
+               [&] { op.StepInPlace(s, dt, Scheme::ExponentialTrapezoid, load); }),
+           "a state of another operator is refused");
+    Expect(Throws<std::invalid_argument>([&] { (void)op.Export(s, "m0"); }),
+           "exporting another operator's state is refused");
   }
   (void)hipFree(delta_dev);
   return failures;

@@ -62,7 +62,7 @@ def step_check(oracle):
     kw = dict(max_iterations=2, rtol=1e-14, anderson_memory=6, anderson_rate=0.5)
     op = st.Viscoelastic3D(earth, lmax, None, st.StepOptions.default(**kw))
     op.set_delta(op.delta_field(st.delta_coefficients(6, op.nodes, seed=5, rms=0.7, lmax=lmax)))
-    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.export("itau"),
+    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.delta_grid(),
                      theta=op.theta, wtheta=op.wtheta)
     load, _ = st.zonal_load_rate(lmax, earth)
     state = op.zero_state()

@@ -64,8 +64,11 @@ class OracleStep:
     """The oracle's problem for one (scheme, dt) on an Earth; keeps every
     array alive while the ctypes structure points at it."""
 
-    def __init__(self, oracle, earth, lmax, scheme, dt, itau_grid, theta=None, wtheta=None,
+    def __init__(self, oracle, earth, lmax, scheme, dt, delta, theta=None, wtheta=None,
                  eps=1e-8):
+        """delta: the lateral field on every memory node's grid, host array
+        [node][theta][phi] (the one the operator was given), or None for
+        zero; the oracle forms itau = itau_ref(node) exp(-delta) itself."""
         self.oracle = oracle
         self.keep = []
         elements, ngll = earth.mesh()
@@ -121,7 +124,7 @@ class OracleStep:
             radius=self.radius, jac=jac,
             pweight=earth.export("point_weight"), pvalue=earth.export("point_value"),
             pderiv=earth.export("point_deriv"), mu=self.mu_node, itau_ref=self.itau_ref,
-            itau=np.ascontiguousarray(itau_grid),
+            itau=self._itau(oracle, delta, nodes, nt * nph),
             ndof=np.array(ndof, dtype=np.int32), kd=np.array(kd, dtype=np.int32),
             fe=np.array(fe, dtype=np.int32), pinned=np.array(pinned, dtype=np.int32),
             band_off=np.array(band_off, dtype=np.int64), bands=np.concatenate(bands),
@@ -161,6 +164,19 @@ class OracleStep:
         self.state.coef = self.coef.ctypes.data_as(DP)
         self.state.has_strain = 0
         self.state.has_coef = 0
+
+    def _itau(self, oracle, delta, nodes, per_node):
+        itau = np.zeros(nodes * per_node)
+        d = None
+        if delta is not None:
+            d = np.ascontiguousarray(delta, dtype=np.float64).ravel()
+            assert d.size == itau.size, "delta holds nodes x ntheta x nphi values"
+        oracle.sl3o_lateral_itau(ctypes.c_int64(nodes), ctypes.c_int64(per_node),
+                                 self.itau_ref.ctypes.data_as(DP),
+                                 d.ctypes.data_as(DP) if d is not None else None,
+                                 itau.ctypes.data_as(DP))
+        self.delta = d
+        return itau
 
     def step(self, dt, scheme, amp, max_iterations=40, rtol=1e-11, damping=1.0,
              anderson_memory=5, anderson_rate=0.5, tamp=None):

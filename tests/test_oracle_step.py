@@ -17,24 +17,41 @@ def earth():
     return st.Earth(small_earth_spec())
 
 
-def itau_field(earth, lmax, delta_scale, seed=4):
-    from step_problem import oracle_library  # noqa: F401
-
+def delta_grid(earth, lmax, delta_scale, seed=4):
+    """A lateral field on every memory node's grid: a node-dependent mix of
+    sin^2(theta) cos(2 phi) and cos(theta), [node][theta][phi]."""
     elements, ngll = earth.mesh()
     first, count = (int(v) for v in earth.export("memory_elements"))
-    np_ = ngll - 1
-    nodes = count * np_
+    nodes = count * (ngll - 1)
     theta, w, nt, nph = grid_of(lmax)
-    itau_ref = earth.export("point_itau").reshape(elements, np_)[first: first + count].ravel()
     rng = np.random.default_rng(seed)
     phi = 2 * np.pi * np.arange(nph) / nph
     T, P = np.meshgrid(theta, phi, indexing="ij")
     out = np.zeros((nodes, nt, nph))
     for k in range(nodes):
         a, b = rng.normal(size=2)
-        delta = delta_scale * (a * np.sin(T) ** 2 * np.cos(2 * P) + b * np.cos(T))
-        out[k] = itau_ref[k] * np.exp(-delta)
+        out[k] = delta_scale * (a * np.sin(T) ** 2 * np.cos(2 * P) + b * np.cos(T))
     return out.ravel()
+
+
+def test_lateral_itau_is_the_reference_scaled_by_exp_minus_delta(oracle, earth):
+    """itau = itau_ref(node) exp(-delta) (src/Viscoelastic3D.cpp:250-283):
+    delta = 0 leaves the reference's inverse time, delta = ln 2 halves it,
+    and a field's factor is exp(-delta) point by point."""
+    lmax = 6
+    load_dt = 1.0
+    zero = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, load_dt, None)
+    per = zero.ntheta * zero.nphi
+    ref = np.repeat(zero.itau_ref, per)
+    assert np.array_equal(zero.arrays["itau"], ref)
+    half = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, load_dt,
+                      np.full(zero.nodes * per, np.log(2.0)))
+    assert np.max(np.abs(half.arrays["itau"] - 0.5 * ref)) <= 1e-15 * np.max(ref)
+    d = delta_grid(earth, lmax, 0.7)
+    field = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, load_dt, d)
+    nz = ref > 0
+    ratio = field.arrays["itau"][nz] / ref[nz]
+    assert np.max(np.abs(np.log(ratio) + d[nz])) <= 1e-13
 
 
 def test_zero_lateral_step_closes_in_two_passes(oracle, earth):
@@ -42,8 +59,7 @@ def test_zero_lateral_step_closes_in_two_passes(oracle, earth):
     length, time, mass = earth.scales()
     dt = 100 * st.YEAR / time
     load, _ = st.zonal_load_rate(lmax, earth)
-    p = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt,
-                   itau_field(earth, lmax, 0.0))
+    p = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, None)
     for k in range(2):
         amp = amplitudes(load, lmax, (k + 1) * dt)
         info = p.step(dt, st.EXPONENTIAL_TRAPEZOID, amp, rtol=1e-12)
@@ -57,8 +73,8 @@ def test_lateral_step_converges_to_its_fixed_point(oracle, earth):
     length, time, mass = earth.scales()
     dt = 100 * st.YEAR / time
     load, _ = st.zonal_load_rate(lmax, earth)
-    itau = itau_field(earth, lmax, 0.7)
-    p = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, itau)
+    p = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt,
+                   delta_grid(earth, lmax, 0.7))
     infos = []
     for k in range(2):
         amp = amplitudes(load, lmax, (k + 1) * dt)

@@ -148,7 +148,7 @@ def test_conjugate_gradient_steps_match_the_oracle_fixed_point(earth, oracle):
         solver=st.CONJUGATE_GRADIENTS, solve_rtol=1e-13))
     coef = st.delta_coefficients(6, op.nodes, seed=11, rms=1.5, lmax=lmax)
     op.set_delta(op.delta_field(coef))
-    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.export("itau"),
+    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.delta_grid(),
                      theta=op.theta, wtheta=op.wtheta)
     load, _ = st.zonal_load_rate(lmax, earth)
     state = op.zero_state()

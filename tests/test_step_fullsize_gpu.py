@@ -1,8 +1,15 @@
 """The step at the benchmark's full size (lmax 128, the 194 x 388 grid, 128
 elements of 6 GLL nodes, 280 Gauss memory nodes, the (1 + l/8)^-3 lateral
-field of rms 0.7, exponential trapezoid over 100 yr, the zonal load rate),
-where the oracle is too slow to run: two independent closures of the same
-linear system, the within-step fixed point (with adaptive Anderson mixing)
+field of rms 0.7 with AR(1) 0.9 across the nodes, seed 2026, exponential
+trapezoid over 100 yr, the zonal load rate).
+
+Against the oracle, on the same inputs: two steps each capped at two
+passes (StartStep, the passes with the adaptive Anderson mixer engaging on
+the second, Finish), the strain coefficients, memory and strain grids
+compared after each step.
+
+Converged steps, where the oracle would take minutes: two independent
+closures of the same linear system, the within-step fixed point (with adaptive Anderson mixing)
 and the preconditioned conjugate gradients, must land on the same state.
 Both converge; their strain coefficients agree to 1e-9 (measured 5.7e-11
 with rtol 1e-11 and solve_rtol 1e-13) and their memory on the grid to 1e-7
@@ -11,6 +18,7 @@ largest of a block)."""
 
 import numpy as np
 import pytest
+from step_problem import OracleStep, amplitudes
 
 import sl3dpp_amd.step as st
 
@@ -65,3 +73,33 @@ def test_full_size_fixed_point_and_conjugate_gradients_agree(earth):
         gm = rel(cg_m[b], fp_m[b])
         print(f"[full] memory block {b} gap {gm:.2e}", flush=True)
         assert gm < 1e-7
+
+
+def test_full_size_passes_match_the_oracle(earth, oracle):
+    L, T, M = earth.scales()
+    dt = 100.0 * st.YEAR / T
+    kw = dict(max_iterations=2, rtol=1e-14, anderson_memory=6, anderson_rate=0.5)
+    op = st.Viscoelastic3D(earth, LMAX, None, st.StepOptions.default(**kw))
+    coef = st.delta_coefficients(64, op.nodes, seed=2026, rms=0.7, ar=0.9, lmax=LMAX)
+    op.set_delta(op.delta_field(coef))
+    print("[full] building the oracle problem", flush=True)
+    ref = OracleStep(oracle, earth, LMAX, st.EXPONENTIAL_TRAPEZOID, dt, op.delta_grid(),
+                     theta=op.theta, wtheta=op.wtheta)
+    load, _ = st.zonal_load_rate(LMAX, earth)
+    state = op.zero_state()
+    for k in range(2):
+        print(f"[full] oracle parity step {k}", flush=True)
+        info = op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+        rinfo = ref.step(dt, st.EXPONENTIAL_TRAPEZOID, amplitudes(load, LMAX, (k + 1) * dt), **kw)
+        assert info.iterations == rinfo.iterations == 2
+        assert info.mixed_passes == rinfo.mixed_passes
+        assert abs(info.residual - rinfo.residual) <= 1e-6 * rinfo.residual
+        gap = rel(state.export("coefficients"), ref.coef)
+        print(f"[full] step {k}: residual {info.residual:.3e} / {rinfo.residual:.3e}, "
+              f"mixed {info.mixed_passes}, coefficients {gap:.2e}", flush=True)
+        assert gap < 1e-10
+        for b in range(3):
+            gm = rel(state.export(f"m{b}"), ref.m[b])
+            gs = rel(state.export(f"strain{b}"), ref.strain[b])
+            print(f"[full] step {k} block {b}: memory {gm:.2e}, strain {gs:.2e}", flush=True)
+            assert gm < 1e-10 and gs < 1e-10

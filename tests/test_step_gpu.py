@@ -99,6 +99,29 @@ def test_gsh_transforms_match_oracle(earth, oracle, lmax, spin):
         assert rel(b[node], c) < 1e-12  # round trip
 
 
+@pytest.mark.parametrize("lmax", [8, 12])
+def test_lateral_itau_matches_the_oracle(earth, oracle, lmax):
+    """ItauKernel (the operator's itau = itau_ref(node) exp(-delta),
+    src/Viscoelastic3D.cpp:250-283) against the oracle's, which forms it
+    from the same delta field on the host, and against numpy's."""
+    print(f"[step] itau lmax {lmax}", flush=True)
+    op = make_operator(earth, lmax, 0.9, seed=23)
+    got = op.export("itau")
+    delta = op.delta_grid()
+    assert delta is not None and np.max(np.abs(delta)) > 0.1
+    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, 1.0, delta,
+                     theta=op.theta, wtheta=op.wtheta)
+    want = ref.arrays["itau"]
+    assert got.shape == want.shape
+    assert rel(got, want) < 1e-14
+    per = op.ntheta * op.nphi
+    direct = np.repeat(ref.itau_ref, per) * np.exp(-delta)
+    assert rel(got, direct) < 1e-14
+    # delta = 0 again: the reference profile itself
+    op.set_delta(None)
+    assert np.array_equal(op.export("itau"), np.repeat(ref.itau_ref, per))
+
+
 def run_pair(earth, oracle, lmax, delta_scale, steps, max_iterations, rtol, histories=False,
              tide=False):
     L, T, M = earth.scales()
@@ -106,8 +129,7 @@ def run_pair(earth, oracle, lmax, delta_scale, steps, max_iterations, rtol, hist
     opts = st.StepOptions.default(max_iterations=max_iterations, rtol=rtol,
                                   anderson_memory=6, anderson_rate=0.5)
     op = make_operator(earth, lmax, delta_scale, opts)
-    itau = op.export("itau")
-    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, itau,
+    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.delta_grid(),
                      theta=op.theta, wtheta=op.wtheta)
     load, _ = st.zonal_load_rate(lmax, earth)
     tide_fn = tidal_rate(lmax, earth) if tide else None
@@ -200,8 +222,8 @@ def test_sdirk2_steps_match_oracle(earth, oracle):
     kw = dict(max_iterations=80, rtol=1e-11, anderson_memory=6, anderson_rate=0.5)
     opts = st.StepOptions.default(**kw)
     op = make_operator(earth, lmax, 0.7, opts)
-    itau = op.export("itau")
-    ref = OracleStep(oracle, earth, lmax, st.BACKWARD_EULER, st.SDIRK2_GAMMA * dt, itau,
+    ref = OracleStep(oracle, earth, lmax, st.BACKWARD_EULER, st.SDIRK2_GAMMA * dt,
+                     op.delta_grid(),
                      theta=op.theta, wtheta=op.wtheta)
     load, _ = st.zonal_load_rate(lmax, earth)
     state = op.zero_state()
@@ -302,7 +324,7 @@ def test_step_solutions_match_oracle(earth, oracle):
     dt = 100 * st.YEAR / T
     opts = st.StepOptions.default(max_iterations=1, rtol=1e-14)
     op = make_operator(earth, lmax, 0.7, opts)
-    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.export("itau"),
+    ref = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, op.delta_grid(),
                      theta=op.theta, wtheta=op.wtheta)
     load, _ = st.zonal_load_rate(lmax, earth)
     tide = tidal_rate(lmax, earth)
