@@ -38,6 +38,9 @@ const char* sl3_version(void);
 /* Waits for every call issued so far on the calling thread's library
  * stream (one per host thread). */
 int sl3_synchronize(void);
+/* Copies bytes between device addresses on the library's stream (for a
+ * caller holding views, sl3_step_view, and nothing else of HIP). */
+int sl3_copy(void* dst_dev, const void* src_dev, int64_t bytes);
 
 /* Kernel timing with device events on the library's stream (off by
  * default): enable, run, then read a kernel's total milliseconds and launch
@@ -358,6 +361,79 @@ int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int
 /* The state's strain coefficients (device layout, the next step's start),
  * borrowed until the state is stepped or destroyed; NULL before a step. */
 int sl3_state_spectral(const sl3_state* state, const double** spectral_dev);
+
+/* OpenStep (include/sl3dpp/Viscoelastic3D.hpp:461-647) at the boundary:
+ * Step is sl3_step_start + sl3_step_close + sl3_step_finish, and a driver
+ * (Evolution3D, the adjoint and tangent sweeps) calls the pieces itself.
+ * The open step lives in the operator's buffers: one open step per
+ * operator at a time, the state kept alive until it is finished. */
+/* StartStep (src/Viscoelastic3D.cpp:1223-1378), one stage (exponential
+ * trapezoid or backward Euler; Sdirk2 is opened stage by stage): the
+ * state's memory becomes the decayed memory m~ in place; the lateral weight,
+ * beta, the fixed memory force and the start strain (the iterate, view
+ * "strain") are formed, the channels gathered at the end time. */
+int sl3_step_start(sl3_operator* op, sl3_state* state, double dt, int scheme,
+                   sl3_load_fn load, sl3_load_fn tide, void* user);
+/* One pass of the within-step fixed point (Respond, AddChannels, StrainOf:
+ * src/Viscoelastic3D.cpp:1758-1786, 1029-1073): the strain coefficients
+ * (device layout) the step's solutions take for s_dev, into next_dev. */
+int sl3_step_pass(sl3_operator* op, const double* s_dev, double* next_dev);
+/* A driver's pass (OpenStep::Pass, :540-547): next = pass(s), device
+ * layout; return 0, or non-zero for a failure that ends Converge with
+ * SL3_EINVAL. It may call sl3_step_pass and add its own terms. */
+typedef int (*sl3_pass_fn)(const double* s_dev, double* next_dev, void* user);
+/* Converge (src/Viscoelastic3D.cpp:1916-1974): s = pass(s) from the start
+ * strain (or start_dev), adaptively Anderson-mixed under the operator's
+ * options; pass NULL is sl3_step_pass. The result is the iterate (view
+ * "strain"). */
+int sl3_step_converge(sl3_operator* op, sl3_pass_fn pass, void* user,
+                      const double* start_dev, sl3_step_info* info);
+/* Close (:2008-2015): the step under its channels as the operator's solver
+ * says (the fixed point, or the conjugate gradients). */
+int sl3_step_close(sl3_operator* op, const double* start_dev, sl3_step_info* info);
+/* Finish (:1885-1914) with strain_dev (device coefficients; NULL for the
+ * iterate): m+ = m~ + beta d, the strain grid and coefficients kept, the
+ * state at the end time; the step is closed. */
+int sl3_step_finish(sl3_operator* op, const double* strain_dev);
+/* Drops the open step unfinished: the state keeps the decayed memory. */
+int sl3_step_cancel(sl3_operator* op);
+/* OpenStep::AddMemory (:576-583): increment_dev[3] (grid blocks as the
+ * state's m0 m1 m2) joins the decayed memory and the fixed force. */
+int sl3_step_add_memory(sl3_operator* op, const double* const* increment_dev);
+/* OpenStep::AddForce's tensor (:585-598): an analysed tensor (device
+ * layout) joins the fixed force. */
+int sl3_step_add_force(sl3_operator* op, const double* analysed_dev);
+/* The open step's device buffers, borrowed (OpenStep's friends weight_,
+ * beta_, memory_, fixed_: include/sl3dpp/Viscoelastic3D.hpp:614-646):
+ * "weight", "beta" (grid values), "fixed", "strain" (device-layout spectral
+ * tensors: the fixed memory force, the iterate), "memory0".."memory2" (the
+ * decayed memory m~, open step only); count in doubles. */
+int sl3_step_view(sl3_operator* op, const char* name, const double** dev,
+                  int64_t* count);
+/* OpenStep::RelativeChange (:1636-1669) of two device spectral tensors. */
+int sl3_step_relative_change(sl3_operator* op, const double* next_dev,
+                             const double* now_dev, double* out);
+
+/* GridCoupling (include/sl3dpp/GridCoupling.hpp:166-384) over the
+ * operator's memory nodes. Solutions and forces are in the reference's
+ * Solutions layout: every task's ndof x (l + 1) complex columns
+ * (column-major), spheroidal l = 0..lmax then toroidal 1..lmax, end to
+ * end (sl3_coupling_solution_doubles); spectral tensors in the device
+ * layout. */
+int sl3_coupling_solution_doubles(const sl3_operator* op, int64_t* count);
+/* SpheroidalForce + ToroidalForce (src/GridCoupling.cpp:691-809): the
+ * memory force of an analysed tensor on every degree system. */
+int sl3_coupling_force(sl3_operator* op, const double* weighted_dev, double* forces_dev);
+/* AddStrainOfSolution over every degree (src/GridCoupling.cpp:595-689;
+ * StrainOf without channels): the strain coefficients of solutions. */
+int sl3_coupling_strain(sl3_operator* op, const double* solutions_dev, double* spectral_dev);
+/* Multiply (src/GridCoupling.cpp:497-509): a grid tensor (blocks as m0 m1
+ * m2) times a weight per grid point, in place. */
+int sl3_coupling_multiply(sl3_operator* op, double* c0_dev, double* c1_dev, double* r_dev,
+                          const double* weight_dev);
+/* ScaleByNode (src/GridCoupling.cpp:524-540): each node's coefficients
+ * times factor[node] (host, one per memory node), in place. */
+int sl3_coupling_scale_by_node(sl3_operator* op, const double* factor, double* spectral_dev);
 
 /* Host copies for checks and outputs. Names:
  *   state:    "coefficients" (the strain coefficients, reference l-major

@@ -171,4 +171,16 @@ int sl3_synchronize(void) {
   return sl3amd::Guard([] { SL3_HIP(hipStreamSynchronize(sl3amd::Stream())); });
 }
 
+int sl3_copy(void* dst_dev, const void* src_dev, int64_t bytes) {
+  return sl3amd::Guard([&] {
+    if (bytes < 0 || ((dst_dev == nullptr || src_dev == nullptr) && bytes > 0)) {
+      throw sl3amd::ArgumentError("sl3_copy: null address or negative size");
+    }
+    if (bytes > 0) {
+      SL3_HIP(hipMemcpyAsync(dst_dev, src_dev, static_cast<size_t>(bytes),
+                             hipMemcpyDeviceToDevice, sl3amd::Stream()));
+    }
+  });
+}
+
 }  // extern "C"

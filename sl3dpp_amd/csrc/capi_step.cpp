@@ -436,6 +436,180 @@ int sl3_step_guarded(sl3_operator* op, sl3_state* state, double dt, int scheme,
   });
 }
 
+namespace {
+
+Channels WrapChannels(sl3_load_fn load, sl3_load_fn tide, void* user) {
+  auto wrap = [user](sl3_load_fn fn) {
+    LoadFn out;
+    if (fn) {
+      out = [fn, user](int l, double t, double* columns) {
+        const int r = fn(l, t, columns, user);
+        if (r < 0) {
+          throw ArgumentError("a load channel reported a failure at degree " +
+                              std::to_string(l));
+        }
+        return r != 0;
+      };
+    }
+    return out;
+  };
+  return Channels{wrap(load), wrap(tide)};
+}
+
+void FillInfo(sl3_step_info* info, const StepInfo& r) {
+  if (info == nullptr) return;
+  info->iterations = r.iterations;
+  info->residual = r.residual;
+  info->converged = r.converged ? 1 : 0;
+  info->mixed_passes = r.mixedPasses;
+  info->start_ms = info->passes_ms = info->finish_ms = 0.0f;
+  info->solve_iterations = r.solveIterations;
+  info->warm_fallback = r.warmFallback ? 1 : 0;
+}
+
+}  // namespace
+
+int sl3_step_start(sl3_operator* op, sl3_state* state, double dt, int scheme, sl3_load_fn load,
+                   sl3_load_fn tide, void* user) {
+  return Guard([&] {
+    CheckOwner(op, state);
+    op->op->OpenStart(*state->state, dt, scheme, WrapChannels(load, tide, user));
+  });
+}
+
+int sl3_step_pass(sl3_operator* op, const double* s_dev, double* next_dev) {
+  return Guard([&] {
+    Check(op != nullptr && s_dev != nullptr && next_dev != nullptr, "null argument");
+    Check(s_dev != next_dev, "a pass writes a tensor other than its input");
+    op->op->OpenPass(s_dev, next_dev);
+  });
+}
+
+int sl3_step_converge(sl3_operator* op, sl3_pass_fn pass, void* user, const double* start_dev,
+                      sl3_step_info* info) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    Viscoelastic3D::PassFn fn;
+    if (pass != nullptr) {
+      fn = [pass, user](const double* s, double* next) {
+        if (pass(s, next, user) != 0) throw ArgumentError("the caller's pass reported a failure");
+      };
+    }
+    FillInfo(info, op->op->OpenConverge(fn, start_dev, 0.0));
+  });
+}
+
+int sl3_step_close(sl3_operator* op, const double* start_dev, sl3_step_info* info) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    FillInfo(info, op->op->OpenClose(start_dev, 0.0));
+  });
+}
+
+int sl3_step_finish(sl3_operator* op, const double* strain_dev) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    op->op->OpenFinish(strain_dev);
+    SL3_HIP(hipStreamSynchronize(Stream()));
+  });
+}
+
+int sl3_step_cancel(sl3_operator* op) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    op->op->OpenCancel();
+  });
+}
+
+int sl3_step_add_memory(sl3_operator* op, const double* const* increment_dev) {
+  return Guard([&] {
+    Check(op != nullptr && increment_dev != nullptr, "null argument");
+    for (int b = 0; b < 3; ++b) Check(increment_dev[b] != nullptr, "null increment block");
+    op->op->OpenAddMemory(increment_dev);
+  });
+}
+
+int sl3_step_add_force(sl3_operator* op, const double* analysed_dev) {
+  return Guard([&] {
+    Check(op != nullptr && analysed_dev != nullptr, "null argument");
+    op->op->OpenAddForce(analysed_dev);
+  });
+}
+
+int sl3_step_view(sl3_operator* op, const char* name, const double** dev, int64_t* count) {
+  return Guard([&] {
+    Check(op != nullptr && name != nullptr && dev != nullptr, "null argument");
+    auto& v = *op->op;
+    const auto& c = v.GetCoupling();
+    const std::string key(name);
+    DeviceState* st = v.OpenState();
+    if (key == "weight" || key == "beta") {
+      *dev = key == "weight" ? v.Weight() : v.Beta();
+      if (count) *count = static_cast<int64_t>(c.GridValueCount());
+    } else if (key == "fixed" || key == "strain") {
+      *dev = key == "fixed" ? v.FixedForce() : v.Iterate();
+      if (count) *count = static_cast<int64_t>(c.SpectralDoubles());
+    } else if (key.size() == 7 && key.rfind("memory", 0) == 0 && key[6] >= '0' && key[6] <= '2') {
+      Check(st != nullptr, "no step is open on this operator");
+      const int b = key[6] - '0';
+      *dev = st->m[b].data();
+      if (count) *count = static_cast<int64_t>(st->m[b].size());
+    } else {
+      throw ArgumentError("unknown step view: " + key);
+    }
+  });
+}
+
+int sl3_step_relative_change(sl3_operator* op, const double* next_dev, const double* now_dev,
+                             double* out) {
+  return Guard([&] {
+    Check(op != nullptr && next_dev != nullptr && now_dev != nullptr && out != nullptr,
+          "null argument");
+    *out = op->op->RelativeChangeOf(next_dev, now_dev);
+  });
+}
+
+int sl3_coupling_solution_doubles(const sl3_operator* op, int64_t* count) {
+  return Guard([&] {
+    Check(op != nullptr && count != nullptr, "null argument");
+    *count = static_cast<int64_t>(op->op->SolutionDoubles());
+  });
+}
+
+int sl3_coupling_force(sl3_operator* op, const double* weighted_dev, double* forces_dev) {
+  return Guard([&] {
+    Check(op != nullptr && weighted_dev != nullptr && forces_dev != nullptr, "null argument");
+    op->op->ForceOf(weighted_dev, forces_dev);
+  });
+}
+
+int sl3_coupling_strain(sl3_operator* op, const double* solutions_dev, double* spectral_dev) {
+  return Guard([&] {
+    Check(op != nullptr && solutions_dev != nullptr && spectral_dev != nullptr, "null argument");
+    op->op->StrainOfSolutions(solutions_dev, spectral_dev);
+  });
+}
+
+int sl3_coupling_multiply(sl3_operator* op, double* c0_dev, double* c1_dev, double* r_dev,
+                          const double* weight_dev) {
+  return Guard([&] {
+    Check(op != nullptr && c0_dev && c1_dev && r_dev && weight_dev, "null argument");
+    LaunchMultiply(c0_dev, c1_dev, r_dev, weight_dev, op->op->GetCoupling().GridValueCount(),
+                   Stream());
+  });
+}
+
+int sl3_coupling_scale_by_node(sl3_operator* op, const double* factor, double* spectral_dev) {
+  return Guard([&] {
+    Check(op != nullptr && factor != nullptr && spectral_dev != nullptr, "null argument");
+    const auto& c = op->op->GetCoupling();
+    DeviceArray<double> f;
+    f.Upload(factor, c.NodeCount(), Stream());
+    LaunchScaleByNode(f.data(), spectral_dev, c.SpectralDoubles() / 2, c.NodeCount(), Stream());
+    SL3_HIP(hipStreamSynchronize(Stream()));
+  });
+}
+
 int sl3_state_export(const sl3_operator* op, const sl3_state* state, const char* name,
                      double* dst, int64_t* count) {
   return Guard([&] {

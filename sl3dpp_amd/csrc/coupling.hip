@@ -233,12 +233,12 @@ __global__ void __launch_bounds__(1024)
       const double* im = response + ts.fOff + static_cast<size_t>(l + 1 + m) * ts.nmPad;
       if (d[0] >= 0) {
         Ui = C(re[d[0]], im[d[0]]);
-        if (ts.unitOff >= 0) Ui = Ui + units[ts.dmOff + d[0]] * a;
+        if (units != nullptr && ts.unitOff >= 0) Ui = Ui + units[ts.dmOff + d[0]] * a;
         if (tide) Ui = Ui + tideUnits[ts.dmOff + d[0]] * at;
       }
       if (d[1] >= 0) {
         Vi = C(re[d[1]], im[d[1]]);
-        if (ts.unitOff >= 0) Vi = Vi + units[ts.dmOff + d[1]] * a;
+        if (units != nullptr && ts.unitOff >= 0) Vi = Vi + units[ts.dmOff + d[1]] * a;
         if (tide) Vi = Vi + tideUnits[ts.dmOff + d[1]] * at;
       }
       if (tor1) {
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(1024)
   DegreeTask tt;
   if (tor1) tt = tor[l - 1];
   const bool tide = tideAmp != nullptr;
-  const bool unitsOn = ts.unitOff >= 0;
+  const bool unitsOn = units != nullptr && ts.unitOff >= 0;  // no load channel: none
   {
     const double* re = response + ts.fOff + static_cast<size_t>(m) * ts.nmPad;
     const double* im = response + ts.fOff + static_cast<size_t>(l + 1 + m) * ts.nmPad;
@@ -427,6 +427,35 @@ __global__ void CombineKernel(double* dst, double a, const double* src, double b
     const double first = a * dst[i];
     const double second = b * src[i];
     dst[i] = first + second;
+  }
+}
+
+// GridCoupling::ScaleByNode: complex value i of a [row][node] tensor
+// belongs to node i mod nodes.
+__global__ void ScaleByNodeKernel(const double* __restrict__ mu, double* spectral, size_t count,
+                                  int nodes) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += stride) {
+    const double f = mu[i % nodes];
+    spectral[2 * i] *= f;
+    spectral[2 * i + 1] *= f;
+  }
+}
+
+// GridCoupling::Multiply: the tensor's two complex and one real component
+// times the weight at each point.
+__global__ void MultiplyKernel(double* c0, double* c1, double* r, const double* __restrict__ w,
+                               size_t points) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < points;
+       i += stride) {
+    const double f = w[i];
+    c0[2 * i] *= f;
+    c0[2 * i + 1] *= f;
+    c1[2 * i] *= f;
+    c1[2 * i + 1] *= f;
+    r[i] *= f;
   }
 }
 
@@ -726,6 +755,27 @@ void LaunchCombine(double* dst, double a, const double* src, double b, size_t co
   ProfileScope prof("combine", stream);
   hipLaunchKernelGGL(CombineKernel, dim3(GridFor(count)), dim3(kThreads), 0, stream, dst, a,
                      src, b, count);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchScaleByNode(const double* mu, double* spectral, size_t count, int nodes,
+                       hipStream_t stream) {
+  if (count == 0) return;
+  if (nodes < 1 || count % static_cast<size_t>(nodes) != 0) {
+    throw ArgumentError("a [row][node] tensor holds whole rows of nodes");
+  }
+  ProfileScope prof("scale_by_node", stream);
+  hipLaunchKernelGGL(ScaleByNodeKernel, dim3(GridFor(count)), dim3(kThreads), 0, stream, mu,
+                     spectral, count, nodes);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchMultiply(double* c0, double* c1, double* r, const double* weight, size_t points,
+                    hipStream_t stream) {
+  if (points == 0) return;
+  ProfileScope prof("multiply", stream);
+  hipLaunchKernelGGL(MultiplyKernel, dim3(GridFor(points)), dim3(kThreads), 0, stream, c0, c1,
+                     r, weight, points);
   SL3_HIP(hipGetLastError());
 }
 

@@ -133,6 +133,17 @@ void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double*
 void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,
                        hipStream_t stream);
 
+// GridCoupling::ScaleByNode (src/GridCoupling.cpp:524-540) on a device-
+// layout spectral tensor ([row][node] complex, count complex values):
+// every coefficient of a node times mu[node].
+void LaunchScaleByNode(const double* mu, double* spectral, size_t count, int nodes,
+                       hipStream_t stream);
+// GridCoupling::Multiply (src/GridCoupling.cpp:497-509) on a grid tensor:
+// c0, c1 (complex) and r times the weight, point by point (points values
+// per block, the weight per point).
+void LaunchMultiply(double* c0, double* c1, double* r, const double* weight, size_t points,
+                    hipStream_t stream);
+
 // Reference layout (per node, l-major, SurfaceGrid::SpinIndex / Index)
 // <-> device layout, for one block over `nodes` nodes.
 void LaunchToDeviceLayout(const double* ref, double* dev, const int* rowOffset,

@@ -177,7 +177,62 @@ __global__ void __launch_bounds__(kThreads)
       scale[t.scaleOff + d] * f[t.fOff + static_cast<long long>(i) * t.colsPad + c];
 }
 
+// Memory-dof force rows (row-major nmPad x colsPad at fOff; Re of
+// m = 0..l, then Im) ->
+// the reference's solution layout (ndof x (l + 1) complex, column-major, at
+// solOff[task]); the other dofs are left as they are.
+__global__ void __launch_bounds__(kThreads)
+    RowsToSolutionsKernel(const DegreeTask* tasks, const int* dm, const long long* solOff,
+                          const double* rows, double* sol) {
+  const DegreeTask t = tasks[blockIdx.y];
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int orders = t.l + 1;
+  if (idx >= static_cast<long long>(t.nm) * orders) return;
+  const int i = static_cast<int>(idx / orders), m = static_cast<int>(idx % orders);
+  const int d = dm[t.dmOff + i];
+  const double* row = rows + t.fOff + static_cast<long long>(i) * t.colsPad;
+  double* z = sol + solOff[blockIdx.y] + 2 * (static_cast<long long>(m) * t.ndof + d);
+  z[0] = row[m];
+  z[1] = row[orders + m];
+}
+
+// The inverse gather into the response layout (column-major: column c of
+// a task at fOff + c nmPad, Re of m = 0..l then Im; padding untouched).
+__global__ void __launch_bounds__(kThreads)
+    SolutionsToRowsKernel(const DegreeTask* tasks, const int* dm, const long long* solOff,
+                          const double* sol, double* rows) {
+  const DegreeTask t = tasks[blockIdx.y];
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int orders = t.l + 1;
+  if (idx >= static_cast<long long>(t.nm) * orders) return;
+  const int i = static_cast<int>(idx / orders), m = static_cast<int>(idx % orders);
+  const int d = dm[t.dmOff + i];
+  const double* z = sol + solOff[blockIdx.y] + 2 * (static_cast<long long>(m) * t.ndof + d);
+  rows[t.fOff + static_cast<long long>(m) * t.nmPad + i] = z[0];
+  rows[t.fOff + static_cast<long long>(orders + m) * t.nmPad + i] = z[1];
+}
+
 }  // namespace
+
+void LaunchRowsToSolutions(const DegreeTask* tasks_dev, int taskCount, int nmMax, int lmax,
+                           const int* dm, const long long* solOff, const double* rows,
+                           double* solutions, hipStream_t stream) {
+  if (taskCount == 0 || nmMax == 0) return;
+  const long long total = static_cast<long long>(nmMax) * (lmax + 1);
+  hipLaunchKernelGGL(RowsToSolutionsKernel, dim3(CeilDiv(total, kThreads), taskCount),
+                     dim3(kThreads), 0, stream, tasks_dev, dm, solOff, rows, solutions);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchSolutionsToRows(const DegreeTask* tasks_dev, int taskCount, int nmMax, int lmax,
+                           const int* dm, const long long* solOff, const double* solutions,
+                           double* rows, hipStream_t stream) {
+  if (taskCount == 0 || nmMax == 0) return;
+  const long long total = static_cast<long long>(nmMax) * (lmax + 1);
+  hipLaunchKernelGGL(SolutionsToRowsKernel, dim3(CeilDiv(total, kThreads), taskCount),
+                     dim3(kThreads), 0, stream, tasks_dev, dm, solOff, solutions, rows);
+  SL3_HIP(hipGetLastError());
+}
 
 std::vector<ResponseWork> ResponseWorkList(const std::vector<DegreeTask>& tasks) {
   std::vector<ResponseWork> work;

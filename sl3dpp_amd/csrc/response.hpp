@@ -54,4 +54,16 @@ void LaunchExpandForce(const DegreeTask* tasks_dev, int taskCount, int nmMax,
                        int colsMax, const int* dm, const double* scale,
                        const double* f, double* rhs, hipStream_t stream);
 
+// The force rows (row-major) -> and the response layout (column-major) <-
+// the reference's solution layout (every task's ndof x (l + 1) complex
+// columns, column-major, task t at solOff[t]): the boundary's force and
+// strain maps. Dofs outside the memory dofs (and the
+// rows' padding) are left as they are.
+void LaunchRowsToSolutions(const DegreeTask* tasks_dev, int taskCount, int nmMax, int lmax,
+                           const int* dm, const long long* solOff, const double* rows,
+                           double* solutions, hipStream_t stream);
+void LaunchSolutionsToRows(const DegreeTask* tasks_dev, int taskCount, int nmMax, int lmax,
+                           const int* dm, const long long* solOff, const double* solutions,
+                           double* rows, hipStream_t stream);
+
 }  // namespace sl3amd

@@ -589,7 +589,14 @@ void Viscoelastic3D::BuildTasks() {
   };
   for (int l = 0; l <= L; ++l) add(l, 0);
   for (int l = 1; l <= L; ++l) add(l, 1);
+  std::vector<long long> solOff;
+  solutionDoubles_ = 0;
+  for (const auto& t : tasks_) {
+    solOff.push_back(static_cast<long long>(solutionDoubles_));
+    solutionDoubles_ += 2 * static_cast<size_t>(t.ndof) * (t.l + 1);
+  }
   hipStream_t s = Stream();
+  solutionOffset_.Upload(solOff.data(), solOff.size(), s);
   tasksDev_.Upload(tasks_.data(), tasks_.size(), s);
   dofs_.Upload(dofsHost_.data(), dofsHost_.size(), s);
   dofsMem_.Upload(dofsMemHost_.data(), dofsMemHost_.size(), s);
@@ -1130,6 +1137,31 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const Channel
 StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
                                   const Channels& channels, const double* start,
                                   double warmGuard) {
+  OpenStart(state, dt, scheme, channels);
+  StepInfo info;
+  try {
+    info = OpenClose(start, warmGuard);
+  } catch (...) {
+    open_.state = nullptr;
+    throw;
+  }
+  OpenFinish(nullptr);
+  hipStream_t s = Stream();
+  SL3_HIP(hipStreamSynchronize(s));
+  SL3_HIP(hipEventElapsedTime(&lastTiming.startStep, events_[0], events_[1]));
+  SL3_HIP(hipEventElapsedTime(&lastTiming.passes, events_[1], events_[2]));
+  SL3_HIP(hipEventElapsedTime(&lastTiming.finish, events_[2], events_[3]));
+  return info;
+}
+
+// StartStep (src/Viscoelastic3D.cpp:1223-1378).
+void Viscoelastic3D::OpenStart(DeviceState& state, double dt, int scheme,
+                               const Channels& channels) {
+  if (open_.state != nullptr) throw ArgumentError("a step is already open on this operator");
+  if (!(dt >= 0.0)) throw ArgumentError("the step length must be non-negative");
+  if (scheme != kExponentialTrapezoid && scheme != kBackwardEuler) {
+    throw ArgumentError("a step is opened stage by stage: exponential trapezoid or backward Euler");
+  }
   hipStream_t s = Stream();
   const size_t S = coupling_.SpectralDoubles();
   double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
@@ -1179,21 +1211,61 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
   // the first pass's start force (the fixed point's only)
   if (!cg) coupling_.AnalyseFromFourier(A, analysed_.data(), fixed_.data(), true, s);
   SL3_HIP(hipEventRecord(events_[1], s));
+  open_.state = &state;
+  open_.flavour = &f;
+  open_.scheme = scheme;
+  open_.dt = dt;
+  open_.tEnd = tEnd;
+  open_.startForce = !cg;
+}
 
-  // Converge (:1916-1974), or the conjugate-gradient solve (:2008-2015)
+void Viscoelastic3D::RequireOpen() const {
+  if (open_.state == nullptr) throw ArgumentError("no step is open on this operator");
+}
+
+// One pass of the open step: Respond + AddChannels + StrainOf
+// (src/Viscoelastic3D.cpp:1758-1786, 1029-1073).
+void Viscoelastic3D::OpenPass(const double* s, double* next) {
+  RequireOpen();
+  const bool useStartForce = open_.startForce && s == s_.data();
+  open_.startForce = false;
+  Pass(s, next, useStartForce, *open_.flavour);
+}
+
+StepInfo Viscoelastic3D::OpenClose(const double* start, double warmGuard) {
+  RequireOpen();
   StepInfo info;
   residualHistory.clear();
-  if (cg) info = CloseByConjugateGradients(f);  // takes no warm start, as the reference's
+  if (options_.solver == kConjugateGradients) {
+    info = CloseByConjugateGradients(*open_.flavour);  // takes no warm start, as the reference's
+    SL3_HIP(hipEventRecord(events_[2], Stream()));
+    return info;
+  }
+  return OpenConverge(PassFn(), start, warmGuard);
+}
+
+// Converge (src/Viscoelastic3D.cpp:1916-1974): the fixed point s = pass(s)
+// from the start strain (or `start`), adaptively Anderson-mixed; the
+// result in s_.
+StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
+                                      double warmGuard) {
+  RequireOpen();
+  hipStream_t s = Stream();
+  const size_t S = coupling_.SpectralDoubles();
+  StepInfo info;
+  residualHistory.clear();
   bool warm = false;
-  const bool guarded = start != nullptr && !cg && warmGuard > 0.0;
+  const bool guarded = start != nullptr && warmGuard > 0.0;
   if (guarded) {
     // the step's own start, kept in case the warm one is dropped
     coldStart_.Resize(S);
     LaunchCopy(coldStart_.data(), s_.data(), S, s);
   }
-  if (start != nullptr && !cg) {
+  const bool startForce = open_.startForce;
+  if (start != nullptr) {
     LaunchCopy(s_.data(), start, S, s);
     warm = true;
+    open_.startForce = false;
   }
   int first = 0;  // the first pass of the sequence being converged
   const bool adaptive = options_.andersonRate > 0.0;
@@ -1224,8 +1296,13 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
     sw.slot = slot;
     return sw;
   };
-  for (int it = 0; it < (cg ? 0 : options_.maxIterations); ++it) {
-    Pass(s_.data(), sNext_.data(), it == 0 && !warm, f);
+  for (int it = 0; it < options_.maxIterations; ++it) {
+    if (pass) {
+      pass(s_.data(), sNext_.data());
+    } else {
+      open_.startForce = false;
+      Pass(s_.data(), sNext_.data(), it == 0 && !warm && startForce, *open_.flavour);
+    }
     // Once the mixer runs (and without damping) every pass that does not
     // converge takes a sweep over the same two iterates: the sweep then also
     // forms the pass's RelativeChange, one read of the data and one
@@ -1309,8 +1386,19 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
     }
   }
   SL3_HIP(hipEventRecord(events_[2], s));
+  return info;
+}
 
-  // Finish (:1900-1914): d = Synthesise(strain), m+ = m~ + beta d
+// Finish (src/Viscoelastic3D.cpp:1885-1914): d = Synthesise(strain),
+// m+ = m~ + beta d, the strain and its coefficients kept, the state at the
+// end time. strain: device coefficients, or null for the iterate.
+void Viscoelastic3D::OpenFinish(const double* strain) {
+  RequireOpen();
+  DeviceState& state = *open_.state;
+  hipStream_t s = Stream();
+  const size_t S = coupling_.SpectralDoubles();
+  if (strain != nullptr && strain != s_.data()) LaunchCopy(s_.data(), strain, S, s);
+  double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
   coupling_.SynthesiseToFourier(s_.data(), A, true, s);
   for (int b = 0; b < 3; ++b) {
     if (!state.hasStrain) state.strain[b].Resize(coupling_.GridDoubles(b));
@@ -1327,13 +1415,62 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
   if (state.coefficients.size() != S) state.coefficients.Resize(S);
   LaunchCopy(state.coefficients.data(), s_.data(), S, s);
   state.hasCoefficients = true;
-  state.t = tEnd;
+  state.t = open_.tEnd;
   SL3_HIP(hipEventRecord(events_[3], s));
-  SL3_HIP(hipStreamSynchronize(s));
-  SL3_HIP(hipEventElapsedTime(&lastTiming.startStep, events_[0], events_[1]));
-  SL3_HIP(hipEventElapsedTime(&lastTiming.passes, events_[1], events_[2]));
-  SL3_HIP(hipEventElapsedTime(&lastTiming.finish, events_[2], events_[3]));
-  return info;
+  open_.state = nullptr;
+}
+
+void Viscoelastic3D::ForceOf(const double* weighted, double* forces) {
+  hipStream_t s = Stream();
+  boundaryRows_.Resize(std::max<size_t>(fDoubles_, 1));
+  LaunchFill(boundaryRows_.data(), boundaryRows_.size(), 0.0, s);
+  LaunchForce(tasksDev_.data(), coupling_.Geometry(),
+              coupling_.View(const_cast<double*>(weighted)), dofsMem_.data(),
+              forceWork_.data(), forceWorkCount_, boundaryRows_.data(), s);
+  LaunchFill(forces, solutionDoubles_, 0.0, s);
+  LaunchRowsToSolutions(tasksDev_.data(), static_cast<int>(tasks_.size()), nmMax_,
+                        coupling_.LMax(), dm_.data(), solutionOffset_.data(),
+                        boundaryRows_.data(), forces, s);
+}
+
+void Viscoelastic3D::StrainOfSolutions(const double* solutions, double* spectral) {
+  hipStream_t s = Stream();
+  boundaryRows_.Resize(std::max<size_t>(fDoubles_, 1));
+  LaunchFill(boundaryRows_.data(), boundaryRows_.size(), 0.0, s);
+  LaunchSolutionsToRows(tasksDev_.data(), static_cast<int>(tasks_.size()), nmMax_,
+                        coupling_.LMax(), dm_.data(), solutionOffset_.data(), solutions,
+                        boundaryRows_.data(), s);
+  const int L = coupling_.LMax();
+  LaunchStrain(tasksDev_.data(), tasksDev_.data() + (L + 1), L, coupling_.Geometry(),
+               dofsMem_.data(), boundaryRows_.data(), nullptr, nullptr, nullptr, nullptr,
+               coupling_.View(spectral), s);
+}
+
+// OpenStep::AddMemory (include/sl3dpp/Viscoelastic3D.hpp:576-583): an
+// increment of the decayed memory joins it and the fixed force
+// (mu Analyse(increment), node by node); the next pass forms its force
+// afresh.
+void Viscoelastic3D::OpenAddMemory(const double* const increment[3]) {
+  RequireOpen();
+  hipStream_t s = Stream();
+  DeviceState& state = *open_.state;
+  double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
+  for (int b = 0; b < 3; ++b) {
+    LaunchCombine(state.m[b].data(), 1.0, increment[b], 1.0, state.m[b].size(), s);
+  }
+  coupling_.Analyse(increment, sNext_.data(), B, s);
+  LaunchScaleByNode(muNode_.data(), sNext_.data(), coupling_.SpectralDoubles() / 2,
+                    coupling_.NodeCount(), s);
+  LaunchCombine(fixed_.data(), 1.0, sNext_.data(), 1.0, coupling_.SpectralDoubles(), s);
+  open_.startForce = false;
+}
+
+// OpenStep::AddForce's tensor (:585-598): an analysed tensor joins the
+// fixed force; the next pass forms its force afresh.
+void Viscoelastic3D::OpenAddForce(const double* analysed) {
+  RequireOpen();
+  LaunchCombine(fixed_.data(), 1.0, analysed, 1.0, coupling_.SpectralDoubles(), Stream());
+  open_.startForce = false;
 }
 
 void Viscoelastic3D::LateralForce(const double* response, bool channels, double* force,

@@ -229,6 +229,54 @@ class Viscoelastic3D {
   // One pass: Respond(s) + AddChannels + StrainOf into sNext.
   void Pass(const double* s, double* sNext, bool useStartForce, Flavour& f);
 
+  // OpenStep (include/sl3dpp/Viscoelastic3D.hpp:461-647) on this
+  // operator's buffers, one open step at a time; Step is OpenStart +
+  // OpenClose + OpenFinish.
+  // StartStep (src/Viscoelastic3D.cpp:1223-1378), one stage (exponential
+  // trapezoid or backward Euler): the state's memory becomes the decayed
+  // memory m~ in place until OpenFinish; the weight, beta, the fixed force,
+  // the channels at the end time and the start strain (the iterate) are
+  // formed.
+  void OpenStart(DeviceState& state, double dt, int scheme, const Channels& channels);
+  bool IsOpen() const { return open_.state != nullptr; }
+  // One pass of the open step (Respond, AddChannels, StrainOf) of s into
+  // next (device spectral tensors).
+  void OpenPass(const double* s, double* next);
+  using PassFn = std::function<void(const double* s, double* next)>;
+  // Converge (:1916-1974) with a caller's pass (empty: OpenPass) from the
+  // start strain or `start`; the result is the iterate.
+  StepInfo OpenConverge(const PassFn& pass, const double* start, double warmGuard);
+  // Close (:2008-2015): the fixed point, or the conjugate gradients.
+  StepInfo OpenClose(const double* start, double warmGuard);
+  // Finish (:1885-1914) with `strain` (device coefficients; null: the
+  // iterate): m+ = m~ + beta d, the strain and coefficients, the end time.
+  void OpenFinish(const double* strain);
+  // AddMemory (increment per grid block, device) and AddForce's tensor
+  // (an analysed tensor, device layout) (:576-598).
+  void OpenAddMemory(const double* const increment[3]);
+  void OpenAddForce(const double* analysed);
+  // Drops the open step without finishing it (the state keeps the decayed
+  // memory m~ the start wrote).
+  void OpenCancel() { open_.state = nullptr; }
+  DeviceState* OpenState() const { return open_.state; }
+  // RelativeChange (src/Viscoelastic3D.cpp:1636-1669) over every entry.
+  double RelativeChangeOf(const double* next, const double* now) {
+    return RelativeChange(next, now, true);
+  }
+  // The iterate (the start strain after OpenStart, the converged strain
+  // after OpenConverge), device layout.
+  double* Iterate() { return s_.data(); }
+
+  // GridCoupling's maps at the boundary, in the reference's solution
+  // layout (every task's ndof x (l + 1) complex columns, column-major, end
+  // to end): SpheroidalForce / ToroidalForce (src/GridCoupling.cpp:
+  // 691-809) of a device-layout spectral tensor, and StrainOf's
+  // AddStrainOfSolution (:595-689) of solutions (no channels). Both use
+  // their own buffers, not the step's.
+  size_t SolutionDoubles() const { return solutionDoubles_; }
+  void ForceOf(const double* weighted, double* forces);
+  void StrainOfSolutions(const double* solutions, double* spectral);
+
  private:
   StepInfo StepOnce(DeviceState& state, double dt, int scheme, const Channels& channels,
                     const double* start, double warmGuard);
@@ -297,7 +345,20 @@ class Viscoelastic3D {
     double dt = 0.0;
   } lastStep_;  // the flavour of the last pass, for the solution export
   bool hasTide_ = false;
+  // the open step: its state, flavour, end time, and whether analysed_
+  // holds the start strain's force for the first pass
+  struct {
+    DeviceState* state = nullptr;
+    Flavour* flavour = nullptr;
+    int scheme = 0;
+    double dt = 0.0, tEnd = 0.0;
+    bool startForce = false;
+  } open_;
+  void RequireOpen() const;
   DeviceArray<double> force_, response_;  // per pass, on the memory dofs
+  DeviceArray<double> boundaryRows_;      // ForceOf / StrainOfSolutions
+  DeviceArray<long long> solutionOffset_;  // per task, in the solution layout
+  size_t solutionDoubles_ = 0;
   DeviceArray<double> rhs_;               // full solves for export only
   bool rhsCurrent_ = false;               // rhs_ holds the solves of force_
   DeviceArray<double> fourierA_[3], fourierB_[3];

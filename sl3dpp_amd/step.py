@@ -89,6 +89,15 @@ class StepInfo(ctypes.Structure):
 
 LOAD_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.c_double,
                            c_double_p, ctypes.c_void_p)
+def device_copy(dst, src, nbytes):
+    """bytes from one device address to another on the library's stream
+    (sl3_copy), synchronised."""
+    check(lib.sl3_copy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_int64(nbytes)))
+    check(lib.sl3_synchronize())
+
+
+# sl3_pass_fn: (s_dev, next_dev, user) -> 0 or a failure
+PASS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
 
 lib.sl3_step_options_default.restype = None
 
@@ -369,6 +378,97 @@ class Viscoelastic3D:
         torch.cuda.synchronize()
         check(lib.sl3_gsh_analyse(self.handle, spin, ctypes.c_void_p(field.data_ptr()),
                                   ctypes.c_void_p(coefficients.data_ptr())))
+
+    # ---- OpenStep (include/sl3dpp/Viscoelastic3D.hpp:461-647) -------------
+
+    def start_step(self, state, dt, scheme=EXPONENTIAL_TRAPEZOID, load=None, tide=None):
+        """StartStep: the state's memory becomes m~ until finish_step."""
+        import torch
+
+        torch.cuda.synchronize()
+        check(lib.sl3_step_start(self.handle, state.handle, ctypes.c_double(dt), scheme, load,
+                                 tide, None))
+
+    def step_pass(self, s, nxt):
+        """One pass (Respond, AddChannels, StrainOf) of device tensor s into
+        nxt."""
+        check(lib.sl3_step_pass(self.handle, ctypes.c_void_p(s), ctypes.c_void_p(nxt)))
+
+    def converge(self, pass_fn=None, start=None):
+        """Converge with a pass (a PASS_FN or None for step_pass)."""
+        info = StepInfo()
+        ptr = ctypes.c_void_p(start.data_ptr()) if start is not None else None
+        check(lib.sl3_step_converge(self.handle, pass_fn, None, ptr, ctypes.byref(info)))
+        return info
+
+    def close_step(self, start=None):
+        info = StepInfo()
+        ptr = ctypes.c_void_p(start.data_ptr()) if start is not None else None
+        check(lib.sl3_step_close(self.handle, ptr, ctypes.byref(info)))
+        return info
+
+    def finish_step(self, strain=None):
+        ptr = ctypes.c_void_p(strain.data_ptr()) if strain is not None else None
+        check(lib.sl3_step_finish(self.handle, ptr))
+
+    def cancel_step(self):
+        check(lib.sl3_step_cancel(self.handle))
+
+    def step_view(self, name):
+        """(device pointer, doubles) of an open step's buffer."""
+        ptr = ctypes.c_void_p()
+        count = ctypes.c_int64()
+        check(lib.sl3_step_view(self.handle, name.encode(), ctypes.byref(ptr),
+                                ctypes.byref(count)))
+        return ptr.value, count.value
+
+    def add_force(self, analysed):
+        check(lib.sl3_step_add_force(self.handle, ctypes.c_void_p(analysed.data_ptr())))
+
+    def add_memory(self, blocks):
+        arr = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks])
+        check(lib.sl3_step_add_memory(self.handle, arr))
+
+    def relative_change(self, nxt, now):
+        out = ctypes.c_double()
+        check(lib.sl3_step_relative_change(self.handle, ctypes.c_void_p(nxt.data_ptr()),
+                                           ctypes.c_void_p(now.data_ptr()), ctypes.byref(out)))
+        return out.value
+
+    # ---- GridCoupling (include/sl3dpp/GridCoupling.hpp:166-384) -----------
+
+    def solution_doubles(self):
+        n = ctypes.c_int64()
+        check(lib.sl3_coupling_solution_doubles(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def coupling_force(self, weighted, forces):
+        """SpheroidalForce + ToroidalForce of a device-layout tensor into the
+        Solutions layout (every task end to end)."""
+        check(lib.sl3_coupling_force(self.handle, ctypes.c_void_p(weighted.data_ptr()),
+                                     ctypes.c_void_p(forces.data_ptr())))
+
+    def coupling_strain(self, solutions, spectral):
+        check(lib.sl3_coupling_strain(self.handle, ctypes.c_void_p(solutions.data_ptr()),
+                                      ctypes.c_void_p(spectral.data_ptr())))
+
+    def multiply(self, blocks, weight):
+        check(lib.sl3_coupling_multiply(self.handle, *[ctypes.c_void_p(b.data_ptr())
+                                                        for b in blocks],
+                                        ctypes.c_void_p(weight.data_ptr())))
+
+    def scale_by_node(self, factor, spectral):
+        f = np.ascontiguousarray(factor, dtype=np.float64)
+        check(lib.sl3_coupling_scale_by_node(self.handle, f.ctypes.data_as(c_double_p),
+                                             ctypes.c_void_p(spectral.data_ptr())))
+
+    def to_reference(self, device, reference):
+        check(lib.sl3_spectral_to_reference(self.handle, ctypes.c_void_p(device.data_ptr()),
+                                            ctypes.c_void_p(reference.data_ptr())))
+
+    def from_reference(self, reference, device):
+        check(lib.sl3_spectral_from_reference(self.handle, ctypes.c_void_p(reference.data_ptr()),
+                                              ctypes.c_void_p(device.data_ptr())))
 
     def delta_field(self, coefficients):
         """delta on every node's grid from per-node spin-0 coefficients

@@ -148,9 +148,11 @@ def forces_gap(got, want, pinned_of):
 
 
 def general_load(lmax, earth, seed=97):
-    """A constant load with every order: seeded m >= 0 columns of a real
-    field (m = 0 real), switched on at t = 0, of the zonal load's size.
-    Returns the LOAD_FN and the amplitudes per (l, m)."""
+    """A load with every order growing linearly from zero at t = 0: seeded
+    m >= 0 columns of a real field (m = 0 real) times t / (100 yr), of the
+    zonal load's size. Zero at t = 0, so the zero state is its own elastic
+    start (the step's start strain from rest is then zero on every path).
+    Returns the LOAD_FN."""
     _, rate = st.zonal_load_rate(lmax, earth)
     length, time, mass = earth.scales()
     sigma = mass / length ** 2
@@ -162,12 +164,15 @@ def general_load(lmax, earth, seed=97):
         cols[0] = cols[0].real
         amp[l] = cols
 
+    unit = 100.0 * st.YEAR / time
+
     def load(l, t, columns, user):
         if l not in amp:
             return 0
+        f = t / unit
         for m in range(l + 1):
-            columns[2 * m] = amp[l][m].real
-            columns[2 * m + 1] = amp[l][m].imag
+            columns[2 * m] = amp[l][m].real * f
+            columns[2 * m + 1] = amp[l][m].imag * f
         return 1
 
     return st.LOAD_FN(load)

@@ -9,7 +9,7 @@
  * the zero-lateral step (tests/TestViscoelastic3D.cpp:90-196): with
    delta = 0 the oracle's 3D step (transforms, grid weights, Converge,
    Finish) gives the degree-basis stepper's solutions over two steps under a
-   load with every order (1e-10), exponential trapezoid and backward Euler.
+   load with every order growing from zero (1e-10), exponential trapezoid and backward Euler.
 
 The point-shear systems are themselves pinned: the elastic block by the
 traction numbers against pyslfp (test_traction_numbers.py) and the point

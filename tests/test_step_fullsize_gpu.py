@@ -3,9 +3,9 @@ elements of 6 GLL nodes, 280 Gauss memory nodes, the (1 + l/8)^-3 lateral
 field of rms 0.7 with AR(1) 0.9 across the nodes, seed 2026, exponential
 trapezoid over 100 yr, the zonal load rate).
 
-Against the oracle, on the same inputs: two steps each capped at two
-passes (StartStep, the passes with the adaptive Anderson mixer engaging on
-the second, Finish), the strain coefficients, memory and strain grids
+Against the oracle, on the same inputs: two steps each capped at six
+passes (StartStep, Picard passes until the adaptive Anderson mixer engages,
+mixed passes, Finish), the strain coefficients, memory and strain grids
 compared after each step.
 
 Converged steps, where the oracle would take minutes: two independent
@@ -78,7 +78,7 @@ def test_full_size_fixed_point_and_conjugate_gradients_agree(earth):
 def test_full_size_passes_match_the_oracle(earth, oracle):
     L, T, M = earth.scales()
     dt = 100.0 * st.YEAR / T
-    kw = dict(max_iterations=2, rtol=1e-14, anderson_memory=6, anderson_rate=0.5)
+    kw = dict(max_iterations=6, rtol=1e-14, anderson_memory=6, anderson_rate=0.5)
     op = st.Viscoelastic3D(earth, LMAX, None, st.StepOptions.default(**kw))
     coef = st.delta_coefficients(64, op.nodes, seed=2026, rms=0.7, ar=0.9, lmax=LMAX)
     op.set_delta(op.delta_field(coef))
@@ -91,8 +91,8 @@ def test_full_size_passes_match_the_oracle(earth, oracle):
         print(f"[full] oracle parity step {k}", flush=True)
         info = op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
         rinfo = ref.step(dt, st.EXPONENTIAL_TRAPEZOID, amplitudes(load, LMAX, (k + 1) * dt), **kw)
-        assert info.iterations == rinfo.iterations == 2
-        assert info.mixed_passes == rinfo.mixed_passes
+        assert info.iterations == rinfo.iterations == 6
+        assert info.mixed_passes == rinfo.mixed_passes > 0
         assert abs(info.residual - rinfo.residual) <= 1e-6 * rinfo.residual
         gap = rel(state.export("coefficients"), ref.coef)
         print(f"[full] step {k}: residual {info.residual:.3e} / {rinfo.residual:.3e}, "
