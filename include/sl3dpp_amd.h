@@ -107,6 +107,22 @@ int sl3_band_lu_solve_batched(int count, const int* n, const int* kl,
                               const int64_t* b_off, const int64_t* row_stride,
                               const int64_t* col_stride, double* b_dev);
 
+/* sl3_band_lu_solve_batched that allocates nothing. It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops). The per-call system
+ * descriptors and work list go to a 64 KiB device array of the library's
+ * own (one per GPU, held until the process ends) instead of buffers
+ * allocated per call; the kernels and results are sl3_band_lu_solve_
+ * batched's. Limits: at most 512 systems and 4096 blocks of 64 right-hand
+ * sides per call (SL3_EINVAL beyond, before any launch). Calls that
+ * overlap on one GPU (from several host threads) race on that array. */
+int sl3_band_lu_solve_batched_noalloc(int count, const int* n, const int* kl,
+                                      const int* ku, const int64_t* ab_off,
+                                      const double* ab_dev, const int64_t* piv_off,
+                                      const int* piv_dev, const int* nrhs,
+                                      const int64_t* b_off, const int64_t* row_stride,
+                                      const int64_t* col_stride, double* b_dev);
+
 /* ===================================================================== *
  * The Earth and its radial set-up (host): a layered body with properties *
  * linear in radius per layer, meshed into GLL elements, one Maxwell       *
