@@ -106,7 +106,6 @@ int sl3_band_lu_solve_batched(int count, const int* n, const int* kl,
                               const int* piv_dev, const int* nrhs,
                               const int64_t* b_off, const int64_t* row_stride,
                               const int64_t* col_stride, double* b_dev);
-
 /* sl3_band_lu_solve_batched that allocates nothing. It has no counterpart
  * in the reference: it is for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops). The per-call system
@@ -235,8 +234,9 @@ int sl3_degree_operator_shape(const sl3_degree_operator* op, int* ndof, int* kd,
 /* Host arrays by name: "band" (the unscaled upper band, (kd+1) x ndof),
  * "scale" (Factorise's diagonal scaling), "dofs" (per global mesh node: U,
  * V, phi for spheroidal, W for toroidal, -1 where absent, as doubles), and
- * for the spheroidal system "load", "traction", "traction_tangential"
- * (DegreeOperator::Load, Traction, TractionTangential,
+ * for the spheroidal system "load", "load_force", "load_potential",
+ * "tangential", "traction", "traction_tangential" (DegreeOperator::Load
+ * (Both, Force, Potential), Tangential, Traction, TractionTangential,
  * src/DegreeOperator.cpp:960-997) and "tide:<p>" (Tide(p), p = -1 for l,
  * :999-1038). dst NULL returns the size through *count. */
 int sl3_degree_operator_export(const sl3_degree_operator* op, const char* name,

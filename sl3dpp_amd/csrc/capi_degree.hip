@@ -188,6 +188,12 @@ int sl3_degree_operator_export(const sl3_degree_operator* op, const char* name, 
       }
     } else if (op->sph && key == "load") {
       v = op->sph->Load();
+    } else if (op->sph && key == "load_force") {
+      v = op->sph->LoadForce();
+    } else if (op->sph && key == "load_potential") {
+      v = op->sph->LoadPotential();
+    } else if (op->sph && key == "tangential") {
+      v = op->sph->Tangential();
     } else if (op->sph && key == "traction") {
       v = op->sph->Traction();
     } else if (op->sph && key == "traction_tangential") {

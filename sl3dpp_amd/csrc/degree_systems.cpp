@@ -413,6 +413,29 @@ std::vector<double> DegreeOperator::Load() const {
   return b;
 }
 
+std::vector<double> DegreeOperator::LoadForce() const {
+  std::vector<double> b(DofCount(), 0.0);
+  const double a = material_->radius;
+  b[SurfaceDof(Component::U)] = -material_->surfaceGravity * a * a;
+  return b;
+}
+
+std::vector<double> DegreeOperator::LoadPotential() const {
+  std::vector<double> b(DofCount(), 0.0);
+  const double a = material_->radius;
+  if (SurfaceDof(Component::Phi) >= 0) b[SurfaceDof(Component::Phi)] = -a * a;
+  return b;
+}
+
+std::vector<double> DegreeOperator::Tangential() const {
+  std::vector<double> b(DofCount(), 0.0);
+  const double a = material_->radius;
+  if (SurfaceDof(Component::V) >= 0) {
+    b[SurfaceDof(Component::V)] = -material_->surfaceGravity * a * a * l_ * (l_ + 1.0);
+  }
+  return b;
+}
+
 std::vector<double> DegreeOperator::Traction() const {
   std::vector<double> b(DofCount(), 0.0);
   const double a = material_->radius;

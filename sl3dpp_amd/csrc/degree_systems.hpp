@@ -97,6 +97,11 @@ class DegreeOperator {
   // load (a surface mass: pressure g_s and the potential's jump), a unit
   // normal traction and a unit tangential traction.
   std::vector<double> Load() const;
+  // Its two parts (LoadPart::Force, LoadPart::Potential) and the unit
+  // tangential load g a^2 l(l + 1) at V(a) (src/DegreeOperator.cpp:960-980).
+  std::vector<double> LoadForce() const;
+  std::vector<double> LoadPotential() const;
+  std::vector<double> Tangential() const;
   std::vector<double> Traction() const;
   std::vector<double> TractionTangential() const;
   // The force of the external potential psi = (r/a)^p (p = l for a

@@ -93,32 +93,53 @@ def read_deck(path):
 
 class DeckModel:
     """EarthModel::FromDeck for an elastic deck: per layer the columns joined
-    linearly, moduli A = rho vph^2, C = rho vpv^2, L = rho vsv^2,
-    N = rho vsh^2, F = eta (A - 2L), gravity from the density's mass
-    integral (SI)."""
+    between the deck's rows (join "linear", or "cubic": not-a-knot cubic
+    splines, planetmodel's join of a deck's columns, a parabola through
+    three rows and a line through two), moduli A = rho vph^2,
+    C = rho vpv^2, L = rho vsv^2, N = rho vsh^2, F = eta (A - 2L), gravity
+    from the density's mass integral taken on the join's segments (SI)."""
 
-    def __init__(self, layers):
+    def __init__(self, layers, join="linear"):
+        if join not in ("linear", "cubic"):
+            raise ValueError("join: linear or cubic")
         self.layers = layers
+        self.join = join
         self.boundaries = [layers[0][0, 0]] + [ly[-1, 0] for ly in layers]
         self.radius = self.boundaries[-1]
-        # mass below each layer's bottom: exact for a linear density per row
-        self.mass_below = [0.0]
-        for ly in layers:
-            self.mass_below.append(self.mass_below[-1] + self._mass(ly, ly[0, 0], ly[-1, 0]))
+        self.splines = []
+        if join == "cubic":
+            from scipy.interpolate import CubicSpline
 
-    @staticmethod
-    def _mass(ly, r0, r1):
-        """4 pi int_r0^r1 rho r^2 dr over one layer, rho linear per row."""
+            for ly in layers:
+                if len(ly) < 2:
+                    self.splines.append(None)
+                    continue
+                self.splines.append(CubicSpline(ly[:, 0], ly[:, 1:], axis=0,
+                                                bc_type="not-a-knot"))
+        # mass below each layer's bottom
+        self.mass_below = [0.0]
+        for i, ly in enumerate(layers):
+            self.mass_below.append(self.mass_below[-1] + self._mass(i, ly[0, 0], ly[-1, 0]))
+
+    def _rho(self, layer, r):
+        return self.values(layer, r)[0]
+
+    def _mass(self, layer, r0, r1):
+        """4 pi int_r0^r1 rho r^2 dr over one layer: per segment between rows
+        by 4-point Gauss-Legendre, exact for a density up to cubic."""
+        ly = self.layers[layer]
+        x, w = np.polynomial.legendre.leggauss(4)
         total = 0.0
-        for k in range(len(ly) - 1):
-            a, b = max(ly[k, 0], r0), min(ly[k + 1, 0], r1)
+        for k in range(max(len(ly) - 1, 1)):
+            lo = ly[k, 0]
+            hi = ly[k + 1, 0] if len(ly) > 1 else ly[k, 0]
+            a, b = max(lo, r0), min(hi, r1)
             if b <= a:
                 continue
-            x0, x1 = ly[k, 0], ly[k + 1, 0]
-            s = (ly[k + 1, 1] - ly[k, 1]) / (x1 - x0)
-            c = ly[k, 1] - s * x0  # rho = c + s r
-            total += 4.0 * math.pi * (c * (b ** 3 - a ** 3) / 3.0 + s * (b ** 4 - a ** 4) / 4.0)
-        return total
+            for xi, wi in zip(x, w):
+                r = 0.5 * (a + b) + 0.5 * (b - a) * xi
+                total += 0.5 * (b - a) * wi * self._rho(layer, r) * r * r
+        return 4.0 * math.pi * total
 
     def layer_of(self, r, above):
         """The layer holding r; at a boundary the one above or below."""
@@ -131,6 +152,11 @@ class DeckModel:
     def values(self, layer, r):
         """(rho, drho/dr, vpv, vsv, vph, vsh, eta) in `layer` at r."""
         ly = self.layers[layer]
+        if self.join == "cubic" and self.splines[layer] is not None:
+            sp = self.splines[layer]
+            row = sp(r)
+            drho = float(sp(r, 1)[0])
+            return (float(row[0]), drho, *[float(v) for v in row[1:]])
         k = int(np.clip(np.searchsorted(ly[:, 0], r, side="right") - 1, 0, len(ly) - 2)) \
             if len(ly) > 1 else 0
         if len(ly) == 1:
@@ -146,7 +172,7 @@ class DeckModel:
         if r <= 0.0:
             return 0.0
         ly = self.layers[layer]
-        m = self.mass_below[layer] + self._mass(ly, ly[0, 0], r)
+        m = self.mass_below[layer] + self._mass(layer, ly[0, 0], r)
         return G_SI * m / (r * r)
 
     def is_fluid(self, layer):
@@ -220,9 +246,9 @@ class Earth:
             self.handle = None
 
     @classmethod
-    def from_deck(cls, path, lmax=None, drmax=None, ngll=5):
+    def from_deck(cls, path, lmax=None, drmax=None, ngll=5, join="linear"):
         """EarthModel::FromDeck + RadialMesh::Create + Material::Create."""
-        model = DeckModel(read_deck(path))
+        model = DeckModel(read_deck(path), join=join)
         edges, layer = mesh_edges(model, lmax=lmax, drmax=drmax)
         xi = gll_nodes(ngll)
         E = len(edges) - 1
@@ -311,6 +337,50 @@ class DegreeOperator:
         y = np.empty_like(x)
         check(lib.sl3_degree_operator_apply(self.handle, _dp(x), nrhs, _dp(y)))
         return y
+
+
+LOVE_CHANNELS = ("u", "phi", "v", "t")
+
+
+def love_columns(op):
+    """ComputeLoveNumbers' right-hand sides (src/LoveNumbers.cpp:116-128):
+    the load's force and potential parts, the tangential load and the
+    tide (r/a)^l."""
+    return np.concatenate([op.export("load_force"), op.export("load_potential"),
+                           op.export("tangential"), op.export("tide:-1")])
+
+
+def love_numbers_from(earth, l, op, x):
+    """The twelve generalised Love numbers of degree l from the solutions x
+    (4 x ndof) of love_columns: h, l, k (surface U, V, phi) of each channel
+    in SI (src/LoveNumbers.cpp:138-156)."""
+    L, T, M = earth.length, earth.time, earth.mass
+    h_load, k_load = L ** 3 / M, L ** 4 / (M * T * T)
+    h_pot, k_pot = T * T / L, 1.0
+    top = (len(earth.edges) - 1) * (earth.ngll - 1)
+    iu, iv, ip = op.dof(top, U), op.dof(top, V), op.dof(top, PHI)
+    out = {}
+    for c, name in enumerate(LOVE_CHANNELS):
+        hf, kf = (h_pot, k_pot) if name == "t" else (h_load, k_load)
+        val = lambda d: x[c, d] if d >= 0 else 0.0  # noqa: E731
+        out["h_" + name] = val(iu) * hf
+        out["l_" + name] = val(iv) * hf
+        out["k_" + name] = val(ip) * kf
+    return out
+
+
+def love_numbers(earth, lmax, device="cuda"):
+    """ComputeLoveNumbers (src/LoveNumbers.cpp:87-175) without the axial
+    moments: per degree the four channels solved together on the GPU."""
+    import torch
+
+    table = []
+    for l in range(lmax + 1):
+        op = DegreeOperator(earth, l)
+        b = torch.tensor(love_columns(op), dtype=torch.float64, device=device)
+        op.solve(b, 4)
+        table.append(love_numbers_from(earth, l, op, b.cpu().numpy().reshape(4, op.ndof)))
+    return table
 
 
 def traction_numbers(earth, lmax, gravitating, device="cuda"):
