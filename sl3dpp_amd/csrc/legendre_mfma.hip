@@ -71,6 +71,7 @@ struct Args {
   const int* firstNode;  // per degree: its first node (or null)
   int orderCount, colBlocks, rowBlocks, thetaBlocks;
   int lmax, nodes, ntheta;
+  int fstride;  // the Fourier layout's colatitude stride
 };
 
 __device__ inline int LMinOf(int m, int spin) {
@@ -147,7 +148,7 @@ __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
         const int node = node0 + nodeLocal, th = k0 + k;
         pf[q] = 0.0;
         if (node < a.nodes && th < a.ntheta) {
-          pf[q] = F[((mi * a.nodes + node) * a.ntheta + th) * 2 + part];
+          pf[q] = F[((mi * a.nodes + node) * a.fstride + th) * 2 + part];
         }
       }
     };
@@ -292,7 +293,7 @@ __global__ void __launch_bounds__(kThreads) LegendreSynthesisMfmaKernel(Args a) 
   const int c = c0 + wave * 16 + lane % 16;
   if (c < cols) {
     const int node = c >> 1, part = c & 1;
-    double* dst = Fo + ((mi * a.nodes + node) * a.ntheta) * 2 + part;
+    double* dst = Fo + ((mi * a.nodes + node) * a.fstride) * 2 + part;
 #pragma unroll
     for (int t = 0; t < kThetaBlock / 16; ++t) {
 #pragma unroll
@@ -320,6 +321,7 @@ Args MakeArgs(const LegendrePlan& p) {
   a.lmax = p.lmax;
   a.nodes = p.nodes;
   a.ntheta = p.ntheta;
+  a.fstride = p.fstride;
   return a;
 }
 

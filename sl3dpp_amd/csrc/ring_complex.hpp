@@ -189,8 +189,8 @@ __device__ inline void ComplexPassBody(const RingLaunch& a, const double2* __res
   // unconditional where the order exists (a ring past the last reads the
   // group's first ring), and issue together.
   // byte offsets within the block: (m + L) nodes ntheta + theta, below 4 GiB
-  const unsigned mStride = 16u * static_cast<unsigned>(a.nodes) * a.ntheta;
-  const double2* F = in + static_cast<size_t>(node) * a.ntheta;
+  const unsigned mStride = 16u * static_cast<unsigned>(a.nodes) * a.fstride;
+  const double2* F = in + static_cast<size_t>(node) * a.fstride;
   const unsigned thIn = 16u * static_cast<unsigned>(ln.valid ? th : th0);
   auto at = [&](int m, unsigned thb) { return static_cast<unsigned>(m + L) * mStride + thb; };
   const double2 h0 = LoadAt(F, at(ln.m2, thIn));
@@ -325,7 +325,7 @@ __device__ inline void ComplexPassBody(const RingLaunch& a, const double2* __res
   // G(k) = u0 + C_k - i S_k at order m2 + 4 k, G(P - k) = u0 + C_k + i S_k at
   // m2 - 4 k, scaled by (2 pi / nphi) w_theta, stored where |order| <= L
   const double scale = ln.valid ? 2.0 * 3.14159265358979323846 / nphi * a.wtheta[th] : 0.0;
-  double2* G = out + static_cast<size_t>(node) * a.ntheta;
+  double2* G = out + static_cast<size_t>(node) * a.fstride;
   const int m2 = Opaque(ln.m2), quad = Opaque(ln.quad);
   const unsigned thOut = 16u * static_cast<unsigned>(Opaque(th));
   if (ln.valid && quad == 0) StoreAt(G, at(m2, thOut), make_double2(scale * g0r, scale * g0i));

@@ -314,8 +314,8 @@ __device__ inline void CopyOut(const Geometry& g, const RingLaunch& a, const dou
   const int th = th0 + ring;
   if (th >= a.ntheta) return;
   const double w = scale * a.wtheta[th];
-  double2* dst = reinterpret_cast<double2*>(out) + static_cast<size_t>(node) * a.ntheta + th;
-  const size_t mStride = static_cast<size_t>(a.nodes) * a.ntheta;
+  double2* dst = reinterpret_cast<double2*>(out) + static_cast<size_t>(node) * a.fstride + th;
+  const size_t mStride = static_cast<size_t>(a.nodes) * a.fstride;
   for (int mo0 = static_cast<int>(threadIdx.x) / rpg; mo0 < mCount; mo0 += kBatch * moStep) {
     double2 res[kBatch];
 #pragma unroll
@@ -451,7 +451,7 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
         const int mo = mo0 + u * moStep;
         v[u] = make_double2(0.0, 0.0);
         if (mo < g.orders && th < a.ntheta) {
-          v[u] = F[(static_cast<size_t>(mo) * a.nodes + node) * a.ntheta + th];
+          v[u] = F[(static_cast<size_t>(mo) * a.nodes + node) * a.fstride + th];
         }
       }
 #pragma unroll

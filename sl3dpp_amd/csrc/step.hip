@@ -204,6 +204,7 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
   plan_.lmax = lmax;
   plan_.nodes = geometry_.nodes;
   plan_.ntheta = grid_.NTheta();
+  plan_.fstride = FourierStride(grid_.NTheta());
 
   const int nphi = grid_.NPhi();
   dft_.R = nphi % 4 == 0 ? 4 : 2;
@@ -285,7 +286,7 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
 
 size_t Coupling::FourierDoubles(int block) const {
   return static_cast<size_t>(grid_.Layout(block).Orders()) * geometry_.nodes *
-         grid_.NTheta() * 2;
+         FourierStride(grid_.NTheta()) * 2;
 }
 
 SpectralView Coupling::View(double* spectral) const {
@@ -356,6 +357,7 @@ void Coupling::Ring(RingLaunch r, hipStream_t s) const {
   r.lmax = lmax_;
   r.nodes = geometry_.nodes;
   r.ntheta = grid_.NTheta();
+  r.fstride = FourierStride(grid_.NTheta());
   r.nphi = grid_.NPhi();
   r.wtheta = wtheta_.data();
   LaunchRing(r, s);
@@ -368,6 +370,7 @@ void Coupling::RingPass(double* const in[3], double* const out[3], const double*
   r.lmax = lmax_;
   r.nodes = geometry_.nodes;
   r.ntheta = grid_.NTheta();
+  r.fstride = FourierStride(grid_.NTheta());
   r.nphi = grid_.NPhi();
   r.wtheta = wtheta_.data();
   r.op = kRingPass;

@@ -30,6 +30,7 @@ constexpr int kThreads = 256;
 struct RingArgs {
   DftTables t;
   int lmax, nodes, ntheta, nphi;
+  int fstride;  // the Fourier layout's colatitude stride
   int block;  // 0, 1 complex spins; 2 real spin 0
   int mlo;
   double scale;           // analysis: 2 pi / nphi
@@ -206,12 +207,12 @@ __device__ void FoldForInverse(const RingShared& sh, const RingArgs& a, int node
       if (real) {
         const int am = m < 0 ? -m : m;
         const double* src =
-            a.fourierIn + ((static_cast<size_t>(am) * a.nodes + node) * a.ntheta + th) * 2;
+            a.fourierIn + ((static_cast<size_t>(am) * a.nodes + node) * a.fstride + th) * 2;
         f = make_double2(src[0], m < 0 ? -src[1] : src[1]);
       } else {
         const double* src =
             a.fourierIn +
-            ((static_cast<size_t>(m - a.mlo) * a.nodes + node) * a.ntheta + th) * 2;
+            ((static_cast<size_t>(m - a.mlo) * a.nodes + node) * a.fstride + th) * 2;
         f = make_double2(src[0], src[1]);
       }
       // e^{+2 pi i r m / nphi}
@@ -250,7 +251,7 @@ __device__ void CombineForward(const RingShared& sh, const RingArgs& a,
     }
     const double w = a.scale * a.wtheta[th];
     double* dst =
-        out + ((static_cast<size_t>(m - a.mlo) * a.nodes + node) * a.ntheta + th) * 2;
+        out + ((static_cast<size_t>(m - a.mlo) * a.nodes + node) * a.fstride + th) * 2;
     dst[0] = s.x * w;
     dst[1] = s.y * w;
   }
@@ -462,6 +463,7 @@ void LaunchRing(const RingLaunch& r, hipStream_t stream) {
   a.lmax = r.lmax;
   a.nodes = r.nodes;
   a.ntheta = r.ntheta;
+  a.fstride = r.fstride;
   a.nphi = r.nphi;
   a.block = r.block;
   a.mlo = r.block == 2 ? 0 : -r.lmax;

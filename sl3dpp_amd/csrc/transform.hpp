@@ -17,7 +17,13 @@ struct LegendrePlan {
   const int* orders = nullptr;  // (block, m) per launch z, device
   int orderCount = 0;
   int lmax = 0, nodes = 0, ntheta = 0;
+  // the Fourier layout's colatitude stride, ntheta rounded up to 8 (whole
+  // 128-byte lines per (order, node) row of complex values)
+  int fstride = 0;
 };
+
+// ntheta rounded up to 8 complex values (128 bytes).
+inline int FourierStride(int ntheta) { return (ntheta + 7) / 8 * 8; }
 
 // The longitude DFT's tables: nphi = R P; cos/sin(2 pi j / P) and
 // cos/sin(2 pi j / nphi), device pointers.
@@ -53,6 +59,7 @@ enum RingOp {
 struct RingLaunch {
   DftTables tables;
   int lmax = 0, nodes = 0, ntheta = 0, nphi = 0;
+  int fstride = 0;  // the Fourier layout's colatitude stride (FourierStride)
   int block = 0;
   int op = kRingAnalyse;
   const double* wtheta = nullptr;

@@ -16,6 +16,7 @@ import math
 import numpy as np
 
 from . import check, lib
+from .step import Earth as _StepEarth
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -208,8 +209,9 @@ def gll_nodes(n):
     return np.concatenate([[-1.0], np.sort(inner.real), [1.0]])
 
 
-class Earth:
-    """An Earth handle from nodal arrays (sl3_earth_create_nodal)."""
+class Earth(_StepEarth):
+    """An Earth handle from nodal arrays (sl3_earth_create_nodal), with the
+    layered Earth's host queries (scales, mesh, export, degree_system)."""
 
     def __init__(self, edges, ngll, rho, drho, gravity, A, C, F, L, N, fluid,
                  branch_mu=None, branch_itau=None, length=0.0, density=0.0):
@@ -240,14 +242,12 @@ class Earth:
         check(lib.sl3_earth_scales(self.handle, ctypes.byref(L_), ctypes.byref(t), ctypes.byref(m)))
         self.length, self.time, self.mass = L_.value, t.value, m.value
 
-    def __del__(self, _lib=lib):
-        if getattr(self, "handle", None):
-            _lib.sl3_earth_destroy(self.handle)
-            self.handle = None
-
     @classmethod
-    def from_deck(cls, path, lmax=None, drmax=None, ngll=5, join="linear"):
-        """EarthModel::FromDeck + RadialMesh::Create + Material::Create."""
+    def from_deck(cls, path, lmax=None, drmax=None, ngll=5, join="linear", viscosity=None):
+        """EarthModel::FromDeck + RadialMesh::Create + Material::Create, and
+        with viscosity(layer, model) -> eta (Pa s) or None per layer,
+        Rheology::Maxwell's one branch: mu the Voigt shear, itau = mu / eta
+        (include/sl3dpp/Rheology.hpp:100-190)."""
         model = DeckModel(read_deck(path), join=join)
         edges, layer = mesh_edges(model, lmax=lmax, drmax=drmax)
         xi = gll_nodes(ngll)
@@ -272,9 +272,24 @@ class Earth:
                 arrays["A"][e, i], arrays["C"][e, i] = A, C
                 arrays["L"][e, i], arrays["N"][e, i] = L, N
                 arrays["F"][e, i] = eta * (A - 2.0 * L)
-        return cls(edges, ngll, arrays["rho"], arrays["drho"], arrays["g"], arrays["A"],
-                   arrays["C"], arrays["F"], arrays["L"], arrays["N"], fluid,
-                   length=model.radius)
+        branch_mu = branch_itau = None
+        if viscosity is not None:
+            mu_v = (arrays["C"] + arrays["A"] - 2.0 * arrays["F"] + 5.0 * arrays["N"]
+                    + 6.0 * arrays["L"]) / 15.0
+            branch_mu = np.zeros((E, ngll))
+            branch_itau = np.zeros((E, ngll))
+            for e in range(E):
+                eta = viscosity(layer[e], model)
+                if eta is not None and not fluid[e]:
+                    branch_mu[e] = mu_v[e]
+                    branch_itau[e] = mu_v[e] / eta
+        earth = cls(edges, ngll, arrays["rho"], arrays["drho"], arrays["g"], arrays["A"],
+                    arrays["C"], arrays["F"], arrays["L"], arrays["N"], fluid,
+                    branch_mu=branch_mu, branch_itau=branch_itau, length=model.radius)
+        earth.model = model
+        earth.layer = layer
+        earth.branch_itau = branch_itau
+        return earth
 
 
 class DegreeOperator:

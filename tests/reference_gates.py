@@ -194,10 +194,13 @@ class DegreeBasisStepper:
         self.nsol = sum(2 * s["ndof"] * (l + 1)
                         for (l, kind), s in zip(tasks_of(ref.lmax), ref.systems) if kind == 0)
 
-    def step(self, dt, scheme, amp):
+    def step(self, dt, scheme, amp, problem=None):
+        """One step; problem: another OracleStep of the same Earth and lmax
+        (another (scheme, dt) flavour's bands), or None for this one's."""
+        ref = problem if problem is not None else self.ref
         sol = np.zeros(self.nsol)
         status = self.oracle.sl3o_step_degree_basis(
-            ctypes.byref(self.ref.problem), ctypes.byref(self.state), ctypes.c_double(dt),
+            ctypes.byref(ref.problem), ctypes.byref(self.state), ctypes.c_double(dt),
             scheme, amp.ctypes.data_as(DP), sol.ctypes.data_as(DP))
         assert status == 0
         sph = [s for (l, kind), s in zip(tasks_of(self.ref.lmax), self.ref.systems) if kind == 0]
