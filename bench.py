@@ -34,10 +34,10 @@ PEAK_FP64 = 78.6e12     # flop/s, MI355X dense FP64 (vector and matrix)
 # x 2.0, WRITE_SIZE x 1.0); profiles/step_lmax128_hbm_traffic.md. Only valid for
 # the default workload.
 MEASURED_TRAFFIC = {
-    "ring_pass": 1.6235e9,
-    "response": 5.458e8,
-    "legendre_analysis": 8.424e8,
-    "legendre_synthesis": 7.061e8,
+    "ring_pass": 1.4058e9,
+    "response": 5.459e8,
+    "legendre_analysis": 8.448e8,
+    "legendre_synthesis": 7.159e8,
 }
 
 
@@ -71,6 +71,10 @@ def parse():
                          "seeded), "
                          "stepped --steps times each by 4 worker threads; value is their "
                          "aggregate steps/s")
+    ap.add_argument("--maxwell1d-steps", type=int, default=200,
+                    help="steps of the batched 1D Maxwell evolution leg (lmax 256, 8 ice caps "
+                         "ramped to zero, 100 yr; 0 skips it)")
+    ap.add_argument("--maxwell1d-lmax", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -386,6 +390,9 @@ def main():
     learned = None
     if args.learned_train > 0 and not cg:
         learned = learned_leg(op, dt, load, args)
+    maxwell1d = None
+    if args.maxwell1d_steps > 0:
+        maxwell1d = maxwell1d_leg(args)
     out = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -416,6 +423,7 @@ def main():
         "cpu_baseline": cpu,
         "extrapolated_start": extrapolated,
         "learned": learned,
+        "maxwell1d": maxwell1d,
         "kernels": breakdown,
     }
     print(json.dumps(out), flush=True)
@@ -487,6 +495,138 @@ def extrapolated_leg(op, dt, load, args, plain_steps_per_s, plain_final):
     log(f"extrapolated start (order {args.extrapolate_order}): {out['steps_per_s']:.2f} steps/s, "
         f"passes {passes}, x{out['speedup_vs_plain']:.2f} of plain, final state vs plain {diff:.1e}")
     return out
+
+
+def ice_caps_pattern(lmax, earth, caps=8, seed=1956):
+    """The surface mass of a seeded sum of Gaussian ice caps as complex
+    coefficients per (l, m >= 0) of the orthonormal spherical harmonics
+    (non-dimensional surface mass; (Re, Im) doubles at l(l+1)/2 + m). Cap
+    k: thickness H_k exp(-(gamma / a_k)^2) of ice (917 kg m^-3) at angular
+    distance gamma from a uniform random centre, a_k uniform in 5-15
+    degrees, H_k uniform in 1-3 km. Its Legendre series h_l P_l(cos gamma)
+    by Gauss-Legendre quadrature, rotated to the centre by the addition
+    theorem: c_lm = h_l 4 pi / (2l + 1) conj(Y_lm(centre))."""
+    import numpy as np
+    from scipy.special import sph_harm_y
+
+    length, time, mass = earth.scales()
+    sigma = mass / length ** 2
+    rng = np.random.default_rng(seed)
+    x, w = np.polynomial.legendre.leggauss(4 * (lmax + 1))
+    gamma = np.arccos(x)
+    # P_l(x) for l = 0..lmax by the three-term recurrence
+    P = np.zeros((lmax + 1, x.size))
+    P[0] = 1.0
+    if lmax > 0:
+        P[1] = x
+    for l in range(2, lmax + 1):
+        P[l] = ((2 * l - 1) * x * P[l - 1] - (l - 1) * P[l - 2]) / l
+    ls = np.arange(lmax + 1)
+    out = np.zeros((lmax + 1) * (lmax + 2) // 2, dtype=np.complex128)
+    ll = np.concatenate([np.full(l + 1, l) for l in ls])
+    mm = np.concatenate([np.arange(l + 1) for l in ls])
+    for _ in range(caps):
+        a = np.radians(rng.uniform(5.0, 15.0))
+        H = rng.uniform(1000.0, 3000.0)
+        theta0 = np.arccos(rng.uniform(-1.0, 1.0))
+        phi0 = rng.uniform(0.0, 2.0 * np.pi)
+        h = 917.0 * H * np.exp(-(gamma / a) ** 2) / sigma
+        hl = (2 * ls + 1) / 2.0 * (P * (w * h)).sum(axis=1)
+        Y = sph_harm_y(ll, mm, theta0, phi0)
+        out += (hl[ll] * 4.0 * np.pi / (2 * ll + 1)) * np.conj(Y)
+    out[mm == 0] = out[mm == 0].real
+    return np.stack([out.real, out.imag], axis=-1).ravel()
+
+
+def maxwell1d_leg(args):
+    """The batched 1D Maxwell evolution (ViscoelasticOperator's Step for
+    every spheroidal degree, sl3_maxwell1d_*): the same Earth (70 / 56 / 2
+    elements x 6 GLL nodes, 5e20 / 1e22 Pa s split at 5701 km, elastic
+    lithosphere), lmax 256 with every order as columns, the exponential
+    trapezoid, 100 yr steps under 8 seeded Gaussian ice caps ramped linearly
+    to zero over the run; from rest with the elastic start (untimed), then
+    the timed steps, each one replay of the captured step graph. Then the
+    same steps as stream launches (the graph's gain), and 10 profiled
+    stream-launched steps for the per-kernel split and the roofline of the
+    largest."""
+    import numpy as np
+    import torch
+
+    import sl3dpp_amd
+    import sl3dpp_amd.step as st
+    from sl3dpp_amd import radial
+
+    lmax, steps = args.maxwell1d_lmax, args.maxwell1d_steps
+    earth = st.Earth(st.three_region_earth(70, 56, 2, 6))
+    length, tscale, mass = earth.scales()
+    dt = 100.0 * st.YEAR / tscale
+    t0 = time.perf_counter()
+    op = radial.ViscoelasticOperator(earth, lmax)
+    op.set_load(ice_caps_pattern(lmax, earth), np.linspace(1.0, 0.0, steps + 1))
+    op.evolve(dt, st.EXPONENTIAL_TRAPEZOID, 0)  # the elastic start, both flavours built
+    op.evolve(dt, st.EXPONENTIAL_TRAPEZOID, 1, graph=True)  # the graph captured
+    setup_s = time.perf_counter() - t0
+    log(f"maxwell1d: lmax {lmax}, {op.nodes} memory nodes, {op.memory_doubles} memory and "
+        f"{op.solution_doubles} solution doubles, set-up {setup_s:.1f} s")
+    timings = {}
+    for graph in (True, False):
+        op.reset()
+        op.evolve(dt, st.EXPONENTIAL_TRAPEZOID, 0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.evolve(dt, st.EXPONENTIAL_TRAPEZOID, steps, graph=graph)
+        timings[graph] = time.perf_counter() - t0
+        if graph:
+            final = op.export("memory")
+            surface = op.surface(radial.U).cpu().numpy()
+    # the per-kernel split: 10 stream-launched steps with events
+    op.reset()
+    op.evolve(dt, st.EXPONENTIAL_TRAPEZOID, 0)
+    sl3dpp_amd.synchronize()
+    sl3dpp_amd.profile_reset()
+    sl3dpp_amd.profile_select(None)
+    sl3dpp_amd.profile_enable(True)
+    op.evolve(dt, st.EXPONENTIAL_TRAPEZOID, 10, graph=False)
+    sl3dpp_amd.profile_enable(False)
+    kernels = sl3dpp_amd.profile_read()
+    M, S = 8.0 * op.memory_doubles, 8.0 * op.solution_doubles
+    # algorithmic HBM bytes per launch: the force reads m and d and writes
+    # every right-hand side; the solves read and write them once (the
+    # factors, about 50 MB, left out); the update reads the right-hand
+    # sides and reads and writes m and d
+    work = {"maxwell_force": 2 * M + S, "band_solve": 2 * S, "maxwell_update": S + 4 * M}
+    split = {}
+    for name, (total_ms, launches) in kernels.items():
+        entry = {"total_ms": round(total_ms, 3), "launches": launches}
+        if name in work and launches > 0:
+            # the band solves are two launches a step (Cholesky, LU)
+            per_step = total_ms / 10.0 * 1e-3
+            entry["hbm_frac"] = work[name] / per_step / PEAK_HBM
+        split[name] = entry
+    top = max((k for k in split if k in work), key=lambda k: split[k]["total_ms"])
+    per_step = split[top]["total_ms"] / 10.0 * 1e-3
+    total = timings[True]
+    return {
+        "workload": (f"batched 1D Maxwell evolution, lmax {lmax} (every order of every "
+                     "spheroidal degree), 128 elements (70/56/2) x 6 GLL nodes, 280 Gauss "
+                     f"memory nodes, exponential trapezoid, {steps} steps of 100 yr, 8 seeded "
+                     "Gaussian ice caps (5-15 deg, 1-3 km) ramped linearly to zero"),
+        "steps": steps,
+        "steps_per_s": steps / total,
+        "ms_per_step": 1e3 * total / steps,
+        "seconds": total,
+        "stream_launch_ms_per_step": 1e3 * timings[False] / steps,
+        "graph_speedup": timings[False] / total,
+        "setup_s": setup_s,
+        "memory_doubles": op.memory_doubles,
+        "solution_doubles": op.solution_doubles,
+        "roofline": {"bound": "hbm", "kernel": top, "achieved": work[top] / per_step / 1e9,
+                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": work[top] / per_step / PEAK_HBM, "work_per_step": work[top]},
+        "kernels": split,
+        "final_memory_rms": float(np.sqrt(np.mean(final ** 2))),
+        "surface_u_l2_m0": float(surface[2 * 3] * length),
+    }
 
 
 def learned_leg(op, dt, load, args):

@@ -475,6 +475,53 @@ int sl3_spectral_from_reference(const sl3_operator* op, const double* reference_
                                 double* device_dev);
 
 /* ===================================================================== *
+ * ViscoelasticOperator (include/sl3dpp/Viscoelastic.hpp:163-301): the    *
+ * radially symmetric Maxwell evolution in the degree basis, every        *
+ * spheroidal degree 0..lmax with all its orders as columns (2 (l + 1)    *
+ * real columns: Re of m = 0..l, then Im), one branch, Gauss memory       *
+ * points, batched on the GPU. The load is one surface-mass channel: a    *
+ * fixed pattern times a ramp sampled at the step times.                  *
+ * ===================================================================== */
+typedef struct sl3_maxwell1d sl3_maxwell1d;
+/* ViscoelasticOperator(rheology, lmax, options) with ZeroState: every
+ * degree 0..lmax, eps the graded truncation level (1e-8). Shares the
+ * Earth. */
+int sl3_maxwell1d_create(const sl3_earth* earth, int lmax, double eps, sl3_maxwell1d** out);
+int sl3_maxwell1d_destroy(sl3_maxwell1d* h);
+/* Back to ZeroState (t = 0, no memory, no strain). */
+int sl3_maxwell1d_reset(sl3_maxwell1d* h);
+/* The load channel (the LoadCallable of Step, include/sl3dpp/
+ * Viscoelastic.hpp:287-291): pattern, (lmax+1)(lmax+2) doubles, complex
+ * (Re, Im) per (l, m >= 0) at l(l+1)/2 + m, a non-dimensional surface mass;
+ * ramp[k] its factor at step time t0 + k dt (ramp_count >= the steps to be
+ * taken + 1). Host arrays. */
+int sl3_maxwell1d_set_load(sl3_maxwell1d* h, const double* pattern, int64_t pattern_count,
+                           const double* ramp, int64_t ramp_count);
+/* steps calls of Step(state, dt, scheme, load) (src/Viscoelastic.cpp:
+ * 1016-1075), scheme SL3_EXPONENTIAL_TRAPEZOID or SL3_BACKWARD_EULER; the
+ * first from ZeroState takes its elastic start under the load at t0 (the
+ * zero-length step). use_graph != 0 replays one captured hipGraph per
+ * step; both give the same bits. */
+int sl3_maxwell1d_evolve(sl3_maxwell1d* h, double dt, int scheme, int steps, int use_graph);
+/* Sizes: memory (and strain) doubles, solution doubles, memory nodes, the
+ * steps taken. Any may be NULL. */
+int sl3_maxwell1d_shape(const sl3_maxwell1d* h, int64_t* memory_doubles,
+                        int64_t* solution_doubles, int* nodes, int* steps_taken);
+/* One degree's layout: its dof count, first memory node, and offsets into
+ * the memory ([node][scalar M, R, S][col]) and the solutions ([dof][col],
+ * row-major). */
+int sl3_maxwell1d_degree(const sl3_maxwell1d* h, int l, int* ndof, int* first_node,
+                         int64_t* memory_offset, int64_t* solution_offset);
+/* Host copies by name: "memory", "strain" (the state's cache), "solutions"
+ * (the last step's, unscaled, with the channel). count in doubles. */
+int sl3_maxwell1d_export(const sl3_maxwell1d* h, const char* name, double* dst, int64_t count);
+/* ViscoelasticOperator::Surface (include/sl3dpp/Viscoelastic.hpp:262-267):
+ * component 0 U, 1 V, 2 phi of the last solutions at the surface per
+ * degree and column, out_dev[l (l+1) + col], (lmax+1)(lmax+2) doubles on
+ * the device. */
+int sl3_maxwell1d_surface(const sl3_maxwell1d* h, int component, double* out_dev);
+
+/* ===================================================================== *
  * The learned warm start (no counterpart in the reference: it feeds the  *
  * `start` of OpenStep::Converge, src/Viscoelastic3D.cpp:1916-1921).      *
  * ===================================================================== */

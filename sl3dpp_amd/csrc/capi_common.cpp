@@ -82,6 +82,12 @@ ProfileScope::ProfileScope(const char* name, hipStream_t stream)
   if (!p.enabled) return;
   std::lock_guard<std::mutex> lock(p.mutex);
   if (!p.only.empty() && p.only.count(name_) == 0) return;
+  // launches recorded into a graph are not timed one by one
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream_, &capture) != hipSuccess ||
+      capture != hipStreamCaptureStatusNone) {
+    return;
+  }
   slot_ = p.Take();
   SL3_HIP(hipEventRecord(p.pool[slot_], stream_));
 }

@@ -4,9 +4,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <future>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "sl3dpp_amd.h"
 
@@ -76,6 +81,21 @@ class ProfileScope {
   hipStream_t stream_;
   int slot_ = -1;
 };
+
+// body(i) for i in [0, count) on up to 8 host threads.
+template <class F>
+void ParallelFor(int count, F&& body) {
+  const int threads =
+      std::max(1, std::min<int>(8, static_cast<int>(std::thread::hardware_concurrency())));
+  std::vector<std::future<void>> jobs;
+  std::atomic<int> next{0};
+  for (int t = 0; t < threads; ++t) {
+    jobs.push_back(std::async(std::launch::async, [&] {
+      for (int i = next++; i < count; i = next++) body(i);
+    }));
+  }
+  for (auto& j : jobs) j.get();
+}
 
 inline int CeilDiv(long long a, long long b) {
   return static_cast<int>((a + b - 1) / b);

@@ -95,19 +95,6 @@ void GramLeastSquares(int m, const double* gram, const double* rhs, double* gamm
   }
 }
 
-template <class F>
-void ParallelFor(int count, F&& body) {
-  const int threads = std::max(1, std::min<int>(8, static_cast<int>(std::thread::hardware_concurrency())));
-  std::vector<std::future<void>> jobs;
-  std::atomic<int> next{0};
-  for (int t = 0; t < threads; ++t) {
-    jobs.push_back(std::async(std::launch::async, [&] {
-      for (int i = next++; i < count; i = next++) body(i);
-    }));
-  }
-  for (auto& j : jobs) j.get();
-}
-
 }  // namespace
 
 // ---- Coupling ---------------------------------------------------------------

@@ -2,8 +2,9 @@
 an elastic deck (EarthModel::FromDeck, RadialMesh::Create, Material::Create,
 src/EarthModel.cpp:86-200, src/RadialMesh.cpp:13-200), one degree's radial
 system (DegreeOperator / ToroidalOperator, include/sl3dpp/DegreeOperator.hpp:
-94-384) factorised and solved on the GPU, and the static responses built on
-it: the traction numbers (ComputeTractionNumbers, src/LoveNumbers.cpp:
+94-384) factorised and solved on the GPU, the batched radially symmetric
+Maxwell evolution (ViscoelasticOperator, include/sl3dpp/Viscoelastic.hpp:
+163-301), and the static responses built on the degree systems: the traction numbers (ComputeTractionNumbers, src/LoveNumbers.cpp:
 463-512) and the load Love numbers of a static elastic solve.
 
 Deck values between rows are joined linearly within a layer; the reference
@@ -352,6 +353,99 @@ class DegreeOperator:
         y = np.empty_like(x)
         check(lib.sl3_degree_operator_apply(self.handle, _dp(x), nrhs, _dp(y)))
         return y
+
+
+class ViscoelasticOperator:
+    """The radially symmetric Maxwell evolution of every spheroidal degree
+    0..lmax on the GPU (sl3_maxwell1d_*): ViscoelasticOperator's ZeroState,
+    Step under one load channel and Surface (include/sl3dpp/Viscoelastic.hpp:
+    163-301). Each degree carries 2 (l + 1) real columns (Re of m = 0..l,
+    then Im); the load is a pattern of surface-mass coefficients (complex
+    per (l, m >= 0) at l(l+1)/2 + m, non-dimensional) times a ramp sampled
+    at the step times."""
+
+    def __init__(self, earth, lmax, eps=1e-8):
+        self.earth = earth
+        self.lmax = lmax
+        h = ctypes.c_void_p()
+        check(lib.sl3_maxwell1d_create(earth.handle, lmax, ctypes.c_double(eps), ctypes.byref(h)))
+        self.handle = h
+        mem, sol, nodes = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        check(lib.sl3_maxwell1d_shape(self.handle, ctypes.byref(mem), ctypes.byref(sol),
+                                      ctypes.byref(nodes), None))
+        self.memory_doubles, self.solution_doubles = mem.value, sol.value
+        self.nodes = nodes.value
+        self.degrees = []
+        for l in range(lmax + 1):
+            nd, fn = ctypes.c_int(), ctypes.c_int()
+            mo, so = ctypes.c_int64(), ctypes.c_int64()
+            check(lib.sl3_maxwell1d_degree(self.handle, l, ctypes.byref(nd), ctypes.byref(fn),
+                                           ctypes.byref(mo), ctypes.byref(so)))
+            self.degrees.append({"ndof": nd.value, "first_node": fn.value,
+                                 "memory_offset": mo.value, "solution_offset": so.value})
+
+    def __del__(self, _lib=lib):
+        if getattr(self, "handle", None):
+            _lib.sl3_maxwell1d_destroy(self.handle)
+            self.handle = None
+
+    def reset(self):
+        check(lib.sl3_maxwell1d_reset(self.handle))
+
+    def set_load(self, pattern, ramp):
+        """pattern: complex (l, m) coefficients, a ((lmax+1)(lmax+2)/2,)
+        complex array or its (Re, Im) doubles; ramp: the factor at every
+        step time."""
+        p = np.asarray(pattern)
+        if np.iscomplexobj(p):
+            p = np.stack([p.real, p.imag], axis=-1)
+        self._pattern = np.ascontiguousarray(p, dtype=np.float64).ravel()
+        self._ramp = np.ascontiguousarray(ramp, dtype=np.float64).ravel()
+        check(lib.sl3_maxwell1d_set_load(self.handle, _dp(self._pattern),
+                                         ctypes.c_int64(self._pattern.size), _dp(self._ramp),
+                                         ctypes.c_int64(self._ramp.size)))
+
+    def evolve(self, dt, scheme, steps, graph=True):
+        check(lib.sl3_maxwell1d_evolve(self.handle, ctypes.c_double(dt), scheme, steps,
+                                       int(graph)))
+
+    def steps_taken(self):
+        k = ctypes.c_int()
+        check(lib.sl3_maxwell1d_shape(self.handle, None, None, None, ctypes.byref(k)))
+        return k.value
+
+    def export(self, name):
+        n = self.solution_doubles if name == "solutions" else self.memory_doubles
+        out = np.empty(n)
+        check(lib.sl3_maxwell1d_export(self.handle, name.encode(), _dp(out), ctypes.c_int64(n)))
+        return out
+
+    def memory_of(self, flat, l):
+        """Degree l's block of a memory or strain export as (nodes, 3, cols)."""
+        d = self.degrees[l]
+        cols = 2 * (l + 1)
+        n = self.nodes * 3 * cols
+        return flat[d["memory_offset"]: d["memory_offset"] + n].reshape(self.nodes, 3, cols)
+
+    def solution_of(self, flat, l):
+        """Degree l's block of the solutions as (ndof, cols)."""
+        d = self.degrees[l]
+        cols = 2 * (l + 1)
+        n = d["ndof"] * cols
+        return flat[d["solution_offset"]: d["solution_offset"] + n].reshape(d["ndof"], cols)
+
+    def surface(self, component, out_dev=None):
+        """Surface U (0), V (1) or phi (2) of the last solutions per degree
+        and column, a ((lmax+1)(lmax+2),) device tensor (degree l's columns
+        at l(l+1) onwards)."""
+        import torch
+
+        if out_dev is None:
+            out_dev = torch.empty((self.lmax + 1) * (self.lmax + 2), dtype=torch.float64,
+                                  device="cuda")
+        check(lib.sl3_maxwell1d_surface(self.handle, component,
+                                        ctypes.c_void_p(out_dev.data_ptr())))
+        return out_dev
 
 
 LOVE_CHANNELS = ("u", "phi", "v", "t")
