@@ -79,6 +79,9 @@ def parse():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run device copies of 1 GiB (the HBM byte counters' scale)")
+    ap.add_argument("--band-solve", action="store_true",
+                    help="only time the batched band Cholesky solve on one system of the "
+                         "1D evolution's longest shape (n 1281, kd 17, 2 columns, row-major)")
     return ap.parse_args()
 
 
@@ -283,6 +286,8 @@ def main():
         log(f"copy calibration: 13 copies of 1 GiB; {ms:.3f} ms each, "
             f"{2 * 2 ** 30 / (ms * 1e-3) / 1e12:.2f} TB/s (read + write)")
         return 0
+    if args.band_solve:
+        return band_solve_probe()
     if args.realisations > 1:
         return batched(args)
     if args.uneven > 0:
@@ -495,6 +500,49 @@ def extrapolated_leg(op, dt, load, args, plain_steps_per_s, plain_final):
     log(f"extrapolated start (order {args.extrapolate_order}): {out['steps_per_s']:.2f} steps/s, "
         f"passes {passes}, x{out['speedup_vs_plain']:.2f} of plain, final state vs plain {diff:.1e}")
     return out
+
+
+def band_solve_probe(n=1281, kd=17, nrhs=2, reps=20):
+    """The band Cholesky solve alone on one diagonally dominant system of
+    the 1D evolution's longest shape; prints microseconds per launch and
+    nanoseconds per row of the two sweeps (HIP events on the library
+    stream)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import sl3dpp_amd
+    from sl3dpp_amd import check, lib
+
+    I, I64 = ctypes.c_int, ctypes.c_int64
+    rng = np.random.default_rng(1)
+    ld = kd + 1
+    ab = np.zeros((n, ld))
+    ab[:, :kd] = 0.1 * rng.normal(size=(n, kd))
+    ab[:, kd] = 4.0 + 0.2 * kd
+    ab_dev = torch.tensor(ab.ravel(), device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    one = lambda v, t: (t * 1)(v)  # noqa: E731
+    check(lib.sl3_band_cholesky_factor_batched(1, one(n, I), one(kd, I), one(0, I64),
+                                               ctypes.c_void_p(ab_dev.data_ptr()),
+                                               ctypes.c_void_p(info.data_ptr())))
+    b = torch.randn(n * nrhs, dtype=torch.float64, device="cuda")
+    args = (1, one(n, I), one(kd, I), one(0, I64), ctypes.c_void_p(ab_dev.data_ptr()),
+            one(nrhs, I), one(0, I64), one(nrhs, I64), one(1, I64), ctypes.c_void_p(b.data_ptr()))
+    check(lib.sl3_band_cholesky_solve_batched(*args))
+    sl3dpp_amd.synchronize()
+    sl3dpp_amd.profile_reset()
+    sl3dpp_amd.profile_select(["band_solve"])
+    sl3dpp_amd.profile_enable(True)
+    for _ in range(reps):
+        check(lib.sl3_band_cholesky_solve_batched(*args))
+    sl3dpp_amd.profile_enable(False)
+    total_ms, launches = sl3dpp_amd.profile_read()["band_solve"]
+    us = 1e3 * total_ms / launches
+    log(f"band solve n {n} kd {kd} nrhs {nrhs}: {us:.1f} us per launch, "
+        f"{1e3 * us / (2 * n):.1f} ns per row")
+    return 0
 
 
 def ice_caps_pattern(lmax, earth, caps=8, seed=1956):
