@@ -255,6 +255,8 @@ def cpu_child(path):
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--cpu-child":
         return cpu_child(sys.argv[2])
+    if len(sys.argv) >= 3 and sys.argv[1] == "--uneven-cpu-child":
+        return uneven_cpu_child(sys.argv[2:])
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: the step "
@@ -809,6 +811,9 @@ def uneven(args, workers=4, seed=48):
         f"{total / elapsed:.1f} steps/s")
     assert all(c["converged"] for c in cases), "a realisation's step did not converge"
     log(f"uneven batch: {total} steps in {elapsed:.2f} s")
+    cpu, errors = None, None
+    if not args.no_cpu_baseline:
+        cpu, errors = uneven_cpu_baseline(cases, args)
     out = {
         "metric": METRIC + " (uneven batch of independent realisations)",
         "value": total / elapsed,
@@ -830,10 +835,128 @@ def uneven(args, workers=4, seed=48):
                               "passes": c["passes"]} for c in cases],
             "one_at_a_time_steps_per_s": total / sequential,
             "speedup_vs_one_at_a_time": sequential / elapsed,
+            "realisation_errors": errors,
         },
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def uneven_cpu_baseline(cases, args, cap=2):
+    """The uneven batch's CPU baseline and per-realisation errors: for the
+    first realisation drawn at each lmax, one step from rest capped at `cap`
+    passes on the GPU (its operator, lateral field, load and tolerance) and
+    the oracle's same capped step from the same inputs, in a child process
+    on the host's cores (as cpu_baseline). The oracle's per-pass and
+    StartStep + Finish times, scaled by each realisation's measured passes,
+    give the CPU time of the whole batch; the relative gaps between the two
+    capped steps (strain coefficients, memory and strain grids) are each
+    sampled realisation's error against the FP64 CPU restatement."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    import sl3dpp_amd.step as st
+
+    sample, seen = [], set()
+    for i, c in enumerate(cases):
+        if c["lmax"] not in seen:
+            seen.add(c["lmax"])
+            sample.append(i)
+    cores, nproc, allowed = physical_cores()
+    threads = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    with tempfile.TemporaryDirectory() as tmp:
+        files = []
+        for i in sample:
+            c = cases[i]
+            op = c["op"]
+            op.set_options(max_iterations=cap)
+            state = op.zero_state()
+            op.step(state, c["dt"], st.EXPONENTIAL_TRAPEZOID, c["load"])
+            op.set_options(max_iterations=200)
+            path = os.path.join(tmp, f"r{i}.npz")
+            np.savez(path, delta=op.delta_grid(), theta=op.theta, wtheta=op.wtheta,
+                     lmax=c["lmax"], elements=c["elements"], dt=c["dt"], rtol=args.rtol,
+                     cap=cap, coef=state.export("coefficients"),
+                     **{f"m{b}": state.export(f"m{b}") for b in range(3)},
+                     **{f"d{b}": state.export(f"strain{b}") for b in range(3)})
+            files.append(path)
+            del state
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores",
+                   OMP_PROC_BIND="close")
+        log(f"uneven cpu baseline: the oracle on {len(files)} sampled realisations, "
+            f"{threads} threads ...")
+        # the child's progress goes straight to this process's stderr
+        run = subprocess.run([sys.executable, os.path.abspath(__file__), "--uneven-cpu-child"]
+                             + files, env=env, stdout=subprocess.PIPE, text=True, timeout=1800)
+        if run.returncode != 0:
+            raise RuntimeError("uneven cpu child failed")
+        results = json.loads(run.stdout.strip().splitlines()[-1])
+    per_lmax = {r["lmax"]: r for r in results}
+    cpu_seconds = 0.0
+    for c in cases:
+        r = per_lmax[c["lmax"]]
+        # the sampled realisation's per-pass cost scaled by mesh size
+        scale = c["op"].nodes / r["nodes"]
+        for passes in c["passes"]:
+            cpu_seconds += scale * (r["once_s"] + passes * r["per_pass_s"])
+    total = sum(len(c["passes"]) for c in cases)
+    errors = [{"lmax": r["lmax"], "elements": r["elements"], "passes": r["passes"],
+               "coefficients": r["coefficients"], "memory": r["memory"], "strain": r["strain"]}
+              for r in results]
+    cpu = {
+        "value": total / cpu_seconds,
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "nproc": nproc,
+        "sample": (f"the oracle (FP64 C/OpenMP, direct longitude DFTs) on one realisation per "
+                   f"lmax ({len(results)}), one step from rest capped at {cap} passes, "
+                   f"{threads} threads (OMP_PLACES=cores, OMP_PROC_BIND=close); per-pass and "
+                   f"StartStep+Finish times scaled to every realisation's passes and memory "
+                   f"nodes"),
+    }
+    return cpu, errors
+
+
+def uneven_cpu_child(paths):
+    """uneven_cpu_baseline's oracle steps, in this (child) process."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from step_problem import OracleStep, amplitudes, oracle_library
+
+    import sl3dpp_amd.step as st
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+    meshes = {64: (35, 27, 2), 96: (52, 42, 2), 128: (70, 56, 2)}
+    oracle = oracle_library()
+    out = []
+    for path in paths:
+        w = np.load(path)
+        lmax, elements = int(w["lmax"]), int(w["elements"])
+        dt, rtol, cap = float(w["dt"]), float(w["rtol"]), int(w["cap"])
+        earth = st.Earth(st.three_region_earth(*meshes[elements], 6))
+        load, _ = st.zonal_load_rate(lmax, earth)
+        log(f"uneven cpu child: lmax {lmax}, {elements} elements ...")
+        p = OracleStep(oracle, earth, lmax, st.EXPONENTIAL_TRAPEZOID, dt, w["delta"],
+                       theta=w["theta"], wtheta=w["wtheta"])
+        info = p.step(dt, st.EXPONENTIAL_TRAPEZOID, amplitudes(load, lmax, dt),
+                      max_iterations=cap, rtol=rtol, anderson_memory=6, anderson_rate=0.5)
+        r = {"lmax": lmax, "elements": elements, "nodes": p.nodes, "passes": info.iterations,
+             "per_pass_s": info.passes_ms / 1e3 / max(info.iterations, 1),
+             "once_s": (info.start_ms + info.finish_ms) / 1e3,
+             "coefficients": rel(w["coef"], p.coef),
+             "memory": max(rel(w[f"m{b}"], p.m[b]) for b in range(3)),
+             "strain": max(rel(w[f"d{b}"], p.strain[b]) for b in range(3))}
+        log(f"uneven cpu child: lmax {lmax}: {r['per_pass_s']:.2f} s/pass, errors "
+            f"{r['coefficients']:.1e} {r['memory']:.1e} {r['strain']:.1e}")
+        out.append(r)
     print(json.dumps(out), flush=True)
     return 0
 
