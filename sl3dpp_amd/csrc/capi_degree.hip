@@ -20,22 +20,24 @@ void Check(bool ok, const std::string& what) {
   if (!ok) throw ArgumentError(what);
 }
 
-// x(i, k) *= scale(i) over ndof x nrhs column-major values; the pinned row
-// (if any) set to zero (DegreeOperator::BeginSolve / FinishSolve,
+// x(i, k) *= scale(i) over ndof x nrhs column-major values; the pinned rows
+// (the toroidal degree-1 system's, one per solid region: at most two, -1
+// for none) set to zero (DegreeOperator::BeginSolve / FinishSolve,
 // src/DegreeOperator.cpp:861-901).
 __global__ void ScaleRowsKernel(double* __restrict__ x, const double* __restrict__ scale, int n,
-                                long long count, int pinned) {
+                                long long count, int pinned0, int pinned1) {
   const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (i >= count) return;
   const int row = static_cast<int>(i % n);
-  x[i] = row == pinned ? 0.0 : x[i] * scale[row];
+  x[i] = row == pinned0 || row == pinned1 ? 0.0 : x[i] * scale[row];
 }
 
-void ScaleRows(double* x, const double* scale, int n, int nrhs, int pinned, hipStream_t s) {
+void ScaleRows(double* x, const double* scale, int n, int nrhs, const int pinned[2],
+               hipStream_t s) {
   const long long count = static_cast<long long>(n) * nrhs;
   if (count == 0) return;
   hipLaunchKernelGGL(ScaleRowsKernel, dim3(CeilDiv(count, 256)), dim3(256), 0, s, x, scale, n,
-                     count, pinned);
+                     count, pinned[0], pinned[1]);
   SL3_HIP(hipGetLastError());
 }
 
@@ -53,6 +55,7 @@ struct sl3_degree_operator {
   bool factored = false;
   bool cholesky = false;
   int kd = 0, ndof = 0, pinned = -1;
+  int pins[2] = {-1, -1};  // every pinned dof
   DeviceArray<double> factor, scale;
   DeviceArray<int> piv;
 };
@@ -142,8 +145,10 @@ int sl3_degree_operator_create(const sl3_earth* earth, int l, int kind,
       d->tor = std::make_unique<ToroidalOperator>(material, l, o.eps, shear);
       d->ndof = d->tor->DofCount();
       d->kd = d->tor->Kd();
-      Check(d->tor->Pinned().size() <= 1, "one solid region above the fluid core is supported");
-      d->pinned = d->tor->Pinned().empty() ? -1 : d->tor->Pinned()[0];
+      const auto& pins = d->tor->Pinned();
+      Check(pins.size() <= 2, "at most two solid regions (inner core, mantle) are supported");
+      d->pinned = pins.empty() ? -1 : pins[0];
+      for (size_t k = 0; k < pins.size(); ++k) d->pins[k] = pins[k];
     }
     *out = d.release();
   });
@@ -250,13 +255,14 @@ int sl3_degree_operator_solve(sl3_degree_operator* op, double* b_dev, int nrhs,
     }
     if (used_cholesky) *used_cholesky = op->cholesky ? 1 : 0;
     if (nrhs == 0) return;
-    ScaleRows(b_dev, op->scale.data(), op->ndof, nrhs, op->pinned, s);
+    ScaleRows(b_dev, op->scale.data(), op->ndof, nrhs, op->pins, s);
     const int status = op->cholesky
                            ? sl3_band_cholesky_solve(op->factor.data(), op->ndof, op->kd, b_dev, nrhs)
                            : sl3_band_lu_solve(op->factor.data(), op->ndof, op->kd, op->kd,
                                                op->piv.data(), b_dev, nrhs);
     if (status != SL3_OK) throw HipError(sl3_last_error());
-    ScaleRows(b_dev, op->scale.data(), op->ndof, nrhs, -1, s);
+    const int none[2] = {-1, -1};
+    ScaleRows(b_dev, op->scale.data(), op->ndof, nrhs, none, s);
     SL3_HIP(hipStreamSynchronize(s));
   });
 }

@@ -269,6 +269,9 @@ int sl3_earth_degree_system(const sl3_earth* earth, int l, int kind, int scheme,
       if (ndof) *ndof = op.DofCount();
       if (kd) *kd = op.Kd();
       if (first_element) *first_element = op.FirstElement();
+      // one pinned dof per solid region at degree 1: the first (the
+      // innermost region's) here; an Earth with a solid inner core has a
+      // second, at the mantle's top, which the device step pins as well
       if (pinned) *pinned = op.Pinned().empty() ? -1 : op.Pinned()[0];
       if (band) std::memcpy(band, op.Band().data(), sizeof(double) * op.Band().size());
       if (dofs) {

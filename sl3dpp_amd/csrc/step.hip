@@ -519,10 +519,11 @@ void Viscoelastic3D::BuildTasks() {
     t.firstElement = coupling_.FirstElementOfDegree(l);
     t.firstNode = coupling_.FirstNodes()[l];
     t.ndof = kind == 0 ? sph[l]->DofCount() : tor[l]->DofCount();
-    t.pinned = (kind == 1 && !tor[l]->Pinned().empty()) ? tor[l]->Pinned()[0] : -1;
-    if (kind == 1 && tor[l]->Pinned().size() > 1) {
-      throw ArgumentError("one solid region above the fluid core is supported");
-    }
+    // the toroidal degree-1 system pins one dof per solid region (its
+    // rigid rotation): none of them is a memory dof
+    const std::vector<int> pins = kind == 1 ? tor[l]->Pinned() : std::vector<int>{};
+    auto pinned = [&](int d) { return std::find(pins.begin(), pins.end(), d) != pins.end(); };
+    t.pinned = pins.empty() ? -1 : pins.back();
     t.rhsOff = static_cast<long long>(rhsDoubles_);
     rhsDoubles_ += static_cast<size_t>(t.ndof) * t.cols;
     t.scaleOff = static_cast<long long>(scaleDoubles_);
@@ -551,14 +552,14 @@ void Viscoelastic3D::BuildTasks() {
     std::vector<int> dm;
     for (size_t k = static_cast<size_t>(t.dofOff); k < dofsHost_.size(); ++k) {
       const int d = dofsHost_[k];
-      if (d >= 0 && d != t.pinned) dm.push_back(d);
+      if (d >= 0 && !pinned(d)) dm.push_back(d);
     }
     std::sort(dm.begin(), dm.end());
     dm.erase(std::unique(dm.begin(), dm.end()), dm.end());
     for (size_t k = static_cast<size_t>(t.dofOff); k < dofsHost_.size(); ++k) {
       const int d = dofsHost_[k];
       int row = -1;
-      if (d >= 0 && d != t.pinned) {
+      if (d >= 0 && !pinned(d)) {
         row = static_cast<int>(std::lower_bound(dm.begin(), dm.end(), d) - dm.begin());
       }
       dofsMemHost_.push_back(row);

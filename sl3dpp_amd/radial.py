@@ -182,6 +182,125 @@ class DeckModel:
         return bool(np.all(ly[:, 3] == 0.0) and np.all(ly[:, 5] == 0.0))
 
 
+def read_legacy_deck(path):
+    """Deck::ReadLegacySl3d (src/Deck.cpp:103-170): the legacy gia1d/sl3d
+    model (a title line; `nknot ntau`, ntau = 1; rows `r rho kappa[GPa]
+    mu[GPa] log10(viscosity)`, 0 meaning no viscosity), split into layers
+    where a radius repeats. Returns (title, layers): per layer a (rows x 5)
+    array r (m), rho, kappa (Pa), mu (Pa), viscosity (Pa s, NaN where
+    absent)."""
+    with open(path) as f:
+        title = f.readline().strip()
+        head = f.readline().split()
+        nknot, ntau = int(head[0]), int(head[1])
+        if ntau != 1:
+            raise ValueError(f"{path}: only ntau = 1 legacy files are supported")
+        values = [float(x) for x in f.read().split()]
+    data = np.asarray(values[: 5 * nknot], dtype=np.float64).reshape(nknot, 5)
+    data[:, 2:4] *= 1e9
+    with np.errstate(over="ignore"):
+        data[:, 4] = np.where(data[:, 4] == 0.0, np.nan, 10.0 ** data[:, 4])
+    layers, start = [], 0
+    for i in range(1, nknot):
+        if data[i, 0] == data[i - 1, 0]:
+            layers.append(data[start:i])
+            start = i
+    layers.append(data[start:])
+    for ly in layers:
+        absent = np.isnan(ly[:, 4])
+        if absent.any() and not absent.all():
+            raise ValueError(f"{path}: a layer mixes zero and non-zero viscosity")
+    return title, layers
+
+
+class LegacyDeckModel(DeckModel):
+    """EarthModel::FromDeck with Options::spline = Natural on a legacy deck
+    (src/EarthModel.cpp:86-260): rho, kappa and mu joined per layer by
+    natural cubic splines (a line through two rows), the viscosity in its
+    logarithm by straight lines (planetmodel's `log-linear` kind for a
+    viscosity column); isotropic moduli A = C = kappa + 4 mu / 3, F = kappa
+    - 2 mu / 3, L = N = mu (NodalModuli::Isotropic, src/Material.cpp:9-24);
+    gravity from the density's mass integral on the spline's segments."""
+
+    def __init__(self, layers):
+        from scipy.interpolate import CubicSpline
+
+        self.layers = layers
+        self.join = "natural"
+        self.boundaries = [layers[0][0, 0]] + [ly[-1, 0] for ly in layers]
+        self.radius = self.boundaries[-1]
+        self.splines = [CubicSpline(ly[:, 0], ly[:, 1:4], axis=0, bc_type="natural")
+                        if len(ly) > 1 else None for ly in layers]
+        self.mass_below = [0.0]
+        for i, ly in enumerate(layers):
+            self.mass_below.append(self.mass_below[-1] + self._mass(i, ly[0, 0], ly[-1, 0]))
+
+    def _rho(self, layer, r):
+        return self.isotropic(layer, r)[0]
+
+    def isotropic(self, layer, r):
+        """(rho, drho/dr, kappa, mu) in `layer` at r."""
+        sp = self.splines[layer]
+        if sp is None:
+            row = self.layers[layer][0]
+            return row[1], 0.0, row[2], row[3]
+        v = sp(r)
+        return float(v[0]), float(sp(r, 1)[0]), float(v[1]), float(v[2])
+
+    def viscosity(self, layer, r):
+        """The viscosity (Pa s) at r, None where the layer has none: the
+        logarithm joined by straight lines between the rows."""
+        ly = self.layers[layer]
+        if np.isnan(ly[0, 4]):
+            return None
+        if len(ly) == 1:
+            return float(ly[0, 4])
+        return float(np.exp(np.interp(r, ly[:, 0], np.log(ly[:, 4]))))
+
+    def is_fluid(self, layer):
+        return bool(np.all(self.layers[layer][:, 3] == 0.0))
+
+
+def graded_mesh_edges(model, lmax, eps=1e-8):
+    """RadialMesh::CreateGraded (src/RadialMesh.cpp:59-111): from the
+    surface down, elements of width max(0.1 a / (lmax + 1), 0.1 a
+    ln(1 - d/a) / ln(eps)) at depth d (non-dimensional radii, the surface
+    one), every model boundary an element boundary, the last one or two
+    elements above a boundary spanning what is left (split in two where it
+    exceeds the width). Returns the edges in metres and each element's
+    layer."""
+    a_si = model.radius
+    b = [x / a_si for x in model.boundaries]
+    a = b[-1]
+    floor = 0.1 * a / (lmax + 1)
+    log_eps = math.log(eps)
+
+    def width(r):
+        depth = a - r
+        if depth <= 0.0:
+            return floor
+        return max(floor, 0.1 * a * math.log1p(-depth / a) / log_eps)
+
+    edges = [a]
+    for s in range(len(b) - 2, -1, -1):
+        lo = b[s]
+        r = edges[-1]
+        while True:
+            w = width(r)
+            if r - lo <= 2.0 * w:
+                n = 1 if r - lo <= w else 2
+                for k in range(1, n + 1):
+                    edges.append(r + (lo - r) * (k / n))
+                edges[-1] = lo
+                break
+            r -= w
+            edges.append(r)
+    edges.reverse()
+    layer = [model.layer_of(0.5 * (edges[e] + edges[e + 1]) * a_si, True)
+             for e in range(len(edges) - 1)]
+    return [x * a_si for x in edges], layer
+
+
 def mesh_edges(model, lmax=None, drmax=None):
     """RadialMesh::Create(model, {lmax} or {drmax}): every model boundary an
     element boundary, each span cut uniformly into elements no wider than
@@ -291,6 +410,48 @@ class Earth(_StepEarth):
         earth.layer = layer
         earth.branch_itau = branch_itau
         return earth
+
+
+def legacy_earth(path, lmax, ngll=5, eps=1e-8):
+    """The legacy sl3d model as the Fortran gate builds it
+    (benchmarks/fortran_gate.cpp:52-57): Deck::ReadLegacySl3d,
+    EarthModel::FromDeck with natural splines, RadialMesh::CreateGraded
+    (lmax, ngll 5), Material::Create and Rheology::FromModel (one Maxwell
+    branch where a layer holds a viscosity: mu the shear modulus, itau =
+    mu / eta at the nodes; src/Rheology.cpp:129-163, 288-326)."""
+    _, layers = read_legacy_deck(path)
+    model = LegacyDeckModel(layers)
+    edges, layer = graded_mesh_edges(model, lmax, eps)
+    xi = gll_nodes(ngll)
+    E = len(edges) - 1
+    arrays = {k: np.zeros((E, ngll)) for k in ("rho", "drho", "g", "A", "C", "F", "L", "N")}
+    branch_mu, branch_itau = np.zeros((E, ngll)), np.zeros((E, ngll))
+    fluid = np.zeros(E, dtype=np.int32)
+    for e in range(E):
+        k = layer[e]
+        fluid[e] = model.is_fluid(k)
+        lo, hi = edges[e], edges[e + 1]
+        for i in range(ngll):
+            r = lo if i == 0 else hi if i == ngll - 1 else lo + (xi[i] + 1.0) * 0.5 * (hi - lo)
+            rho, drho, kappa, mu = model.isotropic(k, r)
+            arrays["rho"][e, i], arrays["drho"][e, i] = rho, drho
+            arrays["g"][e, i] = model.gravity(k, r)
+            arrays["A"][e, i] = arrays["C"][e, i] = kappa + 4.0 * mu / 3.0
+            arrays["F"][e, i] = kappa - 2.0 * mu / 3.0
+            arrays["L"][e, i] = arrays["N"][e, i] = mu
+            eta = model.viscosity(k, r)
+            if eta is not None and not fluid[e]:
+                branch_mu[e, i] = mu
+                branch_itau[e, i] = mu / eta
+    viscous = bool(np.any(branch_itau > 0.0))
+    earth = Earth(edges, ngll, arrays["rho"], arrays["drho"], arrays["g"], arrays["A"],
+                  arrays["C"], arrays["F"], arrays["L"], arrays["N"], fluid,
+                  branch_mu=branch_mu if viscous else None,
+                  branch_itau=branch_itau if viscous else None, length=model.radius)
+    earth.model = model
+    earth.layer = layer
+    earth.branch_itau = branch_itau
+    return earth
 
 
 class DegreeOperator:
