@@ -419,6 +419,17 @@ int sl3_step_add_memory(sl3_operator* op, const double* const* increment_dev);
 /* OpenStep::AddForce's tensor (:585-598): an analysed tensor (device
  * layout) joins the fixed force. */
 int sl3_step_add_force(sl3_operator* op, const double* analysed_dev);
+/* The discrete 3D adjoint's transposed pass of the open step
+ * (src/AdjointDiscrete3D.cpp:998-1037: passWith's map without the sea-level
+ * solve and the measurement duals): out = Analyse(W x Synthesise(
+ * StrainOf(SolveForces(Entry(z)))) + src), with Entry the forces of z
+ * (SpheroidalForces, ToroidalForce), StrainOf without channels, W the open
+ * step's lateral weight and src a grid tensor (src_dev[3], grid blocks as
+ * the state's m0 m1 m2; NULL for none). Device-layout spectral tensors; z
+ * and out distinct. The adjoint of sl3_step_pass's linear part in the
+ * quadrature-weighted pairing of spectral tensors. */
+int sl3_step_transposed_pass(sl3_operator* op, const double* z_dev,
+                             const double* const* src_dev, double* out_dev);
 /* The open step's device buffers, borrowed (OpenStep's friends weight_,
  * beta_, memory_, fixed_: include/sl3dpp/Viscoelastic3D.hpp:614-646):
  * "weight", "beta" (grid values), "fixed", "strain" (device-layout spectral

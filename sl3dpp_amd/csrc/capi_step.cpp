@@ -539,6 +539,16 @@ int sl3_step_add_force(sl3_operator* op, const double* analysed_dev) {
   });
 }
 
+int sl3_step_transposed_pass(sl3_operator* op, const double* z_dev,
+                             const double* const* src_dev, double* out_dev) {
+  return Guard([&] {
+    Check(op != nullptr && z_dev != nullptr && out_dev != nullptr, "null argument");
+    Check(z_dev != out_dev, "z and out must not overlap");
+    op->op->OpenTransposedPass(z_dev, src_dev, out_dev);
+    SL3_HIP(hipStreamSynchronize(Stream()));
+  });
+}
+
 int sl3_step_view(sl3_operator* op, const char* name, const double** dev, int64_t* count) {
   return Guard([&] {
     Check(op != nullptr && name != nullptr && dev != nullptr, "null argument");

@@ -1223,6 +1223,48 @@ void Viscoelastic3D::OpenPass(const double* s, double* next) {
   Pass(s, next, useStartForce, *open_.flavour);
 }
 
+void Viscoelastic3D::OpenTransposedPass(const double* z, const double* const src[3],
+                                        double* out) {
+  RequireOpen();
+  hipStream_t st = Stream();
+  Flavour& f = *open_.flavour;
+  const size_t S = coupling_.SpectralDoubles();
+  if (zeroAmp_.size() != 2 * ampDoubles_) {
+    zeroAmp_.Resize(2 * ampDoubles_);
+    LaunchFill(zeroAmp_.data(), zeroAmp_.size(), 0.0, st);
+  }
+  strainT_.Resize(S);
+  double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
+  double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
+  // the source's analysis first (it shares the Fourier buffers)
+  const double* addend = nullptr;
+  if (src != nullptr) {
+    sourceT_.Resize(S);
+    for (int b = 0; b < 3; ++b) {
+      RingLaunch r;
+      r.block = b;
+      r.op = kRingAnalyse;
+      r.gridIn = src[b];
+      r.fourierOut = B[b];
+      coupling_.Ring(r, st);
+    }
+    coupling_.AnalyseFromFourier(B, sourceT_.data(), nullptr, true, st);
+    addend = sourceT_.data();
+  }
+  // Entry and SolveForces: the forces of z and the memory response
+  SolveAll(z, f);
+  // StrainOf of the dual solutions, no channels
+  const auto* sph = tasksDev_.data();
+  const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
+  LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), dofsMem_.data(),
+               response_.data(), f.unitsMem.data(), zeroAmp_.data(),
+               f.unitsMem.data() + dmHost_.size(), nullptr, coupling_.View(strainT_.data()), st);
+  // Exit's synthesis, the lateral weight, the analysis (+ the source)
+  coupling_.SynthesiseToFourier(strainT_.data(), A, true, st);
+  coupling_.RingPass(A, B, weight_.data(), st);
+  coupling_.AnalyseFromFourier(B, out, addend, true, st);
+}
+
 StepInfo Viscoelastic3D::OpenClose(const double* start, double warmGuard) {
   RequireOpen();
   StepInfo info;

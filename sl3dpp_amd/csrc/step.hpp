@@ -255,6 +255,14 @@ class Viscoelastic3D {
   // (an analysed tensor, device layout) (:576-598).
   void OpenAddMemory(const double* const increment[3]);
   void OpenAddForce(const double* analysed);
+  // The discrete adjoint's transposed pass of the open step
+  // (src/AdjointDiscrete3D.cpp:998-1037, without the sea-level and
+  // measurement duals): out = Analyse(W Synthesise(StrainOf(SolveForces(
+  // Entry(z)))) + src), Entry the forces of z, StrainOf without channels,
+  // src a grid tensor (per block, device; null for none). The pass's own
+  // pieces in the other order: the adjoint of OpenPass's linear part in
+  // the quadrature-weighted pairing of spectral tensors.
+  void OpenTransposedPass(const double* z, const double* const src[3], double* out);
   // Drops the open step without finishing it (the state keeps the decayed
   // memory m~ the start wrote).
   void OpenCancel() { open_.state = nullptr; }
@@ -337,6 +345,9 @@ class Viscoelastic3D {
   std::list<std::unique_ptr<Flavour>> flavours_;
   // the open step's buffers
   DeviceArray<double> weight_, beta_, fixed_, analysed_, amp_, s_, sNext_;
+  // the transposed pass's buffers: zero channel amplitudes, its strain, the
+  // source's analysis
+  DeviceArray<double> zeroAmp_, strainT_, sourceT_;
   DeviceArray<double> coldStart_;  // the step's own start under a guarded warm start
   size_t ampDoubles_ = 0;  // one channel's amplitudes, (L + 1)(L + 2)
   std::vector<double> ampHost_;  // the last amplitudes gathered (load, then tide)

@@ -429,6 +429,18 @@ class Viscoelastic3D:
         arr = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks])
         check(lib.sl3_step_add_memory(self.handle, arr))
 
+    def transposed_pass(self, z, out, src=None):
+        """The discrete adjoint's transposed pass of the open step
+        (src/AdjointDiscrete3D.cpp:998-1037 without the sea-level duals):
+        out = Analyse(W Synthesise(StrainOf(SolveForces(Entry(z)))) + src),
+        device-layout spectral tensors z and out, src three grid blocks or
+        None."""
+        arr = None
+        if src is not None:
+            arr = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in src])
+        check(lib.sl3_step_transposed_pass(self.handle, ctypes.c_void_p(z.data_ptr()), arr,
+                                           ctypes.c_void_p(out.data_ptr())))
+
     def relative_change(self, nxt, now):
         out = ctypes.c_double()
         check(lib.sl3_step_relative_change(self.handle, ctypes.c_void_p(nxt.data_ptr()),
