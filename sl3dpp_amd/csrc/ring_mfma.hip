@@ -677,15 +677,20 @@ __device__ inline void RingBody(const RingLaunch& a, int node, int th0) {
 
 // Every class runs two waves per SIMD (two workgroups per CU, as the LDS
 // tile allows): a register budget of 256 per lane. The step start fits it
-// by reading m~ back for its second analysis instead of holding it.
+// by reading m~ back for its second analysis instead of holding it. Five
+// tiles (64 < H <= 80: lmax 192's P = 145) hold a fifth of each per-point
+// array more and their tables fill most of the LDS, so one workgroup per CU
+// and one wave per SIMD: the budget is 512.
+template <int JT>
+constexpr int WavesPerEu() { return JT >= 5 ? 1 : 2; }
 template <int JT, bool REAL, int CLASS, bool FULL>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerEu<JT>())))
     RingMfmaKernel(RingLaunch a) {
   RingBody<JT, REAL, CLASS, FULL>(a, blockIdx.y, blockIdx.x * MakeGeometry<REAL>(a).ringsPerGroup);
 }
 
 template <int JT, bool REAL, bool FULL>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerEu<JT>())))
     RingMfmaStartKernel(RingLaunch a) {
   RingBody<JT, REAL, kClassStart, FULL>(a, blockIdx.y,
                                         blockIdx.x * MakeGeometry<REAL>(a).ringsPerGroup);
@@ -700,7 +705,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
 constexpr int kPassGroups = 3;
 
 template <int JT, bool FULL, int KIN>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerEu<JT>())))
     RingMfmaPassKernel(RingLaunch base, RingBlocks blocks) {
   const int p = blockIdx.x;
   const int st = (p / (8 * kPassGroups)) * 8 + p % 8;  // the super-tile
@@ -809,12 +814,17 @@ void LaunchRingPassMfma(const RingLaunch& r, const RingBlocks& blocks, hipStream
   // count
   if (r.tables.jt == 3 && full && kIn == 8) {
     SL3_RING_PASS(3, true, 8);
+  } else if (r.tables.jt == 5 && !full && kIn == 12) {
+    // lmax 192 (P = 145, H = 72, kmax 48): the complex blocks hold 12
+    // operand blocks instead of 20
+    SL3_RING_PASS(5, false, 12);
   } else {
     switch (r.tables.jt) {
       case 1: if (full) SL3_RING_PASS(1, true, 0); else SL3_RING_PASS(1, false, 0); break;
       case 2: if (full) SL3_RING_PASS(2, true, 0); else SL3_RING_PASS(2, false, 0); break;
       case 3: if (full) SL3_RING_PASS(3, true, 0); else SL3_RING_PASS(3, false, 0); break;
       case 4: if (full) SL3_RING_PASS(4, true, 0); else SL3_RING_PASS(4, false, 0); break;
+      case 5: if (full) SL3_RING_PASS(5, true, 0); else SL3_RING_PASS(5, false, 0); break;
       default: throw ArgumentError("ring pass: no matrix-core tiling for this grid");
     }
   }
@@ -835,6 +845,7 @@ void LaunchRingMfma(const RingLaunch& r, hipStream_t stream) {
     case 2: LaunchJT<2>(r, grid, lds, stream); break;
     case 3: LaunchJT<3>(r, grid, lds, stream); break;
     case 4: LaunchJT<4>(r, grid, lds, stream); break;
+    case 5: LaunchJT<5>(r, grid, lds, stream); break;
     default: throw ArgumentError("ring transform: no matrix-core tiling for this grid");
   }
   SL3_HIP(hipGetLastError());

@@ -43,7 +43,7 @@ struct DftTables {
 };
 
 // The largest jt the matrix-core ring kernel is instantiated for.
-constexpr int kRingMfmaMaxTiles = 4;
+constexpr int kRingMfmaMaxTiles = 5;
 
 enum RingOp {
   kRingSynthesise = 0,        // F -> grid

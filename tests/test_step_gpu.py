@@ -58,8 +58,8 @@ def make_operator(earth, lmax, delta_scale, options=None, seed=11):
 # generic ring kernel); lmax 64: two 16-row tiles per P-point transform and
 # two 64-degree blocks in the Legendre analysis; lmax 140: ntheta 212, two
 # colatitude blocks in the Legendre synthesis (and P = 106, the generic
-# ring kernel).
-@pytest.mark.parametrize("lmax", [8, 10, 12, 32, 64, 140])
+# ring kernel); lmax 192: five 16-row tiles (P = 145, H = 72).
+@pytest.mark.parametrize("lmax", [8, 10, 12, 32, 64, 140, 192])
 @pytest.mark.parametrize("spin", [-2, -1, 0])
 def test_gsh_transforms_match_oracle(earth, oracle, lmax, spin):
     import torch
@@ -159,8 +159,8 @@ def run_pair(earth, oracle, lmax, delta_scale, steps, max_iterations, rtol, hist
 # all frequencies (H = 48), the workload's grid, with the complex blocks'
 # contraction fixed at compile time; lmax 96 (P = 73, H = 36) and 160
 # (P = 121, H = 60): three and four tiles with padding rows and a run-time
-# contraction length; lmax 192 (P = 145, H = 72: five tiles, beyond the
-# matrix-core kernels, so the generic longitude kernel at full size; 290
+# contraction length; lmax 192 (P = 145, H = 72: five tiles, one wave per
+# SIMD, the complex blocks' contraction fixed at 12 blocks; 290
 # colatitudes in two synthesis blocks); lmax 2 (R = 2), 3 (P = 3: one
 # frequency) and 4 (P = 4, even): the smallest grids of each kernel
 @pytest.mark.parametrize("lmax", [2, 3, 4, 8, 10, 12, 32, 64, 96, 128, 160, 192])
