@@ -76,6 +76,8 @@ def parse():
                          "ramped to zero, 100 yr; 0 skips it)")
     ap.add_argument("--maxwell1d-lmax", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph-leg", action="store_true",
+                    help="skip the leg that replays every pass from a captured hipGraph")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run device copies of 1 GiB (the HBM byte counters' scale)")
@@ -391,6 +393,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and not cg:
         cpu = cpu_baseline(op, earth, dt, load, float(np.mean(passes)), args.rtol)
+    graphs = None
+    if not cg and not args.no_graph_leg:
+        graphs = graph_passes_leg(op, dt, load, args, steps_per_s, plain_final)
     extrapolated = None
     if args.extrapolate_order > 0 and not cg:
         extrapolated = extrapolated_leg(op, dt, load, args, steps_per_s, plain_final)
@@ -428,6 +433,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "graph_passes": graphs,
         "extrapolated_start": extrapolated,
         "learned": learned,
         "maxwell1d": maxwell1d,
@@ -435,6 +441,50 @@ def main():
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def graph_passes_leg(op, dt, load, args, plain_steps_per_s, plain_final):
+    """The timed trajectory again (zero state, the same warm-up and timed
+    steps) with every pass replayed from a captured hipGraph; its final state
+    equals the plain trajectory's bit for bit."""
+    import numpy as np
+    import torch
+
+    import sl3dpp_amd
+    import sl3dpp_amd.step as st
+
+    op.set_options(graph_passes=1)
+    try:
+        state = op.zero_state()
+        for k in range(args.warmup):
+            op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+        passes = []
+        torch.cuda.synchronize()
+        sl3dpp_amd.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            info = op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+            passes.append(info.iterations)
+            assert info.converged, "a step of graph passes did not converge"
+        sl3dpp_amd.synchronize()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        final = state.export("coefficients")
+    finally:
+        op.set_options(graph_passes=0)
+    out = {
+        "steps_per_s": args.steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "passes_per_step": passes,
+        "speedup_vs_plain": (args.steps / elapsed) / plain_steps_per_s,
+        "final_state_equals_plain": bool(np.array_equal(final, plain_final)),
+        "what": ("the timed trajectory again from the zero state with each pass (synthesis, "
+                 "ring pass, analysis, force, response, strain) replayed from a hipGraph "
+                 "captured on its first use"),
+    }
+    log(f"graph passes: {out['steps_per_s']:.2f} steps/s, x{out['speedup_vs_plain']:.3f} of "
+        f"plain, final state equal: {out['final_state_equals_plain']}")
+    return out
 
 
 def extrapolated_leg(op, dt, load, args, plain_steps_per_s, plain_final):

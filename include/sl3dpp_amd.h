@@ -274,6 +274,10 @@ typedef struct {
   double solve_rtol;    /* 1e-11: the relative residual in the preconditioned
                            energy norm the conjugate gradients stop at */
   int solve_max_iterations; /* 200 */
+  int graph_passes;     /* 0: a pass's kernels are launched one by one; 1:
+                           each pass replays a hipGraph captured on its first
+                           use (per input/output tensor pair, flavour and tide
+                           channel), the same kernels and results */
 } sl3_step_options;
 
 typedef struct {

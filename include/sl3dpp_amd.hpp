@@ -85,6 +85,8 @@ class Viscoelastic3D {
     StepSolver solver = StepSolver::FixedPoint;
     double solve_rtol = 1e-11;
     int solve_max_iterations = 200;
+    /* not in the reference: each pass replays a captured hipGraph */
+    bool graph_passes = false;
   };
 
   /* Viscoelastic3D::StepInfo (:357-375). */
@@ -332,6 +334,7 @@ class Viscoelastic3D {
                                                                      : SL3_FIXED_POINT;
     c.solve_rtol = o.solve_rtol;
     c.solve_max_iterations = o.solve_max_iterations;
+    c.graph_passes = o.graph_passes ? 1 : 0;
     return c;
   }
 

@@ -74,6 +74,7 @@ StepOptions ToOptions(const sl3_step_options* o) {
   s.solver = o->solver;
   s.solveRtol = o->solve_rtol;
   s.solveMaxIterations = o->solve_max_iterations;
+  s.graphPasses = o->graph_passes != 0;
   Check(s.solver == kFixedPoint || s.solver == kConjugateGradients,
         "solver: SL3_FIXED_POINT or SL3_CONJUGATE_GRADIENTS");
   Check(s.solveMaxIterations >= 0, "solve_max_iterations >= 0");
@@ -294,6 +295,7 @@ void sl3_step_options_default(sl3_step_options* o) {
   o->solver = s.solver;
   o->solve_rtol = s.solveRtol;
   o->solve_max_iterations = s.solveMaxIterations;
+  o->graph_passes = s.graphPasses ? 1 : 0;
 }
 
 int sl3_operator_create(const sl3_earth* earth, int lmax, const double* delta_dev,

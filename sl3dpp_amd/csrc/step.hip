@@ -453,6 +453,7 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
 }
 
 Viscoelastic3D::~Viscoelastic3D() {
+  DropPassGraphs(0);
   for (auto& e : events_)
     if (e != nullptr) (void)hipEventDestroy(e);
   if (readEvent_ != nullptr) (void)hipEventDestroy(readEvent_);
@@ -612,8 +613,12 @@ Flavour& Viscoelastic3D::FlavourOf(int scheme, double dt) {
       return *flavours_.front();
     }
   }
-  while (flavours_.size() >= 4) flavours_.pop_back();
+  while (flavours_.size() >= 4) {
+    DropPassGraphs(flavours_.back()->id);
+    flavours_.pop_back();
+  }
   auto f = std::make_unique<Flavour>();
+  f->id = ++flavourCount_;
   f->scheme = scheme;
   f->dt = dt;
   f->pointShear = earth_->reference.PointShearModulus(scheme, dt);
@@ -987,7 +992,57 @@ double Viscoelastic3D::RelativeChange(const double* next, const double* now, boo
   return r[0] > 0.0 ? std::sqrt(r[1]) / std::sqrt(r[0]) : 0.0;
 }
 
+void Viscoelastic3D::DropPassGraphs(unsigned long long flavour) {
+  for (auto it = passGraphs_.begin(); it != passGraphs_.end();) {
+    if (flavour == 0 || it->flavour == flavour) {
+      if (it->exec != nullptr) (void)hipGraphExecDestroy(it->exec);
+      it = passGraphs_.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+// A pass replays the graph captured for its tensors, flavour and tide
+// channel (captured on first use: the kernels and their arguments are
+// those LaunchPass records). The first pass after a stored strain's start
+// reads the start's own force (analysed_) and is launched directly.
 void Viscoelastic3D::Pass(const double* s, double* sNext, bool useStartForce, Flavour& f) {
+  rhsCurrent_ = false;
+  if (!options_.graphPasses || useStartForce) {
+    LaunchPass(s, sNext, useStartForce, f);
+    return;
+  }
+  hipStream_t st = Stream();
+  const bool tide = TideAmplitudes() != nullptr;
+  PassGraph* g = nullptr;
+  for (auto& p : passGraphs_) {
+    if (p.s == s && p.next == sNext && p.flavour == f.id && p.tide == tide) g = &p;
+  }
+  if (g == nullptr) {
+    hipGraph_t graph = nullptr;
+    SL3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      LaunchPass(s, sNext, false, f);
+    } catch (...) {
+      hipGraph_t partial = nullptr;
+      (void)hipStreamEndCapture(st, &partial);
+      if (partial != nullptr) (void)hipGraphDestroy(partial);
+      throw;
+    }
+    SL3_HIP(hipStreamEndCapture(st, &graph));
+    PassGraph p{s, sNext, f.id, tide, nullptr};
+    const hipError_t made = hipGraphInstantiate(&p.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SL3_HIP(made);
+    passGraphs_.push_back(p);
+    g = &passGraphs_.back();
+  }
+  SL3_HIP(hipGraphLaunch(g->exec, st));
+}
+
+void Viscoelastic3D::LaunchPass(const double* s, double* sNext, bool useStartForce,
+                                Flavour& f) {
   hipStream_t st = Stream();
   if (!useStartForce) {
     double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};

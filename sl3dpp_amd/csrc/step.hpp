@@ -42,6 +42,9 @@ struct StepOptions {
   int solver = kFixedPoint;
   double solveRtol = 1e-11;
   int solveMaxIterations = 200;
+  // each pass (synthesis, ring pass, analysis, force, response, strain)
+  // replays a hipGraph captured on its first use
+  bool graphPasses = false;
 };
 
 // Viscoelastic3D::StepInfo (include/sl3dpp/Viscoelastic3D.hpp:357-375).
@@ -136,6 +139,7 @@ class Coupling {
 // The factorised systems of one (scheme, dt): Viscoelastic3D::Flavour
 // (src/Viscoelastic3D.cpp:158-167, 535-558).
 struct Flavour {
+  unsigned long long id = 0;  // unique per operator: keys the captured passes
   int scheme = 0;
   double dt = 0.0;
   std::vector<double> pointShear;
@@ -226,8 +230,11 @@ class Viscoelastic3D {
   const std::vector<double>& ItauRefNode() const { return itauRefHost_; }
   int CholeskyFallbacks(int scheme, double dt);
 
-  // One pass: Respond(s) + AddChannels + StrainOf into sNext.
+  // One pass: Respond(s) + AddChannels + StrainOf into sNext; with
+  // graphPasses a replay of the pass captured for (s, sNext, f, tide).
   void Pass(const double* s, double* sNext, bool useStartForce, Flavour& f);
+  // The captured passes held (tests).
+  int PassGraphCount() const { return static_cast<int>(passGraphs_.size()); }
 
   // OpenStep (include/sl3dpp/Viscoelastic3D.hpp:461-647) on this
   // operator's buffers, one open step at a time; Step is OpenStart +
@@ -382,6 +389,22 @@ class Viscoelastic3D {
   double* pinned_ = nullptr;
   hipEvent_t readEvent_ = nullptr;
   void ReadBack(double* dst, const double* src, int count);
+  // The pass's kernels on the stream (what Pass captures).
+  void LaunchPass(const double* s, double* sNext, bool useStartForce, Flavour& f);
+  // A pass captured as a graph, keyed by the tensors it reads and writes,
+  // its flavour and whether the tide channel is in use (the pointers its
+  // kernels take; every other buffer is fixed for the operator's life).
+  struct PassGraph {
+    const double* s = nullptr;
+    double* next = nullptr;
+    unsigned long long flavour = 0;
+    bool tide = false;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<PassGraph> passGraphs_;
+  unsigned long long flavourCount_ = 0;
+  // Destroys the captured passes of one flavour (0: all).
+  void DropPassGraphs(unsigned long long flavour);
 };
 
 }  // namespace sl3amd

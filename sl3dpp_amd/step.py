@@ -60,6 +60,7 @@ class StepOptions(ctypes.Structure):
         ("solver", ctypes.c_int),
         ("solve_rtol", ctypes.c_double),
         ("solve_max_iterations", ctypes.c_int),
+        ("graph_passes", ctypes.c_int),
     ]
 
     @classmethod
