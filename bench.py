@@ -390,6 +390,12 @@ def main():
                 roofline["fp64_flops_per_launch"] = flops[1]
                 roofline["fp64_frac"] = flops[1] / avg_s / PEAK_FP64
     breakdown = {k: {"total_ms": round(v[0], 3), "launches": v[1]} for k, v in kernels.items()}
+    t = op.export("tuning")
+    tuning = {"legendre_analysis_rows": int(t[0]), "legendre_synthesis_theta": int(t[1]),
+              "analysis_ms": {str(r): round(float(v), 4)
+                              for r, v in zip((16, 32, 64, 128), t[2:6])},
+              "synthesis_ms": {str(r): round(float(v), 4)
+                               for r, v in zip((32, 64, 112, 208), t[6:10])}}
     cpu = None
     if not args.no_cpu_baseline and not cg:
         cpu = cpu_baseline(op, earth, dt, load, float(np.mean(passes)), args.rtol)
@@ -433,6 +439,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "graph_passes": graphs,
         "extrapolated_start": extrapolated,
         "learned": learned,

@@ -55,6 +55,16 @@ int sl3_profile_reset(void);
 int sl3_profile_read(const char* kernel, double* total_ms, int64_t* launches);
 int sl3_profile_names(char* buffer, int64_t capacity);
 
+/* The tile-shape tuner (no counterpart in the reference): an operator times
+ * every tile shape of its Legendre kernels on its own shape when it is
+ * made (degrees per analysis workgroup 16, 32, 64 or 128; colatitudes per
+ * synthesis workgroup 32, 64, 112 or 208) and keeps the fastest; a shape met
+ * before in the process takes the choice made then. Every choice gives the
+ * same bits. This call fixes the shapes of operators made from now on
+ * (0: tune), for tests and timing studies; the operator export "tuning"
+ * reports the choice and the timings. */
+int sl3_tuning_set(int analysis_rows, int synthesis_theta);
+
 /* ===================================================================== *
  * Banded Cholesky and LU, FP64, LAPACK band storage (column-major bands, *
  * 0-based pivots). Replace sl3dpp::detail::BandCholeskyFactor,          *
@@ -476,7 +486,11 @@ int sl3_coupling_scale_by_node(sl3_operator* op, const double* factor, double* s
  *             solution of a task, ndof x (l+1) complex column-major),
  *             "tasks" (per task: l, kind, ndof, first memory node, memory
  *             dofs), "residuals" (last step), "solve_energy" (the last
- *             conjugate-gradient solve's <K^-1 f, f>, its residuals' scale).
+ *             conjugate-gradient solve's <K^-1 f, f>, its residuals' scale),
+ *             "tuning" (the Legendre tile shapes chosen, analysis rows and
+ *             synthesis colatitudes, then the ms per launch timed for the
+ *             analysis choices 16, 32, 64, 128 and the synthesis choices
+ *             32, 64, 112, 208).
  * dst NULL returns the size in doubles through *count. */
 int sl3_state_export(const sl3_operator* op, const sl3_state* state,
                      const char* name, double* dst, int64_t* count);

@@ -653,6 +653,10 @@ int sl3_state_export(const sl3_operator* op, const sl3_state* state, const char*
   });
 }
 
+int sl3_tuning_set(int analysis_rows, int synthesis_theta) {
+  return Guard([&] { SetTuningOverride(analysis_rows, synthesis_theta); });
+}
+
 int sl3_operator_export(sl3_operator* op, const char* name, double* dst, int64_t* count) {
   return Guard([&] {
     Check(op != nullptr && name != nullptr, "null argument");
@@ -669,6 +673,7 @@ int sl3_operator_export(sl3_operator* op, const char* name, double* dst, int64_t
     if (key == "solve_energy") return ExportHost(v.solveEnergy, dst, count);
     if (key == "mu_node") return ExportHost(v.MuNode(), dst, count);
     if (key == "itau_ref_node") return ExportHost(v.ItauRefNode(), dst, count);
+    if (key == "tuning") return ExportHost(c.Tuning(), dst, count);
     if (key == "tasks") {
       std::vector<double> t;
       for (const auto& task : v.Tasks()) {

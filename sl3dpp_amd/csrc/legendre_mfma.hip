@@ -39,14 +39,11 @@ constexpr int kWave = 64;
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;
 constexpr int kCols = 64;        // columns per workgroup, 16 per wave
-constexpr int kRowBlock = 64;    // analysis: degrees per workgroup
-constexpr int kThetaBlock = 208;  // synthesis: colatitudes per workgroup
 constexpr int kKa = 16;          // analysis: colatitudes per staged chunk
 constexpr int kKs = 16;          // synthesis: degrees per staged chunk
-// staged elements per thread: analysis table and data, synthesis table and data
-constexpr int kTa = kRowBlock * kKa / kThreads;
+// staged elements per thread: analysis data, synthesis data (the tables'
+// counts follow the tile shape)
 constexpr int kFa = kCols * kKa / kThreads;
-constexpr int kTs = kKs * kThetaBlock / kThreads;
 constexpr int kSs = kKs * kCols / kThreads;
 
 typedef double Acc4 __attribute__((ext_vector_type(4)));
@@ -94,10 +91,16 @@ __device__ inline bool DecodeAnalysis(const Args& a, int& z, int& cb, int& rb) {
   return z < a.orderCount;
 }
 
-// One workgroup: 64 degrees x 64 columns of one order (each wave 16
-// columns, four 16-degree tiles), the colatitudes streamed in chunks of 16
-// with the next chunk's loads in flight under the products.
+// One workgroup: ROWS degrees x 64 columns of one order (each wave 16
+// columns, ROWS / 16 16-degree tiles), the colatitudes streamed in chunks
+// of 16 with the next chunk's loads in flight under the products. Every
+// output sums its colatitudes in the same order whatever ROWS is: the tile
+// shape changes the work per workgroup and the data's reuse, not a bit of
+// the result.
+template <int ROWS>
 __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
+  constexpr int kRowBlock = ROWS;
+  constexpr int kTa = kRowBlock * kKa / kThreads;
   // padded rows: the staging stores run along the first index
   __shared__ double Ts[kKa][kRowBlock + 1];  // [theta][degree row]
   __shared__ double Fs[kKa][kCols + 1];      // [theta][column]
@@ -208,11 +211,16 @@ __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
   }
 }
 
-// One workgroup: 64 colatitudes x 64 columns of one order (each wave 16
-// columns, four 16-colatitude tiles), the degrees up to the block's reach
-// streamed in chunks of 16 with the next chunk's loads in flight under the
-// products; the colatitude blocks of a column block adjacent on one XCD.
+// One workgroup: THETA colatitudes x 64 columns of one order (each wave 16
+// columns, THETA / 16 16-colatitude tiles), the degrees up to the block's
+// reach streamed in chunks of 16 with the next chunk's loads in flight
+// under the products; the colatitude blocks of a column block adjacent on
+// one XCD. Every output sums its degrees in the same order whatever THETA
+// is.
+template <int THETA>
 __global__ void __launch_bounds__(kThreads) LegendreSynthesisMfmaKernel(Args a) {
+  constexpr int kThetaBlock = THETA;
+  constexpr int kTs = kKs * kThetaBlock / kThreads;
   __shared__ double Ts[kKs][kThetaBlock];  // [degree][theta]
   __shared__ double Ss[kKs][kCols];        // [degree][column]
   int z, cb, tb;
@@ -316,8 +324,6 @@ Args MakeArgs(const LegendrePlan& p) {
   a.orders = reinterpret_cast<const OrderRef*>(p.orders);
   a.orderCount = p.orderCount;
   a.colBlocks = CeilDiv(2 * p.nodes, kCols);
-  a.rowBlocks = CeilDiv(p.lmax + 1, kRowBlock);
-  a.thetaBlocks = CeilDiv(p.ntheta, kThetaBlock);
   a.lmax = p.lmax;
   a.nodes = p.nodes;
   a.ntheta = p.ntheta;
@@ -341,9 +347,17 @@ void LaunchLegendreSynthesisMfma(const LegendrePlan& p, const double* const spec
   }
   a.reach = reach;
   if (a.orderCount == 0) return;
+  const int theta = p.synthesisTheta > 0 ? p.synthesisTheta : kDefaultSynthesisTheta;
+  a.thetaBlocks = CeilDiv(p.ntheta, theta);
   a.rowBlocks = a.thetaBlocks;  // the block index the decode splits off
-  hipLaunchKernelGGL(LegendreSynthesisMfmaKernel, GridOf(a, a.thetaBlocks), dim3(kThreads), 0,
-                     stream, a);
+  const dim3 grid = GridOf(a, a.thetaBlocks);
+  switch (theta) {
+    case 32: hipLaunchKernelGGL(LegendreSynthesisMfmaKernel<32>, grid, dim3(kThreads), 0, stream, a); break;
+    case 64: hipLaunchKernelGGL(LegendreSynthesisMfmaKernel<64>, grid, dim3(kThreads), 0, stream, a); break;
+    case 112: hipLaunchKernelGGL(LegendreSynthesisMfmaKernel<112>, grid, dim3(kThreads), 0, stream, a); break;
+    case 208: hipLaunchKernelGGL(LegendreSynthesisMfmaKernel<208>, grid, dim3(kThreads), 0, stream, a); break;
+    default: throw ArgumentError("Legendre synthesis: colatitude block 32, 64, 112 or 208");
+  }
   SL3_HIP(hipGetLastError());
 }
 
@@ -359,8 +373,16 @@ void LaunchLegendreAnalysisMfma(const LegendrePlan& p, const double* const fouri
   a.reach = reach;
   a.firstNode = firstNode;
   if (a.orderCount == 0) return;
-  hipLaunchKernelGGL(LegendreAnalysisMfmaKernel, GridOf(a, a.rowBlocks), dim3(kThreads), 0,
-                     stream, a);
+  const int rows = p.analysisRows > 0 ? p.analysisRows : kDefaultAnalysisRows;
+  a.rowBlocks = CeilDiv(p.lmax + 1, rows);
+  const dim3 grid = GridOf(a, a.rowBlocks);
+  switch (rows) {
+    case 16: hipLaunchKernelGGL(LegendreAnalysisMfmaKernel<16>, grid, dim3(kThreads), 0, stream, a); break;
+    case 32: hipLaunchKernelGGL(LegendreAnalysisMfmaKernel<32>, grid, dim3(kThreads), 0, stream, a); break;
+    case 64: hipLaunchKernelGGL(LegendreAnalysisMfmaKernel<64>, grid, dim3(kThreads), 0, stream, a); break;
+    case 128: hipLaunchKernelGGL(LegendreAnalysisMfmaKernel<128>, grid, dim3(kThreads), 0, stream, a); break;
+    default: throw ArgumentError("Legendre analysis: degree block 16, 32, 64 or 128");
+  }
   SL3_HIP(hipGetLastError());
 }
 

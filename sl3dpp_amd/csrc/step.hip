@@ -2,11 +2,14 @@
 #include "step.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <limits>
 #include <cstring>
 #include <future>
+#include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 
@@ -405,6 +408,117 @@ void Coupling::Analyse(const double* const grid[3], double* spectral,
   AnalyseFromFourier(fourier, spectral, nullptr, false, s);
 }
 
+// ---- The tile-shape tuner --------------------------------------------------
+
+namespace {
+struct TuneCache {
+  std::mutex mutex;
+  std::map<std::array<long long, 4>, std::vector<double>> chosen;
+  int analysisRows = 0, synthesisTheta = 0;  // overrides, 0: tune
+};
+TuneCache& Tunes() {
+  static TuneCache c;
+  return c;
+}
+}  // namespace
+
+void SetTuningOverride(int analysisRows, int synthesisTheta) {
+  auto valid = [](int v, const int (&choices)[kTileChoices]) {
+    return v == 0 || std::find(choices, choices + kTileChoices, v) != choices + kTileChoices;
+  };
+  if (!valid(analysisRows, kAnalysisRowsChoices) ||
+      !valid(synthesisTheta, kSynthesisThetaChoices)) {
+    throw ArgumentError("tile shapes: analysis rows 16, 32, 64, 128; synthesis colatitudes "
+                        "32, 64, 112, 208; 0 to tune");
+  }
+  auto& c = Tunes();
+  std::lock_guard<std::mutex> lock(c.mutex);
+  c.analysisRows = analysisRows;
+  c.synthesisTheta = synthesisTheta;
+}
+
+// Each choice: one untimed launch, then five between two events; the
+// launches write the same outputs from the same inputs, and every choice's
+// outputs are the same bits (the tiles change the work per workgroup, not
+// the order of any sum).
+void Coupling::TuneLegendre(const double* spectral, double* const fourier[3],
+                            double* spectralOut, hipStream_t s) {
+  auto& cache = Tunes();
+  int forcedRows, forcedTheta;
+  {
+    std::lock_guard<std::mutex> lock(cache.mutex);
+    forcedRows = cache.analysisRows;
+    forcedTheta = cache.synthesisTheta;
+  }
+  long long truncation = 0;
+  for (size_t i = 0; i < reachHost_.size(); ++i) truncation = truncation * 31 + reachHost_[i];
+  for (size_t i = 0; i < firstNodeHost_.size(); ++i) {
+    truncation = truncation * 37 + firstNodeHost_[i];
+  }
+  const std::array<long long, 4> key{lmax_, geometry_.nodes, grid_.NTheta(), truncation};
+  std::vector<double> result;
+  {
+    std::lock_guard<std::mutex> lock(cache.mutex);
+    auto it = cache.chosen.find(key);
+    if (it != cache.chosen.end()) result = it->second;
+  }
+  if (result.empty()) {
+    result.assign(2 + 2 * kTileChoices, 0.0);
+    hipEvent_t ev[2];
+    SL3_HIP(hipEventCreate(&ev[0]));
+    SL3_HIP(hipEventCreate(&ev[1]));
+    auto time = [&](auto&& launch) {
+      launch();
+      SL3_HIP(hipEventRecord(ev[0], s));
+      for (int r = 0; r < 5; ++r) launch();
+      SL3_HIP(hipEventRecord(ev[1], s));
+      SL3_HIP(hipEventSynchronize(ev[1]));
+      float ms = 0.0f;
+      SL3_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      return ms / 5.0;
+    };
+    const double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
+                               spectral + spectralOffset_[2]};
+    double* outs[3] = {spectralOut + spectralOffset_[0], spectralOut + spectralOffset_[1],
+                       spectralOut + spectralOffset_[2]};
+    const double* fin[3] = {fourier[0], fourier[1], fourier[2]};
+    double best = 0.0;  // the synthesis choices, then the analysis choices
+    for (int i = 0; i < kTileChoices; ++i) {
+      LegendrePlan p = plan_;
+      p.synthesisTheta = kSynthesisThetaChoices[i];
+      const double ms = time([&] {
+        LaunchLegendreSynthesisMfma(p, blocks, fourier, reach_.data(), s);
+      });
+      result[2 + kTileChoices + i] = ms;
+      if (i == 0 || ms < best) {
+        best = ms;
+        result[1] = p.synthesisTheta;
+      }
+    }
+    for (int i = 0; i < kTileChoices; ++i) {
+      LegendrePlan p = plan_;
+      p.analysisRows = kAnalysisRowsChoices[i];
+      const double ms = time([&] {
+        LaunchLegendreAnalysisMfma(p, fin, outs, nullptr, reach_.data(), firstNode_.data(), s);
+      });
+      result[2 + i] = ms;
+      if (i == 0 || ms < best) {
+        best = ms;
+        result[0] = p.analysisRows;
+      }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    std::lock_guard<std::mutex> lock(cache.mutex);
+    cache.chosen.emplace(key, result);
+  }
+  tuning_ = result;
+  plan_.analysisRows = forcedRows > 0 ? forcedRows : static_cast<int>(result[0]);
+  plan_.synthesisTheta = forcedTheta > 0 ? forcedTheta : static_cast<int>(result[1]);
+  tuning_[0] = plan_.analysisRows;
+  tuning_[1] = plan_.synthesisTheta;
+}
+
 // ---- Viscoelastic3D ---------------------------------------------------------
 
 Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
@@ -445,6 +559,12 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
   amp_.Resize(2 * ampDoubles_);
   partials_.Resize(1024 * 19);  // the sweep's 16 sums and the residual's 3 maxima per block
   result_.Resize(19);
+  // the Legendre kernels' tile shapes, timed on the pass's own buffers
+  LaunchFill(s_.data(), S, 0.0, s);
+  {
+    double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
+    coupling_.TuneLegendre(s_.data(), A, analysed_.data(), s);
+  }
   for (auto& e : events_) SL3_HIP(hipEventCreate(&e));
   SL3_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 32 * sizeof(double),
                         hipHostMallocDefault));

@@ -104,6 +104,15 @@ class Coupling {
   // all three blocks: one launch where the grid allows it.
   void RingPass(double* const in[3], double* const out[3], const double* weight,
                 hipStream_t s) const;
+  // The tuner: times every tile shape of the Legendre kernels on this
+  // coupling's own shape (the degree truncation included) and keeps the
+  // fastest; shapes met before take the choice made then. spectral: a
+  // zeroed tensor, fourier and spectralOut: the pass's buffers (written).
+  void TuneLegendre(const double* spectral, double* const fourier[3], double* spectralOut,
+                    hipStream_t s);
+  // [analysis rows, synthesis colatitudes] chosen, then the ms per launch
+  // of each analysis choice and each synthesis choice.
+  const std::vector<double>& Tuning() const { return tuning_; }
   // One block's Legendre stage alone (the C-ABI transforms).
   void SynthesiseBlockToFourier(int block, const double* spectral, double* fourier,
                                 hipStream_t s) const;
@@ -134,7 +143,12 @@ class Coupling {
   LegendrePlan plan_;
   DeviceArray<long long> runFull_, runCompact_;
   ActiveRuns runs_;
+  std::vector<double> tuning_;
 };
+
+// Tile shapes fixed for every coupling made from now on (0: tuned); for
+// tests and timing studies.
+void SetTuningOverride(int analysisRows, int synthesisTheta);
 
 // The factorised systems of one (scheme, dt): Viscoelastic3D::Flavour
 // (src/Viscoelastic3D.cpp:158-167, 535-558).

@@ -20,7 +20,18 @@ struct LegendrePlan {
   // the Fourier layout's colatitude stride, ntheta rounded up to 8 (whole
   // 128-byte lines per (order, node) row of complex values)
   int fstride = 0;
+  // the matrix-core kernels' tile shapes (0: the defaults below): degrees
+  // per analysis workgroup (16, 32, 64, 128), colatitudes per synthesis
+  // workgroup (32, 64, 112, 208); picked per shape by the tuner
+  int analysisRows = 0;
+  int synthesisTheta = 0;
 };
+
+constexpr int kDefaultAnalysisRows = 64;
+constexpr int kDefaultSynthesisTheta = 208;
+constexpr int kTileChoices = 4;
+constexpr int kAnalysisRowsChoices[kTileChoices] = {16, 32, 64, 128};
+constexpr int kSynthesisThetaChoices[kTileChoices] = {32, 64, 112, 208};
 
 // ntheta rounded up to 8 complex values (128 bytes).
 inline int FourierStride(int ntheta) { return (ntheta + 7) / 8 * 8; }

@@ -49,3 +49,15 @@ def test_learned_entry_points_reject_bad_arguments_without_a_device():
                                 ctypes.c_double(0.1), None) == -1
     count = ctypes.c_int64()
     assert lib.sl3_spectral_rows(None, 4, ctypes.byref(count)) == -1
+
+
+def test_tuning_override_rejects_unknown_tile_shapes_without_a_device():
+    """sl3_tuning_set checks its shapes before anything else: status -1 for
+    a shape the kernels are not built for; 0, 0 (tune) is accepted."""
+    import sl3dpp_amd
+
+    lib = sl3dpp_amd.lib
+    assert lib.sl3_tuning_set(48, 0) == -1
+    assert lib.sl3_tuning_set(0, 100) == -1
+    assert lib.sl3_tuning_set(16, 33) == -1
+    assert lib.sl3_tuning_set(0, 0) == 0
