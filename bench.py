@@ -713,6 +713,7 @@ def maxwell1d_leg(args):
     top = max((k for k in split if k in work), key=lambda k: split[k]["total_ms"])
     per_step = split[top]["total_ms"] / 10.0 * 1e-3
     total = timings[True]
+    tuning = op.export("tuning")
     return {
         "workload": (f"batched 1D Maxwell evolution, lmax {lmax} (every order of every "
                      "spheroidal degree), 128 elements (70/56/2) x 6 GLL nodes, 280 Gauss "
@@ -731,6 +732,9 @@ def maxwell1d_leg(args):
                      "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": work[top] / per_step / PEAK_HBM, "work_per_step": work[top]},
         "kernels": split,
+        "band_panel": {"rows": int(tuning[0]),
+                       "ms_per_solve": {str(r): round(float(v), 4)
+                                        for r, v in zip((4, 8, 16), tuning[1:])}},
         "final_memory_rms": float(np.sqrt(np.mean(final ** 2))),
         "surface_u_l2_m0": float(surface[2 * 3] * length),
     }

@@ -59,11 +59,13 @@ int sl3_profile_names(char* buffer, int64_t capacity);
  * every tile shape of its Legendre kernels on its own shape when it is
  * made (degrees per analysis workgroup 16, 32, 64 or 128; colatitudes per
  * synthesis workgroup 32, 64, 112 or 208) and keeps the fastest; a shape met
- * before in the process takes the choice made then. Every choice gives the
- * same bits. This call fixes the shapes of operators made from now on
- * (0: tune), for tests and timing studies; the operator export "tuning"
- * reports the choice and the timings. */
-int sl3_tuning_set(int analysis_rows, int synthesis_theta);
+ * before in the process takes the choice made then. The 1D evolution
+ * times the band panel of its batched Cholesky solve (rows staged per
+ * panel: 4, 8 or 16) per flavour the same way. Every choice gives the same
+ * bits. This call fixes the shapes of operators and flavours made from now
+ * on (0: tune), for tests and timing studies; the exports "tuning" report
+ * the choice and the timings. */
+int sl3_tuning_set(int analysis_rows, int synthesis_theta, int band_panel_rows);
 
 /* ===================================================================== *
  * Banded Cholesky and LU, FP64, LAPACK band storage (column-major bands, *
@@ -542,7 +544,9 @@ int sl3_maxwell1d_shape(const sl3_maxwell1d* h, int64_t* memory_doubles,
 int sl3_maxwell1d_degree(const sl3_maxwell1d* h, int l, int* ndof, int* first_node,
                          int64_t* memory_offset, int64_t* solution_offset);
 /* Host copies by name: "memory", "strain" (the state's cache), "solutions"
- * (the last step's, unscaled, with the channel). count in doubles. */
+ * (the last step's, unscaled, with the channel), "tuning" (4 doubles: the
+ * last evolution's band panel rows, then the ms per solve timed for panels
+ * of 4, 8 and 16 rows). count in doubles. */
 int sl3_maxwell1d_export(const sl3_maxwell1d* h, const char* name, double* dst, int64_t count);
 /* ViscoelasticOperator::Surface (include/sl3dpp/Viscoelastic.hpp:262-267):
  * component 0 U, 1 V, 2 phi of the last solutions at the surface per

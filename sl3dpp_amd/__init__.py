@@ -81,12 +81,13 @@ def profile_select(names=None):
     check(lib.sl3_profile_select(",".join(names).encode() if names else None))
 
 
-def tuning_set(analysis_rows=0, synthesis_theta=0):
+def tuning_set(analysis_rows=0, synthesis_theta=0, band_panel_rows=0):
     """Fixes the Legendre kernels' tile shapes of operators made from now
     on (degrees per analysis workgroup 16, 32, 64 or 128; colatitudes per
-    synthesis workgroup 32, 64, 112 or 208); 0 lets each operator's tuner
-    choose."""
-    check(lib.sl3_tuning_set(int(analysis_rows), int(synthesis_theta)))
+    synthesis workgroup 32, 64, 112 or 208) and the 1D evolution's band
+    panel (4, 8 or 16 rows); 0 lets the tuner choose."""
+    check(lib.sl3_tuning_set(int(analysis_rows), int(synthesis_theta),
+                             int(band_panel_rows)))
 
 
 def profile_reset():

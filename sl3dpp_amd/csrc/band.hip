@@ -239,7 +239,10 @@ __device__ inline void StoreColumn(const double* xs, double* x, long long rs, in
 // KD updates are independent of one another. Every row takes its terms
 // oldest first (t = KD..1), the order of the ring path's sums; the
 // reciprocal multiply differs from the ring path's division by at most an
-// ulp per row. The band columns of kFixedRows rows (and the KD after them)
+// ulp per row. The band columns of R rows (a panel: 4, 8 or 16, picked per
+// batch by the tuner; every row's arithmetic is the same whatever R is,
+// so the panel changes the staging, not a bit of the result) and the KD
+// after them
 // are staged in LDS, double-buffered: the next block's columns and
 // right-hand side values are loaded into registers while the current block
 // is substituted. Rows past either end of the matrix only ever occupy
@@ -247,7 +250,7 @@ __device__ inline void StoreColumn(const double* xs, double* x, long long rs, in
 // instruction every four cycles and a row costs about 65 (17 FMAs, the
 // LDS reads, the block's share of staging), so one long system's sweep is
 // issue-bound: 465 cycles a row measured at n = 1281, kd = 17.
-constexpr int kFixedRows = 8;
+constexpr int kDefaultPanelRows = 8;
 
 // One staged block of forward steps r = j0 + q: y_r = acc[0] U(r, r)^-1,
 // then U(r, r + k) y_r off the pending rows, and row r + KD enters with its
@@ -255,7 +258,7 @@ constexpr int kFixedRows = 8;
 // band entries are read from LDS one step ahead (the scheduling barrier
 // keeps those reads ahead of the step before), so LDS latency stays off
 // the rows' serial chain.
-template <int KD, bool FULL>
+template <int KD, bool FULL, int R>
 __device__ inline void ForwardSteps(const double* fac, const double* rinv, const double* bin,
                                     double* acc, double* out, int rows) {
   constexpr int ld = KD + 1;
@@ -264,8 +267,8 @@ __device__ inline void ForwardSteps(const double* fac, const double* rinv, const
 #pragma unroll
   for (int k = 1; k <= KD; ++k) cf[k] = fac[k * ld + KD - k];  // column k's entry KD - k
 #pragma unroll
-  for (int q = 0; q < kFixedRows; ++q) {
-    if (q + 1 < kFixedRows) {
+  for (int q = 0; q < R; ++q) {
+    if (q + 1 < R) {
       nf[0] = rinv[q + 1];
 #pragma unroll
       for (int k = 1; k <= KD; ++k) nf[k] = fac[(q + 1 + k) * ld + KD - k];
@@ -290,17 +293,17 @@ __device__ inline void ForwardSteps(const double* fac, const double* rinv, const
 // acc[0] U(j, j)^-1, then U(j - k, j) x_j off the pending rows, and row
 // j - KD enters with its value yin[q]. fac holds columns j0 .. of the
 // block; band entries are read one step ahead as in ForwardSteps.
-template <int KD, bool FULL>
+template <int KD, bool FULL, int R>
 __device__ inline void BackwardSteps(const double* fac, const double* rinv, const double* yin,
                                      double* acc, double* out, int rows) {
   constexpr int ld = KD + 1;
   double cf[KD + 1], nf[KD + 1];  // U(j - KD + k, j) at [k], k < KD; [KD]: the reciprocal
-  constexpr int q0 = kFixedRows - 1;
+  constexpr int q0 = R - 1;
 #pragma unroll
   for (int k = 0; k < KD; ++k) cf[k] = fac[q0 * ld + k];
   cf[KD] = rinv[q0];
 #pragma unroll
-  for (int q = kFixedRows - 1; q >= 0; --q) {
+  for (int q = R - 1; q >= 0; --q) {
     if (q > 0) {
 #pragma unroll
       for (int k = 0; k < KD; ++k) nf[k] = fac[(q - 1) * ld + k];
@@ -323,11 +326,10 @@ __device__ inline void BackwardSteps(const double* fac, const double* rinv, cons
   }
 }
 
-template <int KD>
+template <int KD, int R>
 __device__ void CholeskySweepsFixed(const double* __restrict__ u, double* __restrict__ x,
                                     long long rs, int n, bool active, int lane, double* lds) {
   constexpr int ld = KD + 1;
-  constexpr int R = kFixedRows;
   constexpr int F = (R + KD) * ld;  // band doubles a block stages, at most
   constexpr int FL = (F + kWave - 1) / kWave;
   double acc[KD], cur[R], nxt[R], out[R], fn[FL];
@@ -379,9 +381,9 @@ __device__ void CholeskySweepsFixed(const double* __restrict__ u, double* __rest
         }
       }
       if (rows == R) {
-        ForwardSteps<KD, true>(fac, rinv, cur, acc, out, rows);
+        ForwardSteps<KD, true, R>(fac, rinv, cur, acc, out, rows);
       } else {
-        ForwardSteps<KD, false>(fac, rinv, cur, acc, out, rows);
+        ForwardSteps<KD, false, R>(fac, rinv, cur, acc, out, rows);
         // the last block: its values go out here
         if (active) {
 #pragma unroll
@@ -443,9 +445,9 @@ __device__ void CholeskySweepsFixed(const double* __restrict__ u, double* __rest
         }
       }
       if (rows == R) {
-        BackwardSteps<KD, true>(fac, rinv, cur, acc, out, rows);
+        BackwardSteps<KD, true, R>(fac, rinv, cur, acc, out, rows);
       } else {
-        BackwardSteps<KD, false>(fac, rinv, cur, acc, out, rows);
+        BackwardSteps<KD, false, R>(fac, rinv, cur, acc, out, rows);
       }
       if (active && j0 == 0) {
         // the last block (the only one that may be partial) goes out here
@@ -464,6 +466,7 @@ __device__ void CholeskySweepsFixed(const double* __restrict__ u, double* __rest
 // Solves: block b handles columns tasks[b].y .. +63 of system tasks[b].x.
 // LDS: the ring RingSlots(kd + 1) x 64, lane fastest; the staged band
 // columns (kSolveRows + kd) x (kd + 1); the staged values kSolveRows x 64.
+template <int PANEL>
 __global__ void __launch_bounds__(kWave)
     CholeskySolveKernel(const BandSystem* systems, const int2* tasks,
                         const double* ab, double* b) {
@@ -481,10 +484,10 @@ __global__ void __launch_bounds__(kWave)
   // the widths of the step's systems: spheroidal 3 ngll - 1 and toroidal
   // ngll - 1 at ngll 5 and 6
   switch (kd) {
-    case 4: CholeskySweepsFixed<4>(u, x, rs, n, active, lane, lds); return;
-    case 5: CholeskySweepsFixed<5>(u, x, rs, n, active, lane, lds); return;
-    case 14: CholeskySweepsFixed<14>(u, x, rs, n, active, lane, lds); return;
-    case 17: CholeskySweepsFixed<17>(u, x, rs, n, active, lane, lds); return;
+    case 4: CholeskySweepsFixed<4, PANEL>(u, x, rs, n, active, lane, lds); return;
+    case 5: CholeskySweepsFixed<5, PANEL>(u, x, rs, n, active, lane, lds); return;
+    case 14: CholeskySweepsFixed<14, PANEL>(u, x, rs, n, active, lane, lds); return;
+    case 17: CholeskySweepsFixed<17, PANEL>(u, x, rs, n, active, lane, lds); return;
     default: break;
   }
   // a power-of-two ring of at least kd + 1 slots: slot i at i & mask
@@ -626,16 +629,64 @@ void LaunchLuFactor(const BandSystem* systems_dev, int count, int klMax,
 
 void LaunchCholeskySolve(const BandSystem* systems_dev, const int2* tasks_dev,
                          int tasks, int kdMax, const double* ab_dev,
-                         double* b_dev, hipStream_t stream) {
+                         double* b_dev, hipStream_t stream, int panelRows) {
   if (tasks == 0) return;
   ProfileScope prof297("band_solve", stream);
+  const int panel = panelRows > 0 ? panelRows : kDefaultPanelRows;
   const size_t ring = RingSlots(kdMax + 1) * kWave + (kSolveRows + kdMax) * (kdMax + 1) +
                       kSolveRows * kWave;
-  const size_t fixed = 2 * (kFixedRows + kdMax) * (kdMax + 1) + kFixedRows;
+  const size_t fixed = 2 * static_cast<size_t>(panel + kdMax) * (kdMax + 1) + panel;
   const size_t lds = sizeof(double) * std::max(ring, fixed);
-  hipLaunchKernelGGL(CholeskySolveKernel, dim3(tasks), dim3(kWave), lds, stream, systems_dev,
-                     tasks_dev, ab_dev, b_dev);
+  switch (panel) {
+    case 4:
+      hipLaunchKernelGGL(CholeskySolveKernel<4>, dim3(tasks), dim3(kWave), lds, stream,
+                         systems_dev, tasks_dev, ab_dev, b_dev);
+      break;
+    case 8:
+      hipLaunchKernelGGL(CholeskySolveKernel<8>, dim3(tasks), dim3(kWave), lds, stream,
+                         systems_dev, tasks_dev, ab_dev, b_dev);
+      break;
+    case 16:
+      hipLaunchKernelGGL(CholeskySolveKernel<16>, dim3(tasks), dim3(kWave), lds, stream,
+                         systems_dev, tasks_dev, ab_dev, b_dev);
+      break;
+    default:
+      throw ArgumentError("band solve: panel of 4, 8 or 16 rows");
+  }
   SL3_HIP(hipGetLastError());
+}
+
+// Each panel: one untimed solve, then five between two events, in place on
+// b_dev (the caller's scratch: its values are solved over and over).
+int TuneCholeskySolvePanel(const BandSystem* systems_dev, const int2* tasks_dev, int tasks,
+                           int kdMax, const double* ab_dev, double* b_dev, hipStream_t stream,
+                           double* ms_out) {
+  if (tasks == 0) return kDefaultPanelRows;
+  hipEvent_t ev[2];
+  SL3_HIP(hipEventCreate(&ev[0]));
+  SL3_HIP(hipEventCreate(&ev[1]));
+  int choice = kDefaultPanelRows;
+  double best = 0.0;
+  for (int i = 0; i < kPanelChoices; ++i) {
+    const int panel = kPanelRowsChoices[i];
+    LaunchCholeskySolve(systems_dev, tasks_dev, tasks, kdMax, ab_dev, b_dev, stream, panel);
+    SL3_HIP(hipEventRecord(ev[0], stream));
+    for (int r = 0; r < 5; ++r) {
+      LaunchCholeskySolve(systems_dev, tasks_dev, tasks, kdMax, ab_dev, b_dev, stream, panel);
+    }
+    SL3_HIP(hipEventRecord(ev[1], stream));
+    SL3_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.0f;
+    SL3_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    if (ms_out != nullptr) ms_out[i] = ms / 5.0;
+    if (i == 0 || ms < best) {
+      best = ms;
+      choice = panel;
+    }
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return choice;
 }
 
 void LaunchLuSolve(const BandSystem* systems_dev, const int2* tasks_dev,

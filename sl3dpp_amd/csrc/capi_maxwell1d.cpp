@@ -102,6 +102,8 @@ int sl3_maxwell1d_export(const sl3_maxwell1d* h, const char* name, double* dst, 
       v = h->impl->Strain();
     } else if (std::strcmp(name, "solutions") == 0) {
       v = h->impl->Solutions();
+    } else if (std::strcmp(name, "tuning") == 0) {
+      v = h->impl->Tuning();
     } else {
       throw ArgumentError(std::string("unknown array '") + name + "'");
     }

@@ -653,8 +653,8 @@ int sl3_state_export(const sl3_operator* op, const sl3_state* state, const char*
   });
 }
 
-int sl3_tuning_set(int analysis_rows, int synthesis_theta) {
-  return Guard([&] { SetTuningOverride(analysis_rows, synthesis_theta); });
+int sl3_tuning_set(int analysis_rows, int synthesis_theta, int band_panel_rows) {
+  return Guard([&] { SetTuningOverride(analysis_rows, synthesis_theta, band_panel_rows); });
 }
 
 int sl3_operator_export(sl3_operator* op, const char* name, double* dst, int64_t* count) {

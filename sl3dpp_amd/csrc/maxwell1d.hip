@@ -478,6 +478,17 @@ const Flavour1D& Maxwell1D::FlavourOf(int scheme, double dt) {
     for (size_t i = 0; i < unit.size(); ++i) unit[i] *= scale[i];  // FinishSolve
     f->units.Upload(unit.data(), unit.size(), s);
   }
+  // the step's band panel, timed on a scratch batch of the step's layout
+  {
+    DeviceArray<double> scratch(std::max<size_t>(rhsDoubles_, 1));
+    LaunchFill(scratch.data(), scratch.size(), 0.0, s);
+    f->panelRows = TuneCholeskySolvePanel(f->choleskyDev.data(), f->choleskyTasks.data(),
+                                          f->choleskyTaskCount, f->kdMax, f->factors.data(),
+                                          scratch.data(), s, f->panelMs);
+    const int forced = BandPanelOverride();
+    if (forced > 0) f->panelRows = forced;
+    SL3_HIP(hipStreamSynchronize(s));  // the scratch is freed next
+  }
   // the per-node step weights (include/sl3dpp/StepWeights.hpp:98-144)
   std::vector<double> ew(nodes_), aw(nodes_), bw(nodes_);
   for (int node = 0; node < nodes_; ++node) {
@@ -546,7 +557,7 @@ void Maxwell1D::EnqueueStep(const Flavour1D& f, hipStream_t s) {
     SL3_HIP(hipGetLastError());
   }
   LaunchCholeskySolve(f.choleskyDev.data(), f.choleskyTasks.data(), f.choleskyTaskCount,
-                      f.kdMax, f.factors.data(), rhs_.data(), s);
+                      f.kdMax, f.factors.data(), rhs_.data(), s, f.panelRows);
   LaunchLuSolve(f.luDev.data(), f.luTasks.data(), f.luTaskCount, f.kvMax, f.factors.data(),
                 f.pivots.data(), rhs_.data(), s);
   {
@@ -582,6 +593,8 @@ void Maxwell1D::Evolve(double dt, int scheme, int steps, bool graph) {
     last_ = &elastic;
   }
   const Flavour1D& f = FlavourOf(scheme, dt);
+  tuning_.assign(1, f.panelRows);
+  tuning_.insert(tuning_.end(), f.panelMs, f.panelMs + kPanelChoices);
   if (steps == 0) return;
   if (graph) {
     if (graphFlavour_ != &f) {

@@ -50,6 +50,10 @@ struct Flavour1D {
   // per memory node: e = the memory's decay, alpha, beta (m+ = e m +
   // alpha d0 + beta d)
   DeviceArray<double> e, alpha, beta;
+  // the Cholesky solve's band panel (rows per staged panel) and the ms per
+  // solve the tuner timed for each choice
+  int panelRows = 0;
+  double panelMs[kPanelChoices] = {};
 };
 
 class Maxwell1D {
@@ -81,6 +85,9 @@ class Maxwell1D {
   std::vector<double> Memory() const;
   std::vector<double> Strain() const;
   std::vector<double> Solutions() const;
+  // The step flavour's band panel and the ms per solve timed for each
+  // choice (4, 8, 16 rows), of the last Evolve.
+  std::vector<double> Tuning() const { return tuning_; }
   int StepIndex() const { return stepIndex_; }
   // One component's (0 U, 1 V, 2 phi) surface value of the last solutions
   // per degree and column, at out[l (l + 1) + col] ((lmax+1)(lmax+2)
@@ -111,6 +118,7 @@ class Maxwell1D {
   DeviceArray<int> counter_;  // the device's step index
   int stepIndex_ = 0;
   double t_ = 0.0;
+  std::vector<double> tuning_;
   bool hasStrain_ = false;
   std::vector<std::unique_ptr<Flavour1D>> flavours_;
   const Flavour1D* last_ = nullptr;  // the last step's flavour

@@ -414,7 +414,7 @@ namespace {
 struct TuneCache {
   std::mutex mutex;
   std::map<std::array<long long, 4>, std::vector<double>> chosen;
-  int analysisRows = 0, synthesisTheta = 0;  // overrides, 0: tune
+  int analysisRows = 0, synthesisTheta = 0, bandPanelRows = 0;  // overrides, 0: tune
 };
 TuneCache& Tunes() {
   static TuneCache c;
@@ -422,19 +422,27 @@ TuneCache& Tunes() {
 }
 }  // namespace
 
-void SetTuningOverride(int analysisRows, int synthesisTheta) {
-  auto valid = [](int v, const int (&choices)[kTileChoices]) {
-    return v == 0 || std::find(choices, choices + kTileChoices, v) != choices + kTileChoices;
+void SetTuningOverride(int analysisRows, int synthesisTheta, int bandPanelRows) {
+  auto valid = [](int v, const int* choices, int count) {
+    return v == 0 || std::find(choices, choices + count, v) != choices + count;
   };
-  if (!valid(analysisRows, kAnalysisRowsChoices) ||
-      !valid(synthesisTheta, kSynthesisThetaChoices)) {
+  if (!valid(analysisRows, kAnalysisRowsChoices, kTileChoices) ||
+      !valid(synthesisTheta, kSynthesisThetaChoices, kTileChoices) ||
+      !valid(bandPanelRows, kPanelRowsChoices, kPanelChoices)) {
     throw ArgumentError("tile shapes: analysis rows 16, 32, 64, 128; synthesis colatitudes "
-                        "32, 64, 112, 208; 0 to tune");
+                        "32, 64, 112, 208; band panel rows 4, 8, 16; 0 to tune");
   }
   auto& c = Tunes();
   std::lock_guard<std::mutex> lock(c.mutex);
   c.analysisRows = analysisRows;
   c.synthesisTheta = synthesisTheta;
+  c.bandPanelRows = bandPanelRows;
+}
+
+int BandPanelOverride() {
+  auto& c = Tunes();
+  std::lock_guard<std::mutex> lock(c.mutex);
+  return c.bandPanelRows;
 }
 
 // Each choice: one untimed launch, then five between two events; the

@@ -146,9 +146,6 @@ class Coupling {
   std::vector<double> tuning_;
 };
 
-// Tile shapes fixed for every coupling made from now on (0: tuned); for
-// tests and timing studies.
-void SetTuningOverride(int analysisRows, int synthesisTheta);
 
 // The factorised systems of one (scheme, dt): Viscoelastic3D::Flavour
 // (src/Viscoelastic3D.cpp:158-167, 535-558).

@@ -576,7 +576,7 @@ class ViscoelasticOperator:
         return k.value
 
     def export(self, name):
-        n = self.solution_doubles if name == "solutions" else self.memory_doubles
+        n = {"solutions": self.solution_doubles, "tuning": 4}.get(name, self.memory_doubles)
         out = np.empty(n)
         check(lib.sl3_maxwell1d_export(self.handle, name.encode(), _dp(out), ctypes.c_int64(n)))
         return out

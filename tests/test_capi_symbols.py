@@ -53,11 +53,12 @@ def test_learned_entry_points_reject_bad_arguments_without_a_device():
 
 def test_tuning_override_rejects_unknown_tile_shapes_without_a_device():
     """sl3_tuning_set checks its shapes before anything else: status -1 for
-    a shape the kernels are not built for; 0, 0 (tune) is accepted."""
+    a shape the kernels are not built for; zeros (tune) are accepted."""
     import sl3dpp_amd
 
     lib = sl3dpp_amd.lib
-    assert lib.sl3_tuning_set(48, 0) == -1
-    assert lib.sl3_tuning_set(0, 100) == -1
-    assert lib.sl3_tuning_set(16, 33) == -1
-    assert lib.sl3_tuning_set(0, 0) == 0
+    assert lib.sl3_tuning_set(48, 0, 0) == -1
+    assert lib.sl3_tuning_set(0, 100, 0) == -1
+    assert lib.sl3_tuning_set(16, 33, 0) == -1
+    assert lib.sl3_tuning_set(0, 0, 12) == -1
+    assert lib.sl3_tuning_set(0, 0, 0) == 0

@@ -18,7 +18,8 @@ at once, include/sl3dpp/Viscoelastic.hpp:163-301):
    (degree 1's system, the worst conditioned, sits near 1e-11 once the
    load has ramped to zero);
  * graph replays and stream launches give the same bits, and a reset
-   trajectory repeats itself bit for bit;
+   trajectory repeats itself bit for bit; every band panel the tuner
+   chooses from gives the same bits;
  * the argument checks (scheme, ramp length, pattern size)."""
 
 import ctypes
@@ -188,6 +189,37 @@ def test_graph_replays_equal_stream_launches(c1_small):
     op.evolve(dt, st.EXPONENTIAL_TRAPEZOID, C1_STEPS - 3, graph=True)
     assert np.array_equal(op.export("memory"), out[0][0])
     assert op.steps_taken() == C1_STEPS
+
+
+def test_every_band_panel_gives_the_same_bits(c1_small):
+    """The tuner's band panels (4, 8, 16 rows staged per panel of the
+    batched Cholesky solve) give the same trajectory bit for bit; an
+    evolution reports the panel it used and the timings that chose it."""
+    earth = c1_small
+    length, time, mass = earth.scales()
+    dt = 100.0 * st.YEAR / time
+    pattern = seeded_pattern(C1_LMAX, earth)
+    ramp = np.linspace(1.0, 0.0, C1_STEPS + 1)
+    out = []
+    for panel in (4, 8, 16, 0):
+        sl3dpp_amd.tuning_set(band_panel_rows=panel)
+        try:
+            op = radial.ViscoelasticOperator(earth, C1_LMAX)
+            op.set_load(pattern, ramp)
+            op.evolve(dt, st.EXPONENTIAL_TRAPEZOID, C1_STEPS, graph=True)
+        finally:
+            sl3dpp_amd.tuning_set()
+        t = op.export("tuning")
+        if panel:
+            assert int(t[0]) == panel
+        else:  # tuned: the fastest panel timed
+            assert int(t[0]) in (4, 8, 16)
+            assert t[1 + (4, 8, 16).index(int(t[0]))] == np.min(t[1:])
+        assert np.all(t[1:] > 0)
+        out.append((op.export("memory"), op.export("strain"), op.export("solutions")))
+    for run in out[1:]:
+        for a, b in zip(out[0], run):
+            assert np.array_equal(a, b)
 
 
 def test_surface_equals_the_solutions_surface_rows(c1_small):
