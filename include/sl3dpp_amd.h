@@ -280,7 +280,7 @@ typedef struct {
   int max_iterations;   /* 40 */
   double rtol;          /* 1e-11 */
   double damping;       /* 1 */
-  int anderson_memory;  /* 5 */
+  int anderson_memory;  /* 5; at most 16 */
   double anderson_rate; /* 0.5; <= 0 for Anderson from the first pass */
   int solver;           /* SL3_FIXED_POINT */
   double solve_rtol;    /* 1e-11: the relative residual in the preconditioned

@@ -15,9 +15,11 @@
 
 #include "oracle.h"
 
+/* m <= 16 */
 int sl3o_gram_least_squares(int m, const double* gram, const double* rhs,
                             double* gamma) {
-  double a[64], v[64];
+  double a[256], v[256];
+  if (m < 1 || m > 16) return -1;
   for (int i = 0; i < m * m; ++i) a[i] = gram[i];
   for (int i = 0; i < m; ++i)
     for (int j = 0; j < m; ++j) v[i * m + j] = i == j ? 1.0 : 0.0;
@@ -149,7 +151,7 @@ int sl3o_mixer_update(sl3o_mixer* mx, size_t n, double* x, const double* gx) {
     mx->gram[slot * mx->memory + i] = column[i];
     mx->gram[i * mx->memory + slot] = column[i];
   }
-  double g[64], gamma[16];
+  double g[256], gamma[16];
   for (int i = 0; i < m; ++i)
     for (int j = 0; j < m; ++j) g[i * m + j] = mx->gram[i * mx->memory + j];
   sl3o_gram_least_squares(m, g, rhs, gamma);

@@ -78,7 +78,7 @@ StepOptions ToOptions(const sl3_step_options* o) {
   Check(s.solver == kFixedPoint || s.solver == kConjugateGradients,
         "solver: SL3_FIXED_POINT or SL3_CONJUGATE_GRADIENTS");
   Check(s.solveMaxIterations >= 0, "solve_max_iterations >= 0");
-  Check(s.andersonMemory >= 0 && s.andersonMemory <= 8, "anderson memory in [0, 8]");
+  Check(s.andersonMemory >= 0 && s.andersonMemory <= kAndersonMax, "anderson memory in [0, 16]");
   Check(s.maxIterations >= 1, "max_iterations >= 1");
   Check(s.eps > 0.0 && s.eps < 1.0, "eps in (0, 1)");
   Check(s.oversampling >= 1.0, "oversampling >= 1");

@@ -13,6 +13,8 @@
 //    fixed block order so that results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <vector>
 
 #include "common.hpp"
@@ -198,7 +200,8 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// One workgroup per (degree, order), one thread per memory node.
+// One workgroup per (degree, order), a thread per memory node (beyond the
+// workgroup's 1024 threads, the nodes 1024 apart in turn).
 __global__ void __launch_bounds__(1024)
     StrainKernel(const DegreeTask* sph, const DegreeTask* tor, int lmax,
                  RadialGeometry g, const int* dofs, const double* response,
@@ -209,83 +212,83 @@ __global__ void __launch_bounds__(1024)
   while (l * (l + 1) / 2 > pair) --l;
   while ((l + 1) * (l + 2) / 2 <= pair) ++l;
   const int m = pair - l * (l + 1) / 2;
-  const int node = threadIdx.x;
-  if (node >= g.nodes) return;
   const DegreeTask ts = sph[l];
-  Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
-  if (node >= ts.firstNode) {
-    const int elem = g.elem0 + node / g.np, q = node % g.np;
-    const double r = g.radius[node];
-    const double jac = g.jac[elem - g.elem0];
-    Cplx U = C(0, 0), V = C(0, 0), W = C(0, 0), rdU = C(0, 0), rdV = C(0, 0),
-         rdW = C(0, 0);
-    const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
-    const Cplx at = tideAmp ? C(tideAmp[2 * pair], tideAmp[2 * pair + 1]) : C(0.0, 0.0);
-    const bool tide = tideAmp != nullptr;
-    const bool tor1 = l >= 1;
-    DegreeTask tt;
-    if (tor1) tt = tor[l - 1];
-    for (int i = 0; i < g.ngll; ++i) {
-      const int* d = dofs + ts.dofOff + ((elem - g.elem0) * g.ngll + i) * 2;
-      Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
-      // the response columns m (Re) and l + 1 + m (Im), column-major
-      const double* re = response + ts.fOff + static_cast<size_t>(m) * ts.nmPad;
-      const double* im = response + ts.fOff + static_cast<size_t>(l + 1 + m) * ts.nmPad;
-      if (d[0] >= 0) {
-        Ui = C(re[d[0]], im[d[0]]);
-        if (units != nullptr && ts.unitOff >= 0) Ui = Ui + units[ts.dmOff + d[0]] * a;
-        if (tide) Ui = Ui + tideUnits[ts.dmOff + d[0]] * at;
-      }
-      if (d[1] >= 0) {
-        Vi = C(re[d[1]], im[d[1]]);
-        if (units != nullptr && ts.unitOff >= 0) Vi = Vi + units[ts.dmOff + d[1]] * a;
-        if (tide) Vi = Vi + tideUnits[ts.dmOff + d[1]] * at;
-      }
-      if (tor1) {
-        const int dw = dofs[tt.dofOff + ((elem - g.elem0) * g.ngll + i) * 2];
-        if (dw >= 0) {
-          const double* bt = response + tt.fOff;
-          Wi = C(bt[static_cast<size_t>(m) * tt.nmPad + dw],
-                 bt[static_cast<size_t>(l + 1 + m) * tt.nmPad + dw]);
+  for (int node = threadIdx.x; node < g.nodes; node += blockDim.x) {
+    Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
+    if (node >= ts.firstNode) {
+      const int elem = g.elem0 + node / g.np, q = node % g.np;
+      const double r = g.radius[node];
+      const double jac = g.jac[elem - g.elem0];
+      Cplx U = C(0, 0), V = C(0, 0), W = C(0, 0), rdU = C(0, 0), rdV = C(0, 0),
+           rdW = C(0, 0);
+      const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
+      const Cplx at = tideAmp ? C(tideAmp[2 * pair], tideAmp[2 * pair + 1]) : C(0.0, 0.0);
+      const bool tide = tideAmp != nullptr;
+      const bool tor1 = l >= 1;
+      DegreeTask tt;
+      if (tor1) tt = tor[l - 1];
+      for (int i = 0; i < g.ngll; ++i) {
+        const int* d = dofs + ts.dofOff + ((elem - g.elem0) * g.ngll + i) * 2;
+        Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
+        // the response columns m (Re) and l + 1 + m (Im), column-major
+        const double* re = response + ts.fOff + static_cast<size_t>(m) * ts.nmPad;
+        const double* im = response + ts.fOff + static_cast<size_t>(l + 1 + m) * ts.nmPad;
+        if (d[0] >= 0) {
+          Ui = C(re[d[0]], im[d[0]]);
+          if (units != nullptr && ts.unitOff >= 0) Ui = Ui + units[ts.dmOff + d[0]] * a;
+          if (tide) Ui = Ui + tideUnits[ts.dmOff + d[0]] * at;
         }
+        if (d[1] >= 0) {
+          Vi = C(re[d[1]], im[d[1]]);
+          if (units != nullptr && ts.unitOff >= 0) Vi = Vi + units[ts.dmOff + d[1]] * a;
+          if (tide) Vi = Vi + tideUnits[ts.dmOff + d[1]] * at;
+        }
+        if (tor1) {
+          const int dw = dofs[tt.dofOff + ((elem - g.elem0) * g.ngll + i) * 2];
+          if (dw >= 0) {
+            const double* bt = response + tt.fOff;
+            Wi = C(bt[static_cast<size_t>(m) * tt.nmPad + dw],
+                   bt[static_cast<size_t>(l + 1 + m) * tt.nmPad + dw]);
+          }
+        }
+        const double v = g.value[q * g.ngll + i];
+        U = U + v * Ui;
+        V = V + v * Vi;
+        W = W + v * Wi;
+        const double bb = r * g.deriv[q * g.ngll + i] / jac;
+        rdU = rdU + bb * Ui;
+        rdV = rdV + bb * Vi;
+        rdW = rdW + bb * Wi;
       }
-      const double v = g.value[q * g.ngll + i];
-      U = U + v * Ui;
-      V = V + v * Vi;
-      W = W + v * Wi;
-      const double bb = r * g.deriv[q * g.ngll + i] / jac;
-      rdU = rdU + bb * Ui;
-      rdV = rdV + bb * Vi;
-      rdW = rdW + bb * Wi;
+      // DeviatoricComponents
+      const double zeta2 = l * (l + 1.0);
+      const double zeta = sqrt(zeta2);
+      A[2] = Div(rdU - U + (0.5 * zeta2) * V, 3.0 * r);
+      if (l >= 1) {
+        const Cplx tangential = rdV - V + U;
+        const Cplx twist = W - rdW;
+        const double den = 2.0 * kSqrt2 * r;
+        A[1] = Div(zeta * (tangential + TimesI(twist)), den);
+        A[3] = Div(zeta * (tangential - TimesI(twist)), den);
+      }
+      if (l >= 2) {
+        const double f = 0.5 * zeta * sqrt(zeta2 - 2.0) / r;
+        A[0] = f * (V - TimesI(W));
+        A[4] = f * (V + TimesI(W));
+      }
     }
-    // DeviatoricComponents
-    const double zeta2 = l * (l + 1.0);
-    const double zeta = sqrt(zeta2);
-    A[2] = Div(rdU - U + (0.5 * zeta2) * V, 3.0 * r);
-    if (l >= 1) {
-      const Cplx tangential = rdV - V + U;
-      const Cplx twist = W - rdW;
-      const double den = 2.0 * kSqrt2 * r;
-      A[1] = Div(zeta * (tangential + TimesI(twist)), den);
-      A[3] = Div(zeta * (tangential - TimesI(twist)), den);
-    }
+    // Scatter (src/GridCoupling.cpp:595-611), every slot written once.
+    const double sign = (m & 1) ? -1.0 : 1.0;
     if (l >= 2) {
-      const double f = 0.5 * zeta * sqrt(zeta2 - 2.0) / r;
-      A[0] = f * (V - TimesI(W));
-      A[4] = f * (V + TimesI(W));
+      Store(out, 0, l, m, node, g.nodes, A[0]);
+      if (m > 0) Store(out, 0, l, -m, node, g.nodes, sign * Conj(A[4]));
     }
+    if (l >= 1) {
+      Store(out, 1, l, m, node, g.nodes, A[1]);
+      if (m > 0) Store(out, 1, l, -m, node, g.nodes, sign * Conj(A[3]));
+    }
+    Store(out, 2, l, m, node, g.nodes, A[2]);
   }
-  // Scatter (src/GridCoupling.cpp:595-611), every slot written once.
-  const double sign = (m & 1) ? -1.0 : 1.0;
-  if (l >= 2) {
-    Store(out, 0, l, m, node, g.nodes, A[0]);
-    if (m > 0) Store(out, 0, l, -m, node, g.nodes, sign * Conj(A[4]));
-  }
-  if (l >= 1) {
-    Store(out, 1, l, m, node, g.nodes, A[1]);
-    if (m > 0) Store(out, 1, l, -m, node, g.nodes, sign * Conj(A[3]));
-  }
-  Store(out, 2, l, m, node, g.nodes, A[2]);
 }
 
 // StrainKernel with the workgroup's operands staged in LDS first: the
@@ -343,70 +346,70 @@ __global__ void __launch_bounds__(1024)
     }
   }
   __syncthreads();
-  const int node = threadIdx.x;
-  if (node >= g.nodes) return;
-  Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
-  if (node >= ts.firstNode) {
-    const int elem = g.elem0 + node / g.np, q = node % g.np;
-    const double r = g.radius[node];
-    const double jac = g.jac[elem - g.elem0];
-    Cplx U = C(0, 0), V = C(0, 0), W = C(0, 0), rdU = C(0, 0), rdV = C(0, 0),
-         rdW = C(0, 0);
-    const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
-    const Cplx at = tide ? C(tideAmp[2 * pair], tideAmp[2 * pair + 1]) : C(0.0, 0.0);
-    for (int i = 0; i < g.ngll; ++i) {
-      const int slot = (elem - g.elem0) * g.ngll + i;
-      const int d0 = sDof[2 * slot], d1 = sDof[2 * slot + 1];
-      Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
-      if (d0 >= 0) {
-        Ui = C(sRe[d0], sIm[d0]);
-        if (unitsOn) Ui = Ui + sUnit[d0] * a;
-        if (tide) Ui = Ui + sTide[d0] * at;
+  for (int node = threadIdx.x; node < g.nodes; node += blockDim.x) {
+    Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
+    if (node >= ts.firstNode) {
+      const int elem = g.elem0 + node / g.np, q = node % g.np;
+      const double r = g.radius[node];
+      const double jac = g.jac[elem - g.elem0];
+      Cplx U = C(0, 0), V = C(0, 0), W = C(0, 0), rdU = C(0, 0), rdV = C(0, 0),
+           rdW = C(0, 0);
+      const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
+      const Cplx at = tide ? C(tideAmp[2 * pair], tideAmp[2 * pair + 1]) : C(0.0, 0.0);
+      for (int i = 0; i < g.ngll; ++i) {
+        const int slot = (elem - g.elem0) * g.ngll + i;
+        const int d0 = sDof[2 * slot], d1 = sDof[2 * slot + 1];
+        Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
+        if (d0 >= 0) {
+          Ui = C(sRe[d0], sIm[d0]);
+          if (unitsOn) Ui = Ui + sUnit[d0] * a;
+          if (tide) Ui = Ui + sTide[d0] * at;
+        }
+        if (d1 >= 0) {
+          Vi = C(sRe[d1], sIm[d1]);
+          if (unitsOn) Vi = Vi + sUnit[d1] * a;
+          if (tide) Vi = Vi + sTide[d1] * at;
+        }
+        if (tor1) {
+          const int dw = tDof[slot];
+          if (dw >= 0) Wi = C(tRe[dw], tIm[dw]);
+        }
+        const double v = g.value[q * g.ngll + i];
+        U = U + v * Ui;
+        V = V + v * Vi;
+        W = W + v * Wi;
+        const double bb = r * g.deriv[q * g.ngll + i] / jac;
+        rdU = rdU + bb * Ui;
+        rdV = rdV + bb * Vi;
+        rdW = rdW + bb * Wi;
       }
-      if (d1 >= 0) {
-        Vi = C(sRe[d1], sIm[d1]);
-        if (unitsOn) Vi = Vi + sUnit[d1] * a;
-        if (tide) Vi = Vi + sTide[d1] * at;
+      const double zeta2 = l * (l + 1.0);
+      const double zeta = sqrt(zeta2);
+      A[2] = Div(rdU - U + (0.5 * zeta2) * V, 3.0 * r);
+      if (l >= 1) {
+        const Cplx tangential = rdV - V + U;
+        const Cplx twist = W - rdW;
+        const double den = 2.0 * kSqrt2 * r;
+        A[1] = Div(zeta * (tangential + TimesI(twist)), den);
+        A[3] = Div(zeta * (tangential - TimesI(twist)), den);
       }
-      if (tor1) {
-        const int dw = tDof[slot];
-        if (dw >= 0) Wi = C(tRe[dw], tIm[dw]);
+      if (l >= 2) {
+        const double f = 0.5 * zeta * sqrt(zeta2 - 2.0) / r;
+        A[0] = f * (V - TimesI(W));
+        A[4] = f * (V + TimesI(W));
       }
-      const double v = g.value[q * g.ngll + i];
-      U = U + v * Ui;
-      V = V + v * Vi;
-      W = W + v * Wi;
-      const double bb = r * g.deriv[q * g.ngll + i] / jac;
-      rdU = rdU + bb * Ui;
-      rdV = rdV + bb * Vi;
-      rdW = rdW + bb * Wi;
     }
-    const double zeta2 = l * (l + 1.0);
-    const double zeta = sqrt(zeta2);
-    A[2] = Div(rdU - U + (0.5 * zeta2) * V, 3.0 * r);
-    if (l >= 1) {
-      const Cplx tangential = rdV - V + U;
-      const Cplx twist = W - rdW;
-      const double den = 2.0 * kSqrt2 * r;
-      A[1] = Div(zeta * (tangential + TimesI(twist)), den);
-      A[3] = Div(zeta * (tangential - TimesI(twist)), den);
-    }
+    const double sign = (m & 1) ? -1.0 : 1.0;
     if (l >= 2) {
-      const double f = 0.5 * zeta * sqrt(zeta2 - 2.0) / r;
-      A[0] = f * (V - TimesI(W));
-      A[4] = f * (V + TimesI(W));
+      Store(out, 0, l, m, node, g.nodes, A[0]);
+      if (m > 0) Store(out, 0, l, -m, node, g.nodes, sign * Conj(A[4]));
     }
+    if (l >= 1) {
+      Store(out, 1, l, m, node, g.nodes, A[1]);
+      if (m > 0) Store(out, 1, l, -m, node, g.nodes, sign * Conj(A[3]));
+    }
+    Store(out, 2, l, m, node, g.nodes, A[2]);
   }
-  const double sign = (m & 1) ? -1.0 : 1.0;
-  if (l >= 2) {
-    Store(out, 0, l, m, node, g.nodes, A[0]);
-    if (m > 0) Store(out, 0, l, -m, node, g.nodes, sign * Conj(A[4]));
-  }
-  if (l >= 1) {
-    Store(out, 1, l, m, node, g.nodes, A[1]);
-    if (m > 0) Store(out, 1, l, -m, node, g.nodes, sign * Conj(A[3]));
-  }
-  Store(out, 2, l, m, node, g.nodes, A[2]);
 }
 
 __global__ void FillKernel(double* dst, size_t count, double value) {
@@ -584,12 +587,13 @@ __global__ void AndersonFirstKernel(const double* x, const double* gx, double* x
 
 // RESIDUAL: the pass's RelativeChange (max |gx|^2, max |gx - x|^2, a
 // non-finite flag) from the same loads, into residualPartials (3 per block).
-template <bool RESIDUAL>
+// W: the sums' width, 8 or 16 (AndersonWidth).
+template <bool RESIDUAL, int W>
 __global__ void __launch_bounds__(kThreads)
     AndersonSweepKernel(AndersonSweep s, ActiveRuns a, double* partials,
                         double* residualPartials) {
-  __shared__ double red[kThreads][17];
-  double col[8] = {}, gam[8] = {};
+  __shared__ double red[kThreads][2 * W + 1];
+  double col[W] = {}, gam[W] = {};
   double scale = 0.0, diff = 0.0, bad = 0.0;
   double* dFs = s.dF[s.slot];
   double* dGs = s.dG[s.slot];
@@ -608,7 +612,7 @@ __global__ void __launch_bounds__(kThreads)
     Store2(s.xPrev, c, xs);
     Store2(s.fPrev, c, f);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < W; ++i) {
       if (i < s.m) {
         const double2 di = i == s.slot ? df : Load2(s.dF[i], c);
         col[i] += di.x * df.x + di.y * df.y;
@@ -616,18 +620,18 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
   });
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < W; ++i) {
     red[threadIdx.x][i] = col[i];
-    red[threadIdx.x][8 + i] = gam[i];
+    red[threadIdx.x][W + i] = gam[i];
   }
   __syncthreads();
   for (int w = kThreads / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) {
-      for (int i = 0; i < 16; ++i) red[threadIdx.x][i] += red[threadIdx.x + w][i];
+      for (int i = 0; i < 2 * W; ++i) red[threadIdx.x][i] += red[threadIdx.x + w][i];
     }
     __syncthreads();
   }
-  if (threadIdx.x < 16) partials[blockIdx.x * 16 + threadIdx.x] = red[0][threadIdx.x];
+  if (threadIdx.x < 2 * W) partials[blockIdx.x * 2 * W + threadIdx.x] = red[0][threadIdx.x];
   if constexpr (RESIDUAL) {
     __syncthreads();
     red[threadIdx.x][0] = scale;
@@ -679,7 +683,7 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 struct Gamma {
-  double v[8];
+  double v[kAndersonMax];
 };
 
 __global__ void AndersonMixKernel(AndersonSweep s, Gamma gamma, ActiveRuns a) {
@@ -813,8 +817,7 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
                   const double* tideUnits, const double* tideAmp, SpectralView out,
                   hipStream_t stream) {
   const int pairs = (lmax + 1) * (lmax + 2) / 2;
-  if (g.nodes > 1024) throw ArgumentError("at most 1024 memory nodes");
-  const int threads = 64 * CeilDiv(g.nodes, 64);
+  const int threads = std::min(1024, 64 * CeilDiv(g.nodes, 64));
   ProfileScope prof509("strain", stream);
   // every task's memory dofs fit these bounds (see StrainStagedKernel)
   const int table = g.elements * g.ngll;
@@ -870,18 +873,30 @@ void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPr
 void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double* partials,
                          int partialBlocks, double* sums, double* residual,
                          hipStream_t stream) {
+  if (s.m < 1 || s.m > kAndersonMax) throw ArgumentError("Anderson history of 1 to 16");
   ProfileScope prof("anderson", stream);
-  double* residualPartials = partials + static_cast<size_t>(partialBlocks) * 16;
-  if (residual) {
-    hipLaunchKernelGGL(AndersonSweepKernel<true>, dim3(partialBlocks), dim3(kThreads), 0,
-                       stream, s, runs, partials, residualPartials);
+  const int W = AndersonWidth(s.m);
+  double* residualPartials = partials + static_cast<size_t>(partialBlocks) * 2 * W;
+  if (W == 8) {
+    if (residual) {
+      hipLaunchKernelGGL((AndersonSweepKernel<true, 8>), dim3(partialBlocks), dim3(kThreads), 0,
+                         stream, s, runs, partials, residualPartials);
+    } else {
+      hipLaunchKernelGGL((AndersonSweepKernel<false, 8>), dim3(partialBlocks), dim3(kThreads),
+                         0, stream, s, runs, partials, nullptr);
+    }
   } else {
-    hipLaunchKernelGGL(AndersonSweepKernel<false>, dim3(partialBlocks), dim3(kThreads), 0,
-                       stream, s, runs, partials, nullptr);
+    if (residual) {
+      hipLaunchKernelGGL((AndersonSweepKernel<true, kAndersonMax>), dim3(partialBlocks),
+                         dim3(kThreads), 0, stream, s, runs, partials, residualPartials);
+    } else {
+      hipLaunchKernelGGL((AndersonSweepKernel<false, kAndersonMax>), dim3(partialBlocks),
+                         dim3(kThreads), 0, stream, s, runs, partials, nullptr);
+    }
   }
   SL3_HIP(hipGetLastError());
-  hipLaunchKernelGGL(ReducePartialsKernel, dim3(16), dim3(kThreads), 0, stream, partials,
-                     partialBlocks, 16, 0, sums);
+  hipLaunchKernelGGL(ReducePartialsKernel, dim3(2 * W), dim3(kThreads), 0, stream, partials,
+                     partialBlocks, 2 * W, 0, sums);
   SL3_HIP(hipGetLastError());
   if (residual) {
     hipLaunchKernelGGL(ReducePartialsKernel, dim3(3), dim3(kThreads), 0, stream,
@@ -893,7 +908,7 @@ void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double*
 void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,
                        hipStream_t stream) {
   Gamma g{};
-  for (int i = 0; i < s.m && i < 8; ++i) g.v[i] = gamma[i];
+  for (int i = 0; i < s.m && i < kAndersonMax; ++i) g.v[i] = gamma[i];
   ProfileScope prof("anderson", stream);
   hipLaunchKernelGGL(AndersonMixKernel, dim3(2048), dim3(kThreads), 0, stream, s, g, runs);
   SL3_HIP(hipGetLastError());

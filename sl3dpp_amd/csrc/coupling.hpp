@@ -100,6 +100,12 @@ struct ActiveRuns {
   long long total = 0;
 };
 
+// The deepest Anderson history the kernels hold.
+constexpr int kAndersonMax = 16;
+// The sweep's sums for a history of m columns: the Gram column at [0, w),
+// the right-hand side at [w, 2 w), w = 8 up to m = 8, else 16.
+inline int AndersonWidth(int m) { return m <= 8 ? 8 : kAndersonMax; }
+
 // Anderson (AndersonMixer.hpp:123-218) over the active entries: x and gx
 // in the device layout, the history (xPrev, fPrev, dF, dG) compact.
 struct AndersonSweep {
@@ -107,8 +113,8 @@ struct AndersonSweep {
   const double* gx;     // the pass's image
   double* xPrev;
   double* fPrev;
-  double* dF[8];
-  double* dG[8];
+  double* dF[kAndersonMax];
+  double* dG[kAndersonMax];
   int m;                // active history columns
   int slot;             // the slot being written
 };
@@ -124,9 +130,10 @@ void LaunchAndersonFirst(double* x, const double* gx, double* xPrev,
                          double* fPrev, const ActiveRuns& runs, size_t n,
                          hipStream_t stream);
 // The Anderson sweep (differences, the Gram column and right-hand side into
-// sums[0..15]); with residual non-null also the pass's RelativeChange
-// (max |gx|^2, max |gx - x|^2, a non-finite flag) into residual[0..2].
-// partials holds partialBlocks x 19 doubles.
+// sums[0, 2 w), w = AndersonWidth(m)); with residual non-null also the
+// pass's RelativeChange (max |gx|^2, max |gx - x|^2, a non-finite flag)
+// into residual[0..2]. partials holds partialBlocks x (2 kAndersonMax + 3)
+// doubles.
 void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double* partials,
                          int partialBlocks, double* sums, double* residual,
                          hipStream_t stream);

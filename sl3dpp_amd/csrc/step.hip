@@ -50,7 +50,7 @@ int Blocks(size_t n) {
 // least-squares solution through a cyclic Jacobi eigen-decomposition
 // (singular values of a symmetric matrix are its |eigenvalues|).
 void GramLeastSquares(int m, const double* gram, const double* rhs, double* gamma) {
-  double a[64], v[64];
+  double a[kAndersonMax * kAndersonMax], v[kAndersonMax * kAndersonMax];
   for (int i = 0; i < m * m; ++i) a[i] = gram[i];
   for (int i = 0; i < m; ++i)
     for (int j = 0; j < m; ++j) v[i * m + j] = i == j ? 1.0 : 0.0;
@@ -565,8 +565,9 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
   response_.Resize(std::max<size_t>(fDoubles_, 1));
   ampDoubles_ = static_cast<size_t>(lmax + 1) * (lmax + 2);
   amp_.Resize(2 * ampDoubles_);
-  partials_.Resize(1024 * 19);  // the sweep's 16 sums and the residual's 3 maxima per block
-  result_.Resize(19);
+  // per block the sweep's sums (at most 2 kAndersonMax) and the residual's 3 maxima
+  partials_.Resize(1024 * (2 * kAndersonMax + 3));
+  result_.Resize(2 * kAndersonMax + 3);
   // the Legendre kernels' tile shapes, timed on the pass's own buffers
   LaunchFill(s_.data(), S, 0.0, s);
   {
@@ -574,7 +575,7 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
     coupling_.TuneLegendre(s_.data(), A, analysed_.data(), s);
   }
   for (auto& e : events_) SL3_HIP(hipEventCreate(&e));
-  SL3_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 32 * sizeof(double),
+  SL3_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), kReadBackMax * sizeof(double),
                         hipHostMallocDefault));
   SL3_HIP(hipEventCreateWithFlags(&readEvent_, hipEventDisableTiming));
   SL3_HIP(hipStreamSynchronize(s));
@@ -594,7 +595,7 @@ Viscoelastic3D::~Viscoelastic3D() {
 // sleep until an interrupt wakes it (a pass reads its residual this way, and
 // the device idles until the host has launched the next work).
 void Viscoelastic3D::ReadBack(double* dst, const double* src, int count) {
-  if (count < 0 || count > 32) throw ArgumentError("read-back of at most 32 doubles");
+  if (count < 0 || count > kReadBackMax) throw ArgumentError("read-back of at most 40 doubles");
   hipStream_t s = Stream();
   SL3_HIP(hipMemcpyAsync(pinned_, src, count * sizeof(double), hipMemcpyDeviceToHost, s));
   SL3_HIP(hipEventRecord(readEvent_, s));
@@ -1504,7 +1505,7 @@ StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
     sw.gx = sNext_.data();
     sw.xPrev = xPrev_.data();
     sw.fPrev = fPrev_.data();
-    for (int i = 0; i < memory && i < 8; ++i) {
+    for (int i = 0; i < memory && i < kAndersonMax; ++i) {
       sw.dF[i] = history_.data() + static_cast<size_t>(i) * C;
       sw.dG[i] = history_.data() + static_cast<size_t>(memory + i) * C;
     }
@@ -1527,17 +1528,19 @@ StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
     const bool fused = mixing && mixerStarted && options_.damping == 1.0;
     const int m = std::min(count + 1, memory);
     AndersonSweep sw{};
-    double sums[16];
+    double sums[2 * kAndersonMax];
+    const int width = AndersonWidth(m);  // the Gram column at [0, w), the right-hand side after
     double residual;
     if (fused) {
       sw = sweepOf(m);
       LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(),
-                          result_.data() + 16, s);
-      double r[19];
-      ReadBack(r, result_.data(), 19);
-      std::copy(r, r + 16, sums);
-      residual = r[18] != 0.0 ? std::numeric_limits<double>::quiet_NaN()
-                 : r[16] > 0.0 ? std::sqrt(r[17]) / std::sqrt(r[16])
+                          result_.data() + 2 * width, s);
+      double r[2 * kAndersonMax + 3];
+      ReadBack(r, result_.data(), 2 * width + 3);
+      std::copy(r, r + 2 * width, sums);
+      const double* rc = r + 2 * width;
+      residual = rc[2] != 0.0 ? std::numeric_limits<double>::quiet_NaN()
+                 : rc[0] > 0.0 ? std::sqrt(rc[1]) / std::sqrt(rc[0])
                                : 0.0;
     } else {
       // Every pass's image is zero below each degree's truncation; the start
@@ -1582,7 +1585,7 @@ StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
         if (!fused) {
           sw = sweepOf(m);
           LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(), nullptr, s);
-          ReadBack(sums, result_.data(), 16);
+          ReadBack(sums, result_.data(), 2 * width);
         }
         count = m;
         for (int i = 0; i < m; ++i) {
@@ -1590,10 +1593,10 @@ StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
           gram[i * memory + slot] = sums[i];
         }
         slot = (slot + 1) % memory;
-        double g[64], gamma[8];
+        double g[kAndersonMax * kAndersonMax], gamma[kAndersonMax];
         for (int i = 0; i < m; ++i)
           for (int j = 0; j < m; ++j) g[i * m + j] = gram[i * memory + j];
-        GramLeastSquares(m, g, sums + 8, gamma);
+        GramLeastSquares(m, g, sums + width, gamma);
         LaunchAndersonMix(sw, gamma, runs, s);
       }
       ++info.mixedPasses;

@@ -397,6 +397,7 @@ class Viscoelastic3D {
   hipEvent_t events_[4] = {};
   // a pass's few result doubles come back through pinned memory, the host
   // polling an event behind the copy (ReadBack)
+  static constexpr int kReadBackMax = 40;  // the sweep's 2 x 16 sums and 3 maxima
   double* pinned_ = nullptr;
   hipEvent_t readEvent_ = nullptr;
   void ReadBack(double* dst, const double* src, int count);
