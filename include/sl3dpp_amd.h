@@ -274,6 +274,8 @@ int sl3_degree_operator_apply(const sl3_degree_operator* op, const double* x,
 /* Options::StepSolver (include/sl3dpp/Viscoelastic3D.hpp:122-123). */
 enum { SL3_FIXED_POINT = 0, SL3_CONJUGATE_GRADIENTS = 1 };
 
+#define SL3_MAX_TRANSFORM_DEGREES 32
+
 typedef struct {
   double eps;           /* truncation level, 1e-8 */
   double oversampling;  /* 1.5 */
@@ -290,6 +292,20 @@ typedef struct {
                            each pass replays a hipGraph captured on its first
                            use (per input/output tensor pair, flavour and tide
                            channel), the same kernels and results */
+  int transform_degree_count; /* 0 */
+  int transform_degrees[SL3_MAX_TRANSFORM_DEGREES];
+                        /* Options::transform_degrees (include/sl3dpp/
+                           Viscoelastic3D.hpp:172-175, GridCoupling.hpp:
+                           99-107): the degrees in [2, lmax] the transforms
+                           may be cut to with depth. Each memory element's
+                           nodes are transformed at the least listed degree
+                           (or lmax) at or above the largest degree whose
+                           subset reaches the element, on a grid of that
+                           degree (src/GridCoupling.cpp:163-190): the nodes
+                           form blocks, and every grid field over the nodes
+                           lays the blocks' grids end to end
+                           (sl3_operator_blocks). Fixed when the operator is
+                           made. */
 } sl3_step_options;
 
 typedef struct {
@@ -321,9 +337,22 @@ int sl3_operator_set_options(sl3_operator* op, const sl3_step_options* options);
 /* Replaces the lateral field: itau(x) = itau_ref(node) exp(-delta(x)) from
  * delta_dev ([node][theta][phi], device; NULL for delta = 0). */
 int sl3_operator_set_delta(sl3_operator* op, const double* delta_dev);
-/* Memory nodes, grid shape, doubles of a spectral tensor (device layout). */
+/* Memory nodes, grid shape, doubles of a spectral tensor (device layout).
+ * The grid is lmax's; with transform degrees some nodes have smaller ones
+ * (sl3_operator_blocks). */
 int sl3_operator_shape(const sl3_operator* op, int* nodes, int* ntheta,
                        int* nphi, int64_t* spectral_doubles);
+/* The node blocks (GridCoupling's FieldOffset / GridOf, include/sl3dpp/
+ * GridCoupling.hpp:176-230): *count blocks (at most capacity are written;
+ * any array may be NULL), each its first memory node, node count, degree
+ * and grid (ntheta x nphi); *grid_values, the values of a grid field over
+ * every node, the blocks end to end (each block's nodes [node][theta][phi]
+ * on its grid). */
+int sl3_operator_blocks(const sl3_operator* op, int capacity, int* count, int* first_node,
+                        int* nodes, int* lmax, int* ntheta, int* nphi, int64_t* grid_values);
+/* One block's colatitudes and Gauss-Legendre weights (its ntheta each). */
+int sl3_operator_block_geometry(const sl3_operator* op, int block, double* theta,
+                                double* wtheta);
 /* Host arrays: node radii (nodes), colatitudes and their weights (ntheta). */
 int sl3_operator_geometry(const sl3_operator* op, double* radii, double* theta,
                           double* wtheta);

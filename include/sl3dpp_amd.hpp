@@ -87,6 +87,9 @@ class Viscoelastic3D {
     int solve_max_iterations = 200;
     /* not in the reference: each pass replays a captured hipGraph */
     bool graph_passes = false;
+    /* the degrees the transforms may be cut to with depth (at most
+     * SL3_MAX_TRANSFORM_DEGREES) */
+    std::vector<int> transform_degrees{};
   };
 
   /* Viscoelastic3D::StepInfo (:357-375). */
@@ -335,6 +338,13 @@ class Viscoelastic3D {
     c.solve_rtol = o.solve_rtol;
     c.solve_max_iterations = o.solve_max_iterations;
     c.graph_passes = o.graph_passes ? 1 : 0;
+    if (o.transform_degrees.size() > SL3_MAX_TRANSFORM_DEGREES) {
+      throw std::invalid_argument("at most 32 transform degrees");
+    }
+    c.transform_degree_count = static_cast<int>(o.transform_degrees.size());
+    for (size_t i = 0; i < o.transform_degrees.size(); ++i) {
+      c.transform_degrees[i] = o.transform_degrees[i];
+    }
     return c;
   }
 

@@ -95,8 +95,7 @@ static int lmin_of(int b, int m) {
   return am > an ? am : an;
 }
 
-static void tables_build(tables_t* t, const sl3o_problem* p) {
-  const int L = p->lmax, nt = p->ntheta, nph = p->nphi;
+static void tables_build(tables_t* t, int L, int nt, int nph, const double* theta) {
   t->lmax = L;
   t->ntheta = nt;
   t->nphi = nph;
@@ -117,7 +116,7 @@ static void tables_build(tables_t* t, const sl3o_problem* p) {
       if (lmin > L) continue;
       for (int i = 0; i < nt; ++i) {
         double* col = t->d[b] + t->moff[b][m - mlo] + (int64_t)i * (L - lmin + 1);
-        sl3o_wigner_d_column(L, spin_of(b), m, p->theta[i], col);
+        sl3o_wigner_d_column(L, spin_of(b), m, theta[i], col);
         for (int l = lmin; l <= L; ++l) col[l - lmin] *= sqrt((2.0 * l + 1.0) / (4.0 * M_PI));
       }
     }
@@ -142,7 +141,73 @@ static void tables_free(tables_t* t) {
   free(t->sinT);
 }
 
-static tables_t* g_tables = NULL;
+/* Every node's grid: its block's tables and weights, and where its values
+ * start in a grid field. */
+typedef struct {
+  int count;
+  tables_t* tables;       /* per block */
+  const double** wtheta;  /* per block */
+  int* node_block;        /* per node */
+  size_t* node_offset;    /* per node, in grid values */
+  size_t* node_size;      /* per node: its grid's values */
+  size_t values;          /* over every node */
+} grids_t;
+
+static void grids_layout(const sl3o_problem* p, int* count, int* node_block,
+                         size_t* node_offset, size_t* node_size, size_t* values) {
+  const int blocks = p->block_count > 0 ? p->block_count : 1;
+  size_t off = 0;
+  for (int node = 0; node < p->nodes; ++node) {
+    int k = 0;
+    if (p->block_count > 0) {
+      while (k + 1 < blocks && p->block_first_node[k + 1] <= node) ++k;
+    }
+    const size_t fs = p->block_count > 0 ? (size_t)p->block_ntheta[k] * p->block_nphi[k]
+                                         : (size_t)p->ntheta * p->nphi;
+    if (node_block) node_block[node] = k;
+    if (node_offset) node_offset[node] = off;
+    if (node_size) node_size[node] = fs;
+    off += fs;
+  }
+  if (count) *count = blocks;
+  if (values) *values = off;
+}
+
+int64_t sl3o_grid_values(const sl3o_problem* p) {
+  size_t values = 0;
+  grids_layout(p, NULL, NULL, NULL, NULL, &values);
+  return (int64_t)values;
+}
+
+static void grids_build(grids_t* g, const sl3o_problem* p) {
+  g->node_block = (int*)malloc(sizeof(int) * (p->nodes > 0 ? p->nodes : 1));
+  g->node_offset = (size_t*)malloc(sizeof(size_t) * (p->nodes > 0 ? p->nodes : 1));
+  g->node_size = (size_t*)malloc(sizeof(size_t) * (p->nodes > 0 ? p->nodes : 1));
+  grids_layout(p, &g->count, g->node_block, g->node_offset, g->node_size, &g->values);
+  g->tables = (tables_t*)calloc(g->count, sizeof(tables_t));
+  g->wtheta = (const double**)calloc(g->count, sizeof(double*));
+  for (int k = 0; k < g->count; ++k) {
+    if (p->block_count > 0) {
+      tables_build(&g->tables[k], p->block_lmax[k], p->block_ntheta[k], p->block_nphi[k],
+                   p->block_theta[k]);
+      g->wtheta[k] = p->block_wtheta[k];
+    } else {
+      tables_build(&g->tables[k], p->lmax, p->ntheta, p->nphi, p->theta);
+      g->wtheta[k] = p->wtheta;
+    }
+  }
+}
+
+static void grids_free(grids_t* g) {
+  for (int k = 0; k < g->count; ++k) tables_free(&g->tables[k]);
+  free(g->tables);
+  free(g->wtheta);
+  free(g->node_block);
+  free(g->node_offset);
+  free(g->node_size);
+}
+
+static grids_t* g_grids = NULL;
 
 static void synthesise_block(const tables_t* t, int b, const double* c, double* f) {
   const int L = t->lmax, nt = t->ntheta, nph = t->nphi, mlo = mlo_of(b, L);
@@ -224,26 +289,29 @@ static void analyse_block(const tables_t* t, int b, const double* wtheta, const 
   }
 }
 
+/* Each node on its block's grid, its coefficients (a prefix of lmax's in
+ * the reference's l-major order) to the block's degree. */
 static void synthesise_tensor(const sl3o_problem* p, const double* s, double* const grid[3]) {
-  const size_t fs = (size_t)p->ntheta * p->nphi;
 #pragma omp parallel for schedule(dynamic) collapse(2)
   for (int node = 0; node < p->nodes; ++node) {
     for (int b = 0; b < 3; ++b) {
       const double* c = s + block_offset(b, p->lmax, p->nodes) + 2 * node * block_size(b, p->lmax);
-      double* f = grid[b] + (b == 2 ? 1 : 2) * node * fs;
-      synthesise_block(g_tables, b, c, f);
+      double* f = grid[b] + (b == 2 ? 1 : 2) * g_grids->node_offset[node];
+      synthesise_block(&g_grids->tables[g_grids->node_block[node]], b, c, f);
     }
   }
 }
 
+/* The coefficients above a node's block degree are zero. */
 static void analyse_tensor(const sl3o_problem* p, double* const grid[3], double* s) {
-  const size_t fs = (size_t)p->ntheta * p->nphi;
 #pragma omp parallel for schedule(dynamic) collapse(2)
   for (int node = 0; node < p->nodes; ++node) {
     for (int b = 0; b < 3; ++b) {
       double* c = s + block_offset(b, p->lmax, p->nodes) + 2 * node * block_size(b, p->lmax);
-      const double* f = grid[b] + (b == 2 ? 1 : 2) * node * fs;
-      analyse_block(g_tables, b, p->wtheta, f, c);
+      const double* f = grid[b] + (b == 2 ? 1 : 2) * g_grids->node_offset[node];
+      const int k = g_grids->node_block[node];
+      memset(c, 0, sizeof(double) * 2 * block_size(b, p->lmax));
+      analyse_block(&g_grids->tables[k], b, g_grids->wtheta[k], f, c);
     }
   }
 }
@@ -543,8 +611,6 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
               sl3o_info* info) {
   const int L = p->lmax;
   const int ntasks = 2 * L + 1;
-  const size_t fs = (size_t)p->ntheta * p->nphi;
-  const size_t G = fs * p->nodes;
   const int64_t S = sl3o_spectral_doubles(L, p->nodes);
   int status = 0;
   double clock0 = now_ms();
@@ -564,9 +630,10 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
     if (factorise(t, p->bands + p->band_off[ti]) != 0) status = 1;
   }
   if (status != 0) goto done;
-  tables_t tables;
-  tables_build(&tables, p);
-  g_tables = &tables;
+  grids_t grids;
+  grids_build(&grids, p);
+  g_grids = &grids;
+  const size_t G = grids.values;
   for (int l = 0; l <= L; ++l) {
     task_t* t = &tasks[l];
     /* the unit load and tide solved together (:1042-1055) */
@@ -611,7 +678,8 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
         sl3o_trapezoid_weights(href, &e, &a, &bb);
         refBeta = bb;
       }
-      for (size_t x = node * fs; x < (node + 1) * fs; ++x) {
+      const size_t x0 = grids.node_offset[node], x1 = x0 + grids.node_size[node];
+      for (size_t x = x0; x < x1; ++x) {
         const double h = dt * p->itau[x];
         double ew, aw, bw;
         if (scheme == 1) {
@@ -736,8 +804,8 @@ int sl3o_step(const sl3o_problem* p, sl3o_state* st, double dt, int scheme,
     free(fixedS);
     free(analysed);
   }
-  tables_free(&tables);
-  g_tables = NULL;
+  grids_free(&grids);
+  g_grids = NULL;
 done:
   for (int ti = 0; ti < ntasks; ++ti) {
     free(tasks[ti].scale);
@@ -812,21 +880,21 @@ void sl3o_strain_of(const sl3o_problem* p, const double* solutions, double* coef
 }
 
 void sl3o_synthesise_tensor(const sl3o_problem* p, const double* coef, double* const grid[3]) {
-  tables_t tables;
-  tables_build(&tables, p);
-  g_tables = &tables;
+  grids_t grids;
+  grids_build(&grids, p);
+  g_grids = &grids;
   synthesise_tensor(p, coef, grid);
-  g_tables = NULL;
-  tables_free(&tables);
+  g_grids = NULL;
+  grids_free(&grids);
 }
 
 void sl3o_analyse_tensor(const sl3o_problem* p, double* const grid[3], double* coef) {
-  tables_t tables;
-  tables_build(&tables, p);
-  g_tables = &tables;
+  grids_t grids;
+  grids_build(&grids, p);
+  g_grids = &grids;
   analyse_tensor(p, grid, coef);
-  g_tables = NULL;
-  tables_free(&tables);
+  g_grids = NULL;
+  grids_free(&grids);
 }
 
 /* ---- the degree-basis stepper --------------------------------------------- */

@@ -45,6 +45,21 @@ typedef struct {
   const double* loads;
   const int64_t* tide_off;  /* spheroidal tasks' unit tides (ndof), or NULL */
   const double* tides;
+  /* node blocks (GridCoupling's Options::transform_degrees,
+   * src/GridCoupling.cpp:163-190): block_count 0 puts every node on the one
+   * grid above; otherwise block k holds the nodes from block_first_node[k]
+   * to the next block's first node, transformed at degree block_lmax[k] on
+   * its own grid (block_ntheta[k] x block_nphi[k], colatitudes
+   * block_theta[k], weights block_wtheta[k]); a node's coefficients of
+   * degree above its block's are zero, and every grid field (itau, the
+   * state's grids) lays the nodes' grids end to end. */
+  int block_count;
+  const int* block_first_node;
+  const int* block_lmax;
+  const int* block_ntheta;
+  const int* block_nphi;
+  const double* const* block_theta;
+  const double* const* block_wtheta;
 } sl3o_problem;
 
 typedef struct {
@@ -81,6 +96,8 @@ typedef struct {
 
 /* Size in doubles of the reference-layout spectral tensor. */
 int64_t sl3o_spectral_doubles(int lmax, int nodes);
+/* Values of a grid field over every node (the nodes' grids end to end). */
+int64_t sl3o_grid_values(const sl3o_problem* p);
 
 /* One step in place; amp, tamp: the load and tide channels' columns at
  * the step's end time, complex per (l, m) at l(l+1)/2 + m (NULL for none;

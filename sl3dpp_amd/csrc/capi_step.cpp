@@ -75,6 +75,11 @@ StepOptions ToOptions(const sl3_step_options* o) {
   s.solveRtol = o->solve_rtol;
   s.solveMaxIterations = o->solve_max_iterations;
   s.graphPasses = o->graph_passes != 0;
+  Check(o->transform_degree_count >= 0 &&
+            o->transform_degree_count <= SL3_MAX_TRANSFORM_DEGREES,
+        "transform_degree_count in [0, 32]");
+  s.transformDegrees.assign(o->transform_degrees,
+                            o->transform_degrees + o->transform_degree_count);
   Check(s.solver == kFixedPoint || s.solver == kConjugateGradients,
         "solver: SL3_FIXED_POINT or SL3_CONJUGATE_GRADIENTS");
   Check(s.solveMaxIterations >= 0, "solve_max_iterations >= 0");
@@ -296,6 +301,8 @@ void sl3_step_options_default(sl3_step_options* o) {
   o->solve_rtol = s.solveRtol;
   o->solve_max_iterations = s.solveMaxIterations;
   o->graph_passes = s.graphPasses ? 1 : 0;
+  o->transform_degree_count = 0;
+  for (int i = 0; i < SL3_MAX_TRANSFORM_DEGREES; ++i) o->transform_degrees[i] = 0;
 }
 
 int sl3_operator_create(const sl3_earth* earth, int lmax, const double* delta_dev,
@@ -324,6 +331,13 @@ int sl3_operator_set_options(sl3_operator* op, const sl3_step_options* options) 
     const auto& cur = op->op->Options();
     Check(o.eps == cur.eps && o.oversampling == cur.oversampling,
           "eps and oversampling are fixed at construction");
+    auto sorted = [](std::vector<int> d) {
+      std::sort(d.begin(), d.end());
+      d.erase(std::unique(d.begin(), d.end()), d.end());
+      return d;
+    };
+    Check(sorted(o.transformDegrees) == sorted(cur.transformDegrees),
+          "the transform degrees are fixed at construction");
     op->op->MutableOptions() = o;
   });
 }
@@ -344,6 +358,36 @@ int sl3_operator_shape(const sl3_operator* op, int* nodes, int* ntheta, int* nph
     if (ntheta) *ntheta = c.Grid().NTheta();
     if (nphi) *nphi = c.Grid().NPhi();
     if (spectral_doubles) *spectral_doubles = static_cast<int64_t>(c.SpectralDoubles());
+  });
+}
+
+int sl3_operator_blocks(const sl3_operator* op, int capacity, int* count, int* first_node,
+                        int* nodes, int* lmax, int* ntheta, int* nphi, int64_t* grid_values) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    const auto& c = op->op->GetCoupling();
+    if (count) *count = c.BlockCount();
+    if (grid_values) *grid_values = static_cast<int64_t>(c.GridValueCount());
+    for (int i = 0; i < c.BlockCount() && i < capacity; ++i) {
+      const NodeBlock& k = c.Block(i);
+      if (first_node) first_node[i] = k.firstNode;
+      if (nodes) nodes[i] = k.nodes;
+      if (lmax) lmax[i] = k.lmax;
+      if (ntheta) ntheta[i] = k.grid.NTheta();
+      if (nphi) nphi[i] = k.grid.NPhi();
+    }
+  });
+}
+
+int sl3_operator_block_geometry(const sl3_operator* op, int block, double* theta,
+                                double* wtheta) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    const auto& c = op->op->GetCoupling();
+    Check(block >= 0 && block < c.BlockCount(), "no such block");
+    const SurfaceGrid& g = c.Block(block).grid;
+    if (theta) std::memcpy(theta, g.CoLatitudes().data(), sizeof(double) * g.NTheta());
+    if (wtheta) std::memcpy(wtheta, g.CoLatitudeWeights().data(), sizeof(double) * g.NTheta());
   });
 }
 

@@ -106,12 +106,14 @@ SurfaceGrid::SurfaceGrid(int lmax, double oversampling) : lmax_(lmax) {
   for (int b = 0; b < 3; ++b) layouts_[b] = SpinLayout::Make(SpinOf(b), lmax);
 }
 
-std::vector<double> SurfaceGrid::WignerTable(int block) const {
-  const SpinLayout& lay = layouts_[block];
+std::vector<double> SurfaceGrid::WignerTableIn(int block, const SpinLayout& lay) const {
+  if (lay.spin != layouts_[block].spin || lay.lmax < lmax_) {
+    throw ArgumentError("a Wigner table's rows: the block's spin, degree >= the grid's");
+  }
   const int N = lay.spin;
   std::vector<double> table(static_cast<size_t>(lay.Rows()) * ntheta_, 0.0);
   std::vector<double> column(lmax_ + 1);
-  for (int m = lay.mlo; m <= lmax_; ++m) {
+  for (int m = layouts_[block].mlo; m <= lmax_; ++m) {
     const int lmin = lay.LMin(m);
     if (lmin > lmax_) continue;
     for (int i = 0; i < ntheta_; ++i) {

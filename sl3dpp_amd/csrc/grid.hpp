@@ -57,7 +57,11 @@ class SurfaceGrid {
   // block 0: spin -2, 1: spin -1, 2: spin 0 (real)
   static int SpinOf(int block) { return block == 0 ? -2 : (block == 1 ? -1 : 0); }
   // The Wigner table of a block (see the file comment).
-  std::vector<double> WignerTable(int block) const;
+  std::vector<double> WignerTable(int block) const { return WignerTableIn(block, layouts_[block]); }
+  // The same table on this grid's colatitudes with the rows of `rows` (the
+  // layout of a degree >= this grid's): the rows of degrees and orders up to
+  // this grid's degree filled, the others zero.
+  std::vector<double> WignerTableIn(int block, const SpinLayout& rows) const;
 
  private:
   int lmax_, ntheta_, nphi_;

@@ -69,6 +69,10 @@ struct Args {
   int orderCount, colBlocks, rowBlocks, thetaBlocks;
   int lmax, nodes, ntheta;
   int fstride;  // the Fourier layout's colatitude stride
+  // the node block: its first node and the spectral rows' node count; the
+  // Fourier layout's first order per block
+  int nodeOffset, nodesTotal;
+  int fourierMlo[3];
 };
 
 __device__ inline int LMinOf(int m, int spin) {
@@ -116,15 +120,17 @@ __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
   const int c0 = cb * kCols;
   const int node0 = c0 / 2, nodeEnd = min(a.nodes, node0 + kCols / 2);
   // degrees that reach a node of the column block
-  const int lhi = min(lEnd, a.reach ? a.reach[nodeEnd - 1] : a.lmax);
+  const int lhi = min(lEnd, a.reach ? a.reach[a.nodeOffset + nodeEnd - 1] : a.lmax);
   const int rt = lhi >= l0 ? (lhi - l0 + 16) / 16 : 0;  // live 16-degree tiles
   const int row0 = a.rowOffset[b][m - a.mlo[b]] - lmin;  // row of (m, l) = row0 + l
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const double* T = a.table[b];
   const double* F = a.fourierIn[b];
-  const size_t mi = static_cast<size_t>(m - a.mlo[b]);
-  double* out = a.spectralOut[b];
-  const double* add = a.addend[b];
+  const size_t mi = static_cast<size_t>(m - a.fourierMlo[b]);
+  // the block's columns of the spectral rows
+  const size_t rowCols = 2 * static_cast<size_t>(a.nodesTotal);
+  double* out = a.spectralOut[b] + 2 * a.nodeOffset;
+  const double* add = a.addend[b] ? a.addend[b] + 2 * a.nodeOffset : nullptr;
 
   Acc4 acc[kRowBlock / 16];
 #pragma unroll
@@ -203,9 +209,9 @@ __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
         if (l > lEnd) continue;
         const size_t row = static_cast<size_t>(row0 + l);
         double v = l <= lhi ? acc[t][j] : 0.0;
-        if (a.firstNode && (c >> 1) < a.firstNode[l]) v = 0.0;
-        if (add) v += add[row * cols + c];
-        out[row * cols + c] = v;
+        if (a.firstNode && a.nodeOffset + (c >> 1) < a.firstNode[l]) v = 0.0;
+        if (add) v += add[row * rowCols + c];
+        out[row * rowCols + c] = v;
       }
     }
   }
@@ -234,11 +240,13 @@ __global__ void __launch_bounds__(kThreads) LegendreSynthesisMfmaKernel(Args a) 
   const int cols = 2 * a.nodes;
   const int c0 = cb * kCols;
   const int node0 = c0 / 2, nodeEnd = min(a.nodes, node0 + kCols / 2);
-  const int lhi = a.reach ? min(a.lmax, a.reach[nodeEnd - 1]) : a.lmax;
+  const int lhi = a.reach ? min(a.lmax, a.reach[a.nodeOffset + nodeEnd - 1]) : a.lmax;
   const int row0 = a.rowOffset[b][m - a.mlo[b]] - lmin;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const double* T = a.table[b];
-  const double* S = a.spectral[b];
+  // the block's columns of the spectral rows
+  const size_t rowCols = 2 * static_cast<size_t>(a.nodesTotal);
+  const double* S = a.spectral[b] + 2 * a.nodeOffset;
 
   // a chunk of degrees l0 .. l0 + 15 into registers: the table rows over
   // the block's colatitudes and the coefficient rows of its columns
@@ -258,7 +266,7 @@ __global__ void __launch_bounds__(kThreads) LegendreSynthesisMfmaKernel(Args a) 
       const int k = e / kCols, cc = e % kCols;
       const int l = l0 + k, c = c0 + cc;
       ps[q] = 0.0;
-      if (l <= lhi && c < cols) ps[q] = S[static_cast<size_t>(row0 + l) * cols + c];
+      if (l <= lhi && c < cols) ps[q] = S[static_cast<size_t>(row0 + l) * rowCols + c];
     }
   };
   auto store = [&]() {
@@ -295,9 +303,10 @@ __global__ void __launch_bounds__(kThreads) LegendreSynthesisMfmaKernel(Args a) 
     __syncthreads();
   }
 
-  // C[theta][col] -> F[((m - mlo) nodes + node) ntheta + theta] (re, im)
+  // C[theta][col] -> F[((m - mlo) nodes + node) ntheta + theta] (re, im),
+  // the block's orders and nodes
   double* Fo = a.fourier[b];
-  const size_t mi = static_cast<size_t>(m - a.mlo[b]);
+  const size_t mi = static_cast<size_t>(m - a.fourierMlo[b]);
   const int c = c0 + wave * 16 + lane % 16;
   if (c < cols) {
     const int node = c >> 1, part = c & 1;
@@ -320,7 +329,10 @@ Args MakeArgs(const LegendrePlan& p) {
     a.rowOffset[b] = p.rowOffset[b];
     a.mlo[b] = p.mlo[b];
     a.spin[b] = p.spin[b];
+    a.fourierMlo[b] = p.fourierMlo[b];
   }
+  a.nodeOffset = p.nodeOffset;
+  a.nodesTotal = p.nodesTotal > 0 ? p.nodesTotal : p.nodes;
   a.orders = reinterpret_cast<const OrderRef*>(p.orders);
   a.orderCount = p.orderCount;
   a.colBlocks = CeilDiv(2 * p.nodes, kCols);

@@ -32,6 +32,19 @@ __global__ void ItauKernel(const double* delta, const double* itauRef, double* i
   }
 }
 
+// The rows of a spectral block above a node block's degree, at the node
+// block's columns: the addend's values there, or zero.
+__global__ void CutRowsKernel(const int* rows, int count, double* block, const double* addend,
+                              size_t rowDoubles, size_t first, int width) {
+  const size_t total = static_cast<size_t>(count) * width;
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += stride) {
+    const size_t at = static_cast<size_t>(rows[i / width]) * rowDoubles + first + i % width;
+    block[at] = addend ? addend[at] : 0.0;
+  }
+}
+
 // Converge's damping (src/Viscoelastic3D.cpp:1949-1957): n = o + d (n - o).
 __global__ void DampKernel(double* next, const double* now, double d, size_t n) {
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -102,7 +115,8 @@ void GramLeastSquares(int m, const double* gram, const double* rhs, double* gamm
 
 // ---- Coupling ---------------------------------------------------------------
 
-Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps)
+Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps,
+                   const std::vector<int>& transformDegrees)
     : lmax_(lmax), grid_(lmax, oversampling) {
   hipStream_t s = Stream();
   const auto& mesh = *earth.mesh;
@@ -168,55 +182,49 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
   firstNode_.Upload(firstNodeHost_.data(), firstNodeHost_.size(), s);
   reach_.Upload(reachHost_.data(), reachHost_.size(), s);
 
-  std::vector<int> orders;
+  // the spectral layout: the full degree's rows, every node
   spectralOffset_[0] = 0;
   for (int b = 0; b < 3; ++b) {
     const auto& lay = grid_.Layout(b);
-    const auto table = grid_.WignerTable(b);
-    table_[b].Upload(table.data(), table.size(), s);
     rowOffset_[b].Upload(lay.rowOffset.data(), lay.rowOffset.size(), s);
-    for (int m = lay.mlo; m <= lmax; ++m) {
-      orders.push_back(b);
-      orders.push_back(m);
-    }
     spectralOffset_[b + 1] =
         spectralOffset_[b] + static_cast<size_t>(lay.Rows()) * geometry_.nodes * 2;
-    plan_.table[b] = table_[b].data();
-    plan_.rowOffset[b] = rowOffset_[b].data();
-    plan_.mlo[b] = lay.mlo;
-    plan_.spin[b] = lay.spin;
-    SL3_HIP(hipStreamSynchronize(s));  // the host table is freed next
   }
-  orders_.Upload(orders.data(), orders.size(), s);
-  orderCount_ = static_cast<int>(orders.size() / 2);
-  plan_.orders = orders_.data();
-  plan_.orderCount = orderCount_;
-  plan_.lmax = lmax;
-  plan_.nodes = geometry_.nodes;
-  plan_.ntheta = grid_.NTheta();
-  plan_.fstride = FourierStride(grid_.NTheta());
-
-  const int nphi = grid_.NPhi();
-  dft_.R = nphi % 4 == 0 ? 4 : 2;
-  dft_.P = nphi / dft_.R;
-  std::vector<double> cp(dft_.P), sp(dft_.P), cn(nphi), sn(nphi);
-  for (int j = 0; j < dft_.P; ++j) {
-    cp[j] = std::cos(2.0 * kPi * j / dft_.P);
-    sp[j] = std::sin(2.0 * kPi * j / dft_.P);
+  // the node blocks (GridCoupling::TransformDegrees, src/GridCoupling.cpp:
+  // 163-190): each memory element at the least listed degree, or lmax, at or
+  // above the largest degree whose subset reaches it; the reach only grows
+  // outwards, so each degree's nodes are one contiguous block
+  std::vector<int> degrees = transformDegrees;
+  for (int d : degrees) {
+    if (d < 2 || d > lmax) {
+      throw ArgumentError("a transform degree of " + std::to_string(d) + " is outside [2, lmax = " +
+                          std::to_string(lmax) + "]");
+    }
   }
-  for (int j = 0; j < nphi; ++j) {
-    cn[j] = std::cos(2.0 * kPi * j / nphi);
-    sn[j] = std::sin(2.0 * kPi * j / nphi);
+  degrees.push_back(lmax);
+  std::sort(degrees.begin(), degrees.end());
+  degrees.erase(std::unique(degrees.begin(), degrees.end()), degrees.end());
+  for (int node = 0; node < geometry_.nodes; ++node) {
+    const int d = *std::lower_bound(degrees.begin(), degrees.end(), reachHost_[node]);
+    if (blocks_.empty() || blocks_.back()->lmax != d) {
+      if (!blocks_.empty() && blocks_.back()->lmax > d) {
+        throw ArgumentError("the node blocks' degrees must grow outwards");
+      }
+      blocks_.push_back(std::make_unique<NodeBlock>(d, oversampling));
+      blocks_.back()->firstNode = node;
+    }
+    ++blocks_.back()->nodes;
   }
-  cosP_.Upload(cp.data(), cp.size(), s);
-  sinP_.Upload(sp.data(), sp.size(), s);
-  cosN_.Upload(cn.data(), cn.size(), s);
-  sinN_.Upload(sn.data(), sn.size(), s);
-  wtheta_.Upload(grid_.CoLatitudeWeights().data(), grid_.NTheta(), s);
-  dft_.cosP = cosP_.data();
-  dft_.sinP = sinP_.data();
-  dft_.cosN = cosN_.data();
-  dft_.sinN = sinN_.data();
+  for (auto& k : blocks_) {
+    k->fieldOffset = gridValues_;
+    gridValues_ += static_cast<size_t>(k->nodes) * k->grid.FieldSize();
+    for (int b = 0; b < 3; ++b) {
+      k->fourierOffset[b] = fourierDoubles_[b];
+      fourierDoubles_[b] += static_cast<size_t>(k->grid.Layout(b).Orders()) * k->nodes *
+                            FourierStride(k->grid.NTheta()) * 2;
+    }
+    BuildBlock(*k, s);
+  }
   // the active runs of a spectral tensor: per row, in memory order, the
   // nodes at or above the row degree's first memory node
   {
@@ -245,12 +253,80 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
     runs_.total = total;
     SL3_HIP(hipStreamSynchronize(s));  // the host tables are freed next
   }
+  SL3_HIP(hipStreamSynchronize(s));
+}
+
+// A node block's tables: the Wigner tables on its colatitudes (rows in
+// the full degree's layout, up to the block's degree), its orders, its
+// Legendre plan, the longitude DFT's tables of its grid.
+void Coupling::BuildBlock(NodeBlock& k, hipStream_t s) {
+  const SurfaceGrid& g = k.grid;
+  const int lmax = k.lmax;
+  std::vector<int> orders;
+  for (int b = 0; b < 3; ++b) {
+    const auto& lay = g.Layout(b);
+    const auto table = g.WignerTableIn(b, grid_.Layout(b));
+    k.table[b].Upload(table.data(), table.size(), s);
+    for (int m = lay.mlo; m <= lmax; ++m) {
+      orders.push_back(b);
+      orders.push_back(m);
+    }
+    k.plan.table[b] = k.table[b].data();
+    k.plan.rowOffset[b] = rowOffset_[b].data();
+    k.plan.mlo[b] = grid_.Layout(b).mlo;
+    k.plan.spin[b] = lay.spin;
+    k.plan.fourierMlo[b] = lay.mlo;
+    SL3_HIP(hipStreamSynchronize(s));  // the host table is freed next
+  }
+  k.orders.Upload(orders.data(), orders.size(), s);
+  k.plan.orders = k.orders.data();
+  k.plan.orderCount = static_cast<int>(orders.size() / 2);
+  k.plan.lmax = lmax;
+  k.plan.nodes = k.nodes;
+  k.plan.ntheta = g.NTheta();
+  k.plan.fstride = FourierStride(g.NTheta());
+  k.plan.nodeOffset = k.firstNode;
+  k.plan.nodesTotal = geometry_.nodes;
+  for (int b = 0; b < 3; ++b) {
+    std::vector<int> cut;
+    const auto& lay = grid_.Layout(b);
+    for (int m = lay.mlo; m <= lmax_; ++m) {
+      for (int l = std::max(lay.LMin(m), lmax + 1); l <= lmax_; ++l) cut.push_back(lay.Row(l, m));
+    }
+    k.cutRowCount[b] = static_cast<int>(cut.size());
+    if (cut.empty()) continue;
+    k.cutRows[b].Upload(cut.data(), cut.size(), s);
+    SL3_HIP(hipStreamSynchronize(s));  // the host list is freed next
+  }
+
+  DftTables& dft = k.dft;
+  const int nphi = g.NPhi();
+  dft.R = nphi % 4 == 0 ? 4 : 2;
+  dft.P = nphi / dft.R;
+  std::vector<double> cp(dft.P), sp(dft.P), cn(nphi), sn(nphi);
+  for (int j = 0; j < dft.P; ++j) {
+    cp[j] = std::cos(2.0 * kPi * j / dft.P);
+    sp[j] = std::sin(2.0 * kPi * j / dft.P);
+  }
+  for (int j = 0; j < nphi; ++j) {
+    cn[j] = std::cos(2.0 * kPi * j / nphi);
+    sn[j] = std::sin(2.0 * kPi * j / nphi);
+  }
+  k.cosP.Upload(cp.data(), cp.size(), s);
+  k.sinP.Upload(sp.data(), sp.size(), s);
+  k.cosN.Upload(cn.data(), cn.size(), s);
+  k.sinN.Upload(sn.data(), sn.size(), s);
+  k.wtheta.Upload(g.CoLatitudeWeights().data(), g.NTheta(), s);
+  dft.cosP = k.cosP.data();
+  dft.sinP = k.sinP.data();
+  dft.cosN = k.cosN.data();
+  dft.sinN = k.sinN.data();
   // the matrix-core operand tables of the P-point transforms
-  dft_.jt = RingMfmaTiles(dft_.P);
-  if (dft_.jt > 0) {
-    const int P = dft_.P, H = (P - 1) / 2, ks = 4 * dft_.jt;
-    std::vector<double> tc(static_cast<size_t>(dft_.jt) * ks * 64, 0.0), ts(tc.size(), 0.0);
-    for (int t = 0; t < dft_.jt; ++t) {
+  dft.jt = RingMfmaTiles(dft.P);
+  if (dft.jt > 0) {
+    const int P = dft.P, H = (P - 1) / 2, ks = 4 * dft.jt;
+    std::vector<double> tc(static_cast<size_t>(dft.jt) * ks * 64, 0.0), ts(tc.size(), 0.0);
+    for (int t = 0; t < dft.jt; ++t) {
       for (int kb = 0; kb < ks; ++kb) {
         for (int lane = 0; lane < 64; ++lane) {
           const int j = 1 + 16 * t + lane % 16, k = 1 + 4 * kb + lane / 16;
@@ -262,22 +338,19 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
         }
       }
     }
-    tabC_.Upload(tc.data(), tc.size(), s);
-    tabS_.Upload(ts.data(), ts.size(), s);
-    dft_.tabC = tabC_.data();
-    dft_.tabS = tabS_.data();
+    k.tabC.Upload(tc.data(), tc.size(), s);
+    k.tabS.Upload(ts.data(), ts.size(), s);
+    dft.tabC = k.tabC.data();
+    dft.tabS = k.tabS.data();
     SL3_HIP(hipStreamSynchronize(s));  // the host tables are freed next
   }
-  if (RingLds(nphi, dft_.P) > 160 * 1024) {
+  if (RingLds(nphi, dft.P) > 160 * 1024) {
     throw ArgumentError("grid too large for the ring kernel's LDS staging");
   }
   SL3_HIP(hipStreamSynchronize(s));
 }
 
-size_t Coupling::FourierDoubles(int block) const {
-  return static_cast<size_t>(grid_.Layout(block).Orders()) * geometry_.nodes *
-         FourierStride(grid_.NTheta()) * 2;
-}
+size_t Coupling::FourierDoubles(int block) const { return fourierDoubles_[block]; }
 
 SpectralView Coupling::View(double* spectral) const {
   SpectralView v;
@@ -293,7 +366,11 @@ void Coupling::SynthesiseToFourier(const double* spectral, double* const fourier
                                    bool truncated, hipStream_t s) const {
   const double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
                              spectral + spectralOffset_[2]};
-  LaunchLegendreSynthesis(plan_, blocks, fourier, truncated ? reach_.data() : nullptr, s);
+  for (const auto& k : blocks_) {
+    double* out[3] = {fourier[0] + k->fourierOffset[0], fourier[1] + k->fourierOffset[1],
+                      fourier[2] + k->fourierOffset[2]};
+    LaunchLegendreSynthesis(k->plan, blocks, out, truncated ? reach_.data() : nullptr, s);
+  }
 }
 
 void Coupling::AnalyseFromFourier(const double* const fourier[3], double* spectral,
@@ -305,9 +382,32 @@ void Coupling::AnalyseFromFourier(const double* const fourier[3], double* spectr
   if (addend) {
     for (int b = 0; b < 3; ++b) adds[b] = addend + spectralOffset_[b];
   }
-  LaunchLegendreAnalysis(plan_, fourier, blocks, addend ? adds : nullptr,
-                         truncated ? reach_.data() : nullptr,
-                         truncated ? firstNode_.data() : nullptr, s);
+  for (const auto& k : blocks_) {
+    const double* in[3] = {fourier[0] + k->fourierOffset[0], fourier[1] + k->fourierOffset[1],
+                           fourier[2] + k->fourierOffset[2]};
+    LaunchLegendreAnalysis(k->plan, in, blocks, addend ? adds : nullptr,
+                           truncated ? reach_.data() : nullptr,
+                           truncated ? firstNode_.data() : nullptr, s);
+  }
+  // a truncated analysis feeds the force, which reads only the degrees
+  // that reach each node (all within its block's): the rows above the
+  // block's degree are left as they are
+  if (!truncated) FillCutRows(blocks, addend ? adds : nullptr, s);
+}
+
+void Coupling::FillCutRows(double* const spectral[3], const double* const addend[3],
+                           hipStream_t s) const {
+  for (const auto& k : blocks_) {
+    for (int b = 0; b < 3; ++b) {
+      if (spectral[b] == nullptr || k->cutRowCount[b] == 0) continue;
+      const size_t total = static_cast<size_t>(k->cutRowCount[b]) * 2 * k->nodes;
+      hipLaunchKernelGGL(CutRowsKernel, dim3(Blocks(total)), dim3(256), 0, s,
+                         k->cutRows[b].data(), k->cutRowCount[b], spectral[b],
+                         addend ? addend[b] : nullptr, 2 * static_cast<size_t>(geometry_.nodes),
+                         2 * static_cast<size_t>(k->firstNode), 2 * k->nodes);
+      SL3_HIP(hipGetLastError());
+    }
+  }
 }
 
 namespace {
@@ -320,65 +420,92 @@ int OrderStart(const SurfaceGrid& g, int block) {
 
 void Coupling::SynthesiseBlockToFourier(int block, const double* spectral,
                                         double* fourier, hipStream_t s) const {
-  LegendrePlan p = plan_;
-  p.orders = plan_.orders + 2 * OrderStart(grid_, block);
-  p.orderCount = grid_.Layout(block).Orders();
-  const double* blocks[3] = {};
-  double* fouriers[3] = {};
-  blocks[block] = spectral;
-  fouriers[block] = fourier;
-  LaunchLegendreSynthesis(p, blocks, fouriers, nullptr, s);
+  for (const auto& k : blocks_) {
+    LegendrePlan p = k->plan;
+    p.orders = k->plan.orders + 2 * OrderStart(k->grid, block);
+    p.orderCount = k->grid.Layout(block).Orders();
+    const double* blocks[3] = {};
+    double* fouriers[3] = {};
+    blocks[block] = spectral;
+    fouriers[block] = fourier + k->fourierOffset[block];
+    LaunchLegendreSynthesis(p, blocks, fouriers, nullptr, s);
+  }
 }
 
-void Coupling::AnalyseBlockFromFourier(int block, const double* fourier,
-                                       double* spectral, hipStream_t s) const {
-  LegendrePlan p = plan_;
-  p.orders = plan_.orders + 2 * OrderStart(grid_, block);
-  p.orderCount = grid_.Layout(block).Orders();
-  const double* fouriers[3] = {};
+void Coupling::AnalyseBlockFromFourier(int block, const double* fourier, double* spectral,
+                                       hipStream_t s) const {
+  for (const auto& k : blocks_) {
+    LegendrePlan p = k->plan;
+    p.orders = k->plan.orders + 2 * OrderStart(k->grid, block);
+    p.orderCount = k->grid.Layout(block).Orders();
+    const double* fouriers[3] = {};
+    double* blocks[3] = {};
+    blocks[block] = spectral;
+    fouriers[block] = fourier + k->fourierOffset[block];
+    LaunchLegendreAnalysis(p, fouriers, blocks, nullptr, nullptr, nullptr, s);
+  }
   double* blocks[3] = {};
   blocks[block] = spectral;
-  fouriers[block] = fourier;
-  LaunchLegendreAnalysis(p, fouriers, blocks, nullptr, nullptr, nullptr, s);
+  FillCutRows(blocks, nullptr, s);
+}
+
+RingLaunch Coupling::BlockLaunch(RingLaunch r, const NodeBlock& k) const {
+  const SurfaceGrid& g = k.grid;
+  r.tables = k.dft;
+  r.lmax = k.lmax;
+  r.nodes = k.nodes;
+  r.ntheta = g.NTheta();
+  r.fstride = FourierStride(g.NTheta());
+  r.nphi = g.NPhi();
+  r.wtheta = k.wtheta.data();
+  const size_t points = k.fieldOffset;
+  const size_t values = (r.block == 2 ? 1 : 2) * points;  // a spin block's grid component
+  const size_t fourier = k.fourierOffset[r.block];
+  auto at = [](auto* p, size_t offset) { return p != nullptr ? p + offset : p; };
+  r.fourierIn = at(r.fourierIn, fourier);
+  r.fourierOut = at(r.fourierOut, fourier);
+  r.fourierOut2 = at(r.fourierOut2, fourier);
+  r.gridIn = at(r.gridIn, values);
+  r.gridOut = at(r.gridOut, values);
+  r.memIn = at(r.memIn, values);
+  r.memOut = at(r.memOut, values);
+  r.weight = at(r.weight, points);
+  r.itau = at(r.itau, points);
+  r.betaOut = at(r.betaOut, points);
+  r.weightOut = at(r.weightOut, points);
+  r.beta = at(r.beta, points);
+  r.muNode = at(r.muNode, static_cast<size_t>(k.firstNode));
+  r.itauRefNode = at(r.itauRefNode, static_cast<size_t>(k.firstNode));
+  return r;
 }
 
 void Coupling::Ring(RingLaunch r, hipStream_t s) const {
-  r.tables = dft_;
-  r.lmax = lmax_;
-  r.nodes = geometry_.nodes;
-  r.ntheta = grid_.NTheta();
-  r.fstride = FourierStride(grid_.NTheta());
-  r.nphi = grid_.NPhi();
-  r.wtheta = wtheta_.data();
-  LaunchRing(r, s);
+  for (const auto& k : blocks_) LaunchRing(BlockLaunch(r, *k), s);
 }
 
 void Coupling::RingPass(double* const in[3], double* const out[3], const double* weight,
                         hipStream_t s) const {
-  RingLaunch r;
-  r.tables = dft_;
-  r.lmax = lmax_;
-  r.nodes = geometry_.nodes;
-  r.ntheta = grid_.NTheta();
-  r.fstride = FourierStride(grid_.NTheta());
-  r.nphi = grid_.NPhi();
-  r.wtheta = wtheta_.data();
-  r.op = kRingPass;
-  r.weight = weight;
-  if (RingPassFusable(r)) {
-    RingBlocks blocks;
-    for (int b = 0; b < 3; ++b) {
-      blocks.in[b] = in[b];
-      blocks.out[b] = out[b];
+  for (const auto& k : blocks_) {
+    RingLaunch base;
+    base.op = kRingPass;
+    base.weight = weight;
+    const RingLaunch r = BlockLaunch(base, *k);
+    if (RingPassFusable(r)) {
+      RingBlocks blocks;
+      for (int b = 0; b < 3; ++b) {
+        blocks.in[b] = in[b] + k->fourierOffset[b];
+        blocks.out[b] = out[b] + k->fourierOffset[b];
+      }
+      LaunchRingPassMfma(r, blocks, s);
+      continue;
     }
-    LaunchRingPassMfma(r, blocks, s);
-    return;
-  }
-  for (int b = 0; b < 3; ++b) {
-    r.block = b;
-    r.fourierIn = in[b];
-    r.fourierOut = out[b];
-    LaunchRing(r, s);
+    for (int b = 0; b < 3; ++b) {
+      RingLaunch rb = base;
+      rb.block = b;
+      rb.fourierIn = in[b];
+      rb.fourierOut = out[b];
+      LaunchRing(BlockLaunch(rb, *k), s);
+    }
   }
 }
 
@@ -463,68 +590,82 @@ void Coupling::TuneLegendre(const double* spectral, double* const fourier[3],
   for (size_t i = 0; i < firstNodeHost_.size(); ++i) {
     truncation = truncation * 37 + firstNodeHost_[i];
   }
-  const std::array<long long, 4> key{lmax_, geometry_.nodes, grid_.NTheta(), truncation};
-  std::vector<double> result;
-  {
-    std::lock_guard<std::mutex> lock(cache.mutex);
-    auto it = cache.chosen.find(key);
-    if (it != cache.chosen.end()) result = it->second;
-  }
-  if (result.empty()) {
-    result.assign(2 + 2 * kTileChoices, 0.0);
-    hipEvent_t ev[2];
-    SL3_HIP(hipEventCreate(&ev[0]));
-    SL3_HIP(hipEventCreate(&ev[1]));
-    auto time = [&](auto&& launch) {
-      launch();
-      SL3_HIP(hipEventRecord(ev[0], s));
-      for (int r = 0; r < 5; ++r) launch();
-      SL3_HIP(hipEventRecord(ev[1], s));
-      SL3_HIP(hipEventSynchronize(ev[1]));
-      float ms = 0.0f;
-      SL3_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-      return ms / 5.0;
-    };
-    const double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
-                               spectral + spectralOffset_[2]};
-    double* outs[3] = {spectralOut + spectralOffset_[0], spectralOut + spectralOffset_[1],
-                       spectralOut + spectralOffset_[2]};
-    const double* fin[3] = {fourier[0], fourier[1], fourier[2]};
-    double best = 0.0;  // the synthesis choices, then the analysis choices
-    for (int i = 0; i < kTileChoices; ++i) {
-      LegendrePlan p = plan_;
-      p.synthesisTheta = kSynthesisThetaChoices[i];
-      const double ms = time([&] {
-        LaunchLegendreSynthesisMfma(p, blocks, fourier, reach_.data(), s);
-      });
-      result[2 + kTileChoices + i] = ms;
-      if (i == 0 || ms < best) {
-        best = ms;
-        result[1] = p.synthesisTheta;
-      }
+  const double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
+                             spectral + spectralOffset_[2]};
+  double* outs[3] = {spectralOut + spectralOffset_[0], spectralOut + spectralOffset_[1],
+                     spectralOut + spectralOffset_[2]};
+  for (auto& kp : blocks_) {
+    NodeBlock& k = *kp;
+    const std::array<long long, 4> key{
+        lmax_ * 1000003LL + k.lmax, geometry_.nodes * 1000003LL + k.firstNode,
+        k.nodes * 1000003LL + k.grid.NTheta(), truncation};
+    std::vector<double> result;
+    {
+      std::lock_guard<std::mutex> lock(cache.mutex);
+      auto it = cache.chosen.find(key);
+      if (it != cache.chosen.end()) result = it->second;
     }
-    for (int i = 0; i < kTileChoices; ++i) {
-      LegendrePlan p = plan_;
-      p.analysisRows = kAnalysisRowsChoices[i];
-      const double ms = time([&] {
-        LaunchLegendreAnalysisMfma(p, fin, outs, nullptr, reach_.data(), firstNode_.data(), s);
-      });
-      result[2 + i] = ms;
-      if (i == 0 || ms < best) {
-        best = ms;
-        result[0] = p.analysisRows;
+    if (result.empty()) {
+      result.assign(2 + 2 * kTileChoices, 0.0);
+      hipEvent_t ev[2];
+      SL3_HIP(hipEventCreate(&ev[0]));
+      SL3_HIP(hipEventCreate(&ev[1]));
+      auto time = [&](auto&& launch) {
+        launch();
+        SL3_HIP(hipEventRecord(ev[0], s));
+        for (int r = 0; r < 5; ++r) launch();
+        SL3_HIP(hipEventRecord(ev[1], s));
+        SL3_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.0f;
+        SL3_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        return ms / 5.0;
+      };
+      double* fo[3] = {fourier[0] + k.fourierOffset[0], fourier[1] + k.fourierOffset[1],
+                       fourier[2] + k.fourierOffset[2]};
+      const double* fin[3] = {fo[0], fo[1], fo[2]};
+      double best = 0.0;  // the synthesis choices, then the analysis choices
+      for (int i = 0; i < kTileChoices; ++i) {
+        LegendrePlan p = k.plan;
+        p.synthesisTheta = kSynthesisThetaChoices[i];
+        const double ms = time([&] {
+          LaunchLegendreSynthesisMfma(p, blocks, fo, reach_.data(), s);
+        });
+        result[2 + kTileChoices + i] = ms;
+        if (i == 0 || ms < best) {
+          best = ms;
+          result[1] = p.synthesisTheta;
+        }
       }
+      for (int i = 0; i < kTileChoices; ++i) {
+        LegendrePlan p = k.plan;
+        p.analysisRows = kAnalysisRowsChoices[i];
+        const double ms = time([&] {
+          LaunchLegendreAnalysisMfma(p, fin, outs, nullptr, reach_.data(), firstNode_.data(),
+                                     s);
+        });
+        result[2 + i] = ms;
+        if (i == 0 || ms < best) {
+          best = ms;
+          result[0] = p.analysisRows;
+        }
+      }
+      (void)hipEventDestroy(ev[0]);
+      (void)hipEventDestroy(ev[1]);
+      std::lock_guard<std::mutex> lock(cache.mutex);
+      cache.chosen.emplace(key, result);
     }
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-    std::lock_guard<std::mutex> lock(cache.mutex);
-    cache.chosen.emplace(key, result);
+    k.plan.analysisRows = forcedRows > 0 ? forcedRows : static_cast<int>(result[0]);
+    k.plan.synthesisTheta = forcedTheta > 0 ? forcedTheta : static_cast<int>(result[1]);
+    k.tuning = result;
+    k.tuning[0] = k.plan.analysisRows;
+    k.tuning[1] = k.plan.synthesisTheta;
   }
-  tuning_ = result;
-  plan_.analysisRows = forcedRows > 0 ? forcedRows : static_cast<int>(result[0]);
-  plan_.synthesisTheta = forcedTheta > 0 ? forcedTheta : static_cast<int>(result[1]);
-  tuning_[0] = plan_.analysisRows;
-  tuning_[1] = plan_.synthesisTheta;
+}
+
+std::vector<double> Coupling::Tuning() const {
+  std::vector<double> out;
+  for (const auto& k : blocks_) out.insert(out.end(), k->tuning.begin(), k->tuning.end());
+  return out;
 }
 
 // ---- Viscoelastic3D ---------------------------------------------------------
@@ -533,7 +674,7 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
                                const double* delta_dev, StepOptions options)
     : earth_(std::move(earthPtr)),
       options_(options),
-      coupling_(*earth_, lmax, options.oversampling, options.eps) {
+      coupling_(*earth_, lmax, options.oversampling, options.eps, options.transformDegrees) {
   const Earth& earth = *earth_;
   hipStream_t s = Stream();
   const auto& geo = coupling_.Geometry();
@@ -554,6 +695,10 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
   const size_t S = coupling_.SpectralDoubles();
   fixed_.Resize(S);
   analysed_.Resize(S);
+  // the rows no truncated analysis writes (above a node block's degree)
+  // stay zero
+  LaunchFill(fixed_.data(), S, 0.0, s);
+  LaunchFill(analysed_.data(), S, 0.0, s);
   s_.Resize(S);
   sNext_.Resize(S);
   for (int b = 0; b < 3; ++b) {
@@ -611,9 +756,16 @@ void Viscoelastic3D::SetDelta(const double* delta_dev) {
   const size_t G = coupling_.GridValueCount();
   itau_.Resize(G);
   ProfileScope prof381("itau", s);
-  hipLaunchKernelGGL(ItauKernel, dim3(Blocks(G)), dim3(256), 0, s, delta_dev,
-                     itauRefNode_.data(), itau_.data(), coupling_.Grid().FieldSize(), G);
-  SL3_HIP(hipGetLastError());
+  // each node block's nodes on the block's grid
+  for (int i = 0; i < coupling_.BlockCount(); ++i) {
+    const NodeBlock& k = coupling_.Block(i);
+    const size_t per = k.grid.FieldSize(), count = per * k.nodes;
+    hipLaunchKernelGGL(ItauKernel, dim3(Blocks(count)), dim3(256), 0, s,
+                       delta_dev ? delta_dev + k.fieldOffset : nullptr,
+                       itauRefNode_.data() + k.firstNode, itau_.data() + k.fieldOffset, per,
+                       count);
+    SL3_HIP(hipGetLastError());
+  }
   SL3_HIP(hipStreamSynchronize(s));
 }
 

@@ -45,6 +45,9 @@ struct StepOptions {
   // each pass (synthesis, ring pass, analysis, force, response, strain)
   // replays a hipGraph captured on its first use
   bool graphPasses = false;
+  // Options::transform_degrees (include/sl3dpp/Viscoelastic3D.hpp:
+  // 172-175): the degrees the transforms may be cut to with depth
+  std::vector<int> transformDegrees;
 };
 
 // Viscoelastic3D::StepInfo (include/sl3dpp/Viscoelastic3D.hpp:357-375).
@@ -69,15 +72,45 @@ struct Channels {
   LoadFn tide;
 };
 
+// A node block (GridCoupling's Options::transform_degrees,
+// include/sl3dpp/GridCoupling.hpp:23-40, src/GridCoupling.cpp:163-190):
+// contiguous memory nodes transformed at the block's degree on a grid of
+// that degree, with its own Wigner and DFT tables; every grid field lays the
+// blocks end to end, every spin block's Fourier buffer too.
+struct NodeBlock {
+  NodeBlock(int lmax, double oversampling) : lmax(lmax), grid(lmax, oversampling) {}
+  int firstNode = 0, nodes = 0, lmax = 0;
+  SurfaceGrid grid;
+  size_t fieldOffset = 0;        // grid values before the block
+  size_t fourierOffset[3] = {};  // Fourier doubles before the block, per spin block
+  DeviceArray<double> table[3];  // rows in the full degree's layout
+  // the rows of degree above the block's (zero at the block's nodes after
+  // an analysis), per spin block
+  DeviceArray<int> cutRows[3];
+  int cutRowCount[3] = {};
+  DeviceArray<int> orders;
+  DeviceArray<double> wtheta, cosP, sinP, cosN, sinN, tabC, tabS;
+  LegendrePlan plan;
+  DftTables dft;
+  std::vector<double> tuning;
+};
+
 // The device grid coupling (GridCoupling + SurfaceGrid): memory nodes,
-// Wigner tables, DFT tables, layouts.
+// their node blocks, Wigner tables, DFT tables, layouts.
 class Coupling {
  public:
-  Coupling(const Earth& earth, int lmax, double oversampling, double eps);
+  // transformDegrees: the degrees the transforms may be cut to with depth
+  // (Options::transform_degrees; empty: one block at lmax).
+  Coupling(const Earth& earth, int lmax, double oversampling, double eps,
+           const std::vector<int>& transformDegrees = {});
   int LMax() const { return lmax_; }
   int NodeCount() const { return geometry_.nodes; }
+  // the full degree's grid (the layouts, and every node's grid without cuts)
   const SurfaceGrid& Grid() const { return grid_; }
-  size_t GridValueCount() const { return static_cast<size_t>(geometry_.nodes) * grid_.FieldSize(); }
+  int BlockCount() const { return static_cast<int>(blocks_.size()); }
+  const NodeBlock& Block(int i) const { return *blocks_[i]; }
+  // the values of a grid field over every node (the blocks end to end)
+  size_t GridValueCount() const { return gridValues_; }
   // doubles of each grid component (complex: 2 per value) and of a
   // spectral tensor (three blocks end to end, device layout)
   size_t GridDoubles(int block) const { return (block == 2 ? 1 : 2) * GridValueCount(); }
@@ -110,9 +143,9 @@ class Coupling {
   // zeroed tensor, fourier and spectralOut: the pass's buffers (written).
   void TuneLegendre(const double* spectral, double* const fourier[3], double* spectralOut,
                     hipStream_t s);
-  // [analysis rows, synthesis colatitudes] chosen, then the ms per launch
-  // of each analysis choice and each synthesis choice.
-  const std::vector<double>& Tuning() const { return tuning_; }
+  // Per node block: [analysis rows, synthesis colatitudes] chosen, then the
+  // ms per launch of each analysis choice and each synthesis choice.
+  std::vector<double> Tuning() const;
   // One block's Legendre stage alone (the C-ABI transforms).
   void SynthesiseBlockToFourier(int block, const double* spectral, double* fourier,
                                 hipStream_t s) const;
@@ -124,6 +157,14 @@ class Coupling {
                hipStream_t s) const;
 
  private:
+  void BuildBlock(NodeBlock& k, hipStream_t s);
+  // After an analysis: the rows above each block's degree at its nodes
+  // hold the addend (or zero), as if the block's grid carried them.
+  void FillCutRows(double* const spectral[3], const double* const addend[3],
+                   hipStream_t s) const;
+  // A ring launch's fields for one node block: sizes, tables, and every
+  // pointer moved to the block's part.
+  RingLaunch BlockLaunch(RingLaunch r, const NodeBlock& k) const;
   int lmax_;
   SurfaceGrid grid_;
   RadialGeometry geometry_;
@@ -132,18 +173,14 @@ class Coupling {
   std::vector<int> firstElement_;  // per degree
   std::vector<int> firstNodeHost_, reachHost_;
   DeviceArray<double> radius_, jac_, weight_, value_, deriv_;
-  DeviceArray<double> table_[3];
   DeviceArray<int> rowOffset_[3];
-  DeviceArray<int> orders_;
-  int orderCount_ = 0;
   DeviceArray<int> firstNode_, reach_;
-  DeviceArray<double> wtheta_, cosP_, sinP_, cosN_, sinN_, tabC_, tabS_;
-  DftTables dft_;
+  std::vector<std::unique_ptr<NodeBlock>> blocks_;
+  size_t gridValues_ = 0;
+  size_t fourierDoubles_[3] = {};
   size_t spectralOffset_[4] = {};
-  LegendrePlan plan_;
   DeviceArray<long long> runFull_, runCompact_;
   ActiveRuns runs_;
-  std::vector<double> tuning_;
 };
 
 

@@ -16,7 +16,14 @@ struct LegendrePlan {
   int spin[3] = {};
   const int* orders = nullptr;  // (block, m) per launch z, device
   int orderCount = 0;
+  // a node block (its nodes nodeOffset .. nodeOffset + nodes - 1 of the
+  // spectral tensor's nodesTotal, transformed to degree lmax on its own
+  // ntheta colatitudes): the tables and the spectral rows keep the full
+  // degree's row layout (rowOffset, mlo), the block's Fourier coefficients
+  // their own orders (from fourierMlo)
   int lmax = 0, nodes = 0, ntheta = 0;
+  int nodeOffset = 0, nodesTotal = 0;
+  int fourierMlo[3] = {};
   // the Fourier layout's colatitude stride, ntheta rounded up to 8 (whole
   // 128-byte lines per (order, node) row of complex values)
   int fstride = 0;

@@ -48,6 +48,9 @@ class EarthSpec(ctypes.Structure):
     ]
 
 
+MAX_TRANSFORM_DEGREES = 32
+
+
 class StepOptions(ctypes.Structure):
     _fields_ = [
         ("eps", ctypes.c_double),
@@ -61,6 +64,8 @@ class StepOptions(ctypes.Structure):
         ("solve_rtol", ctypes.c_double),
         ("solve_max_iterations", ctypes.c_int),
         ("graph_passes", ctypes.c_int),
+        ("transform_degree_count", ctypes.c_int),
+        ("transform_degrees_", ctypes.c_int * MAX_TRANSFORM_DEGREES),
     ]
 
     @classmethod
@@ -70,6 +75,19 @@ class StepOptions(ctypes.Structure):
         for k, v in overrides.items():
             setattr(o, k, v)
         return o
+
+    @property
+    def transform_degrees(self):
+        return list(self.transform_degrees_[:self.transform_degree_count])
+
+    @transform_degrees.setter
+    def transform_degrees(self, degrees):
+        degrees = list(degrees)
+        if len(degrees) > MAX_TRANSFORM_DEGREES:
+            raise ValueError(f"at most {MAX_TRANSFORM_DEGREES} transform degrees")
+        self.transform_degree_count = len(degrees)
+        for i, d in enumerate(degrees):
+            self.transform_degrees_[i] = int(d)
 
 
 class StepInfo(ctypes.Structure):
@@ -315,6 +333,26 @@ class Viscoelastic3D:
         check(lib.sl3_operator_geometry(self.handle, self.radii.ctypes.data_as(c_double_p),
                                         self.theta.ctypes.data_as(c_double_p),
                                         self.wtheta.ctypes.data_as(c_double_p)))
+        # the node blocks (transform degrees): every grid field lays the
+        # blocks' grids end to end, grid_values values in all
+        count, gv = ctypes.c_int(), ctypes.c_int64()
+        check(lib.sl3_operator_blocks(self.handle, 0, ctypes.byref(count), None, None, None,
+                                      None, None, ctypes.byref(gv)))
+        n = count.value
+        arrays = [(ctypes.c_int * n)() for _ in range(5)]
+        check(lib.sl3_operator_blocks(self.handle, n, ctypes.byref(count), *arrays, None))
+        self.grid_values = gv.value
+        self.blocks = []
+        offset = 0
+        for i in range(n):
+            first, nodes, lmax_b, nt_b, nph_b = (a[i] for a in arrays)
+            th, wt = np.zeros(nt_b), np.zeros(nt_b)
+            check(lib.sl3_operator_block_geometry(self.handle, i, th.ctypes.data_as(c_double_p),
+                                                  wt.ctypes.data_as(c_double_p)))
+            self.blocks.append({"first_node": first, "nodes": nodes, "lmax": lmax_b,
+                                "ntheta": nt_b, "nphi": nph_b, "theta": th, "wtheta": wt,
+                                "field_offset": offset})
+            offset += nodes * nt_b * nph_b
 
     def __del__(self, _lib=lib):  # the default keeps the library through shutdown
         if getattr(self, "handle", None):
@@ -489,8 +527,7 @@ class Viscoelastic3D:
         import torch
 
         coef = torch.from_numpy(np.ascontiguousarray(coefficients).view(np.float64).ravel()).cuda()
-        field = torch.zeros(self.nodes * self.ntheta * self.nphi, dtype=torch.float64,
-                            device="cuda")
+        field = torch.zeros(self.grid_values, dtype=torch.float64, device="cuda")
         self.synthesise(0, coef, field)
         return field
 

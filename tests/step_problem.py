@@ -28,6 +28,9 @@ class Problem(ctypes.Structure):
         ("ndof", IP), ("kd", IP), ("first_element", IP), ("pinned", IP),
         ("band_off", I64P), ("bands", DP), ("dof_off", I64P), ("dofs", IP),
         ("load_off", I64P), ("loads", DP), ("tide_off", I64P), ("tides", DP),
+        ("block_count", ctypes.c_int), ("block_first_node", IP), ("block_lmax", IP),
+        ("block_ntheta", IP), ("block_nphi", IP), ("block_theta", ctypes.POINTER(DP)),
+        ("block_wtheta", ctypes.POINTER(DP)),
     ]
 
 
@@ -65,10 +68,13 @@ class OracleStep:
     array alive while the ctypes structure points at it."""
 
     def __init__(self, oracle, earth, lmax, scheme, dt, delta, theta=None, wtheta=None,
-                 eps=1e-8):
+                 eps=1e-8, blocks=None):
         """delta: the lateral field on every memory node's grid, host array
         [node][theta][phi] (the one the operator was given), or None for
-        zero; the oracle forms itau = itau_ref(node) exp(-delta) itself."""
+        zero; the oracle forms itau = itau_ref(node) exp(-delta) itself.
+        blocks: the operator's node blocks (Viscoelastic3D.blocks) when its
+        transforms are cut with depth: each node on its block's grid, the
+        grids end to end."""
         self.oracle = oracle
         self.keep = []
         elements, ngll = earth.mesh()
@@ -119,12 +125,18 @@ class OracleStep:
                 tides.append(s["tide"])
                 load_off.append(lo)
                 lo += s["ndof"]
+        if blocks is not None and len(blocks) > 1:
+            sizes = [k["ntheta"] * k["nphi"] for k in blocks]
+            per_node = np.concatenate([np.full(k["nodes"], n) for k, n in zip(blocks, sizes)])
+        else:
+            blocks = None
+            per_node = np.full(nodes, nt * nph)
         self.arrays = dict(
             theta=np.ascontiguousarray(theta), wtheta=np.ascontiguousarray(wtheta),
             radius=self.radius, jac=jac,
             pweight=earth.export("point_weight"), pvalue=earth.export("point_value"),
             pderiv=earth.export("point_deriv"), mu=self.mu_node, itau_ref=self.itau_ref,
-            itau=self._itau(oracle, delta, nodes, nt * nph),
+            itau=self._itau(oracle, delta, per_node),
             ndof=np.array(ndof, dtype=np.int32), kd=np.array(kd, dtype=np.int32),
             fe=np.array(fe, dtype=np.int32), pinned=np.array(pinned, dtype=np.int32),
             band_off=np.array(band_off, dtype=np.int64), bands=np.concatenate(bands),
@@ -148,11 +160,27 @@ class OracleStep:
         for name, key in [("band_off", "band_off"), ("dof_off", "dof_off"),
                           ("load_off", "load_off"), ("tide_off", "load_off")]:
             setattr(p, name, a[key].ctypes.data_as(I64P))
+        p.block_count = 0
+        if blocks is not None:
+            n = len(blocks)
+            for key in ("first_node", "lmax", "ntheta", "nphi"):
+                a["block_" + key] = np.array([k[key] for k in blocks], dtype=np.int32)
+                setattr(p, "block_" + key, a["block_" + key].ctypes.data_as(IP))
+            a["block_theta_arrays"] = [np.ascontiguousarray(k["theta"]) for k in blocks]
+            a["block_wtheta_arrays"] = [np.ascontiguousarray(k["wtheta"]) for k in blocks]
+            a["block_theta"] = (DP * n)(*[x.ctypes.data_as(DP) for x in a["block_theta_arrays"]])
+            a["block_wtheta"] = (DP * n)(*[x.ctypes.data_as(DP)
+                                           for x in a["block_wtheta_arrays"]])
+            p.block_theta = ctypes.cast(a["block_theta"], ctypes.POINTER(DP))
+            p.block_wtheta = ctypes.cast(a["block_wtheta"], ctypes.POINTER(DP))
+            p.block_count = n
         self.problem = p
+        self.blocks = blocks
         self.lmax, self.nodes, self.ntheta, self.nphi = lmax, nodes, nt, nph
         oracle.sl3o_spectral_doubles.restype = ctypes.c_int64
         self.spectral_doubles = oracle.sl3o_spectral_doubles(lmax, nodes)
-        G = nodes * nt * nph
+        G = int(np.sum(per_node))
+        self.grid_values = G
         self.m = [np.zeros(2 * G), np.zeros(2 * G), np.zeros(G)]
         self.strain = [np.zeros(2 * G), np.zeros(2 * G), np.zeros(G)]
         self.coef = np.zeros(self.spectral_doubles)
@@ -165,16 +193,26 @@ class OracleStep:
         self.state.has_strain = 0
         self.state.has_coef = 0
 
-    def _itau(self, oracle, delta, nodes, per_node):
-        itau = np.zeros(nodes * per_node)
+    def _itau(self, oracle, delta, per_node):
+        """itau on every node's grid (per_node values each), node runs of
+        equal grids at a time."""
+        itau = np.zeros(int(np.sum(per_node)))
         d = None
         if delta is not None:
             d = np.ascontiguousarray(delta, dtype=np.float64).ravel()
-            assert d.size == itau.size, "delta holds nodes x ntheta x nphi values"
-        oracle.sl3o_lateral_itau(ctypes.c_int64(nodes), ctypes.c_int64(per_node),
-                                 self.itau_ref.ctypes.data_as(DP),
-                                 d.ctypes.data_as(DP) if d is not None else None,
-                                 itau.ctypes.data_as(DP))
+            assert d.size == itau.size, "delta holds a value per point of every node's grid"
+        node, at = 0, 0
+        while node < len(per_node):
+            end = node
+            while end < len(per_node) and per_node[end] == per_node[node]:
+                end += 1
+            n, size = end - node, int(per_node[node])
+            oracle.sl3o_lateral_itau(
+                ctypes.c_int64(n), ctypes.c_int64(size),
+                self.itau_ref[node:end].ctypes.data_as(DP),
+                d[at:at + n * size].ctypes.data_as(DP) if d is not None else None,
+                itau[at:].ctypes.data_as(DP))
+            node, at = end, at + n * size
         self.delta = d
         return itau
 
