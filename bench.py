@@ -64,8 +64,8 @@ def parse():
     ap.add_argument("--learned-eval", type=int, default=6,
                     help="held-out delta seeds the learned leg evaluates (full: 200)")
     ap.add_argument("--learned-epochs", type=int, default=50)
-    ap.add_argument("--extrapolate-order", type=int, default=3,
-                    help="order of the extrapolated-start trajectory leg (0 skips it)")
+    ap.add_argument("--extrapolate-order", type=int, default=4,
+                    help="order of the extrapolated-start trajectory leg, 1 to 5 (0 skips it)")
     ap.add_argument("--uneven", type=int, default=0,
                     help="an uneven batch of this many independent realisations (lmax from "
                          "{32, 64, 96, 128, 192}, meshes of {64, 96, 128} elements x degree 5, "

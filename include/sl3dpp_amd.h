@@ -395,7 +395,7 @@ int sl3_step(sl3_operator* op, sl3_state* state, double dt, int scheme,
 int sl3_step_guarded(sl3_operator* op, sl3_state* state, double dt, int scheme,
                      sl3_load_fn load, sl3_load_fn tide, void* user,
                      const double* start_dev, double guard, sl3_step_info* info);
-/* dst = sum_j coefficients[j] sources[j] over a spectral tensor (1 to 4
+/* dst = sum_j coefficients[j] sources[j] over a spectral tensor (1 to 8
  * terms; dst may be one of the sources). With the states' coefficients as
  * sources this forms the extrapolated warm start of a trajectory's next
  * step (the `start` of OpenStep::Converge, src/Viscoelastic3D.cpp:
@@ -406,7 +406,7 @@ int sl3_spectral_combine(const sl3_operator* op, int count, const double* coeffi
  * src/Viscoelastic3D.cpp:1916-1921; no counterpart in the reference's
  * library): the extrapolator keeps the last order + 1 converged strains of
  * the states it stepped, and sl3_extrapolated_step starts each step from
- * their Lagrange extrapolation in time to the step's end (order 1..3; the
+ * their Lagrange extrapolation in time to the step's end (order 1..5; the
  * step's own start while fewer than two are kept, for Sdirk2, and when the
  * state is not the last one it stepped or has moved since, which starts a
  * new history). The

@@ -829,7 +829,7 @@ int sl3_spectral_scatter(sl3_operator* op, const float* in_dev, int ld, int cut,
 int sl3_extrapolator_create(sl3_operator* op, int order, sl3_extrapolator** out) {
   return Guard([&] {
     Check(op != nullptr && out != nullptr, "null argument");
-    Check(order >= 1 && order <= 3, "extrapolation order must be 1, 2 or 3");
+    Check(order >= 1 && order <= 5, "extrapolation order must be 1 to 5");
     *out = nullptr;
     auto ex = std::make_unique<sl3_extrapolator>();
     ex->op = op;
@@ -919,7 +919,7 @@ int sl3_spectral_combine(const sl3_operator* op, int count, const double* coeffi
                          const double* const* sources_dev, double* dst_dev) {
   return Guard([&] {
     Check(op && coefficients && sources_dev && dst_dev, "null argument");
-    Check(count >= 1 && count <= 4, "combine: 1 to 4 terms");
+    Check(count >= 1 && count <= kSpectralTermsMax, "combine: 1 to 8 terms");
     SpectralTerms t;
     t.count = count;
     for (int j = 0; j < count; ++j) {

@@ -311,7 +311,7 @@ __global__ __launch_bounds__(256) void StandardiseKernel(float* __restrict__ x, 
   }
 }
 
-// dst = sum_j c_j src_j (up to four terms; dst may be one of the sources:
+// dst = sum_j c_j src_j (up to eight terms; dst may be one of the sources:
 // every element reads its sources before its store).
 __global__ __launch_bounds__(256) void CombineTermsKernel(SpectralTerms t, double* dst,
                                                           size_t n) {
@@ -653,7 +653,7 @@ void Standardise(float* x, long long rows, int cols, const float* mean, const fl
 }
 
 void LaunchCombineTerms(const SpectralTerms& t, double* dst, size_t n, hipStream_t s) {
-  if (t.count < 1 || t.count > 4) throw ArgumentError("combine: 1 to 4 terms");
+  if (t.count < 1 || t.count > kSpectralTermsMax) throw ArgumentError("combine: 1 to 8 terms");
   if (n == 0) return;
   ProfileScope prof("combine", s);
   hipLaunchKernelGGL(CombineTermsKernel, dim3(GridOf(n)), dim3(256), 0, s, t, dst, n);

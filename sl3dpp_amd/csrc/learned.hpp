@@ -86,10 +86,11 @@ void Standardise(float* x, long long rows, int cols, const float* mean, const fl
                  bool inverse, hipStream_t s);
 
 // A linear combination of up to four device vectors.
+constexpr int kSpectralTermsMax = 8;
 struct SpectralTerms {
   int count = 0;
-  double c[4] = {};
-  const double* src[4] = {};
+  double c[kSpectralTermsMax] = {};
+  const double* src[kSpectralTermsMax] = {};
 };
 void LaunchCombineTerms(const SpectralTerms& t, double* dst, size_t n, hipStream_t s);
 

@@ -61,10 +61,18 @@ def test_combine_against_numpy(trajectory_problem):
     # dst aliasing a source
     st.spectral_combine(op, [0.5, 2.0], [da.data_ptr(), db.data_ptr()], da)
     np.testing.assert_allclose(da.cpu().numpy(), 0.5 * a + 2 * b, rtol=0, atol=1e-14)
+    # eight terms, the most a combination takes (an order-5 extrapolation
+    # uses six)
+    w = [6.0, -15.0, 20.0, -15.0, 6.0, -1.0, 0.25, 0.5]
+    vs = [rng.normal(size=op.spectral_doubles) for _ in w]
+    ds = [torch.from_numpy(v).cuda() for v in vs]
+    st.spectral_combine(op, w, [d.data_ptr() for d in ds], out)
+    want = sum(c * v for c, v in zip(w, vs))
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=0, atol=1e-12)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 5])
 def test_extrapolated_trajectory_matches_the_plain_one(trajectory_problem, order):
     op, dt, load = trajectory_problem
     plain_state = op.zero_state()
