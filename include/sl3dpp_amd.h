@@ -306,6 +306,20 @@ typedef struct {
                            lays the blocks' grids end to end
                            (sl3_operator_blocks). Fixed when the operator is
                            made. */
+  int realisations;     /* 1; at most 64. Independent realisations of the
+                           step on one Earth and degree (each with its own
+                           lateral field and state; load and tide shared),
+                           stepped together: realisation r's memory nodes are
+                           nodes r N .. r N + N - 1 of every node-indexed
+                           tensor (N = sl3_operator_realisations' radial
+                           nodes), every kernel of a pass takes them all, and
+                           each realisation converges on its own residual and
+                           Anderson history, as if stepped alone
+                           (sl3_step_realisation_infos). With the fixed point
+                           only, no transform degrees and no guarded warm
+                           start; fixed when the operator is made. No
+                           reference counterpart: the reference steps
+                           realisations one operator at a time. */
 } sl3_step_options;
 
 typedef struct {
@@ -337,9 +351,9 @@ int sl3_operator_set_options(sl3_operator* op, const sl3_step_options* options);
 /* Replaces the lateral field: itau(x) = itau_ref(node) exp(-delta(x)) from
  * delta_dev ([node][theta][phi], device; NULL for delta = 0). */
 int sl3_operator_set_delta(sl3_operator* op, const double* delta_dev);
-/* Memory nodes, grid shape, doubles of a spectral tensor (device layout).
- * The grid is lmax's; with transform degrees some nodes have smaller ones
- * (sl3_operator_blocks). */
+/* Memory nodes (of every realisation), grid shape, doubles of a spectral
+ * tensor (device layout). The grid is lmax's; with transform degrees some
+ * nodes have smaller ones (sl3_operator_blocks). */
 int sl3_operator_shape(const sl3_operator* op, int* nodes, int* ntheta,
                        int* nphi, int64_t* spectral_doubles);
 /* The node blocks (GridCoupling's FieldOffset / GridOf, include/sl3dpp/
@@ -356,6 +370,18 @@ int sl3_operator_block_geometry(const sl3_operator* op, int block, double* theta
 /* Host arrays: node radii (nodes), colatitudes and their weights (ntheta). */
 int sl3_operator_geometry(const sl3_operator* op, double* radii, double* theta,
                           double* wtheta);
+/* The realisations stepped side by side (sl3_step_options.realisations)
+ * and the memory nodes of each (the shape's nodes = realisations x
+ * radial_nodes). No reference counterpart. */
+int sl3_operator_realisations(const sl3_operator* op, int* realisations, int* radial_nodes);
+/* Per realisation, the last step's (or converge's) own sl3_step_info
+ * (passes, residual, convergence, mixed passes; the timings zero): *count
+ * realisations, at most capacity written. The info a step returns for
+ * several realisations takes the most passes and mixed passes, the largest
+ * residual, and converged when every realisation converged. No reference
+ * counterpart. */
+int sl3_step_realisation_infos(const sl3_operator* op, int capacity, int* count,
+                               sl3_step_info* infos);
 
 /* LoadCallable3D: write degree l's m >= 0 columns at time t as (re, im)
  * pairs into columns (2(l+1) doubles) and return 1, or return 0 for none.

@@ -90,6 +90,8 @@ class Viscoelastic3D {
     /* the degrees the transforms may be cut to with depth (at most
      * SL3_MAX_TRANSFORM_DEGREES) */
     std::vector<int> transform_degrees{};
+    /* not in the reference: independent realisations stepped side by side */
+    int realisations = 1;
   };
 
   /* Viscoelastic3D::StepInfo (:357-375). */
@@ -345,6 +347,7 @@ class Viscoelastic3D {
     for (size_t i = 0; i < o.transform_degrees.size(); ++i) {
       c.transform_degrees[i] = o.transform_degrees[i];
     }
+    c.realisations = o.realisations;
     return c;
   }
 

@@ -80,6 +80,10 @@ StepOptions ToOptions(const sl3_step_options* o) {
         "transform_degree_count in [0, 32]");
   s.transformDegrees.assign(o->transform_degrees,
                             o->transform_degrees + o->transform_degree_count);
+  s.realisations = o->realisations;
+  Check(s.realisations >= 1 && s.realisations <= kRealisationsMax, "realisations in [1, 64]");
+  Check(s.realisations == 1 || (s.transformDegrees.empty() && s.solver == kFixedPoint),
+        "realisations side by side take the fixed point and no transform degrees");
   Check(s.solver == kFixedPoint || s.solver == kConjugateGradients,
         "solver: SL3_FIXED_POINT or SL3_CONJUGATE_GRADIENTS");
   Check(s.solveMaxIterations >= 0, "solve_max_iterations >= 0");
@@ -303,6 +307,7 @@ void sl3_step_options_default(sl3_step_options* o) {
   o->graph_passes = s.graphPasses ? 1 : 0;
   o->transform_degree_count = 0;
   for (int i = 0; i < SL3_MAX_TRANSFORM_DEGREES; ++i) o->transform_degrees[i] = 0;
+  o->realisations = s.realisations;
 }
 
 int sl3_operator_create(const sl3_earth* earth, int lmax, const double* delta_dev,
@@ -338,6 +343,7 @@ int sl3_operator_set_options(sl3_operator* op, const sl3_step_options* options) 
     };
     Check(sorted(o.transformDegrees) == sorted(cur.transformDegrees),
           "the transform degrees are fixed at construction");
+    Check(o.realisations == cur.realisations, "the realisations are fixed at construction");
     op->op->MutableOptions() = o;
   });
 }
@@ -358,6 +364,34 @@ int sl3_operator_shape(const sl3_operator* op, int* nodes, int* ntheta, int* nph
     if (ntheta) *ntheta = c.Grid().NTheta();
     if (nphi) *nphi = c.Grid().NPhi();
     if (spectral_doubles) *spectral_doubles = static_cast<int64_t>(c.SpectralDoubles());
+  });
+}
+
+int sl3_operator_realisations(const sl3_operator* op, int* realisations, int* radial_nodes) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    const auto& c = op->op->GetCoupling();
+    if (realisations) *realisations = c.Realisations();
+    if (radial_nodes) *radial_nodes = c.RadialNodeCount();
+  });
+}
+
+int sl3_step_realisation_infos(const sl3_operator* op, int capacity, int* count,
+                               sl3_step_info* infos) {
+  return Guard([&] {
+    Check(op != nullptr, "null operator");
+    const auto& all = op->op->RealisationInfos();
+    if (count) *count = static_cast<int>(all.size());
+    for (int r = 0; r < static_cast<int>(all.size()) && r < capacity && infos; ++r) {
+      sl3_step_info& i = infos[r];
+      i = sl3_step_info{};
+      i.iterations = all[r].iterations;
+      i.residual = all[r].residual;
+      i.converged = all[r].converged ? 1 : 0;
+      i.mixed_passes = all[r].mixedPasses;
+      i.solve_iterations = all[r].solveIterations;
+      i.warm_fallback = all[r].warmFallback ? 1 : 0;
+    }
   });
 }
 
@@ -396,7 +430,10 @@ int sl3_operator_geometry(const sl3_operator* op, double* radii, double* theta,
   return Guard([&] {
     Check(op != nullptr, "null operator");
     const auto& c = op->op->GetCoupling();
-    if (radii) std::memcpy(radii, c.NodeRadii().data(), sizeof(double) * c.NodeCount());
+    if (radii) {
+      const auto& r = c.NodeRadii();  // one realisation's nodes, repeated
+      for (int node = 0; node < c.NodeCount(); ++node) radii[node] = r[node % c.RadialNodeCount()];
+    }
     if (theta) {
       std::memcpy(theta, c.Grid().CoLatitudes().data(), sizeof(double) * c.Grid().NTheta());
     }

@@ -111,6 +111,8 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ Cplx rowsL[maxPoints * kChunkOrders][6];
   const int2 wk = work[blockIdx.y];
   const int l = wk.x, mc = wk.y;
+  const int real = blockIdx.z;  // the realisation
+  const int width = g.nodes * g.realisations;
   const int L = w.lmax;
   const DegreeTask ts = tasks[l];
   const bool hasTor = l >= 1;
@@ -135,7 +137,7 @@ __global__ void __launch_bounds__(kThreads)
     const int elem = eStart + p / g.np, q = p % g.np;
     const int node = (elem - g.elem0) * g.np + q;
     Cplx A[5], rows[6];
-    Gather(w, l, mc + mi, node, g.nodes, A);
+    Gather(w, l, mc + mi, real * g.nodes + node, width, A);
     ForceRows(l, A, g.radius[node], rows);
     Cplx* dst = rowsL[mi * maxPoints + p];
 #pragma unroll
@@ -144,8 +146,8 @@ __global__ void __launch_bounds__(kThreads)
   __syncthreads();
   // stage 2: each owned node's sums, order fastest (consecutive threads
   // write consecutive columns of one force row)
-  double* bs = force + ts.fOff;
-  double* bt = force + tt.fOff;
+  double* bs = force + ts.fOff + real * ts.cols;
+  double* bt = force + tt.fOff + real * tt.cols;
   const size_t lds = ts.colsPad, ldt = tt.colsPad;
   for (int e = threadIdx.x; e < owned * mCount; e += blockDim.x) {
     const int mi = e % mCount, gl = e / mCount;
@@ -213,6 +215,8 @@ __global__ void __launch_bounds__(1024)
   while ((l + 1) * (l + 2) / 2 <= pair) ++l;
   const int m = pair - l * (l + 1) / 2;
   const DegreeTask ts = sph[l];
+  const int real = blockIdx.y;  // the realisation
+  const int width = g.nodes * g.realisations;
   for (int node = threadIdx.x; node < g.nodes; node += blockDim.x) {
     Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
     if (node >= ts.firstNode) {
@@ -231,8 +235,9 @@ __global__ void __launch_bounds__(1024)
         const int* d = dofs + ts.dofOff + ((elem - g.elem0) * g.ngll + i) * 2;
         Cplx Ui = C(0, 0), Vi = C(0, 0), Wi = C(0, 0);
         // the response columns m (Re) and l + 1 + m (Im), column-major
-        const double* re = response + ts.fOff + static_cast<size_t>(m) * ts.nmPad;
-        const double* im = response + ts.fOff + static_cast<size_t>(l + 1 + m) * ts.nmPad;
+        const double* re = response + ts.fOff + static_cast<size_t>(real * ts.cols + m) * ts.nmPad;
+        const double* im =
+            response + ts.fOff + static_cast<size_t>(real * ts.cols + l + 1 + m) * ts.nmPad;
         if (d[0] >= 0) {
           Ui = C(re[d[0]], im[d[0]]);
           if (units != nullptr && ts.unitOff >= 0) Ui = Ui + units[ts.dmOff + d[0]] * a;
@@ -247,8 +252,8 @@ __global__ void __launch_bounds__(1024)
           const int dw = dofs[tt.dofOff + ((elem - g.elem0) * g.ngll + i) * 2];
           if (dw >= 0) {
             const double* bt = response + tt.fOff;
-            Wi = C(bt[static_cast<size_t>(m) * tt.nmPad + dw],
-                   bt[static_cast<size_t>(l + 1 + m) * tt.nmPad + dw]);
+            Wi = C(bt[static_cast<size_t>(real * tt.cols + m) * tt.nmPad + dw],
+                   bt[static_cast<size_t>(real * tt.cols + l + 1 + m) * tt.nmPad + dw]);
           }
         }
         const double v = g.value[q * g.ngll + i];
@@ -279,15 +284,16 @@ __global__ void __launch_bounds__(1024)
     }
     // Scatter (src/GridCoupling.cpp:595-611), every slot written once.
     const double sign = (m & 1) ? -1.0 : 1.0;
+    const int col = real * g.nodes + node;
     if (l >= 2) {
-      Store(out, 0, l, m, node, g.nodes, A[0]);
-      if (m > 0) Store(out, 0, l, -m, node, g.nodes, sign * Conj(A[4]));
+      Store(out, 0, l, m, col, width, A[0]);
+      if (m > 0) Store(out, 0, l, -m, col, width, sign * Conj(A[4]));
     }
     if (l >= 1) {
-      Store(out, 1, l, m, node, g.nodes, A[1]);
-      if (m > 0) Store(out, 1, l, -m, node, g.nodes, sign * Conj(A[3]));
+      Store(out, 1, l, m, col, width, A[1]);
+      if (m > 0) Store(out, 1, l, -m, col, width, sign * Conj(A[3]));
     }
-    Store(out, 2, l, m, node, g.nodes, A[2]);
+    Store(out, 2, l, m, col, width, A[2]);
   }
 }
 
@@ -326,9 +332,12 @@ __global__ void __launch_bounds__(1024)
   if (tor1) tt = tor[l - 1];
   const bool tide = tideAmp != nullptr;
   const bool unitsOn = units != nullptr && ts.unitOff >= 0;  // no load channel: none
+  const int real = blockIdx.y;  // the realisation
+  const int width = g.nodes * g.realisations;
   {
-    const double* re = response + ts.fOff + static_cast<size_t>(m) * ts.nmPad;
-    const double* im = response + ts.fOff + static_cast<size_t>(l + 1 + m) * ts.nmPad;
+    const double* re = response + ts.fOff + static_cast<size_t>(real * ts.cols + m) * ts.nmPad;
+    const double* im =
+        response + ts.fOff + static_cast<size_t>(real * ts.cols + l + 1 + m) * ts.nmPad;
     for (int j = threadIdx.x; j < ts.nm; j += blockDim.x) {
       sRe[j] = re[j];
       sIm[j] = im[j];
@@ -339,8 +348,8 @@ __global__ void __launch_bounds__(1024)
     if (tor1) {
       const double* bt = response + tt.fOff;
       for (int j = threadIdx.x; j < tt.nm; j += blockDim.x) {
-        tRe[j] = bt[static_cast<size_t>(m) * tt.nmPad + j];
-        tIm[j] = bt[static_cast<size_t>(l + 1 + m) * tt.nmPad + j];
+        tRe[j] = bt[static_cast<size_t>(real * tt.cols + m) * tt.nmPad + j];
+        tIm[j] = bt[static_cast<size_t>(real * tt.cols + l + 1 + m) * tt.nmPad + j];
       }
       for (int j = threadIdx.x; j < table; j += blockDim.x) tDof[j] = dofs[tt.dofOff + 2 * j];
     }
@@ -400,15 +409,16 @@ __global__ void __launch_bounds__(1024)
       }
     }
     const double sign = (m & 1) ? -1.0 : 1.0;
+    const int col = real * g.nodes + node;
     if (l >= 2) {
-      Store(out, 0, l, m, node, g.nodes, A[0]);
-      if (m > 0) Store(out, 0, l, -m, node, g.nodes, sign * Conj(A[4]));
+      Store(out, 0, l, m, col, width, A[0]);
+      if (m > 0) Store(out, 0, l, -m, col, width, sign * Conj(A[4]));
     }
     if (l >= 1) {
-      Store(out, 1, l, m, node, g.nodes, A[1]);
-      if (m > 0) Store(out, 1, l, -m, node, g.nodes, sign * Conj(A[3]));
+      Store(out, 1, l, m, col, width, A[1]);
+      if (m > 0) Store(out, 1, l, -m, col, width, sign * Conj(A[3]));
     }
-    Store(out, 2, l, m, node, g.nodes, A[2]);
+    Store(out, 2, l, m, col, width, A[2]);
   }
 }
 
@@ -583,6 +593,10 @@ __global__ void AndersonFirstKernel(const double* x, const double* gx, double* x
     Store2(xPrev, c, xv);
     Store2(fPrev, c, make_double2(gv.x - xv.x, gv.y - xv.y));
   });
+}
+
+__global__ void CopyRunsKernel(double* dst, const double* src, ActiveRuns a) {
+  ForActivePairs(a, [&](long long, long long j) { Store2(dst, j, Load2(src, j)); });
 }
 
 // RESIDUAL: the pass's RelativeChange (max |gx|^2, max |gx - x|^2, a
@@ -804,7 +818,7 @@ void LaunchForce(const DegreeTask* tasks_dev, const RadialGeometry& g,
                  int workCount, double* force, hipStream_t stream) {
   if (workCount == 0 || g.elements == 0) return;
   if (g.np > kMaxPoints) throw ArgumentError("at most 6 memory points per element");
-  const dim3 grid(CeilDiv(g.elements, kChunkElements), workCount);
+  const dim3 grid(CeilDiv(g.elements, kChunkElements), workCount, g.realisations);
   ProfileScope prof("force", stream);
   hipLaunchKernelGGL(ForceKernel, grid, dim3(kThreads), 0, stream, tasks_dev, g, weighted,
                      dofs, work_dev, force);
@@ -825,11 +839,11 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
   const size_t lds = sizeof(double) * (4 * static_cast<size_t>(capS) + 2 * capT) +
                      sizeof(int) * (2 * static_cast<size_t>(table) + table);
   if (lds <= 64 * 1024) {
-    hipLaunchKernelGGL(StrainStagedKernel, dim3(pairs), dim3(threads), lds, stream,
+    hipLaunchKernelGGL(StrainStagedKernel, dim3(pairs, g.realisations), dim3(threads), lds, stream,
                        sphTasks_dev, torTasks_dev, lmax, g, dofs, response, units, amp,
                        tideUnits, tideAmp, out, capS, capT);
   } else {
-    hipLaunchKernelGGL(StrainKernel, dim3(pairs), dim3(threads), 0, stream, sphTasks_dev,
+    hipLaunchKernelGGL(StrainKernel, dim3(pairs, g.realisations), dim3(threads), 0, stream, sphTasks_dev,
                        torTasks_dev, lmax, g, dofs, response, units, amp, tideUnits, tideAmp,
                        out);
   }
@@ -867,6 +881,21 @@ void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPr
                      xPrev, fPrev, runs);
   SL3_HIP(hipGetLastError());
   hipLaunchKernelGGL(CopyKernel, dim3(GridFor(n)), dim3(kThreads), 0, stream, x, gx, n);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPrev,
+                         const ActiveRuns& runs, const ActiveRuns& every, hipStream_t stream) {
+  ProfileScope prof("anderson", stream);
+  hipLaunchKernelGGL(AndersonFirstKernel, dim3(1024), dim3(kThreads), 0, stream, x, gx,
+                     xPrev, fPrev, runs);
+  SL3_HIP(hipGetLastError());
+  LaunchCopyRuns(x, gx, every, stream);
+}
+
+void LaunchCopyRuns(double* dst, const double* src, const ActiveRuns& runs, hipStream_t stream) {
+  if (runs.total == 0) return;
+  hipLaunchKernelGGL(CopyRunsKernel, dim3(1024), dim3(kThreads), 0, stream, dst, src, runs);
   SL3_HIP(hipGetLastError());
 }
 

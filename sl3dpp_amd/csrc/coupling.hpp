@@ -41,6 +41,9 @@ struct RadialGeometry {
   int ngll = 0;
   int elem0 = 0;       // first memory element
   int elements = 0;    // memory elements (contiguous)
+  // independent realisations side by side: realisation r's node n is the
+  // spectral column r nodes + n, its degree-task columns r cols + c
+  int realisations = 1;
   const double* radius = nullptr;  // per node
   const double* jac = nullptr;     // per memory element
   const double* weight = nullptr;  // per point (reference weights)
@@ -62,17 +65,18 @@ void LaunchCombine(double* dst, double a, const double* src, double b, size_t co
 
 // Forces of the analysed tensor on every task's memory dofs (rows of
 // nmPad x colsPad at fOff, unscaled: the memory response carries the
-// scaling). tasks: spheroidal l = 0..L, then toroidal l = 1..L. dofs: per
-// task, memory element x GLL node x (U, V) or (W, -), the memory-dof row or
-// -1 (none, or the pinned dof). work: the (degree, first order) chunks of
-// ForceWork(lmax).
+// scaling; realisation r's columns at r cols). tasks: spheroidal l = 0..L,
+// then toroidal l = 1..L. dofs: per task, memory element x GLL node x
+// (U, V) or (W, -), the memory-dof row or -1 (none, or the pinned dof).
+// work: the (degree, first order) chunks of ForceWork(lmax).
 std::vector<int2> ForceWork(int lmax);
 void LaunchForce(const DegreeTask* tasks_dev, const RadialGeometry& g,
                  const SpectralView& weighted, const int* dofs, const int2* work_dev,
                  int workCount, double* force, hipStream_t stream);
 
 // Strain of the responses on the memory dofs into the spectral iterate
-// (every value written), the channels' unit responses (per memory dof at
+// (every value written; realisation r from its response columns r cols,
+// the channels shared), the channels' unit responses (per memory dof at
 // dmOff) added with their amplitudes (complex per (l, m), offset
 // l(l+1)/2 + m): the load's from units and amp, the tide's from tideUnits
 // and tideAmp (tideAmp null: no tide).
@@ -129,6 +133,12 @@ void LaunchRelativeChangeActive(const double* next, const double* now, const Act
 void LaunchAndersonFirst(double* x, const double* gx, double* xPrev,
                          double* fPrev, const ActiveRuns& runs, size_t n,
                          hipStream_t stream);
+// The same with x = gx over the entries of `every` (one realisation's
+// columns of every row) in place of the whole tensor.
+void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPrev,
+                         const ActiveRuns& runs, const ActiveRuns& every, hipStream_t stream);
+// dst = src on the entries of the runs (device layout).
+void LaunchCopyRuns(double* dst, const double* src, const ActiveRuns& runs, hipStream_t stream);
 // The Anderson sweep (differences, the Gram column and right-hand side into
 // sums[0, 2 w), w = AndersonWidth(m)); with residual non-null also the
 // pass's RelativeChange (max |gx|^2, max |gx - x|^2, a non-finite flag)

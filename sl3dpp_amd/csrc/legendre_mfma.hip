@@ -65,7 +65,7 @@ struct Args {
   int spin[3];
   const OrderRef* orders;
   const int* reach;      // per node: the largest degree it carries (or null)
-  const int* firstNode;  // per degree: its first node (or null)
+  const int* firstNode;  // non-null: analysis writes zeros where a node's reach is below the degree
   int orderCount, colBlocks, rowBlocks, thetaBlocks;
   int lmax, nodes, ntheta;
   int fstride;  // the Fourier layout's colatitude stride
@@ -74,6 +74,14 @@ struct Args {
   int nodeOffset, nodesTotal;
   int fourierMlo[3];
 };
+
+// The largest degree any node of a column block carries (workgroup-uniform
+// reads of the per-node reach).
+__device__ inline int ColumnBlockReach(const Args& a, int node0, int nodeEnd) {
+  int r = 0;
+  for (int n = node0; n < nodeEnd; ++n) r = max(r, a.reach[a.nodeOffset + n]);
+  return r;
+}
 
 __device__ inline int LMinOf(int m, int spin) {
   const int am = m < 0 ? -m : m, an = spin < 0 ? -spin : spin;
@@ -119,8 +127,9 @@ __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
   const int cols = 2 * a.nodes;
   const int c0 = cb * kCols;
   const int node0 = c0 / 2, nodeEnd = min(a.nodes, node0 + kCols / 2);
-  // degrees that reach a node of the column block
-  const int lhi = min(lEnd, a.reach ? a.reach[a.nodeOffset + nodeEnd - 1] : a.lmax);
+  // degrees that reach a node of the column block (the largest reach among
+  // its nodes: nodes of several realisations may share a block)
+  const int lhi = min(lEnd, a.reach ? ColumnBlockReach(a, node0, nodeEnd) : a.lmax);
   const int rt = lhi >= l0 ? (lhi - l0 + 16) / 16 : 0;  // live 16-degree tiles
   const int row0 = a.rowOffset[b][m - a.mlo[b]] - lmin;  // row of (m, l) = row0 + l
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -209,7 +218,7 @@ __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
         if (l > lEnd) continue;
         const size_t row = static_cast<size_t>(row0 + l);
         double v = l <= lhi ? acc[t][j] : 0.0;
-        if (a.firstNode && a.nodeOffset + (c >> 1) < a.firstNode[l]) v = 0.0;
+        if (a.firstNode && a.reach[a.nodeOffset + (c >> 1)] < l) v = 0.0;
         if (add) v += add[row * rowCols + c];
         out[row * rowCols + c] = v;
       }
@@ -240,7 +249,7 @@ __global__ void __launch_bounds__(kThreads) LegendreSynthesisMfmaKernel(Args a) 
   const int cols = 2 * a.nodes;
   const int c0 = cb * kCols;
   const int node0 = c0 / 2, nodeEnd = min(a.nodes, node0 + kCols / 2);
-  const int lhi = a.reach ? min(a.lmax, a.reach[a.nodeOffset + nodeEnd - 1]) : a.lmax;
+  const int lhi = a.reach ? min(a.lmax, ColumnBlockReach(a, node0, nodeEnd)) : a.lmax;
   const int row0 = a.rowOffset[b][m - a.mlo[b]] - lmin;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const double* T = a.table[b];

@@ -116,9 +116,16 @@ void GramLeastSquares(int m, const double* gram, const double* rhs, double* gamm
 // ---- Coupling ---------------------------------------------------------------
 
 Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps,
-                   const std::vector<int>& transformDegrees)
+                   const std::vector<int>& transformDegrees, int realisations)
     : lmax_(lmax), grid_(lmax, oversampling) {
   hipStream_t s = Stream();
+  if (realisations < 1 || realisations > kRealisationsMax) {
+    throw ArgumentError("realisations must lie in [1, " + std::to_string(kRealisationsMax) + "]");
+  }
+  if (realisations > 1 && !transformDegrees.empty()) {
+    throw ArgumentError("realisations side by side take the full degree's transforms: no "
+                        "transform degrees");
+  }
   const auto& mesh = *earth.mesh;
   const auto& points = earth.reference.points;
   const int np = points.count, n = mesh.NGll();
@@ -169,6 +176,8 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
     }
     reachHost_[node] = std::max(reach, 0);
   }
+  geometry_.realisations = realisations;
+  const int N = geometry_.nodes, total = NodeCount();
   radius_.Upload(radiusHost_.data(), radiusHost_.size(), s);
   jac_.Upload(jac.data(), jac.size(), s);
   weight_.Upload(points.weight.data(), points.weight.size(), s);
@@ -180,15 +189,19 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
   geometry_.value = value_.data();
   geometry_.deriv = deriv_.data();
   firstNode_.Upload(firstNodeHost_.data(), firstNodeHost_.size(), s);
-  reach_.Upload(reachHost_.data(), reachHost_.size(), s);
+  {
+    std::vector<int> reach(static_cast<size_t>(total));  // every realisation's nodes
+    for (int node = 0; node < total; ++node) reach[node] = reachHost_[node % N];
+    reach_.Upload(reach.data(), reach.size(), s);
+    SL3_HIP(hipStreamSynchronize(s));  // the host copy is freed next
+  }
 
   // the spectral layout: the full degree's rows, every node
   spectralOffset_[0] = 0;
   for (int b = 0; b < 3; ++b) {
     const auto& lay = grid_.Layout(b);
     rowOffset_[b].Upload(lay.rowOffset.data(), lay.rowOffset.size(), s);
-    spectralOffset_[b + 1] =
-        spectralOffset_[b] + static_cast<size_t>(lay.Rows()) * geometry_.nodes * 2;
+    spectralOffset_[b + 1] = spectralOffset_[b] + static_cast<size_t>(lay.Rows()) * total * 2;
   }
   // the node blocks (GridCoupling::TransformDegrees, src/GridCoupling.cpp:
   // 163-190): each memory element at the least listed degree, or lmax, at or
@@ -204,8 +217,8 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
   degrees.push_back(lmax);
   std::sort(degrees.begin(), degrees.end());
   degrees.erase(std::unique(degrees.begin(), degrees.end()), degrees.end());
-  for (int node = 0; node < geometry_.nodes; ++node) {
-    const int d = *std::lower_bound(degrees.begin(), degrees.end(), reachHost_[node]);
+  for (int node = 0; node < total; ++node) {
+    const int d = *std::lower_bound(degrees.begin(), degrees.end(), reachHost_[node % N]);
     if (blocks_.empty() || blocks_.back()->lmax != d) {
       if (!blocks_.empty() && blocks_.back()->lmax > d) {
         throw ArgumentError("the node blocks' degrees must grow outwards");
@@ -225,34 +238,42 @@ Coupling::Coupling(const Earth& earth, int lmax, double oversampling, double eps
     }
     BuildBlock(*k, s);
   }
-  // the active runs of a spectral tensor: per row, in memory order, the
-  // nodes at or above the row degree's first memory node
-  {
+  // the active runs of a spectral tensor: per realisation, per row in
+  // memory order, the realisation's nodes at or above the row degree's first
+  // memory node (every realisation's compact vector laid out alike); and
+  // the runs of every entry of a realisation's columns
+  auto runsOf = [&](bool active, DeviceArray<long long>& fullDev,
+                    DeviceArray<long long>& compactDev, ActiveRuns& out) {
     std::vector<long long> full, compact;
-    long long total = 0;
-    const int nodes = geometry_.nodes;
-    for (int b = 0; b < 3; ++b) {
-      const auto& lay = grid_.Layout(b);
-      for (int m = lay.mlo; m <= lmax; ++m) {
-        for (int l = lay.LMin(m); l <= lmax; ++l) {
-          const int row = lay.Row(l, m);
-          const int fn = std::min(firstNodeHost_[l], nodes);
-          full.push_back(static_cast<long long>(spectralOffset_[b]) +
-                         2LL * (static_cast<long long>(row) * nodes + fn));
-          compact.push_back(total);
-          total += 2LL * (nodes - fn);
+    long long count = 0;
+    for (int r = 0; r < realisations; ++r) {
+      for (int b = 0; b < 3; ++b) {
+        const auto& lay = grid_.Layout(b);
+        for (int m = lay.mlo; m <= lmax; ++m) {
+          for (int l = lay.LMin(m); l <= lmax; ++l) {
+            const int row = lay.Row(l, m);
+            const int fn = active ? std::min(firstNodeHost_[l], N) : 0;
+            full.push_back(static_cast<long long>(spectralOffset_[b]) +
+                           2LL * (static_cast<long long>(row) * total + r * N + fn));
+            if (r == 0) {
+              compact.push_back(count);
+              count += 2LL * (N - fn);
+            }
+          }
         }
       }
     }
-    compact.push_back(total);
-    runFull_.Upload(full.data(), full.size(), s);
-    runCompact_.Upload(compact.data(), compact.size(), s);
-    runs_.full = runFull_.data();
-    runs_.compact = runCompact_.data();
-    runs_.runs = static_cast<int>(full.size());
-    runs_.total = total;
+    compact.push_back(count);
+    fullDev.Upload(full.data(), full.size(), s);
+    compactDev.Upload(compact.data(), compact.size(), s);
+    out.full = fullDev.data();
+    out.compact = compactDev.data();
+    out.runs = static_cast<int>(compact.size() - 1);
+    out.total = count;
     SL3_HIP(hipStreamSynchronize(s));  // the host tables are freed next
-  }
+  };
+  runsOf(true, runFull_, runCompact_, runs_);
+  runsOf(false, everyFull_, everyCompact_, everyRuns_);
   SL3_HIP(hipStreamSynchronize(s));
 }
 
@@ -286,7 +307,7 @@ void Coupling::BuildBlock(NodeBlock& k, hipStream_t s) {
   k.plan.ntheta = g.NTheta();
   k.plan.fstride = FourierStride(g.NTheta());
   k.plan.nodeOffset = k.firstNode;
-  k.plan.nodesTotal = geometry_.nodes;
+  k.plan.nodesTotal = NodeCount();
   for (int b = 0; b < 3; ++b) {
     std::vector<int> cut;
     const auto& lay = grid_.Layout(b);
@@ -403,7 +424,7 @@ void Coupling::FillCutRows(double* const spectral[3], const double* const addend
       const size_t total = static_cast<size_t>(k->cutRowCount[b]) * 2 * k->nodes;
       hipLaunchKernelGGL(CutRowsKernel, dim3(Blocks(total)), dim3(256), 0, s,
                          k->cutRows[b].data(), k->cutRowCount[b], spectral[b],
-                         addend ? addend[b] : nullptr, 2 * static_cast<size_t>(geometry_.nodes),
+                         addend ? addend[b] : nullptr, 2 * static_cast<size_t>(NodeCount()),
                          2 * static_cast<size_t>(k->firstNode), 2 * k->nodes);
       SL3_HIP(hipGetLastError());
     }
@@ -597,7 +618,7 @@ void Coupling::TuneLegendre(const double* spectral, double* const fourier[3],
   for (auto& kp : blocks_) {
     NodeBlock& k = *kp;
     const std::array<long long, 4> key{
-        lmax_ * 1000003LL + k.lmax, geometry_.nodes * 1000003LL + k.firstNode,
+        lmax_ * 1000003LL + k.lmax, NodeCount() * 1000003LL + k.firstNode,
         k.nodes * 1000003LL + k.grid.NTheta(), truncation};
     std::vector<double> result;
     {
@@ -674,12 +695,17 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
                                const double* delta_dev, StepOptions options)
     : earth_(std::move(earthPtr)),
       options_(options),
-      coupling_(*earth_, lmax, options.oversampling, options.eps, options.transformDegrees) {
+      coupling_(*earth_, lmax, options.oversampling, options.eps, options.transformDegrees,
+                options.realisations) {
   const Earth& earth = *earth_;
   hipStream_t s = Stream();
   const auto& geo = coupling_.Geometry();
   const int np = geo.np;
-  for (int node = 0; node < geo.nodes; ++node) {
+  if (geo.realisations > 1 && options_.solver != kFixedPoint) {
+    throw ArgumentError("realisations side by side take the fixed point");
+  }
+  for (int column = 0; column < coupling_.NodeCount(); ++column) {
+    const int node = column % geo.nodes;  // every realisation's nodes alike
     const int e = coupling_.NodeElement()[node];
     const int p = node % np;
     muNodeHost_.push_back(earth.reference.pointMu[static_cast<size_t>(e) * np + p]);
@@ -712,7 +738,7 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
   amp_.Resize(2 * ampDoubles_);
   // per block the sweep's sums (at most 2 kAndersonMax) and the residual's 3 maxima
   partials_.Resize(1024 * (2 * kAndersonMax + 3));
-  result_.Resize(2 * kAndersonMax + 3);
+  result_.Resize(static_cast<size_t>(2 * kAndersonMax + 3) * geo.realisations);
   // the Legendre kernels' tile shapes, timed on the pass's own buffers
   LaunchFill(s_.data(), S, 0.0, s);
   {
@@ -720,7 +746,8 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
     coupling_.TuneLegendre(s_.data(), A, analysed_.data(), s);
   }
   for (auto& e : events_) SL3_HIP(hipEventCreate(&e));
-  SL3_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), kReadBackMax * sizeof(double),
+  SL3_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_),
+                        static_cast<size_t>(kReadBackMax) * geo.realisations * sizeof(double),
                         hipHostMallocDefault));
   SL3_HIP(hipEventCreateWithFlags(&readEvent_, hipEventDisableTiming));
   SL3_HIP(hipStreamSynchronize(s));
@@ -740,7 +767,9 @@ Viscoelastic3D::~Viscoelastic3D() {
 // sleep until an interrupt wakes it (a pass reads its residual this way, and
 // the device idles until the host has launched the next work).
 void Viscoelastic3D::ReadBack(double* dst, const double* src, int count) {
-  if (count < 0 || count > kReadBackMax) throw ArgumentError("read-back of at most 40 doubles");
+  if (count < 0 || count > kReadBackMax * coupling_.Realisations()) {
+    throw ArgumentError("read-back of at most 40 doubles per realisation");
+  }
   hipStream_t s = Stream();
   SL3_HIP(hipMemcpyAsync(pinned_, src, count * sizeof(double), hipMemcpyDeviceToHost, s));
   SL3_HIP(hipEventRecord(readEvent_, s));
@@ -848,7 +877,8 @@ void Viscoelastic3D::BuildTasks() {
     }
     t.nm = static_cast<int>(dm.size());
     t.nmPad = 16 * CeilDiv(t.nm, 16);
-    t.colsPad = 16 * CeilDiv(t.cols, 16);
+    // every realisation's columns side by side
+    t.colsPad = 16 * CeilDiv(geo.realisations * t.cols, 16);
     t.dmOff = static_cast<long long>(dmHost_.size());
     dmHost_.insert(dmHost_.end(), dm.begin(), dm.end());
     t.gOff = static_cast<long long>(gDoubles_);
@@ -1423,6 +1453,7 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const Channel
   CopyState(state, y1);
   const StepInfo first = StepOnce(y1, length, kBackwardEuler, channels, nullptr, 0.0);
   const auto firstHistory = residualHistory;
+  const auto firstInfos = realisationInfos_;
   const Timing firstTiming = lastTiming;
   for (int b = 0; b < 3; ++b) {
     LaunchCombine(state.m[b].data(), 1.0 - r, y1.m[b].data(), r, state.m[b].size(), s);
@@ -1448,12 +1479,19 @@ StepInfo Viscoelastic3D::StepSdirk2(DeviceState& state, double dt, const Channel
   state.hasStrain = staged.hasStrain;
   state.hasCoefficients = staged.hasCoefficients;
   state.t = tn + dt;
-  StepInfo info;
-  info.iterations = first.iterations + second.iterations;
-  info.mixedPasses = first.mixedPasses + second.mixedPasses;
-  info.solveIterations = first.solveIterations + second.solveIterations;
-  info.residual = second.residual;
-  info.converged = first.converged && second.converged;
+  auto stages = [](const StepInfo& a, const StepInfo& b) {
+    StepInfo info;
+    info.iterations = a.iterations + b.iterations;
+    info.mixedPasses = a.mixedPasses + b.mixedPasses;
+    info.solveIterations = a.solveIterations + b.solveIterations;
+    info.residual = b.residual;
+    info.converged = a.converged && b.converged;
+    return info;
+  };
+  StepInfo info = stages(first, second);
+  for (size_t r = 0; r < realisationInfos_.size() && r < firstInfos.size(); ++r) {
+    realisationInfos_[r] = stages(firstInfos[r], realisationInfos_[r]);
+  }
   residualHistory.insert(residualHistory.begin(), firstHistory.begin(), firstHistory.end());
   lastTiming.startStep += firstTiming.startStep;
   lastTiming.passes += firstTiming.passes;
@@ -1472,6 +1510,7 @@ StepInfo Viscoelastic3D::StepOnce(DeviceState& state, double dt, int scheme,
     open_.state = nullptr;
     throw;
   }
+  if (coupling_.Realisations() == 1) realisationInfos_.assign(1, info);
   OpenFinish(nullptr);
   hipStream_t s = Stream();
   SL3_HIP(hipStreamSynchronize(s));
@@ -1619,6 +1658,7 @@ StepInfo Viscoelastic3D::OpenClose(const double* start, double warmGuard) {
 StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
                                       double warmGuard) {
   RequireOpen();
+  if (coupling_.Realisations() > 1) return ConvergeBatch(pass, start, warmGuard);
   hipStream_t s = Stream();
   const size_t S = coupling_.SpectralDoubles();
   StepInfo info;
@@ -1760,6 +1800,186 @@ StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
   return info;
 }
 
+// Converge (src/Viscoelastic3D.cpp:1916-1974) for realisations side by
+// side. Each pass runs every realisation through the same kernels; each
+// realisation then takes what OpenConverge takes alone, on its own runs:
+// its residual (from a fused sweep once its mixer runs), its adaptive
+// switch, its Anderson history and mix, the Picard image copied into its
+// columns of the iterate in place of the buffers' swap. A converged
+// realisation's image becomes its columns of the iterate and nothing
+// writes them again (its columns still go through the passes).
+StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
+                                       double warmGuard) {
+  if (warmGuard > 0.0) {
+    throw ArgumentError("realisations side by side take no guarded warm start");
+  }
+  hipStream_t s = Stream();
+  const int R = coupling_.Realisations();
+  const size_t S = coupling_.SpectralDoubles();
+  residualHistory.clear();
+  const bool startForce = open_.startForce;
+  bool warm = false;
+  if (start != nullptr) {
+    LaunchCopy(s_.data(), start, S, s);
+    warm = true;
+    open_.startForce = false;
+  }
+  const bool adaptive = options_.andersonRate > 0.0;
+  const int memory = options_.andersonMemory;
+  const size_t C = static_cast<size_t>(coupling_.Runs().total);  // per realisation
+  const size_t historyEach = 2 * static_cast<size_t>(memory) * C;
+  if (memory > 0 && history_.size() != historyEach * R) {
+    history_.Resize(historyEach * R);
+    xPrev_.Resize(C * R);
+    fPrev_.Resize(C * R);
+  }
+  struct Control {
+    bool live = true, mixing = false, started = false, fused = false;
+    int count = 0, slot = 0, m = 0, width = 0;
+    double previous = 0.0, residual = 0.0;
+    std::vector<double> gram;
+    StepInfo info;
+  };
+  std::vector<Control> ctl(R);
+  for (auto& c : ctl) {
+    c.mixing = memory > 0 && !adaptive;
+    c.gram.assign(static_cast<size_t>(std::max(memory, 1)) * std::max(memory, 1), 0.0);
+  }
+  auto sweepOf = [&](int r) {
+    AndersonSweep sw{};
+    sw.x = s_.data();
+    sw.gx = sNext_.data();
+    sw.xPrev = xPrev_.data() + r * C;
+    sw.fPrev = fPrev_.data() + r * C;
+    double* h = history_.data() + r * historyEach;
+    for (int i = 0; i < memory && i < kAndersonMax; ++i) {
+      sw.dF[i] = h + static_cast<size_t>(i) * C;
+      sw.dG[i] = h + static_cast<size_t>(memory + i) * C;
+    }
+    sw.m = ctl[r].m;
+    sw.slot = ctl[r].slot;
+    return sw;
+  };
+  const int stride = 2 * kAndersonMax + 3;  // one realisation's results
+  std::vector<double> back(static_cast<size_t>(stride) * R);
+  for (int it = 0; it < options_.maxIterations; ++it) {
+    if (std::none_of(ctl.begin(), ctl.end(), [](const Control& c) { return c.live; })) break;
+    if (pass) {
+      pass(s_.data(), sNext_.data());
+    } else {
+      open_.startForce = false;
+      Pass(s_.data(), sNext_.data(), it == 0 && !warm && startForce, *open_.flavour);
+    }
+    // every live realisation's residual: a fused sweep once its mixer runs
+    // (the sums at [0, 2 w), the maxima after), else RelativeChange (the
+    // maxima at 0; the first pass over every entry)
+    for (int r = 0; r < R; ++r) {
+      Control& c = ctl[r];
+      if (!c.live) continue;
+      c.fused = c.mixing && c.started && options_.damping == 1.0;
+      c.m = std::min(c.count + 1, memory);
+      c.width = AndersonWidth(std::max(c.m, 1));
+      double* out = result_.data() + static_cast<size_t>(r) * stride;
+      if (c.fused) {
+        LaunchAndersonSweep(sweepOf(r), coupling_.Runs(r), partials_.data(), 1024, out,
+                            out + 2 * c.width, s);
+      } else {
+        LaunchRelativeChangeActive(sNext_.data(), s_.data(),
+                                   it == 0 ? coupling_.EveryRuns(r) : coupling_.Runs(r),
+                                   partials_.data(), 1024, out, s);
+      }
+    }
+    ReadBack(back.data(), result_.data(), stride * R);
+    double largest = 0.0;
+    for (int r = 0; r < R; ++r) {
+      Control& c = ctl[r];
+      if (!c.live) continue;
+      const double* rc = back.data() + static_cast<size_t>(r) * stride + (c.fused ? 2 * c.width : 0);
+      c.residual = rc[2] != 0.0 ? std::numeric_limits<double>::quiet_NaN()
+                   : rc[0] > 0.0 ? std::sqrt(rc[1]) / std::sqrt(rc[0])
+                                 : 0.0;
+      c.info.iterations = it + 1;
+      c.info.residual = c.residual;
+      largest = std::isnan(c.residual) || std::isnan(largest) ? std::numeric_limits<double>::quiet_NaN()
+                                                            : std::max(largest, c.residual);
+      if (!std::isfinite(c.residual)) {
+        c.live = false;
+      } else if (c.residual <= options_.rtol) {
+        c.info.converged = true;
+        c.live = false;
+        LaunchCopyRuns(s_.data(), sNext_.data(), coupling_.EveryRuns(r), s);
+      }
+    }
+    residualHistory.push_back(largest);
+    if (options_.damping != 1.0) {
+      ProfileScope prof("damp", s);
+      hipLaunchKernelGGL(DampKernel, dim3(Blocks(S)), dim3(256), 0, s, sNext_.data(),
+                         s_.data(), options_.damping, S);
+      SL3_HIP(hipGetLastError());
+    }
+    // the sweeps a damped pass takes after its residual
+    bool sweeps = false;
+    for (int r = 0; r < R; ++r) {
+      Control& c = ctl[r];
+      if (!c.live) continue;
+      if (adaptive && !c.mixing && memory > 0 && it > 0 &&
+          c.residual > options_.andersonRate * c.previous) {
+        c.mixing = true;
+      }
+      c.previous = c.residual;
+      if (c.mixing && c.started && !c.fused) {
+        LaunchAndersonSweep(sweepOf(r), coupling_.Runs(r), partials_.data(), 1024,
+                            result_.data() + static_cast<size_t>(r) * stride, nullptr, s);
+        sweeps = true;
+      }
+    }
+    if (sweeps) ReadBack(back.data(), result_.data(), stride * R);
+    for (int r = 0; r < R; ++r) {
+      Control& c = ctl[r];
+      if (!c.live) continue;
+      if (!c.mixing) {
+        LaunchCopyRuns(s_.data(), sNext_.data(), coupling_.EveryRuns(r), s);
+        continue;
+      }
+      if (!c.started) {
+        LaunchAndersonFirst(s_.data(), sNext_.data(), xPrev_.data() + r * C,
+                            fPrev_.data() + r * C, coupling_.Runs(r), coupling_.EveryRuns(r), s);
+        c.started = true;
+      } else {
+        const double* sums = back.data() + static_cast<size_t>(r) * stride;
+        const int m = c.m;
+        c.count = m;
+        for (int i = 0; i < m; ++i) {
+          c.gram[c.slot * memory + i] = sums[i];
+          c.gram[i * memory + c.slot] = sums[i];
+        }
+        const AndersonSweep sw = sweepOf(r);  // the slot the sweep wrote
+        c.slot = (c.slot + 1) % memory;
+        double g[kAndersonMax * kAndersonMax], gamma[kAndersonMax];
+        for (int i = 0; i < m; ++i)
+          for (int j = 0; j < m; ++j) g[i * m + j] = c.gram[i * memory + j];
+        GramLeastSquares(m, g, sums + c.width, gamma);
+        LaunchAndersonMix(sw, gamma, coupling_.Runs(r), s);
+      }
+      ++c.info.mixedPasses;
+    }
+  }
+  realisationInfos_.clear();
+  StepInfo info;
+  info.converged = true;
+  for (const auto& c : ctl) {
+    realisationInfos_.push_back(c.info);
+    info.iterations = std::max(info.iterations, c.info.iterations);
+    info.mixedPasses = std::max(info.mixedPasses, c.info.mixedPasses);
+    info.converged = info.converged && c.info.converged;
+    info.residual = std::isnan(c.info.residual) || std::isnan(info.residual)
+                        ? std::numeric_limits<double>::quiet_NaN()
+                        : std::max(info.residual, c.info.residual);
+  }
+  SL3_HIP(hipEventRecord(events_[2], s));
+  return info;
+}
+
 // Finish (src/Viscoelastic3D.cpp:1885-1914): d = Synthesise(strain),
 // m+ = m~ + beta d, the strain and its coefficients kept, the state at the
 // end time. strain: device coefficients, or null for the iterate.
@@ -1791,7 +2011,14 @@ void Viscoelastic3D::OpenFinish(const double* strain) {
   open_.state = nullptr;
 }
 
+void Viscoelastic3D::RequireOneRealisation(const char* what) const {
+  if (coupling_.Realisations() != 1) {
+    throw ArgumentError(std::string(what) + " takes an operator of one realisation");
+  }
+}
+
 void Viscoelastic3D::ForceOf(const double* weighted, double* forces) {
+  RequireOneRealisation("the force in the solution layout");
   hipStream_t s = Stream();
   boundaryRows_.Resize(std::max<size_t>(fDoubles_, 1));
   LaunchFill(boundaryRows_.data(), boundaryRows_.size(), 0.0, s);
@@ -1805,6 +2032,7 @@ void Viscoelastic3D::ForceOf(const double* weighted, double* forces) {
 }
 
 void Viscoelastic3D::StrainOfSolutions(const double* solutions, double* spectral) {
+  RequireOneRealisation("the strain of solutions");
   hipStream_t s = Stream();
   boundaryRows_.Resize(std::max<size_t>(fDoubles_, 1));
   LaunchFill(boundaryRows_.data(), boundaryRows_.size(), 0.0, s);
@@ -1961,6 +2189,7 @@ StepInfo Viscoelastic3D::CloseByConjugateGradients(Flavour& f) {
 }
 
 std::vector<double> Viscoelastic3D::SolutionOfTask(int task) {
+  RequireOneRealisation("the solutions' export");
   const auto& t = tasks_.at(task);
   if (lastStep_.scheme < 0) throw ArgumentError("no step has been taken");
   // the factors of the last step's own flavour (a first step's elastic

@@ -48,7 +48,14 @@ struct StepOptions {
   // Options::transform_degrees (include/sl3dpp/Viscoelastic3D.hpp:
   // 172-175): the degrees the transforms may be cut to with depth
   std::vector<int> transformDegrees;
+  // independent realisations (each its own lateral field and state, the
+  // Earth, degree, load and tide shared) stepped together: every kernel
+  // takes their memory nodes as extra columns, each converges on its own
+  int realisations = 1;
 };
+
+// The most realisations one operator steps together.
+constexpr int kRealisationsMax = 64;
 
 // Viscoelastic3D::StepInfo (include/sl3dpp/Viscoelastic3D.hpp:357-375).
 struct StepInfo {
@@ -100,11 +107,18 @@ struct NodeBlock {
 class Coupling {
  public:
   // transformDegrees: the degrees the transforms may be cut to with depth
-  // (Options::transform_degrees; empty: one block at lmax).
+  // (Options::transform_degrees; empty: one block at lmax). realisations:
+  // independent copies of the memory nodes side by side (realisation r's
+  // node n is column r N + n of every spectral row and node r N + n of
+  // every grid field, N = RadialNodeCount()); not with transformDegrees.
   Coupling(const Earth& earth, int lmax, double oversampling, double eps,
-           const std::vector<int>& transformDegrees = {});
+           const std::vector<int>& transformDegrees = {}, int realisations = 1);
   int LMax() const { return lmax_; }
-  int NodeCount() const { return geometry_.nodes; }
+  // the memory nodes of every realisation (the spectral rows' columns)
+  int NodeCount() const { return geometry_.nodes * geometry_.realisations; }
+  // the memory nodes of one realisation
+  int RadialNodeCount() const { return geometry_.nodes; }
+  int Realisations() const { return geometry_.realisations; }
   // the full degree's grid (the layouts, and every node's grid without cuts)
   const SurfaceGrid& Grid() const { return grid_; }
   int BlockCount() const { return static_cast<int>(blocks_.size()); }
@@ -119,13 +133,26 @@ class Coupling {
   size_t FourierDoubles(int block) const;
   const RadialGeometry& Geometry() const { return geometry_; }
   const std::vector<int>& FirstNodes() const { return firstNodeHost_; }
+  // per node of one realisation
   const std::vector<double>& NodeRadii() const { return radiusHost_; }
   const std::vector<int>& NodeElement() const { return nodeElement_; }
   int FirstElementOfDegree(int l) const { return firstElement_[l]; }
   SpectralView View(double* spectral) const;
   const int* RowOffsetDevice(int block) const { return rowOffset_[block].data(); }
-  // The active (row, node) runs of a spectral tensor (degree truncation).
-  const ActiveRuns& Runs() const { return runs_; }
+  // The active (row, node) runs of a spectral tensor (degree truncation)
+  // of realisation r, their compact vector laid out alike for every
+  // realisation; with one realisation, of the tensor.
+  ActiveRuns Runs(int r = 0) const {
+    ActiveRuns a = runs_;
+    a.full += static_cast<size_t>(r) * runs_.runs;
+    return a;
+  }
+  // Every entry of realisation r's columns, as runs.
+  ActiveRuns EveryRuns(int r) const {
+    ActiveRuns a = everyRuns_;
+    a.full += static_cast<size_t>(r) * everyRuns_.runs;
+    return a;
+  }
 
   // Batched transforms of all three blocks over every node.
   void SynthesiseToFourier(const double* spectral, double* const fourier[3],
@@ -179,8 +206,8 @@ class Coupling {
   size_t gridValues_ = 0;
   size_t fourierDoubles_[3] = {};
   size_t spectralOffset_[4] = {};
-  DeviceArray<long long> runFull_, runCompact_;
-  ActiveRuns runs_;
+  DeviceArray<long long> runFull_, runCompact_, everyFull_, everyCompact_;
+  ActiveRuns runs_, everyRuns_;  // realisation 0's; full offsets per realisation follow
 };
 
 
@@ -260,8 +287,13 @@ class Viscoelastic3D {
     float startStep = 0, passes = 0, finish = 0;
   } lastTiming;
   // Per-pass residuals of the last step (conjugate gradients: the relative
-  // residual before each iteration and after the last).
+  // residual before each iteration and after the last; realisations side
+  // by side: the largest of the pass's unconverged realisations).
   std::vector<double> residualHistory;
+  // Per realisation, the last step's (or converge's) own StepInfo; the
+  // StepInfo returned for several realisations takes the most passes, the
+  // largest residual, and converged when all converged.
+  const std::vector<StepInfo>& RealisationInfos() const { return realisationInfos_; }
   // The last conjugate-gradient solve's scale <x0, f> (x0 = K^-1 f).
   std::vector<double> solveEnergy;
   // The device buffers, for export.
@@ -344,6 +376,14 @@ class Viscoelastic3D {
   StepInfo StepOnce(DeviceState& state, double dt, int scheme, const Channels& channels,
                     const double* start, double warmGuard);
   StepInfo StepSdirk2(DeviceState& state, double dt, const Channels& channels);
+  // Converge for several realisations side by side: every pass takes all
+  // of them (one launch per kernel), each realisation keeps its own
+  // residual, adaptive switch and Anderson history (launched on its runs),
+  // and stops changing once it converges, so that each follows the passes
+  // it takes alone.
+  StepInfo ConvergeBatch(const PassFn& pass, const double* start, double warmGuard);
+  void RequireOneRealisation(const char* what) const;
+  std::vector<StepInfo> realisationInfos_;
   // OpenStep::Forces + Solve + StrainOf (src/Viscoelastic3D.cpp:2008-2015):
   // the step's system by conjugate gradients on the memory dofs, its strain
   // coefficients into s_.
@@ -435,6 +475,7 @@ class Viscoelastic3D {
   // a pass's few result doubles come back through pinned memory, the host
   // polling an event behind the copy (ReadBack)
   static constexpr int kReadBackMax = 40;  // the sweep's 2 x 16 sums and 3 maxima
+  // (per realisation)
   double* pinned_ = nullptr;
   hipEvent_t readEvent_ = nullptr;
   void ReadBack(double* dst, const double* src, int count);

@@ -66,6 +66,7 @@ class StepOptions(ctypes.Structure):
         ("graph_passes", ctypes.c_int),
         ("transform_degree_count", ctypes.c_int),
         ("transform_degrees_", ctypes.c_int * MAX_TRANSFORM_DEGREES),
+        ("realisations", ctypes.c_int),
     ]
 
     @classmethod
@@ -327,6 +328,12 @@ class Viscoelastic3D:
                                      ctypes.byref(nph), ctypes.byref(sd)))
         self.nodes, self.ntheta, self.nphi = nodes.value, nt.value, nph.value
         self.spectral_doubles = sd.value
+        # realisations side by side: realisation r's memory nodes are
+        # nodes r * radial_nodes ... of every node-indexed tensor
+        real, radial = ctypes.c_int(), ctypes.c_int()
+        check(lib.sl3_operator_realisations(self.handle, ctypes.byref(real),
+                                            ctypes.byref(radial)))
+        self.realisations, self.radial_nodes = real.value, radial.value
         self.radii = np.zeros(self.nodes)
         self.theta = np.zeros(self.ntheta)
         self.wtheta = np.zeros(self.ntheta)
@@ -393,6 +400,15 @@ class Viscoelastic3D:
                                    load, tide, None, ptr, ctypes.c_double(guard),
                                    ctypes.byref(info)))
         return info
+
+    def realisation_infos(self):
+        """Per realisation, the last step's own StepInfo."""
+        count = ctypes.c_int()
+        check(lib.sl3_step_realisation_infos(self.handle, 0, ctypes.byref(count), None))
+        infos = (StepInfo * max(count.value, 1))()
+        check(lib.sl3_step_realisation_infos(self.handle, count.value, ctypes.byref(count),
+                                             infos))
+        return list(infos[:count.value])
 
     def export(self, name):
         count = ctypes.c_int64()
