@@ -106,12 +106,13 @@ constexpr int kChunkOrders = 8;
 // shares) in LDS; stage 2 sums each owned GLL node's rows.
 __global__ void __launch_bounds__(kThreads)
     ForceKernel(const DegreeTask* tasks, RadialGeometry g, SpectralView w, const int* dofs,
-                const int2* work, double* force) {
+                const int2* work, double* force, const int* live) {
   constexpr int maxPoints = (kChunkElements + 1) * kMaxPoints;
   __shared__ Cplx rowsL[maxPoints * kChunkOrders][6];
   const int2 wk = work[blockIdx.y];
   const int l = wk.x, mc = wk.y;
   const int real = blockIdx.z;  // the realisation
+  if (live && live[real] == 0) return;  // converged
   const int width = g.nodes * g.realisations;
   const int L = w.lmax;
   const DegreeTask ts = tasks[l];
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(1024)
     StrainKernel(const DegreeTask* sph, const DegreeTask* tor, int lmax,
                  RadialGeometry g, const int* dofs, const double* response,
                  const double* units, const double* amp, const double* tideUnits,
-                 const double* tideAmp, SpectralView out) {
+                 const double* tideAmp, SpectralView out, const int* live) {
   const int pair = blockIdx.x;
   int l = static_cast<int>((sqrt(8.0 * pair + 1.0) - 1.0) * 0.5);
   while (l * (l + 1) / 2 > pair) --l;
@@ -216,6 +217,7 @@ __global__ void __launch_bounds__(1024)
   const int m = pair - l * (l + 1) / 2;
   const DegreeTask ts = sph[l];
   const int real = blockIdx.y;  // the realisation
+  if (live && live[real] == 0) return;  // converged
   const int width = g.nodes * g.realisations;
   for (int node = threadIdx.x; node < g.nodes; node += blockDim.x) {
     Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
@@ -310,7 +312,8 @@ __global__ void __launch_bounds__(1024)
     StrainStagedKernel(const DegreeTask* sph, const DegreeTask* tor, int lmax,
                        RadialGeometry g, const int* dofs, const double* response,
                        const double* units, const double* amp, const double* tideUnits,
-                       const double* tideAmp, SpectralView out, int capS, int capT) {
+                       const double* tideAmp, SpectralView out, int capS, int capT,
+                       const int* live) {
   extern __shared__ double staged[];
   double* sRe = staged;
   double* sIm = sRe + capS;
@@ -333,6 +336,7 @@ __global__ void __launch_bounds__(1024)
   const bool tide = tideAmp != nullptr;
   const bool unitsOn = units != nullptr && ts.unitOff >= 0;  // no load channel: none
   const int real = blockIdx.y;  // the realisation
+  if (live && live[real] == 0) return;  // converged
   const int width = g.nodes * g.realisations;
   {
     const double* re = response + ts.fOff + static_cast<size_t>(real * ts.cols + m) * ts.nmPad;
@@ -815,13 +819,13 @@ std::vector<int2> ForceWork(int lmax) {
 
 void LaunchForce(const DegreeTask* tasks_dev, const RadialGeometry& g,
                  const SpectralView& weighted, const int* dofs, const int2* work_dev,
-                 int workCount, double* force, hipStream_t stream) {
+                 int workCount, double* force, hipStream_t stream, const int* live) {
   if (workCount == 0 || g.elements == 0) return;
   if (g.np > kMaxPoints) throw ArgumentError("at most 6 memory points per element");
   const dim3 grid(CeilDiv(g.elements, kChunkElements), workCount, g.realisations);
   ProfileScope prof("force", stream);
   hipLaunchKernelGGL(ForceKernel, grid, dim3(kThreads), 0, stream, tasks_dev, g, weighted,
-                     dofs, work_dev, force);
+                     dofs, work_dev, force, live);
   SL3_HIP(hipGetLastError());
 }
 
@@ -829,7 +833,7 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
                   int lmax, const RadialGeometry& g, const int* dofs,
                   const double* response, const double* units, const double* amp,
                   const double* tideUnits, const double* tideAmp, SpectralView out,
-                  hipStream_t stream) {
+                  hipStream_t stream, const int* live) {
   const int pairs = (lmax + 1) * (lmax + 2) / 2;
   const int threads = std::min(1024, 64 * CeilDiv(g.nodes, 64));
   ProfileScope prof509("strain", stream);
@@ -841,11 +845,11 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
   if (lds <= 64 * 1024) {
     hipLaunchKernelGGL(StrainStagedKernel, dim3(pairs, g.realisations), dim3(threads), lds, stream,
                        sphTasks_dev, torTasks_dev, lmax, g, dofs, response, units, amp,
-                       tideUnits, tideAmp, out, capS, capT);
+                       tideUnits, tideAmp, out, capS, capT, live);
   } else {
     hipLaunchKernelGGL(StrainKernel, dim3(pairs, g.realisations), dim3(threads), 0, stream, sphTasks_dev,
                        torTasks_dev, lmax, g, dofs, response, units, amp, tideUnits, tideAmp,
-                       out);
+                       out, live);
   }
   SL3_HIP(hipGetLastError());
 }

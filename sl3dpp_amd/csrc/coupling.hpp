@@ -70,9 +70,12 @@ void LaunchCombine(double* dst, double a, const double* src, double b, size_t co
 // (U, V) or (W, -), the memory-dof row or -1 (none, or the pinned dof).
 // work: the (degree, first order) chunks of ForceWork(lmax).
 std::vector<int2> ForceWork(int lmax);
+// live (per realisation, or null): a realisation marked 0 has converged
+// and the maps below skip it.
 void LaunchForce(const DegreeTask* tasks_dev, const RadialGeometry& g,
                  const SpectralView& weighted, const int* dofs, const int2* work_dev,
-                 int workCount, double* force, hipStream_t stream);
+                 int workCount, double* force, hipStream_t stream,
+                 const int* live = nullptr);
 
 // Strain of the responses on the memory dofs into the spectral iterate
 // (every value written; realisation r from its response columns r cols,
@@ -84,7 +87,7 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
                   int lmax, const RadialGeometry& g, const int* dofs,
                   const double* response, const double* units, const double* amp,
                   const double* tideUnits, const double* tideAmp, SpectralView out,
-                  hipStream_t stream);
+                  hipStream_t stream, const int* live = nullptr);
 
 // max |n|^2, max |n - o|^2 and a non-finite flag over n values, into
 // result[0..2] (device).

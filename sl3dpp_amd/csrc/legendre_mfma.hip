@@ -73,6 +73,8 @@ struct Args {
   // Fourier layout's first order per block
   int nodeOffset, nodesTotal;
   int fourierMlo[3];
+  const int* live;  // per realisation, 0: converged, its column blocks skipped
+  int liveNodes;
 };
 
 // The largest degree any node of a column block carries (workgroup-uniform
@@ -81,6 +83,17 @@ __device__ inline int ColumnBlockReach(const Args& a, int node0, int nodeEnd) {
   int r = 0;
   for (int n = node0; n < nodeEnd; ++n) r = max(r, a.reach[a.nodeOffset + n]);
   return r;
+}
+
+// Whether a column block holds a node of a realisation still converging.
+__device__ inline bool AnyLive(const Args& a, int node0, int nodeEnd) {
+  if (a.live == nullptr) return true;
+  const int r0 = (a.nodeOffset + node0) / a.liveNodes;
+  const int r1 = (a.nodeOffset + nodeEnd - 1) / a.liveNodes;
+  for (int r = r0; r <= r1; ++r) {
+    if (a.live[r] != 0) return true;
+  }
+  return false;
 }
 
 __device__ inline int LMinOf(int m, int spin) {
@@ -127,6 +140,7 @@ __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
   const int cols = 2 * a.nodes;
   const int c0 = cb * kCols;
   const int node0 = c0 / 2, nodeEnd = min(a.nodes, node0 + kCols / 2);
+  if (!AnyLive(a, node0, nodeEnd)) return;
   // degrees that reach a node of the column block (the largest reach among
   // its nodes: nodes of several realisations may share a block)
   const int lhi = min(lEnd, a.reach ? ColumnBlockReach(a, node0, nodeEnd) : a.lmax);
@@ -249,6 +263,7 @@ __global__ void __launch_bounds__(kThreads) LegendreSynthesisMfmaKernel(Args a) 
   const int cols = 2 * a.nodes;
   const int c0 = cb * kCols;
   const int node0 = c0 / 2, nodeEnd = min(a.nodes, node0 + kCols / 2);
+  if (!AnyLive(a, node0, nodeEnd)) return;
   const int lhi = a.reach ? min(a.lmax, ColumnBlockReach(a, node0, nodeEnd)) : a.lmax;
   const int row0 = a.rowOffset[b][m - a.mlo[b]] - lmin;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -349,6 +364,8 @@ Args MakeArgs(const LegendrePlan& p) {
   a.nodes = p.nodes;
   a.ntheta = p.ntheta;
   a.fstride = p.fstride;
+  a.live = p.live;
+  a.liveNodes = p.liveNodes;
   return a;
 }
 

@@ -71,11 +71,17 @@ constexpr int kBPer = 4 * kChunk * 16 * kColTiles / kThreads;   // staged B per 
 
 __global__ void __launch_bounds__(kThreads)
     ResponseKernel(const DegreeTask* tasks, const ResponseWork* work, const double* g,
-                   const double* f, double* x) {
+                   const double* f, double* x, const int* live, int realisations) {
   __shared__ double As[kRowTiles * kChunk][kWave];              // [tile][lane]
   __shared__ double Bs[4 * kChunk][16 * kColTiles + 1];         // [k][column]
   const ResponseWork w = work[blockIdx.x];
   const DegreeTask t = tasks[w.task];
+  if (live) {  // skip a tile of converged realisations' columns only
+    const int c0 = 16 * w.cb, c1 = min(c0 + 16 * kColTiles, realisations * t.cols) - 1;
+    bool any = false;
+    for (int r = c0 / t.cols; r <= c1 / t.cols; ++r) any = any || live[r] != 0;
+    if (!any) return;
+  }
   const int rbCount = t.nmPad / 16, k4 = t.nmPad / 4, cbCount = t.colsPad / 16;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int wr = wave / 2, wc = wave % 2;  // the wave's 2 row tiles and 4 column tiles
@@ -272,11 +278,11 @@ void LaunchPackResponse(const DegreeTask* tasks_dev, int taskCount, int nmPadMax
 
 void LaunchResponse(const DegreeTask* tasks_dev, const ResponseWork* work_dev,
                     int workCount, const double* g, const double* f, double* x,
-                    hipStream_t stream) {
+                    hipStream_t stream, const int* live, int realisations) {
   if (workCount == 0) return;
   ProfileScope prof("response", stream);
   hipLaunchKernelGGL(ResponseKernel, dim3(workCount), dim3(kThreads), 0, stream, tasks_dev,
-                     work_dev, g, f, x);
+                     work_dev, g, f, x, live, realisations);
   SL3_HIP(hipGetLastError());
 }
 

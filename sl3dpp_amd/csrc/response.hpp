@@ -43,9 +43,11 @@ void LaunchPackResponse(const DegreeTask* tasks_dev, int taskCount, int nmPadMax
 
 // x = G f for every task (f: nmPad x colsPad row-major at fOff; x the
 // same size at fOff, column-major: x[col][row]).
+// live (per realisation, or null): the column tiles of realisations marked
+// 0 (converged; realisation r's columns at r cols) are skipped.
 void LaunchResponse(const DegreeTask* tasks_dev, const ResponseWork* work_dev,
                     int workCount, const double* g, const double* f, double* x,
-                    hipStream_t stream);
+                    hipStream_t stream, const int* live = nullptr, int realisations = 1);
 
 // The full scaled right-hand sides from the memory-dof force (for a band
 // solve of the whole system): rhs[dm[i]][c] = scale[dm[i]] f[i][c]; rows

@@ -686,6 +686,7 @@ constexpr int WavesPerEu() { return JT >= 5 ? 1 : 2; }
 template <int JT, bool REAL, int CLASS, bool FULL>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerEu<JT>())))
     RingMfmaKernel(RingLaunch a) {
+  if (a.live && a.live[blockIdx.y / a.liveNodes] == 0) return;  // a converged realisation
   RingBody<JT, REAL, CLASS, FULL>(a, blockIdx.y, blockIdx.x * MakeGeometry<REAL>(a).ringsPerGroup);
 }
 
@@ -713,6 +714,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W
   const int groups = (base.ntheta + 15) / 16;
   if (st >= groups * base.nodes) return;
   const int node = st / groups, th16 = (st % groups) * 16;
+  if (base.live && base.live[node / base.liveNodes] == 0) return;  // a converged realisation
   if (sub < 2) {
     complex_lanes::ComplexPassBody<JT, FULL, KIN>(
         base, reinterpret_cast<const double2*>(blocks.in[sub]),

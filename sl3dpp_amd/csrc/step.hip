@@ -384,19 +384,22 @@ SpectralView Coupling::View(double* spectral) const {
 }
 
 void Coupling::SynthesiseToFourier(const double* spectral, double* const fourier[3],
-                                   bool truncated, hipStream_t s) const {
+                                   bool truncated, hipStream_t s, const int* live) const {
   const double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
                              spectral + spectralOffset_[2]};
   for (const auto& k : blocks_) {
     double* out[3] = {fourier[0] + k->fourierOffset[0], fourier[1] + k->fourierOffset[1],
                       fourier[2] + k->fourierOffset[2]};
-    LaunchLegendreSynthesis(k->plan, blocks, out, truncated ? reach_.data() : nullptr, s);
+    LegendrePlan p = k->plan;
+    p.live = live;
+    p.liveNodes = geometry_.nodes;
+    LaunchLegendreSynthesis(p, blocks, out, truncated ? reach_.data() : nullptr, s);
   }
 }
 
 void Coupling::AnalyseFromFourier(const double* const fourier[3], double* spectral,
-                                  const double* addend, bool truncated,
-                                  hipStream_t s) const {
+                                  const double* addend, bool truncated, hipStream_t s,
+                                  const int* live) const {
   double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
                        spectral + spectralOffset_[2]};
   const double* adds[3] = {};
@@ -406,7 +409,10 @@ void Coupling::AnalyseFromFourier(const double* const fourier[3], double* spectr
   for (const auto& k : blocks_) {
     const double* in[3] = {fourier[0] + k->fourierOffset[0], fourier[1] + k->fourierOffset[1],
                            fourier[2] + k->fourierOffset[2]};
-    LaunchLegendreAnalysis(k->plan, in, blocks, addend ? adds : nullptr,
+    LegendrePlan p = k->plan;
+    p.live = live;
+    p.liveNodes = geometry_.nodes;
+    LaunchLegendreAnalysis(p, in, blocks, addend ? adds : nullptr,
                            truncated ? reach_.data() : nullptr,
                            truncated ? firstNode_.data() : nullptr, s);
   }
@@ -505,11 +511,13 @@ void Coupling::Ring(RingLaunch r, hipStream_t s) const {
 }
 
 void Coupling::RingPass(double* const in[3], double* const out[3], const double* weight,
-                        hipStream_t s) const {
+                        hipStream_t s, const int* live) const {
   for (const auto& k : blocks_) {
     RingLaunch base;
     base.op = kRingPass;
     base.weight = weight;
+    base.live = live;
+    base.liveNodes = geometry_.nodes;
     const RingLaunch r = BlockLaunch(base, *k);
     if (RingPassFusable(r)) {
       RingBlocks blocks;
@@ -713,6 +721,10 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
   }
   muNode_.Upload(muNodeHost_.data(), muNodeHost_.size(), s);
   itauRefNode_.Upload(itauRefHost_.data(), itauRefHost_.size(), s);
+  if (geo.realisations > 1) {
+    live_.Resize(geo.realisations);
+    SetAllLive();
+  }
   const size_t G = coupling_.GridValueCount();
   SetDelta(delta_dev);
   BuildTasks();
@@ -1237,9 +1249,9 @@ void Viscoelastic3D::SolveAll(const double* weighted, Flavour& f) {
   rhsCurrent_ = false;
   const SpectralView w = coupling_.View(const_cast<double*>(weighted));
   LaunchForce(tasksDev_.data(), coupling_.Geometry(), w, dofsMem_.data(), forceWork_.data(),
-              forceWorkCount_, force_.data(), s);
+              forceWorkCount_, force_.data(), s, Live());
   LaunchResponse(tasksDev_.data(), work_.data(), workCount_, f.response.data(),
-                 force_.data(), response_.data(), s);
+                 force_.data(), response_.data(), s, Live(), coupling_.Realisations());
 }
 
 // TideColumns' check (src/Viscoelastic3D.cpp:142-154) on degree 1's columns.
@@ -1358,16 +1370,17 @@ void Viscoelastic3D::LaunchPass(const double* s, double* sNext, bool useStartFor
   if (!useStartForce) {
     double* A[3] = {fourierA_[0].data(), fourierA_[1].data(), fourierA_[2].data()};
     double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
-    coupling_.SynthesiseToFourier(s, A, true, st);
-    coupling_.RingPass(A, B, weight_.data(), st);
-    coupling_.AnalyseFromFourier(B, analysed_.data(), fixed_.data(), true, st);
+    coupling_.SynthesiseToFourier(s, A, true, st, Live());
+    coupling_.RingPass(A, B, weight_.data(), st, Live());
+    coupling_.AnalyseFromFourier(B, analysed_.data(), fixed_.data(), true, st, Live());
   }
   SolveAll(analysed_.data(), f);
   const auto* sph = tasksDev_.data();
   const auto* tor = tasksDev_.data() + (coupling_.LMax() + 1);
   LaunchStrain(sph, tor, coupling_.LMax(), coupling_.Geometry(), dofsMem_.data(),
                response_.data(), f.unitsMem.data(), amp_.data(),
-               f.unitsMem.data() + dmHost_.size(), TideAmplitudes(), coupling_.View(sNext), st);
+               f.unitsMem.data() + dmHost_.size(), TideAmplitudes(), coupling_.View(sNext), st,
+               Live());
 }
 
 void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const Channels& channels,
@@ -1536,6 +1549,7 @@ void Viscoelastic3D::OpenStart(DeviceState& state, double dt, int scheme,
   lastStep_ = {scheme, dt};
   rhsCurrent_ = false;
   const double tEnd = state.t + dt;
+  SetAllLive();  // a converge cut short by an error may have left some marked
 
   SL3_HIP(hipEventRecord(events_[0], s));
   // s0 (src/Viscoelastic3D.cpp:1252-1263)
@@ -1816,6 +1830,7 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
   hipStream_t s = Stream();
   const int R = coupling_.Realisations();
   const size_t S = coupling_.SpectralDoubles();
+  SetAllLive();
   residualHistory.clear();
   const bool startForce = open_.startForce;
   bool warm = false;
@@ -1909,6 +1924,8 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
         c.live = false;
         LaunchCopyRuns(s_.data(), sNext_.data(), coupling_.EveryRuns(r), s);
       }
+      // the next passes skip a realisation that stopped
+      if (!c.live) SL3_HIP(hipMemsetAsync(live_.data() + r, 0, sizeof(int), s));
     }
     residualHistory.push_back(largest);
     if (options_.damping != 1.0) {
@@ -1964,6 +1981,7 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
       ++c.info.mixedPasses;
     }
   }
+  SetAllLive();
   realisationInfos_.clear();
   StepInfo info;
   info.converged = true;
@@ -2009,6 +2027,11 @@ void Viscoelastic3D::OpenFinish(const double* strain) {
   state.t = open_.tEnd;
   SL3_HIP(hipEventRecord(events_[3], s));
   open_.state = nullptr;
+}
+
+void Viscoelastic3D::SetAllLive() {
+  if (live_.size() == 0) return;
+  SL3_HIP(hipMemsetAsync(live_.data(), 0xff, live_.size() * sizeof(int), Stream()));
 }
 
 void Viscoelastic3D::RequireOneRealisation(const char* what) const {

@@ -154,16 +154,19 @@ class Coupling {
     return a;
   }
 
-  // Batched transforms of all three blocks over every node.
+  // Batched transforms of all three blocks over every node. live (per
+  // realisation, or null): realisations marked 0 are skipped (their
+  // outputs left as they were).
   void SynthesiseToFourier(const double* spectral, double* const fourier[3],
-                           bool truncated, hipStream_t s) const;
+                           bool truncated, hipStream_t s, const int* live = nullptr) const;
   void AnalyseFromFourier(const double* const fourier[3], double* spectral,
-                          const double* addend, bool truncated, hipStream_t s) const;
+                          const double* addend, bool truncated, hipStream_t s,
+                          const int* live = nullptr) const;
   void Ring(RingLaunch r, hipStream_t s) const;  // fills tables and sizes
   // The pass's longitude stage (Fourier -> ring x weight -> Fourier) over
   // all three blocks: one launch where the grid allows it.
   void RingPass(double* const in[3], double* const out[3], const double* weight,
-                hipStream_t s) const;
+                hipStream_t s, const int* live = nullptr) const;
   // The tuner: times every tile shape of the Legendre kernels on this
   // coupling's own shape (the degree truncation included) and keeps the
   // fastest; shapes met before take the choice made then. spectral: a
@@ -384,6 +387,11 @@ class Viscoelastic3D {
   StepInfo ConvergeBatch(const PassFn& pass, const double* start, double warmGuard);
   void RequireOneRealisation(const char* what) const;
   std::vector<StepInfo> realisationInfos_;
+  // realisations side by side: per realisation nonzero while it converges
+  // (a pass skips the others); every one nonzero outside ConvergeBatch
+  DeviceArray<int> live_;
+  const int* Live() const { return coupling_.Realisations() > 1 ? live_.data() : nullptr; }
+  void SetAllLive();
   // OpenStep::Forces + Solve + StrainOf (src/Viscoelastic3D.cpp:2008-2015):
   // the step's system by conjugate gradients on the memory dofs, its strain
   // coefficients into s_.

@@ -54,6 +54,8 @@ struct RingArgs {
   double dt;
   int scheme;
   double* fourierOut2;   // start: W d0's transform beside the fixed force's
+  const int* live;       // per realisation, 0: converged, skipped (null: none)
+  int liveNodes;
 };
 
 // The LDS of one workgroup: RB rings of nphi complex, their R P-point
@@ -286,6 +288,7 @@ __global__ void __launch_bounds__(kThreads) RingKernel(RingArgs a) {
   const int R = a.t.R, P = a.t.P, nphi = a.nphi;
   const RingShared sh = CarveShared(lds, nphi, P);
   const int node = blockIdx.y;
+  if (a.live && a.live[node / a.liveNodes] == 0) return;  // a converged realisation
   const int th0 = blockIdx.x * kRings;
   const int rings = min(kRings, a.ntheta - th0);
   const bool real = a.block == 2;
@@ -486,6 +489,8 @@ void LaunchRing(const RingLaunch& r, hipStream_t stream) {
   a.itauRefNode = r.itauRefNode;
   a.dt = r.dt;
   a.scheme = r.scheme;
+  a.live = r.live;
+  a.liveNodes = r.liveNodes;
   const dim3 grid(CeilDiv(r.ntheta, kRings), r.nodes);
   ProfileScope prof692("ring", stream);
   hipLaunchKernelGGL(RingKernel, grid, dim3(kThreads), RingLds(r.nphi, r.tables.P),

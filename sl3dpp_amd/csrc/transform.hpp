@@ -32,6 +32,10 @@ struct LegendrePlan {
   // workgroup (32, 64, 112, 208); picked per shape by the tuner
   int analysisRows = 0;
   int synthesisTheta = 0;
+  // realisations side by side: a column block whose nodes (nodeOffset +
+  // node) all have live[node / liveNodes] == 0 is skipped (null: none)
+  const int* live = nullptr;
+  int liveNodes = 1;
 };
 
 constexpr int kDefaultAnalysisRows = 64;
@@ -97,6 +101,10 @@ struct RingLaunch {
   const double* itauRefNode = nullptr;
   double dt = 0.0;
   int scheme = 0;
+  // realisations side by side: live[node / liveNodes] == 0 marks a
+  // converged realisation, whose nodes the pass skips (null: none)
+  const int* live = nullptr;
+  int liveNodes = 1;
 };
 
 // The three blocks' Fourier buffers of one fused pass launch.
