@@ -484,6 +484,17 @@ __global__ void CopyKernel(double* dst, const double* src, size_t count) {
   }
 }
 
+// A reduction's few results to host-mapped memory, then the ticket after a
+// system-scope fence (the host polls the ticket).
+__global__ void PublishKernel(const double* src, int n, HostPublish pub) {
+  if (static_cast<int>(threadIdx.x) < n) pub.values[threadIdx.x] = src[threadIdx.x];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(pub.seq, pub.ticket, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 // Block b covers [n b / B, n (b + 1) / B): a fixed partition of the index
 // range, so the reductions below group the same terms on every run.
 __device__ inline void BlockRange(size_t n, size_t& lo, size_t& hi) {
@@ -763,6 +774,12 @@ int GridFor(size_t n) {
 
 }  // namespace
 
+void LaunchPublish(const double* src, int n, const HostPublish& pub, hipStream_t stream) {
+  if (n < 0 || n > 64) throw ArgumentError("publish at most 64 doubles");
+  hipLaunchKernelGGL(PublishKernel, dim3(1), dim3(64), 0, stream, src, n, pub);
+  SL3_HIP(hipGetLastError());
+}
+
 void LaunchFill(double* dst, size_t count, double value, hipStream_t stream) {
   if (count == 0) return;
   ProfileScope prof475("fill", stream);
@@ -856,7 +873,7 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
 
 void LaunchRelativeChange(const double* next, const double* now, size_t n,
                           double* partials, int partialBlocks, double* result,
-                          hipStream_t stream) {
+                          hipStream_t stream, const HostPublish* pub) {
   ProfileScope prof518("residual", stream);
   hipLaunchKernelGGL(RelativeChangeKernel, dim3(partialBlocks), dim3(kThreads), 0,
                      stream, next, now, n / 2, partials);
@@ -864,11 +881,12 @@ void LaunchRelativeChange(const double* next, const double* now, size_t n,
   hipLaunchKernelGGL(ReducePartialsKernel, dim3(3), dim3(kThreads), 0, stream, partials,
                      partialBlocks, 3, 1, result);
   SL3_HIP(hipGetLastError());
+  if (pub) LaunchPublish(result, 3, *pub, stream);
 }
 
 void LaunchRelativeChangeActive(const double* next, const double* now, const ActiveRuns& runs,
                                 double* partials, int partialBlocks, double* result,
-                                hipStream_t stream) {
+                                hipStream_t stream, const HostPublish* pub) {
   ProfileScope prof("residual", stream);
   hipLaunchKernelGGL(RelativeChangeActiveKernel, dim3(partialBlocks), dim3(kThreads), 0,
                      stream, next, now, runs, partials);
@@ -876,6 +894,7 @@ void LaunchRelativeChangeActive(const double* next, const double* now, const Act
   hipLaunchKernelGGL(ReducePartialsKernel, dim3(3), dim3(kThreads), 0, stream, partials,
                      partialBlocks, 3, 1, result);
   SL3_HIP(hipGetLastError());
+  if (pub) LaunchPublish(result, 3, *pub, stream);
 }
 
 void LaunchAndersonFirst(double* x, const double* gx, double* xPrev, double* fPrev,
@@ -905,7 +924,7 @@ void LaunchCopyRuns(double* dst, const double* src, const ActiveRuns& runs, hipS
 
 void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double* partials,
                          int partialBlocks, double* sums, double* residual,
-                         hipStream_t stream) {
+                         hipStream_t stream, const HostPublish* pub) {
   if (s.m < 1 || s.m > kAndersonMax) throw ArgumentError("Anderson history of 1 to 16");
   ProfileScope prof("anderson", stream);
   const int W = AndersonWidth(s.m);
@@ -935,6 +954,12 @@ void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double*
     hipLaunchKernelGGL(ReducePartialsKernel, dim3(3), dim3(kThreads), 0, stream,
                        residualPartials, partialBlocks, 3, 1, residual);
     SL3_HIP(hipGetLastError());
+  }
+  // the sums then the residual's values, laid out as in the device result
+  // when residual follows the sums directly
+  if (pub) {
+    if (residual && residual != sums + 2 * W) throw ArgumentError("published residual follows the sums");
+    LaunchPublish(sums, 2 * W + (residual ? 3 : 0), *pub, stream);
   }
 }
 

@@ -89,11 +89,22 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
                   const double* tideUnits, const double* tideAmp, SpectralView out,
                   hipStream_t stream, const int* live = nullptr);
 
+// Where a reduction's results also go for the host: a one-workgroup kernel
+// after the reduction writes them to `values` (host-mapped memory), then
+// `ticket` to `seq` (host-mapped) after a system-scope fence; the host polls
+// `seq` (no copy, no event).
+struct HostPublish {
+  double* values = nullptr;
+  unsigned long long* seq = nullptr;
+  unsigned long long ticket = 0;
+};
+void LaunchPublish(const double* src, int n, const HostPublish& pub, hipStream_t stream);
+
 // max |n|^2, max |n - o|^2 and a non-finite flag over n values, into
-// result[0..2] (device).
+// result[0..2] (device; with pub, also published to the host).
 void LaunchRelativeChange(const double* next, const double* now, size_t n,
                           double* partials, int partialBlocks, double* result,
-                          hipStream_t stream);
+                          hipStream_t stream, const HostPublish* pub = nullptr);
 
 // The active entries of a spectral tensor: per row (all blocks, in memory
 // order) the run of nodes at or above the row degree's first memory node;
@@ -129,7 +140,7 @@ struct AndersonSweep {
 // entries of both iterates are zero).
 void LaunchRelativeChangeActive(const double* next, const double* now, const ActiveRuns& runs,
                                 double* partials, int partialBlocks, double* result,
-                                hipStream_t stream);
+                                hipStream_t stream, const HostPublish* pub = nullptr);
 
 // The first mixed pass: xPrev = x, fPrev = gx - x (compact), then x = gx
 // (every entry, n doubles).
@@ -146,10 +157,11 @@ void LaunchCopyRuns(double* dst, const double* src, const ActiveRuns& runs, hipS
 // sums[0, 2 w), w = AndersonWidth(m)); with residual non-null also the
 // pass's RelativeChange (max |gx|^2, max |gx - x|^2, a non-finite flag)
 // into residual[0..2]. partials holds partialBlocks x (2 kAndersonMax + 3)
-// doubles.
+// doubles. With pub, the sums and then the residual's three values (if
+// any; residual must then be sums + 2 w) are also published to the host.
 void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double* partials,
                          int partialBlocks, double* sums, double* residual,
-                         hipStream_t stream);
+                         hipStream_t stream, const HostPublish* pub = nullptr);
 void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,
                        hipStream_t stream);
 

@@ -762,6 +762,17 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
                         static_cast<size_t>(kReadBackMax) * geo.realisations * sizeof(double),
                         hipHostMallocDefault));
   SL3_HIP(hipEventCreateWithFlags(&readEvent_, hipEventDisableTiming));
+  {
+    const size_t R = geo.realisations;
+    const size_t bytes = R * kReadBackMax * sizeof(double) + R * sizeof(unsigned long long);
+    SL3_HIP(hipHostMalloc(reinterpret_cast<void**>(&mapped_), bytes,
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(mapped_, 0, bytes);
+    void* dev = nullptr;
+    SL3_HIP(hipHostGetDevicePointer(&dev, mapped_, 0));
+    mappedValuesDev_ = static_cast<double*>(dev);
+    mappedSeqDev_ = reinterpret_cast<unsigned long long*>(mappedValuesDev_ + R * kReadBackMax);
+  }
   SL3_HIP(hipStreamSynchronize(s));
 }
 
@@ -771,6 +782,7 @@ Viscoelastic3D::~Viscoelastic3D() {
     if (e != nullptr) (void)hipEventDestroy(e);
   if (readEvent_ != nullptr) (void)hipEventDestroy(readEvent_);
   if (pinned_ != nullptr) (void)hipHostFree(pinned_);
+  if (mapped_ != nullptr) (void)hipHostFree(mapped_);
 }
 
 // Copies `count` doubles of a device result to the host through pinned
@@ -790,6 +802,33 @@ void Viscoelastic3D::ReadBack(double* dst, const double* src, int count) {
   }
   SL3_HIP(status);
   std::copy(pinned_, pinned_ + count, dst);
+}
+
+HostPublish Viscoelastic3D::Publish(int r) {
+  HostPublish p;
+  p.values = mappedValuesDev_ + static_cast<size_t>(r) * kReadBackMax;
+  p.seq = mappedSeqDev_ + r;
+  p.ticket = ++ticket_;
+  return p;
+}
+
+// Polls the ticket word; the stream's state is checked now and then, so an
+// error ends the wait, and a stream that went idle without the ticket
+// arriving is an error rather than a hang.
+const double* Viscoelastic3D::Published(int r, unsigned long long ticket) {
+  const size_t R = coupling_.Realisations();
+  auto* seq = reinterpret_cast<unsigned long long*>(mapped_ + R * kReadBackMax) + r;
+  for (unsigned long long spin = 0;; ++spin) {
+    if (__atomic_load_n(seq, __ATOMIC_ACQUIRE) == ticket) break;
+    if ((spin & 1023) == 1023) {
+      const hipError_t q = hipStreamQuery(Stream());
+      if (q == hipSuccess && __atomic_load_n(seq, __ATOMIC_ACQUIRE) != ticket) {
+        throw std::runtime_error("a reduction's result did not reach the host");
+      }
+      if (q != hipSuccess && q != hipErrorNotReady) SL3_HIP(q);
+    }
+  }
+  return mapped_ + static_cast<size_t>(r) * kReadBackMax;
 }
 
 void Viscoelastic3D::SetDelta(const double* delta_dev) {
@@ -1302,15 +1341,15 @@ void Viscoelastic3D::Amplitudes(const Channels& channels, double t) {
 
 double Viscoelastic3D::RelativeChange(const double* next, const double* now, bool everyEntry) {
   hipStream_t s = Stream();
+  const HostPublish pub = Publish(0);
   if (everyEntry) {
     LaunchRelativeChange(next, now, coupling_.SpectralDoubles(), partials_.data(), 1024,
-                         result_.data(), s);
+                         result_.data(), s, &pub);
   } else {
     LaunchRelativeChangeActive(next, now, coupling_.Runs(), partials_.data(), 1024,
-                               result_.data(), s);
+                               result_.data(), s, &pub);
   }
-  double r[3];
-  ReadBack(r, result_.data(), 3);
+  const double* r = Published(0, pub.ticket);
   if (r[2] != 0.0) return std::numeric_limits<double>::quiet_NaN();
   return r[0] > 0.0 ? std::sqrt(r[1]) / std::sqrt(r[0]) : 0.0;
 }
@@ -1739,10 +1778,10 @@ StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
     double residual;
     if (fused) {
       sw = sweepOf(m);
+      const HostPublish pub = Publish(0);
       LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(),
-                          result_.data() + 2 * width, s);
-      double r[2 * kAndersonMax + 3];
-      ReadBack(r, result_.data(), 2 * width + 3);
+                          result_.data() + 2 * width, s, &pub);
+      const double* r = Published(0, pub.ticket);
       std::copy(r, r + 2 * width, sums);
       const double* rc = r + 2 * width;
       residual = rc[2] != 0.0 ? std::numeric_limits<double>::quiet_NaN()
@@ -1790,8 +1829,11 @@ StepInfo Viscoelastic3D::OpenConverge(const PassFn& pass, const double* start,
       } else {
         if (!fused) {
           sw = sweepOf(m);
-          LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(), nullptr, s);
-          ReadBack(sums, result_.data(), 2 * width);
+          const HostPublish pub = Publish(0);
+          LaunchAndersonSweep(sw, runs, partials_.data(), 1024, result_.data(), nullptr, s,
+                              &pub);
+          const double* r = Published(0, pub.ticket);
+          std::copy(r, r + 2 * width, sums);
         }
         count = m;
         for (int i = 0; i < m; ++i) {
@@ -1888,6 +1930,7 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
     // every live realisation's residual: a fused sweep once its mixer runs
     // (the sums at [0, 2 w), the maxima after), else RelativeChange (the
     // maxima at 0; the first pass over every entry)
+    std::vector<unsigned long long> tickets(R, 0);
     for (int r = 0; r < R; ++r) {
       Control& c = ctl[r];
       if (!c.live) continue;
@@ -1895,16 +1938,26 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
       c.m = std::min(c.count + 1, memory);
       c.width = AndersonWidth(std::max(c.m, 1));
       double* out = result_.data() + static_cast<size_t>(r) * stride;
+      const HostPublish pub = Publish(r);
+      tickets[r] = pub.ticket;
       if (c.fused) {
         LaunchAndersonSweep(sweepOf(r), coupling_.Runs(r), partials_.data(), 1024, out,
-                            out + 2 * c.width, s);
+                            out + 2 * c.width, s, &pub);
       } else {
         LaunchRelativeChangeActive(sNext_.data(), s_.data(),
                                    it == 0 ? coupling_.EveryRuns(r) : coupling_.Runs(r),
-                                   partials_.data(), 1024, out, s);
+                                   partials_.data(), 1024, out, s, &pub);
       }
     }
-    ReadBack(back.data(), result_.data(), stride * R);
+    auto collect = [&]() {
+      for (int r = 0; r < R; ++r) {
+        if (tickets[r] == 0) continue;
+        const double* v = Published(r, tickets[r]);
+        std::copy(v, v + stride, back.data() + static_cast<size_t>(r) * stride);
+        tickets[r] = 0;
+      }
+    };
+    collect();
     double largest = 0.0;
     for (int r = 0; r < R; ++r) {
       Control& c = ctl[r];
@@ -1945,12 +1998,14 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
       }
       c.previous = c.residual;
       if (c.mixing && c.started && !c.fused) {
+        const HostPublish pub = Publish(r);
+        tickets[r] = pub.ticket;
         LaunchAndersonSweep(sweepOf(r), coupling_.Runs(r), partials_.data(), 1024,
-                            result_.data() + static_cast<size_t>(r) * stride, nullptr, s);
+                            result_.data() + static_cast<size_t>(r) * stride, nullptr, s, &pub);
         sweeps = true;
       }
     }
-    if (sweeps) ReadBack(back.data(), result_.data(), stride * R);
+    if (sweeps) collect();
     for (int r = 0; r < R; ++r) {
       Control& c = ctl[r];
       if (!c.live) continue;

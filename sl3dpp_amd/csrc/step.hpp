@@ -487,6 +487,17 @@ class Viscoelastic3D {
   double* pinned_ = nullptr;
   hipEvent_t readEvent_ = nullptr;
   void ReadBack(double* dst, const double* src, int count);
+  // The residual's and the sweep's results reach the host without a copy:
+  // a kernel writes them to host-mapped memory (per realisation
+  // kReadBackMax doubles and a ticket word), the host polls the ticket
+  // (Publish, then Published).
+  double* mapped_ = nullptr;  // host-mapped: values, then the ticket words
+  double* mappedValuesDev_ = nullptr;
+  unsigned long long* mappedSeqDev_ = nullptr;
+  unsigned long long ticket_ = 0;
+  HostPublish Publish(int r);
+  // Waits for realisation r's ticket `ticket`; its values then.
+  const double* Published(int r, unsigned long long ticket);
   // The pass's kernels on the stream (what Pass captures).
   void LaunchPass(const double* s, double* sNext, bool useStartForce, Flavour& f);
   // A pass captured as a graph, keyed by the tensors it reads and writes,
