@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--learned-epochs", type=int, default=50)
     ap.add_argument("--extrapolate-order", type=int, default=4,
                     help="order of the extrapolated-start trajectory leg, 1 to 5 (0 skips it)")
+    ap.add_argument("--side-by-side", type=int, default=4,
+                    help="realisations of the leg that steps several delta seeds side by side "
+                         "in one operator, plain and with extrapolated starts (0 skips it)")
     ap.add_argument("--uneven", type=int, default=0,
                     help="an uneven batch of this many independent realisations (lmax from "
                          "{32, 64, 96, 128, 192}, meshes of {64, 96, 128} elements x degree 5, "
@@ -416,6 +419,9 @@ def main():
     extrapolated = None
     if args.extrapolate_order > 0 and not cg:
         extrapolated = extrapolated_leg(op, dt, load, args, steps_per_s, plain_final)
+    side = None
+    if args.side_by_side > 1 and not cg:
+        side = side_by_side_leg(earth, lmax, dt, load, args, steps_per_s, plain_final)
     learned = None
     if args.learned_train > 0 and not cg:
         learned = learned_leg(op, dt, load, args)
@@ -454,6 +460,7 @@ def main():
         "graph_passes": graphs,
         "transforms_cut_with_depth": cut,
         "extrapolated_start": extrapolated,
+        "side_by_side": side,
         "learned": learned,
         "maxwell1d": maxwell1d,
         "kernels": breakdown,
@@ -640,6 +647,95 @@ def extrapolated_leg(op, dt, load, args, plain_steps_per_s, plain_final):
     log(f"extrapolated start later on: {5 / later_s:.2f} steps/s, passes {later}")
     log(f"extrapolated start (order {args.extrapolate_order}): {out['steps_per_s']:.2f} steps/s, "
         f"passes {passes}, x{out['speedup_vs_plain']:.2f} of plain, final state vs plain {diff:.1e}")
+    return out
+
+
+def side_by_side_leg(earth, lmax, dt, load, args, plain_steps_per_s, plain_final):
+    """The batched form: R realisations (delta seeds 2026 .. 2026 + R - 1,
+    the first the timed trajectory's own) side by side in one operator, every
+    pass kernel over all their memory nodes, each converging on its own.
+    The same trajectory (zero state, the same warm-up and timed steps), first
+    plain, then with every step started from the extrapolation in time of
+    the last converged strains (all realisations' columns at once), with
+    steps 11-15 as in the extrapolated leg. Rates are realisation-steps/s;
+    ratios against the plain single-realisation value. The first
+    realisation's plain final state must equal the timed trajectory's, bit
+    for bit."""
+    import numpy as np
+    import torch
+
+    import sl3dpp_amd
+    import sl3dpp_amd.step as st
+
+    R = args.side_by_side
+    opts = st.StepOptions.default(max_iterations=200, rtol=args.rtol, anderson_memory=6,
+                                  anderson_rate=0.5, realisations=R)
+    op = st.Viscoelastic3D(earth, lmax, None, opts)
+    N = op.radial_nodes
+    coef = np.concatenate([st.delta_coefficients(min(64, lmax), N, seed=2026 + i, rms=0.7,
+                                                 ar=0.9, lmax=lmax) for i in range(R)])
+    op.set_delta(op.delta_field(coef))
+
+    def timed(step, state, count):
+        passes = []
+        torch.cuda.synchronize()
+        sl3dpp_amd.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(count):
+            info = step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+            assert info.converged, "a side-by-side step did not converge"
+            passes.append([i.iterations for i in op.realisation_infos()])
+        sl3dpp_amd.synchronize()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, passes
+
+    state = op.zero_state()
+    for _ in range(args.warmup):
+        op.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    plain_s, plain_passes = timed(op.step, state, args.steps)
+    first = state.export("coefficients").reshape(-1)
+    # the reference layout is per block [node][coefficient]: realisation 0's
+    # share of each block is its first N nodes' rows
+    same = True
+    at, at1 = 0, 0
+    for b in range(3):
+        size = 2 * ((lmax + 1) ** 2 - (4 if b == 0 else 1) if b < 2 else (lmax + 1) * (lmax + 2) // 2)
+        block = first[at:at + R * N * size].reshape(R * N, size)
+        same = same and np.array_equal(block[:N].ravel(), plain_final[at1:at1 + N * size])
+        at += R * N * size
+        at1 += N * size
+    del state
+    state = op.zero_state()
+    ws = st.ExtrapolatedStart(op, order=max(args.extrapolate_order, 1))
+    for _ in range(args.warmup):
+        ws.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    extra_s, extra_passes = timed(ws.step, state, args.steps)
+    for _ in range(max(0, 10 - args.warmup - args.steps)):
+        ws.step(state, dt, st.EXPONENTIAL_TRAPEZOID, load)
+    later_s, later_passes = timed(ws.step, state, 5)
+    total = R * args.steps
+    out = {
+        "realisations": R,
+        "plain_steps_per_s": total / plain_s,
+        "plain_passes": plain_passes,
+        "plain_speedup_vs_plain": total / plain_s / plain_steps_per_s,
+        "first_equals_timed_trajectory": bool(same),
+        "extrapolated_order": max(args.extrapolate_order, 1),
+        "extrapolated_steps_per_s": total / extra_s,
+        "extrapolated_passes": extra_passes,
+        "extrapolated_speedup_vs_plain": total / extra_s / plain_steps_per_s,
+        "later_steps": {"first": max(args.warmup + args.steps, 10) + 1, "count": 5,
+                        "steps_per_s": 5 * R / later_s, "passes": later_passes,
+                        "speedup_vs_plain": 5 * R / later_s / plain_steps_per_s},
+        "what": ("R realisations side by side in one operator (delta seeds 2026..), the timed "
+                 "trajectory plain and with extrapolated starts; rates in realisation-steps/s, "
+                 "ratios against the single-realisation plain value"),
+    }
+    log(f"side by side ({R}): plain {out['plain_steps_per_s']:.2f} realisation-steps/s "
+        f"(x{out['plain_speedup_vs_plain']:.2f}), extrapolated {out['extrapolated_steps_per_s']:.2f} "
+        f"(x{out['extrapolated_speedup_vs_plain']:.2f}), later {out['later_steps']['steps_per_s']:.2f} "
+        f"(x{out['later_steps']['speedup_vs_plain']:.2f}); first realisation equal: {same}")
+    assert same, "the first side-by-side realisation differs from the timed trajectory"
     return out
 
 

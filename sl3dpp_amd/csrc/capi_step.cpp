@@ -904,6 +904,14 @@ int sl3_extrapolated_step(sl3_extrapolator* ex, sl3_state* state, double dt, int
     // another time since, starts a new history
     if (!ex->recorded.empty() && (ex->last != state->id || ex->recorded.back().first != t0))
       ex->recorded.clear();
+    if (ex->recorded.empty() && scheme != SL3_SDIRK2 && dt > 0.0) {
+      // a new history starts at the strain the state's step starts from, at
+      // the state's time: the second step is then extrapolated already
+      ex->op->op->StartStrain(*state->state, WrapChannels(load, tide, user),
+                              ex->buffers[0].data());
+      ex->recorded.emplace_back(t0, 0);
+      ex->last = state->id;
+    }
     if (scheme == SL3_SDIRK2 || !(dt > 0.0) || ex->recorded.size() < 2) return;
     // Lagrange extrapolation through the recorded times to t0 + dt
     const double target = t0 + dt;

@@ -1446,6 +1446,22 @@ void Viscoelastic3D::ElasticStrainCoefficients(DeviceState& state, const Channel
                f.unitsMem.data() + dmHost_.size(), TideAmplitudes(), coupling_.View(out), s);
 }
 
+void Viscoelastic3D::StartStrain(DeviceState& state, const Channels& channels, double* out) {
+  if (open_.state != nullptr) throw ArgumentError("a step is open on this operator");
+  hipStream_t s = Stream();
+  if (state.hasCoefficients) {
+    LaunchCopy(out, state.coefficients.data(), coupling_.SpectralDoubles(), s);
+  } else if (state.hasStrain) {
+    double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
+    const double* grid[3] = {state.strain[0].data(), state.strain[1].data(),
+                             state.strain[2].data()};
+    coupling_.Analyse(grid, out, B, s);
+  } else {
+    ElasticStrainCoefficients(state, channels, out);
+  }
+  SL3_HIP(hipStreamSynchronize(s));
+}
+
 StepInfo Viscoelastic3D::Step(DeviceState& state, double dt, int scheme,
                               const Channels& channels, const double* start,
                               double warmGuard) {

@@ -279,6 +279,11 @@ class Viscoelastic3D {
   // non-finite one), the step drops it and converges from its own start.
   StepInfo Step(DeviceState& state, double dt, int scheme, const Channels& channels,
                 const double* start = nullptr, double warmGuard = 0.0);
+  // The strain coefficients a step from `state` starts from (StartStep's
+  // s0, src/Viscoelastic3D.cpp:1252-1263): the state's coefficients, the
+  // analysis of its strain grid, or the elastic strain of its memory and the
+  // channels at its time; into out (device layout). No step may be open.
+  void StartStrain(DeviceState& state, const Channels& channels, double* out);
   // The last pass's solutions on every dof, unscaled with channels, per
   // task (band solves of the last pass's force): host export for the
   // surface outputs and tests.
