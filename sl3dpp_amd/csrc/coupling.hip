@@ -601,8 +601,8 @@ __device__ inline void Store2(double* v, long long i, double2 x) {
   *reinterpret_cast<double2*>(v + i) = x;
 }
 
-__global__ void AndersonFirstKernel(const double* x, const double* gx, double* xPrev,
-                                    double* fPrev, ActiveRuns a) {
+__device__ inline void FirstBody(const double* x, const double* gx, double* xPrev,
+                                 double* fPrev, const ActiveRuns& a) {
   ForActivePairs(a, [&](long long c, long long j) {
     const double2 xv = Load2(x, j), gv = Load2(gx, j);
     Store2(xPrev, c, xv);
@@ -610,25 +610,56 @@ __global__ void AndersonFirstKernel(const double* x, const double* gx, double* x
   });
 }
 
+__global__ void AndersonFirstKernel(const double* x, const double* gx, double* xPrev,
+                                    double* fPrev, ActiveRuns a) {
+  FirstBody(x, gx, xPrev, fPrev, a);
+}
+
 __global__ void CopyRunsKernel(double* dst, const double* src, ActiveRuns a) {
   ForActivePairs(a, [&](long long, long long j) { Store2(dst, j, Load2(src, j)); });
+}
+
+// Entry e of a batch: realisation r's runs and Anderson history.
+__device__ inline ActiveRuns EntryRuns(const ActiveRuns& a, int r) {
+  ActiveRuns v = a;
+  v.full += static_cast<size_t>(r) * a.runs;
+  return v;
+}
+// An entry's history within the base sweep's buffers: dF and dG moved by
+// `history`, xPrev and fPrev by `prev` doubles, its own m and slot (the
+// base stays in the kernel arguments: indexing a modified copy of its
+// pointer arrays would put the copy in scratch memory).
+struct EntryHistory {
+  long long history = 0, prev = 0;
+  int m = 0, slot = 0;
+};
+__device__ inline EntryHistory EntryOf(const RealisationBatch& b, int e) {
+  EntryHistory h;
+  const long long r = b.r[e];
+  h.history = r * b.historyStride;
+  h.prev = r * b.prevStride;
+  h.m = b.m[e];
+  h.slot = b.slot[e];
+  return h;
 }
 
 // RESIDUAL: the pass's RelativeChange (max |gx|^2, max |gx - x|^2, a
 // non-finite flag) from the same loads, into residualPartials (3 per block).
 // W: the sums' width, 8 or 16 (AndersonWidth).
 template <bool RESIDUAL, int W>
-__global__ void __launch_bounds__(kThreads)
-    AndersonSweepKernel(AndersonSweep s, ActiveRuns a, double* partials,
-                        double* residualPartials) {
+__device__ inline void SweepBody(const AndersonSweep& s, const EntryHistory& h,
+                                 const ActiveRuns& a, double* partials,
+                                 double* residualPartials) {
   __shared__ double red[kThreads][2 * W + 1];
   double col[W] = {}, gam[W] = {};
   double scale = 0.0, diff = 0.0, bad = 0.0;
-  double* dFs = s.dF[s.slot];
-  double* dGs = s.dG[s.slot];
+  double* dFs = s.dF[h.slot] + h.history;
+  double* dGs = s.dG[h.slot] + h.history;
+  double* xPrev = s.xPrev + h.prev;
+  double* fPrev = s.fPrev + h.prev;
   ForActivePairs(a, [&](long long c, long long j) {
     const double2 xs = Load2(s.x, j), gx = Load2(s.gx, j);
-    const double2 xp = Load2(s.xPrev, c), fp = Load2(s.fPrev, c);
+    const double2 xp = Load2(xPrev, c), fp = Load2(fPrev, c);
     const double2 f = make_double2(gx.x - xs.x, gx.y - xs.y);
     const double2 df = make_double2(f.x - fp.x, f.y - fp.y);
     if constexpr (RESIDUAL) {
@@ -638,12 +669,12 @@ __global__ void __launch_bounds__(kThreads)
     }
     Store2(dFs, c, df);
     Store2(dGs, c, make_double2(xs.x - xp.x + df.x, xs.y - xp.y + df.y));
-    Store2(s.xPrev, c, xs);
-    Store2(s.fPrev, c, f);
+    Store2(xPrev, c, xs);
+    Store2(fPrev, c, f);
 #pragma unroll
     for (int i = 0; i < W; ++i) {
-      if (i < s.m) {
-        const double2 di = i == s.slot ? df : Load2(s.dF[i], c);
+      if (i < h.m) {
+        const double2 di = i == h.slot ? df : Load2(s.dF[i] + h.history, c);
         col[i] += di.x * df.x + di.y * df.y;
         gam[i] += di.x * f.x + di.y * f.y;
       }
@@ -679,10 +710,31 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// RelativeChange over the active entries (the others are zero in both).
+template <bool RESIDUAL, int W>
 __global__ void __launch_bounds__(kThreads)
-    RelativeChangeActiveKernel(const double* next, const double* now, ActiveRuns a,
-                               double* partials) {
+    AndersonSweepKernel(AndersonSweep s, ActiveRuns a, double* partials,
+                        double* residualPartials) {
+  EntryHistory h;
+  h.m = s.m;
+  h.slot = s.slot;
+  SweepBody<RESIDUAL, W>(s, h, a, partials, residualPartials);
+}
+
+// Every entry's sweep (blockIdx.y the entry), its partials at e entryStride:
+// the sums' partialBlocks x 2 W, then (RESIDUAL) the maxima's x 3.
+template <bool RESIDUAL, int W>
+__global__ void __launch_bounds__(kThreads)
+    AndersonSweepBatchKernel(AndersonSweep base, ActiveRuns a, RealisationBatch b,
+                             double* partials, size_t entryStride) {
+  const int e = blockIdx.y;
+  double* p = partials + e * entryStride;
+  SweepBody<RESIDUAL, W>(base, EntryOf(b, e), EntryRuns(a, b.r[e]), p,
+                         p + static_cast<size_t>(gridDim.x) * 2 * W);
+}
+
+// RelativeChange over the active entries (the others are zero in both).
+__device__ inline void RelativeChangeBody(const double* next, const double* now,
+                                          const ActiveRuns& a, double* partials) {
   __shared__ double s1[kThreads], s2[kThreads], s3[kThreads];
   double scale = 0.0, diff = 0.0, bad = 0.0;
   ForActivePairs(a, [&](long long, long long j) {
@@ -711,20 +763,116 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+__global__ void __launch_bounds__(kThreads)
+    RelativeChangeActiveKernel(const double* next, const double* now, ActiveRuns a,
+                               double* partials) {
+  RelativeChangeBody(next, now, a, partials);
+}
+
+// Every entry's RelativeChange (over its runs, or every[e]: over every entry
+// of its columns), partials at e entryStride.
+__global__ void __launch_bounds__(kThreads)
+    RelativeChangeBatchKernel(const double* next, const double* now, ActiveRuns runs,
+                              ActiveRuns every, RealisationBatch b, double* partials,
+                              size_t entryStride) {
+  const int e = blockIdx.y;
+  RelativeChangeBody(next, now, EntryRuns(b.every[e] ? every : runs, b.r[e]),
+                     partials + e * entryStride);
+}
+
 struct Gamma {
   double v[kAndersonMax];
 };
 
-__global__ void AndersonMixKernel(AndersonSweep s, Gamma gamma, ActiveRuns a) {
+// gamma: the entry's m values (held in registers before the sweep).
+__device__ inline void MixBody(const AndersonSweep& s, const EntryHistory& h,
+                               const double* gamma, const ActiveRuns& a) {
+  double g[kAndersonMax];
+#pragma unroll
+  for (int i = 0; i < kAndersonMax; ++i) g[i] = i < h.m ? gamma[i] : 0.0;
   ForActivePairs(a, [&](long long c, long long j) {
     double2 value = Load2(s.gx, j);
-    for (int i = 0; i < s.m; ++i) {
-      const double2 d = Load2(s.dG[i], c);
-      value.x -= gamma.v[i] * d.x;
-      value.y -= gamma.v[i] * d.y;
+#pragma unroll
+    for (int i = 0; i < kAndersonMax; ++i) {
+      if (i < h.m) {
+        const double2 d = Load2(s.dG[i] + h.history, c);
+        value.x -= g[i] * d.x;
+        value.y -= g[i] * d.y;
+      }
     }
     Store2(s.x, j, value);
   });
+}
+
+__global__ void AndersonMixKernel(AndersonSweep s, Gamma gamma, ActiveRuns a) {
+  EntryHistory h;
+  h.m = s.m;
+  MixBody(s, h, gamma.v, a);
+}
+
+// Every entry's mix, its gamma at gammas + e kAndersonMax (device).
+__global__ void AndersonMixBatchKernel(AndersonSweep base, RealisationBatch b,
+                                       const double* gammas, ActiveRuns a) {
+  const int e = blockIdx.y;
+  MixBody(base, EntryOf(b, e), gammas + e * kAndersonMax, EntryRuns(a, b.r[e]));
+}
+
+// Every entry's first mixed pass (the history of base moved to its
+// realisation); x = gx follows in a launch of its own (the two partition the
+// entries differently: one launch could overwrite x before another
+// workgroup reads it).
+__global__ void AndersonFirstBatchKernel(AndersonSweep base, RealisationBatch b, ActiveRuns a) {
+  const int e = blockIdx.y;
+  const EntryHistory h = EntryOf(b, e);
+  FirstBody(base.x, base.gx, base.xPrev + h.prev, base.fPrev + h.prev, EntryRuns(a, b.r[e]));
+}
+
+__global__ void CopyRunsBatchKernel(double* dst, const double* src, ActiveRuns every,
+                                    RealisationBatch b) {
+  const int e = blockIdx.y;
+  ForActivePairs(EntryRuns(every, b.r[e]),
+                 [&](long long, long long j) { Store2(dst, j, Load2(src, j)); });
+}
+
+// ReducePartialsKernel for every entry (blockIdx.y): column c of entry e's
+// partials (at e entryStride) into out[r stride + offset + c].
+__global__ void __launch_bounds__(kThreads)
+    ReducePartialsBatchKernel(const double* partials, size_t entryStride, int blocks, int width,
+                              int useMax, RealisationBatch b, double* out, int stride,
+                              int offset) {
+  __shared__ double red[kThreads];
+  const int c = blockIdx.x, e = blockIdx.y;
+  const double* p = partials + e * entryStride;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < blocks; k += blockDim.x) {
+    const double v = p[k * width + c];
+    s = useMax ? fmax(s, v) : s + v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double x = red[threadIdx.x], y = red[threadIdx.x + w];
+      red[threadIdx.x] = useMax ? fmax(x, y) : x + y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b.r[e] * stride + offset + c] = red[0];
+}
+
+// Every entry's n results (result + r stride) to the host-mapped values +
+// r valuesStride, then the ticket to seq[r].
+__global__ void PublishBatchKernel(const double* result, int stride, int n, RealisationBatch b,
+                                   HostPublish pub, int valuesStride) {
+  const int r = b.r[blockIdx.x];
+  if (static_cast<int>(threadIdx.x) < n) {
+    pub.values[r * valuesStride + threadIdx.x] = result[r * stride + threadIdx.x];
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(pub.seq + r, pub.ticket, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 __device__ inline void RefIndexToLm(int spin, long long idx, int& l, int& m) {
@@ -961,6 +1109,104 @@ void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double*
     if (residual && residual != sums + 2 * W) throw ArgumentError("published residual follows the sums");
     LaunchPublish(sums, 2 * W + (residual ? 3 : 0), *pub, stream);
   }
+}
+
+void LaunchRelativeChangeBatch(const double* next, const double* now, const ActiveRuns& runs,
+                               const ActiveRuns& every, const RealisationBatch& b,
+                               double* partials, int partialBlocks, double* result, int stride,
+                               hipStream_t stream) {
+  if (b.count == 0) return;
+  if (b.count > kBatchMax) throw ArgumentError("at most 64 realisations per batch");
+  ProfileScope prof("residual", stream);
+  const size_t entryStride = static_cast<size_t>(partialBlocks) * (2 * kAndersonMax + 3);
+  hipLaunchKernelGGL(RelativeChangeBatchKernel, dim3(partialBlocks, b.count), dim3(kThreads), 0,
+                     stream, next, now, runs, every, b, partials, entryStride);
+  SL3_HIP(hipGetLastError());
+  hipLaunchKernelGGL(ReducePartialsBatchKernel, dim3(3, b.count), dim3(kThreads), 0, stream,
+                     partials, entryStride, partialBlocks, 3, 1, b, result, stride, 0);
+  SL3_HIP(hipGetLastError());
+}
+
+int LaunchAndersonSweepBatch(const AndersonSweep& base, const ActiveRuns& runs,
+                             const RealisationBatch& b, bool residual, double* partials,
+                             int partialBlocks, double* result, int stride, hipStream_t stream) {
+  if (b.count == 0) return 0;
+  if (b.count > kBatchMax) throw ArgumentError("at most 64 realisations per batch");
+  int mMax = 1;
+  for (int e = 0; e < b.count; ++e) {
+    if (b.m[e] < 1 || b.m[e] > kAndersonMax) throw ArgumentError("Anderson history of 1 to 16");
+    mMax = std::max(mMax, b.m[e]);
+  }
+  ProfileScope prof("anderson", stream);
+  const int W = AndersonWidth(mMax);
+  const size_t entryStride = static_cast<size_t>(partialBlocks) * (2 * kAndersonMax + 3);
+  const dim3 grid(partialBlocks, b.count);
+  if (W == 8) {
+    if (residual) {
+      hipLaunchKernelGGL((AndersonSweepBatchKernel<true, 8>), grid, dim3(kThreads), 0, stream,
+                         base, runs, b, partials, entryStride);
+    } else {
+      hipLaunchKernelGGL((AndersonSweepBatchKernel<false, 8>), grid, dim3(kThreads), 0, stream,
+                         base, runs, b, partials, entryStride);
+    }
+  } else {
+    if (residual) {
+      hipLaunchKernelGGL((AndersonSweepBatchKernel<true, kAndersonMax>), grid, dim3(kThreads), 0,
+                         stream, base, runs, b, partials, entryStride);
+    } else {
+      hipLaunchKernelGGL((AndersonSweepBatchKernel<false, kAndersonMax>), grid, dim3(kThreads),
+                         0, stream, base, runs, b, partials, entryStride);
+    }
+  }
+  SL3_HIP(hipGetLastError());
+  hipLaunchKernelGGL(ReducePartialsBatchKernel, dim3(2 * W, b.count), dim3(kThreads), 0, stream,
+                     partials, entryStride, partialBlocks, 2 * W, 0, b, result, stride, 0);
+  SL3_HIP(hipGetLastError());
+  if (residual) {
+    hipLaunchKernelGGL(ReducePartialsBatchKernel, dim3(3, b.count), dim3(kThreads), 0, stream,
+                       partials + static_cast<size_t>(partialBlocks) * 2 * W, entryStride,
+                       partialBlocks, 3, 1, b, result, stride, 2 * W);
+    SL3_HIP(hipGetLastError());
+  }
+  return W;
+}
+
+void LaunchAndersonMixBatch(const AndersonSweep& base, const ActiveRuns& runs,
+                            const RealisationBatch& b, const double* gammas,
+                            hipStream_t stream) {
+  if (b.count == 0) return;
+  ProfileScope prof("anderson", stream);
+  hipLaunchKernelGGL(AndersonMixBatchKernel, dim3(2048, b.count), dim3(kThreads), 0, stream,
+                     base, b, gammas, runs);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchAndersonFirstBatch(const AndersonSweep& base, const ActiveRuns& runs,
+                              const ActiveRuns& every, const RealisationBatch& b,
+                              hipStream_t stream) {
+  if (b.count == 0) return;
+  ProfileScope prof("anderson", stream);
+  hipLaunchKernelGGL(AndersonFirstBatchKernel, dim3(1024, b.count), dim3(kThreads), 0, stream,
+                     base, b, runs);
+  SL3_HIP(hipGetLastError());
+  LaunchCopyRunsBatch(base.x, base.gx, every, b, stream);
+}
+
+void LaunchCopyRunsBatch(double* dst, const double* src, const ActiveRuns& every,
+                         const RealisationBatch& b, hipStream_t stream) {
+  if (b.count == 0) return;
+  hipLaunchKernelGGL(CopyRunsBatchKernel, dim3(1024, b.count), dim3(kThreads), 0, stream, dst,
+                     src, every, b);
+  SL3_HIP(hipGetLastError());
+}
+
+void LaunchPublishBatch(const double* result, int stride, int n, const RealisationBatch& b,
+                        const HostPublish& pub, int valuesStride, hipStream_t stream) {
+  if (b.count == 0) return;
+  if (n < 0 || n > 64) throw ArgumentError("publish at most 64 doubles");
+  hipLaunchKernelGGL(PublishBatchKernel, dim3(b.count), dim3(64), 0, stream, result, stride, n, b,
+                     pub, valuesStride);
+  SL3_HIP(hipGetLastError());
 }
 
 void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,

@@ -165,6 +165,50 @@ void LaunchAndersonSweep(const AndersonSweep& s, const ActiveRuns& runs, double*
 void LaunchAndersonMix(const AndersonSweep& s, const double* gamma, const ActiveRuns& runs,
                        hipStream_t stream);
 
+// Several realisations' vector sweeps in one launch (realisations side by
+// side, every realisation's compact vector laid out alike): entry e works
+// on realisation r[e], whose runs are the given runs' full offsets moved by
+// r[e] runs, whose Anderson history is the base sweep's moved by r[e]
+// strides, with its own m and slot (sweeps and mixes) and, for its
+// RelativeChange, every[e]: over every entry of its columns. Each entry's
+// arithmetic is its single-realisation launch's, bit for bit.
+constexpr int kBatchMax = 64;
+struct RealisationBatch {
+  int count = 0;
+  long long prevStride = 0;     // doubles between realisations' xPrev, and fPrev
+  long long historyStride = 0;  // doubles between realisations' dF[i], and dG[i]
+  int r[kBatchMax] = {};
+  int m[kBatchMax] = {};
+  int slot[kBatchMax] = {};
+  int every[kBatchMax] = {};
+};
+// partials: per entry partialBlocks x (2 kAndersonMax + 3) doubles. Results
+// go to result + r stride: RelativeChange's three maxima at 0; a sweep's
+// sums at [0, 2 W) and, with residual, its maxima after them; W =
+// AndersonWidth of the entries' largest m, returned.
+void LaunchRelativeChangeBatch(const double* next, const double* now, const ActiveRuns& runs,
+                               const ActiveRuns& every, const RealisationBatch& b,
+                               double* partials, int partialBlocks, double* result, int stride,
+                               hipStream_t stream);
+int LaunchAndersonSweepBatch(const AndersonSweep& base, const ActiveRuns& runs,
+                             const RealisationBatch& b, bool residual, double* partials,
+                             int partialBlocks, double* result, int stride, hipStream_t stream);
+// gammas: per entry kAndersonMax doubles (device).
+void LaunchAndersonMixBatch(const AndersonSweep& base, const ActiveRuns& runs,
+                            const RealisationBatch& b, const double* gammas,
+                            hipStream_t stream);
+// The first mixed pass of every entry, x = gx over every entry of its columns.
+void LaunchAndersonFirstBatch(const AndersonSweep& base, const ActiveRuns& runs,
+                              const ActiveRuns& every, const RealisationBatch& b,
+                              hipStream_t stream);
+// dst = src over every entry of each entry's columns.
+void LaunchCopyRunsBatch(double* dst, const double* src, const ActiveRuns& every,
+                         const RealisationBatch& b, hipStream_t stream);
+// n values of every entry's result (result + r stride) to pub.values +
+// r valuesStride (host-mapped), then pub.ticket to pub.seq[r].
+void LaunchPublishBatch(const double* result, int stride, int n, const RealisationBatch& b,
+                        const HostPublish& pub, int valuesStride, hipStream_t stream);
+
 // GridCoupling::ScaleByNode (src/GridCoupling.cpp:524-540) on a device-
 // layout spectral tensor ([row][node] complex, count complex values):
 // every coefficient of a node times mu[node].

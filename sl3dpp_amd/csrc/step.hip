@@ -724,6 +724,10 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
   if (geo.realisations > 1) {
     live_.Resize(geo.realisations);
     SetAllLive();
+    const size_t g = static_cast<size_t>(geo.realisations) * kAndersonMax;
+    SL3_HIP(hipHostMalloc(reinterpret_cast<void**>(&gammaHost_), g * sizeof(double),
+                          hipHostMallocDefault));
+    gammaDev_.Resize(g);
   }
   const size_t G = coupling_.GridValueCount();
   SetDelta(delta_dev);
@@ -749,7 +753,8 @@ Viscoelastic3D::Viscoelastic3D(std::shared_ptr<const Earth> earthPtr, int lmax,
   ampDoubles_ = static_cast<size_t>(lmax + 1) * (lmax + 2);
   amp_.Resize(2 * ampDoubles_);
   // per block the sweep's sums (at most 2 kAndersonMax) and the residual's 3 maxima
-  partials_.Resize(1024 * (2 * kAndersonMax + 3));
+  // (per realisation side by side: their sweeps share a launch)
+  partials_.Resize(static_cast<size_t>(geo.realisations) * 1024 * (2 * kAndersonMax + 3));
   result_.Resize(static_cast<size_t>(2 * kAndersonMax + 3) * geo.realisations);
   // the Legendre kernels' tile shapes, timed on the pass's own buffers
   LaunchFill(s_.data(), S, 0.0, s);
@@ -783,6 +788,7 @@ Viscoelastic3D::~Viscoelastic3D() {
   if (readEvent_ != nullptr) (void)hipEventDestroy(readEvent_);
   if (pinned_ != nullptr) (void)hipHostFree(pinned_);
   if (mapped_ != nullptr) (void)hipHostFree(mapped_);
+  if (gammaHost_ != nullptr) (void)hipHostFree(gammaHost_);
 }
 
 // Copies `count` doubles of a device result to the host through pinned
@@ -1918,23 +1924,46 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
     c.mixing = memory > 0 && !adaptive;
     c.gram.assign(static_cast<size_t>(std::max(memory, 1)) * std::max(memory, 1), 0.0);
   }
-  auto sweepOf = [&](int r) {
-    AndersonSweep sw{};
-    sw.x = s_.data();
-    sw.gx = sNext_.data();
-    sw.xPrev = xPrev_.data() + r * C;
-    sw.fPrev = fPrev_.data() + r * C;
-    double* h = history_.data() + r * historyEach;
-    for (int i = 0; i < memory && i < kAndersonMax; ++i) {
-      sw.dF[i] = h + static_cast<size_t>(i) * C;
-      sw.dG[i] = h + static_cast<size_t>(memory + i) * C;
-    }
-    sw.m = ctl[r].m;
-    sw.slot = ctl[r].slot;
-    return sw;
+  // realisation 0's sweep pointers; an entry of a batch moves them to its
+  // realisation's history
+  AndersonSweep base{};
+  base.x = s_.data();
+  base.gx = sNext_.data();
+  base.xPrev = xPrev_.data();
+  base.fPrev = fPrev_.data();
+  for (int i = 0; i < memory && i < kAndersonMax; ++i) {
+    base.dF[i] = history_.data() + static_cast<size_t>(i) * C;
+    base.dG[i] = history_.data() + static_cast<size_t>(memory + i) * C;
+  }
+  auto batch = [&]() {
+    RealisationBatch b;
+    b.prevStride = static_cast<long long>(C);
+    b.historyStride = static_cast<long long>(historyEach);
+    return b;
   };
+  auto add = [&](RealisationBatch& b, int r, int every) {
+    const int e = b.count++;
+    b.r[e] = r;
+    b.m[e] = ctl[r].m;
+    b.slot[e] = ctl[r].slot;
+    b.every[e] = every;
+  };
+  const ActiveRuns runs = coupling_.Runs(0), every = coupling_.EveryRuns(0);
   const int stride = 2 * kAndersonMax + 3;  // one realisation's results
   std::vector<double> back(static_cast<size_t>(stride) * R);
+  // the batch's results to the host (one launch) and back
+  auto collect = [&](const RealisationBatch& b) {
+    if (b.count == 0) return;
+    HostPublish pub;
+    pub.values = mappedValuesDev_;
+    pub.seq = mappedSeqDev_;
+    pub.ticket = ++ticket_;
+    LaunchPublishBatch(result_.data(), stride, stride, b, pub, kReadBackMax, s);
+    for (int e = 0; e < b.count; ++e) {
+      const double* v = Published(b.r[e], pub.ticket);
+      std::copy(v, v + stride, back.data() + static_cast<size_t>(b.r[e]) * stride);
+    }
+  };
   for (int it = 0; it < options_.maxIterations; ++it) {
     if (std::none_of(ctl.begin(), ctl.end(), [](const Control& c) { return c.live; })) break;
     if (pass) {
@@ -1945,39 +1974,31 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
     }
     // every live realisation's residual: a fused sweep once its mixer runs
     // (the sums at [0, 2 w), the maxima after), else RelativeChange (the
-    // maxima at 0; the first pass over every entry)
-    std::vector<unsigned long long> tickets(R, 0);
+    // maxima at 0; the first pass over every entry); one launch of each
+    RealisationBatch fused = batch(), plain = batch(), live = batch();
     for (int r = 0; r < R; ++r) {
       Control& c = ctl[r];
       if (!c.live) continue;
       c.fused = c.mixing && c.started && options_.damping == 1.0;
       c.m = std::min(c.count + 1, memory);
-      c.width = AndersonWidth(std::max(c.m, 1));
-      double* out = result_.data() + static_cast<size_t>(r) * stride;
-      const HostPublish pub = Publish(r);
-      tickets[r] = pub.ticket;
       if (c.fused) {
-        LaunchAndersonSweep(sweepOf(r), coupling_.Runs(r), partials_.data(), 1024, out,
-                            out + 2 * c.width, s, &pub);
+        add(fused, r, 0);
       } else {
-        LaunchRelativeChangeActive(sNext_.data(), s_.data(),
-                                   it == 0 ? coupling_.EveryRuns(r) : coupling_.Runs(r),
-                                   partials_.data(), 1024, out, s, &pub);
+        add(plain, r, it == 0 ? 1 : 0);
       }
+      add(live, r, 0);
     }
-    auto collect = [&]() {
-      for (int r = 0; r < R; ++r) {
-        if (tickets[r] == 0) continue;
-        const double* v = Published(r, tickets[r]);
-        std::copy(v, v + stride, back.data() + static_cast<size_t>(r) * stride);
-        tickets[r] = 0;
-      }
-    };
-    collect();
+    const int fusedWidth = LaunchAndersonSweepBatch(base, runs, fused, true, partials_.data(),
+                                                    1024, result_.data(), stride, s);
+    LaunchRelativeChangeBatch(sNext_.data(), s_.data(), runs, every, plain, partials_.data(), 1024,
+                              result_.data(), stride, s);
+    collect(live);
     double largest = 0.0;
+    RealisationBatch done = batch();
     for (int r = 0; r < R; ++r) {
       Control& c = ctl[r];
       if (!c.live) continue;
+      if (c.fused) c.width = fusedWidth;
       const double* rc = back.data() + static_cast<size_t>(r) * stride + (c.fused ? 2 * c.width : 0);
       c.residual = rc[2] != 0.0 ? std::numeric_limits<double>::quiet_NaN()
                    : rc[0] > 0.0 ? std::sqrt(rc[1]) / std::sqrt(rc[0])
@@ -1991,11 +2012,12 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
       } else if (c.residual <= options_.rtol) {
         c.info.converged = true;
         c.live = false;
-        LaunchCopyRuns(s_.data(), sNext_.data(), coupling_.EveryRuns(r), s);
+        add(done, r, 0);
       }
       // the next passes skip a realisation that stopped
       if (!c.live) SL3_HIP(hipMemsetAsync(live_.data() + r, 0, sizeof(int), s));
     }
+    LaunchCopyRunsBatch(s_.data(), sNext_.data(), every, done, s);
     residualHistory.push_back(largest);
     if (options_.damping != 1.0) {
       ProfileScope prof("damp", s);
@@ -2004,7 +2026,7 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
       SL3_HIP(hipGetLastError());
     }
     // the sweeps a damped pass takes after its residual
-    bool sweeps = false;
+    RealisationBatch damped = batch();
     for (int r = 0; r < R; ++r) {
       Control& c = ctl[r];
       if (!c.live) continue;
@@ -2013,25 +2035,26 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
         c.mixing = true;
       }
       c.previous = c.residual;
-      if (c.mixing && c.started && !c.fused) {
-        const HostPublish pub = Publish(r);
-        tickets[r] = pub.ticket;
-        LaunchAndersonSweep(sweepOf(r), coupling_.Runs(r), partials_.data(), 1024,
-                            result_.data() + static_cast<size_t>(r) * stride, nullptr, s, &pub);
-        sweeps = true;
-      }
+      if (c.mixing && c.started && !c.fused) add(damped, r, 0);
     }
-    if (sweeps) collect();
+    if (damped.count > 0) {
+      const int w = LaunchAndersonSweepBatch(base, runs, damped, false, partials_.data(), 1024,
+                                             result_.data(), stride, s);
+      for (int e = 0; e < damped.count; ++e) ctl[damped.r[e]].width = w;
+      collect(damped);
+    }
+    // each realisation's move: its Picard image, its first mixed pass, or
+    // its mix with the gamma solved here (uploaded for all at once)
+    RealisationBatch picard = batch(), first = batch(), mix = batch();
     for (int r = 0; r < R; ++r) {
       Control& c = ctl[r];
       if (!c.live) continue;
       if (!c.mixing) {
-        LaunchCopyRuns(s_.data(), sNext_.data(), coupling_.EveryRuns(r), s);
+        add(picard, r, 0);
         continue;
       }
       if (!c.started) {
-        LaunchAndersonFirst(s_.data(), sNext_.data(), xPrev_.data() + r * C,
-                            fPrev_.data() + r * C, coupling_.Runs(r), coupling_.EveryRuns(r), s);
+        add(first, r, 0);
         c.started = true;
       } else {
         const double* sums = back.data() + static_cast<size_t>(r) * stride;
@@ -2041,15 +2064,23 @@ StepInfo Viscoelastic3D::ConvergeBatch(const PassFn& pass, const double* start,
           c.gram[c.slot * memory + i] = sums[i];
           c.gram[i * memory + c.slot] = sums[i];
         }
-        const AndersonSweep sw = sweepOf(r);  // the slot the sweep wrote
-        c.slot = (c.slot + 1) % memory;
-        double g[kAndersonMax * kAndersonMax], gamma[kAndersonMax];
+        double g[kAndersonMax * kAndersonMax];
         for (int i = 0; i < m; ++i)
           for (int j = 0; j < m; ++j) g[i * m + j] = c.gram[i * memory + j];
+        double* gamma = gammaHost_ + static_cast<size_t>(mix.count) * kAndersonMax;
+        std::fill(gamma, gamma + kAndersonMax, 0.0);
         GramLeastSquares(m, g, sums + c.width, gamma);
-        LaunchAndersonMix(sw, gamma, coupling_.Runs(r), s);
+        add(mix, r, 0);  // m as swept; the slot no longer matters to the mix
+        c.slot = (c.slot + 1) % memory;
       }
       ++c.info.mixedPasses;
+    }
+    LaunchCopyRunsBatch(s_.data(), sNext_.data(), every, picard, s);
+    LaunchAndersonFirstBatch(base, runs, every, first, s);
+    if (mix.count > 0) {
+      SL3_HIP(hipMemcpyAsync(gammaDev_.data(), gammaHost_,
+                             sizeof(double) * kAndersonMax * mix.count, hipMemcpyHostToDevice, s));
+      LaunchAndersonMixBatch(base, runs, mix, gammaDev_.data(), s);
     }
   }
   SetAllLive();

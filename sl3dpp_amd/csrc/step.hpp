@@ -396,6 +396,10 @@ class Viscoelastic3D {
   // (a pass skips the others); every one nonzero outside ConvergeBatch
   DeviceArray<int> live_;
   const int* Live() const { return coupling_.Realisations() > 1 ? live_.data() : nullptr; }
+  // the mixes' gammas of one pass (per realisation kAndersonMax): pinned on
+  // the host, uploaded in one copy
+  double* gammaHost_ = nullptr;
+  DeviceArray<double> gammaDev_;
   void SetAllLive();
   // OpenStep::Forces + Solve + StrainOf (src/Viscoelastic3D.cpp:2008-2015):
   // the step's system by conjugate gradients on the memory dofs, its strain
