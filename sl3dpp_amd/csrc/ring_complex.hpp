@@ -358,5 +358,237 @@ __device__ inline void ComplexPassBody(const RingLaunch& a, const double2* __res
   }
 }
 
+// The forward half of the pass from the lane's point values (y at n1 = j,
+// P - j and 0, each at n1 + P n2 of its ring): the forward butterflies, the
+// P-point forward transforms, the stores to the Fourier layout at |order| <=
+// L, scaled by (2 pi / nphi) w_theta. The same statements as the pass's.
+template <int JT>
+__device__ inline void ComplexForwardStore(const RingLaunch& a, const Lane& ln, int th, int node,
+                                           const double* cosN, const double* sinN,
+                                           const double* tabC, const double* tabS,
+                                           double (&yjr)[4 * JT], double (&yji)[4 * JT],
+                                           double (&yqr)[4 * JT], double (&yqi)[4 * JT],
+                                           double y0r, double y0i, double2* out, bool full) {
+  constexpr int KS = 4 * JT;
+  const int P = a.tables.P, H = (P - 1) / 2, L = a.lmax, nphi = a.nphi;
+  const int kmax = min(H, (L + 3) / 4);
+  const int tOut = (kmax + 15) / 16;
+  auto jOf = [&](int v) { return 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad; };
+  auto jc = [&](int v) { return full ? jOf(v) : min(jOf(v), H); };
+  const double* cosF = cosN + Opaque(0);
+  const double* sinF = sinN + Opaque(0);
+  const int m2f = Opaque(ln.m2);
+#pragma unroll
+  for (int v = 0; v < KS; ++v) {
+    const int j = jc(v);
+    ForwardButterfly(ln, yjr[v], yji[v], cosF[m2f * j], sinF[m2f * j]);
+    ForwardButterfly(ln, yqr[v], yqi[v], cosF[m2f * (P - j)], sinF[m2f * (P - j)]);
+    if (v % 2 == 1) PhaseFence();
+  }
+  ForwardButterfly(ln, y0r, y0i, 1.0, 0.0);
+  PhaseFence();
+  double sumr = 0.0, sumi = 0.0;
+#pragma unroll
+  for (int v = 0; v < KS; ++v) {
+    const double ar = yjr[v] + yqr[v], ai = yji[v] + yqi[v];
+    const double br = yjr[v] - yqr[v], bi = yji[v] - yqi[v];
+    yjr[v] = ar;
+    yji[v] = ai;
+    yqr[v] = br;
+    yqi[v] = bi;
+    sumr += ar;
+    sumi += ai;
+  }
+  const double g0r = y0r + ColumnSum(sumr), g0i = y0i + ColumnSum(sumi);
+  const double scale = ln.valid ? 2.0 * 3.14159265358979323846 / nphi * a.wtheta[th] : 0.0;
+  double2* G = out + static_cast<size_t>(node) * a.fstride;
+  const unsigned mStride = 16u * static_cast<unsigned>(a.nodes) * a.fstride;
+  auto at = [&](int m, unsigned thb) { return static_cast<unsigned>(m + L) * mStride + thb; };
+  const int m2 = Opaque(ln.m2), quad = Opaque(ln.quad);
+  const unsigned thOut = 16u * static_cast<unsigned>(Opaque(th));
+  if (ln.valid && quad == 0) StoreAt(G, at(m2, thOut), make_double2(scale * g0r, scale * g0i));
+#pragma unroll
+  for (int t = 0; t < JT; ++t) {
+    if (t >= tOut) break;
+    Acc4 Cr = {0.0, 0.0, 0.0, 0.0}, Ci = Cr, Sr = Cr, Si = Cr;
+#pragma unroll
+    for (int kb = 0; kb < KS; ++kb) {
+      const int i = (t * KS + kb) * kWave + ln.lane;
+      const double c = tabC[i], sn = tabS[i];
+      Cr = __builtin_amdgcn_mfma_f64_16x16x4f64(c, yjr[kb], Cr, 0, 0, 0);
+      Ci = __builtin_amdgcn_mfma_f64_16x16x4f64(c, yji[kb], Ci, 0, 0, 0);
+      Sr = __builtin_amdgcn_mfma_f64_16x16x4f64(sn, yqr[kb], Sr, 0, 0, 0);
+      Si = __builtin_amdgcn_mfma_f64_16x16x4f64(sn, yqi[kb], Si, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = 1 + 16 * t + 4 * q + quad;
+      const double cr = y0r + Cr[q], ci = y0i + Ci[q];
+      const int mu = m2 + 4 * k, md = m2 - 4 * k;
+      if (ln.valid && mu <= L) {
+        StoreAt(G, at(mu, thOut), make_double2(scale * (cr + Si[q]), scale * (ci - Sr[q])));
+      }
+      if (ln.valid && md >= -L) {
+        StoreAt(G, at(md, thOut), make_double2(scale * (cr - Si[q]), scale * (ci + Sr[q])));
+      }
+    }
+    PhaseFence();
+  }
+}
+
+// The step start's analyses of a complex block with whole complex values
+// per lane (src/Viscoelastic3D.cpp:1283-1365, one branch), in two launches:
+// START: from d0 (gridIn), the memory m (memIn) and itau, the weights of
+// h = dt itau, m~ = e m + alpha d0 stored (memOut), and y = W d0 (W =
+// mu (beta - beta_ref)) transformed into fourierOut2; otherwise y = mu m~ of
+// the stored memOut transformed into fourierOut. One workgroup: rings th0
+// .. th0 + 15 of one node; each lane's points are the pass's.
+template <int JT, bool FULL, bool START>
+__device__ inline void ComplexStartBody(const RingLaunch& a, int node, int th0) {
+  constexpr int KS = 4 * JT;
+  extern __shared__ double lds[];
+  const int P = a.tables.P, H = (P - 1) / 2, nphi = a.nphi;
+  Lane ln;
+  ln.lane = threadIdx.x % kWave;
+  ln.quad = ln.lane / 16;
+  const int s = ln.lane % 16;
+  ln.m2 = s & 3;
+  const int ra = ln.m2 >> 1, rb = ln.m2 & 1;
+  ln.ring = (threadIdx.x / kWave) * kRingsPerWave + (s >> 2);
+  const int th = th0 + ln.ring;
+  ln.valid = th < a.ntheta;
+  ln.sA = ra == 0 ? 1.0 : -1.0;
+  ln.sB = rb == 0 ? 1.0 : -1.0;
+  ln.rot = ra == 1 && rb == 1;
+  const int n2 = ra + 2 * rb;
+
+  constexpr int kTab = JT * KS * kWave;
+  double* cosN = lds;
+  double* sinN = lds + nphi;
+  double* tabC = lds + 2 * nphi;
+  double* tabS = tabC + kTab;
+  for (int i0 = 0; i0 < nphi; i0 += 2 * kThreads) {
+    double c[2], sn[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = i0 + threadIdx.x + u * kThreads;
+      c[u] = i < nphi ? a.tables.cosN[i] : 0.0;
+      sn[u] = i < nphi ? a.tables.sinN[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = i0 + threadIdx.x + u * kThreads;
+      if (i < nphi) {
+        cosN[i] = c[u];
+        sinN[i] = sn[u];
+      }
+    }
+  }
+  {
+    constexpr int kPerThread = kTab / kThreads;
+    double tc[kPerThread], ts[kPerThread];
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      tc[u] = a.tables.tabC[threadIdx.x + u * kThreads];
+      ts[u] = a.tables.tabS[threadIdx.x + u * kThreads];
+    }
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      tabC[threadIdx.x + u * kThreads] = tc[u];
+      tabS[threadIdx.x + u * kThreads] = ts[u];
+    }
+  }
+
+  // the lane's points j (clamped to H past the last frequency), P - j, 0
+  auto jOf = [&](int v) { return 1 + 16 * (v / 4) + 4 * (v % 4) + ln.quad; };
+  auto live = [&](int v) { return ln.valid && (FULL || jOf(v) <= H); };
+  auto jc = [&](int v) { return FULL ? jOf(v) : min(jOf(v), H); };
+  const size_t base = (static_cast<size_t>(node) * a.ntheta + (ln.valid ? th : th0)) * nphi +
+                      static_cast<size_t>(P) * n2;
+  const double mu = a.muNode[node];
+  double yjr[KS], yji[KS], yqr[KS], yqi[KS], y0r, y0i;
+  if constexpr (START) {
+    const double2* d0 = reinterpret_cast<const double2*>(a.gridIn) + base;
+    const double2* mIn = reinterpret_cast<const double2*>(a.memIn) + base;
+    double2* mOut = reinterpret_cast<double2*>(a.memOut) + base;
+    const double* itau = a.itau + base;
+    const double href = a.dt * a.itauRefNode[node];
+    auto phi1 = [](double h) {
+      return fabs(h) < 1e-3 ? 1.0 - h / 2.0 + h * h / 6.0 - h * h * h / 24.0 +
+                                  h * h * h * h / 120.0
+                            : (1.0 - exp(-h)) / h;
+    };
+    const double refBeta = a.scheme == 1 ? 1.0 - 1.0 / (1.0 + href) : 1.0 - phi1(href);
+    // m~ at a point from its loaded values (stored where `store`), and W d0
+    auto start = [&](int n1, double2 d, double2 m, double it, bool store, double& wr,
+                     double& wi) {
+      const double h = a.dt * it;
+      double ew, aw, bw;
+      if (a.scheme == 1) {
+        const double w = 1.0 / (1.0 + h);
+        ew = w;
+        aw = 0.0;
+        bw = 1.0 - w;
+      } else {
+        const double e = exp(-h);
+        const double p = phi1(h);
+        ew = e;
+        aw = p - e;
+        bw = 1.0 - p;
+      }
+      if (store) mOut[n1] = make_double2(ew * m.x + aw * d.x, ew * m.y + aw * d.y);
+      const double W = mu * (bw - refBeta);
+      wr = W * d.x;
+      wi = W * d.y;
+    };
+    start(0, d0[0], mIn[0], itau[0], ln.valid && ln.quad == 0, y0r, y0i);
+    if (!ln.valid) y0r = y0i = 0.0;
+    // a group's loads issue together: each store of m~ (to the buffer the
+    // memory is read from) orders the loads after it behind it
+    constexpr int kGroup = KS % 6 == 0 ? 6 : 4;
+#pragma unroll
+    for (int g0 = 0; g0 < KS; g0 += kGroup) {
+      double2 dj[kGroup], dq[kGroup], mj[kGroup], mq[kGroup];
+      double hj[kGroup], hq[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const int j = jc(g0 + u);
+        dj[u] = d0[j];
+        dq[u] = d0[P - j];
+        mj[u] = mIn[j];
+        mq[u] = mIn[P - j];
+        hj[u] = itau[j];
+        hq[u] = itau[P - j];
+      }
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const int v = g0 + u, j = jc(v);
+        start(j, dj[u], mj[u], hj[u], live(v), yjr[v], yji[v]);
+        start(P - j, dq[u], mq[u], hq[u], live(v), yqr[v], yqi[v]);
+        if (!live(v)) yjr[v] = yji[v] = yqr[v] = yqi[v] = 0.0;
+      }
+    }
+  } else {
+    const double2* m = reinterpret_cast<const double2*>(a.memOut) + base;
+    const double2 v0 = m[0];
+    y0r = ln.valid ? mu * v0.x : 0.0;
+    y0i = ln.valid ? mu * v0.y : 0.0;
+#pragma unroll
+    for (int v = 0; v < KS; ++v) {
+      const int j = jc(v);
+      const double2 xj = m[j], xq = m[P - j];
+      yjr[v] = live(v) ? mu * xj.x : 0.0;
+      yji[v] = live(v) ? mu * xj.y : 0.0;
+      yqr[v] = live(v) ? mu * xq.x : 0.0;
+      yqi[v] = live(v) ? mu * xq.y : 0.0;
+    }
+  }
+  __syncthreads();  // the tables are staged
+  PhaseFence();
+  ComplexForwardStore<JT>(a, ln, th, node, cosN, sinN, tabC, tabS, yjr, yji, yqr, yqi, y0r, y0i,
+                          reinterpret_cast<double2*>(START ? a.fourierOut2 : a.fourierOut),
+                          FULL);
+}
+
 }  // namespace complex_lanes
 }  // namespace sl3amd

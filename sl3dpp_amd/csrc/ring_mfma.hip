@@ -728,9 +728,31 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W
   }
 }
 
+// The step start of a complex block from a stored strain grid on the
+// complex-lane body: m~ and the analysis of W d0, then that of mu m~.
+template <int JT, bool FULL, bool START>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WavesPerEu<JT>())))
+    RingStartComplexKernel(RingLaunch a) {
+  complex_lanes::ComplexStartBody<JT, FULL, START>(a, blockIdx.y,
+                                                   blockIdx.x * complex_lanes::kRingsPerGroup);
+}
+
 template <int JT, int CLASS, bool FULL>
 void LaunchClass(const RingLaunch& r, dim3 grid, size_t lds, hipStream_t stream) {
   if constexpr (CLASS == kClassStart) {
+    if (r.block != 2 && r.op == kRingStart && r.tables.R == 4 && r.lmax >= 3) {
+      // 16 rings of one node per workgroup, as the staged body's grid for a
+      // complex block; LDS: the twiddles and the operand tables
+      const size_t ldsC = sizeof(double) * (2 * static_cast<size_t>(r.nphi) +
+                                            2 * static_cast<size_t>(JT) * 4 * JT * kWave);
+      const dim3 gridC(CeilDiv(r.ntheta, complex_lanes::kRingsPerGroup), r.nodes);
+      hipLaunchKernelGGL((RingStartComplexKernel<JT, FULL, true>), gridC, dim3(kThreads), ldsC,
+                         stream, r);
+      SL3_HIP(hipGetLastError());
+      hipLaunchKernelGGL((RingStartComplexKernel<JT, FULL, false>), gridC, dim3(kThreads), ldsC,
+                         stream, r);
+      return;
+    }
     if (r.block == 2) {
       hipLaunchKernelGGL((RingMfmaStartKernel<JT, true, FULL>), grid, dim3(kThreads), lds,
                          stream, r);
