@@ -275,6 +275,33 @@ class Viscoelastic3D {
     return out;
   }
 
+  /* Not in the reference: realisations side by side
+   * (Options::realisations). Realisation r's memory nodes are nodes
+   * r N .. r N + N - 1 of every node-indexed tensor, N = RadialNodes(). */
+  int Realisations() const {
+    int r = 0;
+    Check(sl3_operator_realisations(handle_, &r, nullptr));
+    return r;
+  }
+  int RadialNodes() const {
+    int n = 0;
+    Check(sl3_operator_realisations(handle_, nullptr, &n));
+    return n;
+  }
+  /* Per realisation, the last step's own StepInfo. */
+  std::vector<StepInfo> RealisationInfos() const {
+    int count = 0;
+    Check(sl3_step_realisation_infos(handle_, 0, &count, nullptr));
+    std::vector<sl3_step_info> c(static_cast<size_t>(count));
+    Check(sl3_step_realisation_infos(handle_, count, &count, c.data()));
+    std::vector<StepInfo> out;
+    for (const auto& i : c) {
+      out.push_back(StepInfo{i.iterations, i.residual, i.converged != 0, i.mixed_passes,
+                             i.solve_iterations, i.warm_fallback != 0});
+    }
+    return out;
+  }
+
   sl3_operator* Handle() const { return handle_; }
 
  private:

@@ -8,7 +8,8 @@
 //        hold every degree (spheroidal 0..L, toroidal 1..L, ndof x (l + 1));
 //        a negative dt and a tide with a degree-1 column throw
 //        std::invalid_argument; an exception thrown by a load channel comes
-//        out of Step as itself.
+//        out of Step as itself; two realisations side by side report their
+//        own infos, the step's the larger pass count.
 // Prints one line per check and "ok" at the end; exits non-zero on failure.
 #include <hip/hip_runtime_api.h>
 
@@ -228,6 +229,31 @@ int Gpu() {
            "a state of another operator is refused");
     Expect(Throws<std::invalid_argument>([&] { (void)op.Export(s, "m0"); }),
            "exporting another operator's state is refused");
+  }
+  // two realisations side by side (the second without a lateral field: the
+  // first's field and zeros), each with its own info
+  {
+    Viscoelastic3D::Options o3 = options;
+    o3.realisations = 2;
+    Viscoelastic3D pair(earth, lmax, nullptr, o3);
+    Expect(pair.Realisations() == 2 && pair.RadialNodes() == nodes,
+           "an operator of two realisations");
+    double* both = nullptr;
+    const size_t field = static_cast<size_t>(nodes) * ntheta * nphi;
+    if (hipMalloc(&both, sizeof(double) * 2 * field) != hipSuccess) return 1;
+    if (hipMemcpy(both, delta_dev, sizeof(double) * field, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemset(both + field, 0, sizeof(double) * field) != hipSuccess)
+      return 1;
+    sl3dpp_amd::Check(sl3_operator_set_delta(pair.Handle(), both));
+    auto s2 = pair.ZeroState();
+    const auto info = pair.StepInPlace(s2, dt, Scheme::ExponentialTrapezoid, load);
+    const auto per = pair.RealisationInfos();
+    Expect(per.size() == 2 && per[0].converged && per[1].converged && info.converged,
+           "both realisations converge");
+    Expect(per.size() == 2 &&
+               info.iterations == std::max(per[0].iterations, per[1].iterations),
+           "the step's passes are the most of its realisations'");
+    (void)hipFree(both);
   }
   (void)hipFree(delta_dev);
   return failures;
