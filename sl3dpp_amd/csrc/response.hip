@@ -55,26 +55,33 @@ __global__ void __launch_bounds__(kThreads)
   g[t.gOff + idx] = v;
 }
 
-// One workgroup: a 64 x 128 block of one task's x (row tiles rb .. rb + 3,
-// column tiles cb .. cb + 7), the waves 2 x 2 with 2 x 4 matrix-core tiles
-// each. The contraction runs in chunks of 16: each chunk's G tiles (already
-// in operand order) and f rows are staged in LDS and shared by the four
+// One workgroup: an RT x CT block of one task's 16 x 16 tiles of x (row
+// tiles rb .. rb + RT - 1, column tiles cb .. cb + CT - 1), each wave 2 x 4
+// matrix-core tiles: the waves 2 x 2 for a wide task's 64 x 128 blocks (RT
+// 4, CT 8), 4 x 1 for a narrow task's 128 x 64 (RT 8, CT 4; a task of at
+// most four column tiles then keeps all four waves on products). The
+// contraction runs in chunks of 16: each chunk's G tiles (already in
+// operand order) and f rows are staged in LDS and shared by the four
 // waves, the next chunk's loads in flight under the current products.
 // Tiles past the task's padded edge are staged as zeros; x is written
 // column by column (order major), the layout strain recovery reads with
-// neighbouring nodes in neighbouring lanes.
-constexpr int kRowTiles = 4;   // 64 rows per workgroup
-constexpr int kColTiles = 8;   // 128 columns per workgroup
-constexpr int kChunk = 4;      // contraction steps of 4 per staged chunk
-constexpr int kAPer = kRowTiles * kChunk * kWave / kThreads;    // staged A per thread
-constexpr int kBPer = 4 * kChunk * 16 * kColTiles / kThreads;   // staged B per thread
+// neighbouring nodes in neighbouring lanes. Every x element sums its
+// contraction in the same order whatever the tiling.
+constexpr int kChunk = 4;  // contraction steps of 4 per staged chunk
 
-__global__ void __launch_bounds__(kThreads)
-    ResponseKernel(const DegreeTask* tasks, const ResponseWork* work, const double* g,
-                   const double* f, double* x, const int* live, int realisations) {
-  __shared__ double As[kRowTiles * kChunk][kWave];              // [tile][lane]
-  __shared__ double Bs[4 * kChunk][16 * kColTiles + 1];         // [k][column]
-  const ResponseWork w = work[blockIdx.x];
+template <int RT, int CT>
+__device__ inline void ResponseBody(const DegreeTask* tasks, const ResponseWork& w,
+                                    const double* g, const double* f, double* x,
+                                    const int* live, int realisations, double* ldsA,
+                                    double* ldsB) {
+  constexpr int kRowTiles = RT, kColTiles = CT;
+  constexpr int kWavesCol = 4 / (RT / 2);  // waves along the columns
+  constexpr int kAPer = kRowTiles * kChunk * kWave / kThreads;   // staged A per thread
+  constexpr int kBPer = 4 * kChunk * 16 * kColTiles / kThreads;  // staged B per thread
+  constexpr int kBRow = 16 * kColTiles + 1;
+  // As[tile][lane] and Bs[k][column] in the kernel's LDS
+  auto As = [&](int i, int j) -> double& { return ldsA[i * kWave + j]; };
+  auto Bs = [&](int i, int j) -> double& { return ldsB[i * kBRow + j]; };
   const DegreeTask t = tasks[w.task];
   if (live) {  // skip a tile of converged realisations' columns only
     const int c0 = 16 * w.cb, c1 = min(c0 + 16 * kColTiles, realisations * t.cols) - 1;
@@ -84,7 +91,8 @@ __global__ void __launch_bounds__(kThreads)
   }
   const int rbCount = t.nmPad / 16, k4 = t.nmPad / 4, cbCount = t.colsPad / 16;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int wr = wave / 2, wc = wave % 2;  // the wave's 2 row tiles and 4 column tiles
+  // the wave's 2 row tiles and 4 column tiles
+  const int wr = wave / kWavesCol, wc = wave % kWavesCol;
   const long long ld = t.colsPad;
 
   double pa[kAPer], pb[kBPer];
@@ -113,12 +121,12 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int q = 0; q < kAPer; ++q) {
       const int e = threadIdx.x + kThreads * q;
-      As[e / kWave][e % kWave] = pa[q];
+      As(e / kWave, e % kWave) = pa[q];
     }
 #pragma unroll
     for (int q = 0; q < kBPer; ++q) {
       const int e = threadIdx.x + kThreads * q;
-      Bs[e / (16 * kColTiles)][e % (16 * kColTiles)] = pb[q];
+      Bs(e / (16 * kColTiles), e % (16 * kColTiles)) = pb[q];
     }
   };
 
@@ -136,13 +144,13 @@ __global__ void __launch_bounds__(kThreads)
     if (kc + 1 < chunks) load(kc + 1);
 #pragma unroll
     for (int s = 0; s < kChunk; ++s) {
-      const double a0 = As[(2 * wr) * kChunk + s][lane];
-      const double a1 = As[(2 * wr + 1) * kChunk + s][lane];
+      const double a0 = As((2 * wr) * kChunk + s, lane);
+      const double a1 = As((2 * wr + 1) * kChunk + s, lane);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         // tiles past the task's padded edge are skipped (wave-uniform)
         if (w.cb + 4 * wc + c < cbCount) {
-          const double b = Bs[4 * s + lane / 16][(4 * wc + c) * 16 + lane % 16];
+          const double b = Bs(4 * s + lane / 16, (4 * wc + c) * 16 + lane % 16);
           if (w.rb + 2 * wr < rbCount) {
             acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b, acc[0][c], 0, 0, 0);
           }
@@ -168,6 +176,22 @@ __global__ void __launch_bounds__(kThreads)
         x[t.fOff + col * t.nmPad + row] = acc[r][c][j];  // column-major
       }
     }
+  }
+}
+
+// Both tilings in one launch (each item's pad says which), so the narrow
+// tasks' workgroups fill in beside the wide ones'.
+__global__ void __launch_bounds__(kThreads)
+    ResponseKernel(const DegreeTask* tasks, const ResponseWork* work, const double* g,
+                   const double* f, double* x, const int* live, int realisations) {
+  // the larger of the two tilings' staging: A 8 x 4 tiles, B 16 x (16 x 8 + 1)
+  __shared__ double ldsA[8 * kChunk * kWave];
+  __shared__ double ldsB[4 * kChunk * (16 * 8 + 1)];
+  const ResponseWork w = work[blockIdx.x];
+  if (w.pad == 1) {
+    ResponseBody<8, 4>(tasks, w, g, f, x, live, realisations, ldsA, ldsB);
+  } else {
+    ResponseBody<4, 8>(tasks, w, g, f, x, live, realisations, ldsA, ldsB);
   }
 }
 
@@ -241,20 +265,31 @@ void LaunchSolutionsToRows(const DegreeTask* tasks_dev, int taskCount, int nmMax
 }
 
 std::vector<ResponseWork> ResponseWorkList(const std::vector<DegreeTask>& tasks) {
-  std::vector<ResponseWork> work;
+  std::vector<ResponseWork> narrow, wide;
   for (size_t ti = 0; ti < tasks.size(); ++ti) {
     const auto& t = tasks[ti];
-    for (int rb = 0; rb < t.nmPad / 16; rb += kRowTiles) {
-      for (int cb = 0; cb < t.colsPad / 16; cb += kColTiles) {
-        work.push_back({static_cast<int>(ti), rb, cb, 0});
+    const bool thin = t.colsPad / 16 <= kNarrowColTiles;
+    const int rStep = thin ? 8 : 4, cStep = thin ? 4 : 8;
+    for (int rb = 0; rb < t.nmPad / 16; rb += rStep) {
+      for (int cb = 0; cb < t.colsPad / 16; cb += cStep) {
+        (thin ? narrow : wide).push_back({static_cast<int>(ti), rb, cb, thin ? 1 : 0});
       }
     }
   }
   // the longest contractions first, so the short ones fill the tail
-  std::stable_sort(work.begin(), work.end(), [&](const ResponseWork& a, const ResponseWork& b) {
+  auto longest = [&](const ResponseWork& a, const ResponseWork& b) {
     return tasks[a.task].nmPad > tasks[b.task].nmPad;
-  });
-  return work;
+  };
+  std::stable_sort(narrow.begin(), narrow.end(), longest);
+  std::stable_sort(wide.begin(), wide.end(), longest);
+  narrow.insert(narrow.end(), wide.begin(), wide.end());
+  return narrow;
+}
+
+int ResponseNarrowCount(const std::vector<ResponseWork>& work) {
+  int n = 0;
+  while (n < static_cast<int>(work.size()) && work[n].pad == 1) ++n;
+  return n;
 }
 
 void LaunchUnitColumns(const DegreeTask* tasks_dev, int taskCount, int nmMax,
@@ -277,10 +312,11 @@ void LaunchPackResponse(const DegreeTask* tasks_dev, int taskCount, int nmPadMax
 }
 
 void LaunchResponse(const DegreeTask* tasks_dev, const ResponseWork* work_dev,
-                    int workCount, const double* g, const double* f, double* x,
+                    int workCount, int narrowCount, const double* g, const double* f, double* x,
                     hipStream_t stream, const int* live, int realisations) {
   if (workCount == 0) return;
   ProfileScope prof("response", stream);
+  (void)narrowCount;  // the items say their tiling
   hipLaunchKernelGGL(ResponseKernel, dim3(workCount), dim3(kThreads), 0, stream, tasks_dev,
                      work_dev, g, f, x, live, realisations);
   SL3_HIP(hipGetLastError());

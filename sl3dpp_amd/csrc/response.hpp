@@ -26,8 +26,13 @@ struct ResponseWork {
   int pad = 0;
 };
 
-// Every workgroup's work over the tasks, the largest contraction first.
+// Every workgroup's work over the tasks, the largest contraction first:
+// the narrow tasks' (at most kNarrowColTiles column tiles; pad = 1, 128 x 64
+// blocks) first, then the wide tasks' (64 x 128 blocks).
+constexpr int kNarrowColTiles = 4;
 std::vector<ResponseWork> ResponseWorkList(const std::vector<DegreeTask>& tasks);
+// The narrow items at the list's front.
+int ResponseNarrowCount(const std::vector<ResponseWork>& work);
 
 // z (per task at zOff[task], ndof x nm row-major, zero on entry): column j
 // gets scale[dm[j]] in row dm[j] (BeginSolve of the unit vector).
@@ -46,7 +51,7 @@ void LaunchPackResponse(const DegreeTask* tasks_dev, int taskCount, int nmPadMax
 // live (per realisation, or null): the column tiles of realisations marked
 // 0 (converged; realisation r's columns at r cols) are skipped.
 void LaunchResponse(const DegreeTask* tasks_dev, const ResponseWork* work_dev,
-                    int workCount, const double* g, const double* f, double* x,
+                    int workCount, int narrowCount, const double* g, const double* f, double* x,
                     hipStream_t stream, const int* live = nullptr, int realisations = 1);
 
 // The full scaled right-hand sides from the memory-dof force (for a band

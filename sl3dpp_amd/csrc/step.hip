@@ -964,6 +964,7 @@ void Viscoelastic3D::BuildTasks() {
   const auto work = ResponseWorkList(tasks_);
   work_.Upload(work.data(), work.size(), s);
   workCount_ = static_cast<int>(work.size());
+  narrowCount_ = ResponseNarrowCount(work);
   const auto forceWork = ForceWork(L);
   forceWork_.Upload(forceWork.data(), forceWork.size(), s);
   forceWorkCount_ = static_cast<int>(forceWork.size());
@@ -1295,7 +1296,7 @@ void Viscoelastic3D::SolveAll(const double* weighted, Flavour& f) {
   const SpectralView w = coupling_.View(const_cast<double*>(weighted));
   LaunchForce(tasksDev_.data(), coupling_.Geometry(), w, dofsMem_.data(), forceWork_.data(),
               forceWorkCount_, force_.data(), s, Live());
-  LaunchResponse(tasksDev_.data(), work_.data(), workCount_, f.response.data(),
+  LaunchResponse(tasksDev_.data(), work_.data(), workCount_, narrowCount_, f.response.data(),
                  force_.data(), response_.data(), s, Live(), coupling_.Realisations());
 }
 
@@ -2244,7 +2245,8 @@ StepInfo Viscoelastic3D::CloseByConjugateGradients(Flavour& f) {
   cgPartials_.Resize(std::max(pairWorkCount_, 1));
   const ChannelTerms none{};
   auto response = [&](const double* force, double* out) {
-    LaunchResponse(tasksDev_.data(), work_.data(), workCount_, f.response.data(), force, out,
+    LaunchResponse(tasksDev_.data(), work_.data(), workCount_, narrowCount_, f.response.data(),
+                   force, out,
                    s);
   };
   // x0 = K^-1 f (OpenStep::Forces, :1448-1471): the fixed memory force's

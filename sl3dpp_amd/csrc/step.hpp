@@ -445,7 +445,7 @@ class Viscoelastic3D {
   size_t gDoubles_ = 0, fDoubles_ = 0;
   int nmMax_ = 0, nmPadMax_ = 0, colsMax_ = 0;
   DeviceArray<ResponseWork> work_;
-  int workCount_ = 0;
+  int workCount_ = 0, narrowCount_ = 0;
   DeviceArray<int2> forceWork_;  // the force kernel's (degree, order chunk) list
   int forceWorkCount_ = 0;
   DeviceArray<PairWork> pairWork_;  // the pairing's tiles
