@@ -75,6 +75,7 @@ struct Args {
   int fourierMlo[3];
   const int* live;  // per realisation, 0: converged, its column blocks skipped
   int liveNodes;
+  int keepInactive;  // truncated analysis: entries below a degree's truncation untouched
 };
 
 // The largest degree any node of a column block carries (workgroup-uniform
@@ -232,7 +233,10 @@ __global__ void __launch_bounds__(kThreads) LegendreAnalysisMfmaKernel(Args a) {
         if (l > lEnd) continue;
         const size_t row = static_cast<size_t>(row0 + l);
         double v = l <= lhi ? acc[t][j] : 0.0;
-        if (a.firstNode && a.reach[a.nodeOffset + (c >> 1)] < l) v = 0.0;
+        if (a.firstNode && a.reach[a.nodeOffset + (c >> 1)] < l) {
+          if (a.keepInactive) continue;
+          v = 0.0;
+        }
         if (add) v += add[row * rowCols + c];
         out[row * rowCols + c] = v;
       }
@@ -366,6 +370,7 @@ Args MakeArgs(const LegendrePlan& p) {
   a.fstride = p.fstride;
   a.live = p.live;
   a.liveNodes = p.liveNodes;
+  a.keepInactive = p.keepInactive ? 1 : 0;
   return a;
 }
 

@@ -399,7 +399,7 @@ void Coupling::SynthesiseToFourier(const double* spectral, double* const fourier
 
 void Coupling::AnalyseFromFourier(const double* const fourier[3], double* spectral,
                                   const double* addend, bool truncated, hipStream_t s,
-                                  const int* live) const {
+                                  const int* live, bool keepInactive) const {
   double* blocks[3] = {spectral + spectralOffset_[0], spectral + spectralOffset_[1],
                        spectral + spectralOffset_[2]};
   const double* adds[3] = {};
@@ -412,6 +412,7 @@ void Coupling::AnalyseFromFourier(const double* const fourier[3], double* spectr
     LegendrePlan p = k->plan;
     p.live = live;
     p.liveNodes = geometry_.nodes;
+    p.keepInactive = truncated && keepInactive;
     LaunchLegendreAnalysis(p, in, blocks, addend ? adds : nullptr,
                            truncated ? reach_.data() : nullptr,
                            truncated ? firstNode_.data() : nullptr, s);
@@ -1418,7 +1419,7 @@ void Viscoelastic3D::LaunchPass(const double* s, double* sNext, bool useStartFor
     double* B[3] = {fourierB_[0].data(), fourierB_[1].data(), fourierB_[2].data()};
     coupling_.SynthesiseToFourier(s, A, true, st, Live());
     coupling_.RingPass(A, B, weight_.data(), st, Live());
-    coupling_.AnalyseFromFourier(B, analysed_.data(), fixed_.data(), true, st, Live());
+    coupling_.AnalyseFromFourier(B, analysed_.data(), fixed_.data(), true, st, Live(), true);
   }
   SolveAll(analysed_.data(), f);
   const auto* sph = tasksDev_.data();
@@ -1650,8 +1651,11 @@ void Viscoelastic3D::OpenStart(DeviceState& state, double dt, int scheme,
   }
   coupling_.AnalyseFromFourier(B, fixed_.data(), nullptr, true, s);
   const bool cg = options_.solver == kConjugateGradients;
+  // analysed_ stays zero below each degree's truncation (only truncated
+  // analyses write it, and the force reads it only above): its analyses
+  // leave those entries
   // the first pass's start force (the fixed point's only)
-  if (!cg) coupling_.AnalyseFromFourier(A, analysed_.data(), fixed_.data(), true, s);
+  if (!cg) coupling_.AnalyseFromFourier(A, analysed_.data(), fixed_.data(), true, s, nullptr, true);
   SL3_HIP(hipEventRecord(events_[1], s));
   open_.state = &state;
   open_.flavour = &f;

@@ -159,9 +159,11 @@ class Coupling {
   // outputs left as they were).
   void SynthesiseToFourier(const double* spectral, double* const fourier[3],
                            bool truncated, hipStream_t s, const int* live = nullptr) const;
+  // keepInactive (truncated only): the output is zero below each degree's
+  // truncation already and read only above it; those entries are left.
   void AnalyseFromFourier(const double* const fourier[3], double* spectral,
                           const double* addend, bool truncated, hipStream_t s,
-                          const int* live = nullptr) const;
+                          const int* live = nullptr, bool keepInactive = false) const;
   void Ring(RingLaunch r, hipStream_t s) const;  // fills tables and sizes
   // The pass's longitude stage (Fourier -> ring x weight -> Fourier) over
   // all three blocks: one launch where the grid allows it.

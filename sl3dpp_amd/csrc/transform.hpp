@@ -36,6 +36,10 @@ struct LegendrePlan {
   // node) all have live[node / liveNodes] == 0 is skipped (null: none)
   const int* live = nullptr;
   int liveNodes = 1;
+  // a truncated analysis whose output is zero below each degree's
+  // truncation already (and read only above it) leaves those entries as
+  // they are: neither the zeros written nor the addend read there
+  bool keepInactive = false;
 };
 
 constexpr int kDefaultAnalysisRows = 64;
