@@ -339,23 +339,54 @@ __global__ void __launch_bounds__(1024)
   if (live && live[real] == 0) return;  // converged
   const int width = g.nodes * g.realisations;
   {
+    // every operand's loads of a round issued before any of its LDS
+    // stores: one memory latency per round, not one per operand and entry
     const double* re = response + ts.fOff + static_cast<size_t>(real * ts.cols + m) * ts.nmPad;
     const double* im =
         response + ts.fOff + static_cast<size_t>(real * ts.cols + l + 1 + m) * ts.nmPad;
-    for (int j = threadIdx.x; j < ts.nm; j += blockDim.x) {
-      sRe[j] = re[j];
-      sIm[j] = im[j];
-      if (unitsOn) sUnit[j] = units[ts.dmOff + j];
-      if (tide) sTide[j] = tideUnits[ts.dmOff + j];
-    }
-    for (int j = threadIdx.x; j < 2 * table; j += blockDim.x) sDof[j] = dofs[ts.dofOff + j];
-    if (tor1) {
-      const double* bt = response + tt.fOff;
-      for (int j = threadIdx.x; j < tt.nm; j += blockDim.x) {
-        tRe[j] = bt[static_cast<size_t>(real * tt.cols + m) * tt.nmPad + j];
-        tIm[j] = bt[static_cast<size_t>(real * tt.cols + l + 1 + m) * tt.nmPad + j];
+    const double* tre =
+        tor1 ? response + tt.fOff + static_cast<size_t>(real * tt.cols + m) * tt.nmPad : nullptr;
+    const double* tim =
+        tor1 ? response + tt.fOff + static_cast<size_t>(real * tt.cols + l + 1 + m) * tt.nmPad
+             : nullptr;
+    const int nS = ts.nm, nD = 2 * table, nT = tor1 ? tt.nm : 0, nTD = tor1 ? table : 0;
+    const int most = max(max(nS, nD), max(nT, nTD));
+    constexpr int kRound = 4;  // entries per thread and operand in one round
+    for (int base = 0; base < most; base += kRound * static_cast<int>(blockDim.x)) {
+      double vRe[kRound], vIm[kRound], vUnit[kRound], vTide[kRound], vTRe[kRound], vTIm[kRound];
+      int vDof[kRound], vTDof[kRound];
+#pragma unroll
+      for (int u = 0; u < kRound; ++u) {
+        const int j = base + u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+        if (j < nS) {
+          vRe[u] = re[j];
+          vIm[u] = im[j];
+          if (unitsOn) vUnit[u] = units[ts.dmOff + j];
+          if (tide) vTide[u] = tideUnits[ts.dmOff + j];
+        }
+        if (j < nD) vDof[u] = dofs[ts.dofOff + j];
+        if (j < nT) {
+          vTRe[u] = tre[j];
+          vTIm[u] = tim[j];
+        }
+        if (j < nTD) vTDof[u] = dofs[tt.dofOff + 2 * j];
       }
-      for (int j = threadIdx.x; j < table; j += blockDim.x) tDof[j] = dofs[tt.dofOff + 2 * j];
+#pragma unroll
+      for (int u = 0; u < kRound; ++u) {
+        const int j = base + u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+        if (j < nS) {
+          sRe[j] = vRe[u];
+          sIm[j] = vIm[u];
+          if (unitsOn) sUnit[j] = vUnit[u];
+          if (tide) sTide[j] = vTide[u];
+        }
+        if (j < nD) sDof[j] = vDof[u];
+        if (j < nT) {
+          tRe[j] = vTRe[u];
+          tIm[j] = vTIm[u];
+        }
+        if (j < nTD) tDof[j] = vTDof[u];
+      }
     }
   }
   __syncthreads();
