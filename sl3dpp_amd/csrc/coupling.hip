@@ -301,9 +301,11 @@ __global__ void __launch_bounds__(1024)
 
 // StrainKernel with the workgroup's operands staged in LDS first: the
 // response columns m and l + 1 + m of the degree's spheroidal and toroidal
-// tasks, the unit load and tide responses, and both dof tables, each read
-// once with consecutive threads on consecutive entries. A thread then reads
-// its node's GLL values from LDS (no dependent global loads) and computes
+// tasks, the unit load and tide responses, both dof tables and the GLL
+// value and derivative tables, each read once with consecutive threads on
+// consecutive entries (the node's radius, element size and the channel
+// amplitudes are loaded beside them). A thread then reads its node's GLL
+// values from LDS (no dependent global loads) and computes
 // exactly what StrainKernel computes, in the same order. Sizes: capS >=
 // every spheroidal task's memory dofs (at most elements x ngll x 2, each
 // memory dof being some (element, GLL node, component) of the table), capT
@@ -321,7 +323,10 @@ __global__ void __launch_bounds__(1024)
   double* sTide = sUnit + capS;
   double* tRe = sTide + capS;
   double* tIm = tRe + capT;
-  int* sDof = reinterpret_cast<int*>(tIm + capT);
+  const int basis = g.np * g.ngll;
+  double* sValue = tIm + capT;  // g.value, then g.deriv
+  double* sDeriv = sValue + basis;
+  int* sDof = reinterpret_cast<int*>(sDeriv + basis);
   const int table = g.elements * g.ngll;
   int* tDof = sDof + 2 * table;
   const int pair = blockIdx.x;
@@ -337,6 +342,16 @@ __global__ void __launch_bounds__(1024)
   const bool unitsOn = units != nullptr && ts.unitOff >= 0;  // no load channel: none
   const int real = blockIdx.y;  // the realisation
   if (live && live[real] == 0) return;  // converged
+  // the thread's first node's radius and element size and the channels'
+  // amplitudes, loaded before the barrier
+  const int first = threadIdx.x;
+  double firstRadius = 0.0, firstJac = 0.0;
+  if (first < g.nodes) {
+    firstRadius = g.radius[first];
+    firstJac = g.jac[first / g.np];
+  }
+  const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
+  const Cplx at = tide ? C(tideAmp[2 * pair], tideAmp[2 * pair + 1]) : C(0.0, 0.0);
   const int width = g.nodes * g.realisations;
   {
     // every operand's loads of a round issued before any of its LDS
@@ -350,10 +365,11 @@ __global__ void __launch_bounds__(1024)
         tor1 ? response + tt.fOff + static_cast<size_t>(real * tt.cols + l + 1 + m) * tt.nmPad
              : nullptr;
     const int nS = ts.nm, nD = 2 * table, nT = tor1 ? tt.nm : 0, nTD = tor1 ? table : 0;
-    const int most = max(max(nS, nD), max(nT, nTD));
-    constexpr int kRound = 4;  // entries per thread and operand in one round
+    const int most = max(max(max(nS, nD), max(nT, nTD)), basis);
+    constexpr int kRound = 3;  // entries per thread and operand in one round
     for (int base = 0; base < most; base += kRound * static_cast<int>(blockDim.x)) {
       double vRe[kRound], vIm[kRound], vUnit[kRound], vTide[kRound], vTRe[kRound], vTIm[kRound];
+      double vValue[kRound], vDeriv[kRound];
       int vDof[kRound], vTDof[kRound];
 #pragma unroll
       for (int u = 0; u < kRound; ++u) {
@@ -370,6 +386,10 @@ __global__ void __launch_bounds__(1024)
           vTIm[u] = tim[j];
         }
         if (j < nTD) vTDof[u] = dofs[tt.dofOff + 2 * j];
+        if (j < basis) {
+          vValue[u] = g.value[j];
+          vDeriv[u] = g.deriv[j];
+        }
       }
 #pragma unroll
       for (int u = 0; u < kRound; ++u) {
@@ -386,6 +406,10 @@ __global__ void __launch_bounds__(1024)
           tIm[j] = vTIm[u];
         }
         if (j < nTD) tDof[j] = vTDof[u];
+        if (j < basis) {
+          sValue[j] = vValue[u];
+          sDeriv[j] = vDeriv[u];
+        }
       }
     }
   }
@@ -394,12 +418,10 @@ __global__ void __launch_bounds__(1024)
     Cplx A[5] = {C(0, 0), C(0, 0), C(0, 0), C(0, 0), C(0, 0)};
     if (node >= ts.firstNode) {
       const int elem = g.elem0 + node / g.np, q = node % g.np;
-      const double r = g.radius[node];
-      const double jac = g.jac[elem - g.elem0];
+      const double r = node == first ? firstRadius : g.radius[node];
+      const double jac = node == first ? firstJac : g.jac[elem - g.elem0];
       Cplx U = C(0, 0), V = C(0, 0), W = C(0, 0), rdU = C(0, 0), rdV = C(0, 0),
            rdW = C(0, 0);
-      const Cplx a = amp ? C(amp[2 * pair], amp[2 * pair + 1]) : C(0.0, 0.0);
-      const Cplx at = tide ? C(tideAmp[2 * pair], tideAmp[2 * pair + 1]) : C(0.0, 0.0);
       for (int i = 0; i < g.ngll; ++i) {
         const int slot = (elem - g.elem0) * g.ngll + i;
         const int d0 = sDof[2 * slot], d1 = sDof[2 * slot + 1];
@@ -418,11 +440,11 @@ __global__ void __launch_bounds__(1024)
           const int dw = tDof[slot];
           if (dw >= 0) Wi = C(tRe[dw], tIm[dw]);
         }
-        const double v = g.value[q * g.ngll + i];
+        const double v = sValue[q * g.ngll + i];
         U = U + v * Ui;
         V = V + v * Vi;
         W = W + v * Wi;
-        const double bb = r * g.deriv[q * g.ngll + i] / jac;
+        const double bb = r * sDeriv[q * g.ngll + i] / jac;
         rdU = rdU + bb * Ui;
         rdV = rdV + bb * Vi;
         rdW = rdW + bb * Wi;
@@ -1044,8 +1066,9 @@ void LaunchStrain(const DegreeTask* sphTasks_dev, const DegreeTask* torTasks_dev
   // every task's memory dofs fit these bounds (see StrainStagedKernel)
   const int table = g.elements * g.ngll;
   const int capS = 2 * table, capT = table;
-  const size_t lds = sizeof(double) * (4 * static_cast<size_t>(capS) + 2 * capT) +
-                     sizeof(int) * (2 * static_cast<size_t>(table) + table);
+  const size_t lds =
+      sizeof(double) * (4 * static_cast<size_t>(capS) + 2 * capT + 2 * g.np * g.ngll) +
+      sizeof(int) * (2 * static_cast<size_t>(table) + table);
   if (lds <= 64 * 1024) {
     hipLaunchKernelGGL(StrainStagedKernel, dim3(pairs, g.realisations), dim3(threads), lds, stream,
                        sphTasks_dev, torTasks_dev, lmax, g, dofs, response, units, amp,
