@@ -98,6 +98,7 @@ __device__ inline void ForceRows(int l, const Cplx A[5], double r, Cplx rows[6])
 constexpr int kChunkElements = 4;
 constexpr int kMaxPoints = 6;  // memory points per element the LDS holds
 constexpr int kChunkOrders = 8;
+constexpr int kMaxBasis = 96;  // np x ngll GLL table entries the LDS holds
 
 // One workgroup: a chunk of memory elements, one degree l and kChunkOrders
 // orders, for the spheroidal and the toroidal system together (one read of
@@ -109,6 +110,11 @@ __global__ void __launch_bounds__(kThreads)
                 const int2* work, double* force, const int* live) {
   constexpr int maxPoints = (kChunkElements + 1) * kMaxPoints;
   __shared__ Cplx rowsL[maxPoints * kChunkOrders][6];
+  // per staged point its radius and jac weight radius; the GLL value and
+  // derivative tables (when they fit) and the reference weights
+  __shared__ double sRadius[maxPoints], sPoint[maxPoints];
+  __shared__ double sValue[kMaxBasis], sDeriv[kMaxBasis], sWeight[kMaxPoints];
+  static_assert(maxPoints <= kThreads && kMaxBasis <= kThreads, "one thread per staged entry");
   const int2 wk = work[blockIdx.y];
   const int l = wk.x, mc = wk.y;
   const int real = blockIdx.z;  // the realisation
@@ -132,6 +138,22 @@ __global__ void __launch_bounds__(kThreads)
   const int gnEnd = e1 * nm + (e1 == elemEnd ? 1 : 0);
   const int owned = gnEnd - gnFirst;
   const int mCount = min(kChunkOrders, l + 1 - mc);
+  // the geometry stage 2 reads, loaded here and stored after the gathers
+  // (one thread per entry: points, the basis and np all fit the workgroup)
+  const int basis = g.np * g.ngll;
+  const bool stagedBasis = basis <= kMaxBasis;
+  const int t = threadIdx.x;
+  double tRadius = 0.0, tPoint = 0.0, tValue = 0.0, tDeriv = 0.0, tWeight = 0.0;
+  if (t < points) {
+    const int elem = eStart + t / g.np, q = t % g.np;
+    tRadius = g.radius[(elem - g.elem0) * g.np + q];
+    tPoint = g.jac[elem - g.elem0] * g.weight[q] * tRadius;
+  }
+  if (stagedBasis && t < basis) {
+    tValue = g.value[t];
+    tDeriv = g.deriv[t];
+  }
+  if (t < g.np) tWeight = g.weight[t];
   // stage 1: rows of (point, m)
   for (int e = threadIdx.x; e < points * mCount; e += blockDim.x) {
     const int p = e % points, mi = e / points;
@@ -144,7 +166,18 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int k = 0; k < 6; ++k) dst[k] = rows[k];
   }
+  if (t < points) {
+    sRadius[t] = tRadius;
+    sPoint[t] = tPoint;
+  }
+  if (stagedBasis && t < basis) {
+    sValue[t] = tValue;
+    sDeriv[t] = tDeriv;
+  }
+  if (t < g.np) sWeight[t] = tWeight;
   __syncthreads();
+  const double* value = stagedBasis ? sValue : g.value;
+  const double* derivTable = stagedBasis ? sDeriv : g.deriv;
   // stage 2: each owned node's sums, order fastest (consecutive threads
   // write consecutive columns of one force row)
   double* bs = force + ts.fOff + real * ts.cols;
@@ -164,27 +197,27 @@ __global__ void __launch_bounds__(kThreads)
       elems[count] = ea;
       locals[count++] = ia;
     }
+    // the row's dofs first: their loads in flight under the sums
+    const int elemOwn = count > 0 ? elems[count - 1] : ea;
+    const int iOwn = count > 0 ? locals[count - 1] : ia;
+    const int at = ((elemOwn - g.elem0) * g.ngll + iOwn) * 2;
+    const int dU = dofs[ts.dofOff + at], dV = dofs[ts.dofOff + at + 1];
+    const int dW = hasTor ? dofs[tt.dofOff + at] : -1;
     Cplx U = C(0.0, 0.0), V = C(0.0, 0.0), W = C(0.0, 0.0);
     for (int k = 0; k < count; ++k) {
       const int elem = elems[k], i = locals[k];
-      const double jac = g.jac[elem - g.elem0];
       for (int q = 0; q < g.np; ++q) {
         const int p = (elem - eStart) * g.np + q;
-        const int node = (elem - g.elem0) * g.np + q;
-        const double r = g.radius[node];
-        const double point = jac * g.weight[q] * r;
-        const double v = point * g.value[q * g.ngll + i];
-        const double deriv = g.weight[q] * g.deriv[q * g.ngll + i] * r;
+        const double r = sRadius[p];
+        const double point = sPoint[p];  // jac weight[q] r
+        const double v = point * value[q * g.ngll + i];
+        const double deriv = sWeight[q] * derivTable[q * g.ngll + i] * r;
         const Cplx* rw = rowsL[mi * maxPoints + p];
         U = U + v * rw[0] + deriv * rw[3];
         V = V + v * rw[1] + deriv * rw[4];
         W = W + v * rw[2] + deriv * rw[5];
       }
     }
-    const int elem = count > 0 ? elems[count - 1] : ea;
-    const int i = count > 0 ? locals[count - 1] : ia;
-    const int at = ((elem - g.elem0) * g.ngll + i) * 2;
-    const int dU = dofs[ts.dofOff + at], dV = dofs[ts.dofOff + at + 1];
     if (dU >= 0) {
       bs[static_cast<size_t>(dU) * lds + m] = U.re;
       bs[static_cast<size_t>(dU) * lds + l + 1 + m] = U.im;
@@ -194,7 +227,6 @@ __global__ void __launch_bounds__(kThreads)
       bs[static_cast<size_t>(dV) * lds + l + 1 + m] = V.im;
     }
     if (hasTor) {
-      const int dW = dofs[tt.dofOff + at];
       if (dW >= 0) {
         bt[static_cast<size_t>(dW) * ldt + m] = W.re;
         bt[static_cast<size_t>(dW) * ldt + l + 1 + m] = W.im;
